@@ -1,0 +1,579 @@
+/*
+ * Whole-run driver of the oracle: start (start.f90:25-144) and the hydro
+ * loop (hydro.f90:46-87) with the phase drivers timestep.f90, PdV.f90,
+ * advection.f90, update_halo.f90, update_tile_halo.f90 and
+ * field_summary.f90, restated in C.
+ *
+ * TEST INFRASTRUCTURE ONLY (see clover_oracle.h).  The kernels it calls come
+ * from a table: either the C restatement in clover_oracle.c, or the
+ * reference's own Fortran kernels compiled into oracle/_ref/libclover_ref.so
+ * (loaded with dlopen, symbols ref_<kernel>).
+ */
+#include "clover_oracle.h"
+
+#include <dlfcn.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef void (*fn_ideal_gas)(OR_DIMS, const double *, const double *, double *, double *);
+typedef void (*fn_viscosity)(OR_DIMS, const double *, const double *, const double *, const double *,
+                             const double *, const double *, const double *, const double *, double *,
+                             const double *, const double *, const double *);
+typedef void (*fn_calc_dt)(OR_DIMS, double, double, double, double, double, double, double, double,
+                           const double *, const double *, const double *, const double *, const double *,
+                           const double *, const double *, const double *, const double *, const double *,
+                           const double *, const double *, const double *, const double *, const double *,
+                           const double *, const double *, const double *, double *, double *, int *, double *,
+                           double *, double *, int *, int *, int *, int *);
+typedef void (*fn_pdv)(int, OR_DIMS, double, const double *, const double *, const double *, const double *,
+                       const double *, double *, const double *, double *, const double *, const double *,
+                       const double *, const double *, const double *, const double *, const double *,
+                       const double *);
+typedef void (*fn_revert)(OR_DIMS, const double *, double *, const double *, double *);
+typedef void (*fn_accelerate)(OR_DIMS, double, const double *, const double *, const double *, const double *,
+                              const double *, const double *, const double *, const double *, const double *,
+                              const double *, double *, double *, double *);
+typedef void (*fn_flux_calc)(OR_DIMS, double, const double *, const double *, const double *, const double *,
+                             const double *, const double *, const double *, const double *, const double *,
+                             double *, double *, double *);
+typedef void (*fn_advec_cell)(OR_DIMS, int, int, int, const double *, const double *, const double *,
+                              const double *, double *, double *, double *, const double *, double *,
+                              const double *, double *, const double *, double *, double *, double *, double *,
+                              double *, double *, double *);
+typedef void (*fn_advec_mom)(OR_DIMS, double *, const double *, const double *, const double *, const double *,
+                             const double *, const double *, const double *, const double *, double *, double *,
+                             double *, double *, double *, double *, double *, const double *, const double *,
+                             const double *, int, int, int, int);
+typedef void (*fn_reset)(OR_DIMS, double *, const double *, double *, const double *, double *, const double *,
+                         double *, const double *, double *, const double *);
+typedef void (*fn_update_halo)(OR_DIMS, const int *, const int *, double *, double *, double *, double *, double *,
+                               double *, double *, double *, double *, double *, double *, double *, double *,
+                               double *, double *, double *, double *, double *, double *, const int *, int);
+typedef void (*fn_tile_halo)(int, OR_DIMS, double *const *, int, int, int, int, int, int, double *const *,
+                             const int *, int);
+typedef void (*fn_summary)(OR_DIMS, const double *, const double *, const double *, const double *,
+                           const double *, const double *, const double *, double *, double *, double *, double *,
+                           double *);
+
+typedef struct {
+    fn_ideal_gas ideal_gas;
+    fn_viscosity viscosity;
+    fn_calc_dt calc_dt;
+    fn_pdv pdv;
+    fn_revert revert;
+    fn_accelerate accelerate;
+    fn_flux_calc flux_calc;
+    fn_advec_cell advec_cell;
+    fn_advec_mom advec_mom;
+    fn_reset reset;
+    fn_update_halo update_halo;
+    fn_tile_halo tile_halo;
+    fn_summary summary;
+} ktab;
+
+enum { F_D0, F_D1, F_E0, F_E1, F_P, F_Q, F_C, F_U0, F_U1, F_V0, F_V1, F_W0, F_W1, F_VFX, F_VFY, F_VFZ, F_MFX, F_MFY, F_MFZ };
+
+typedef struct {
+    int nx, ny, nz;
+    int left, bottom, back;
+    int nbr[6];
+    double *d0, *d1, *e0, *e1, *p, *q, *c, *vol;
+    double *u0, *u1, *v0, *v1, *w0, *w1;
+    double *w[8]; /* work_array1..7 at w[1..7] */
+    double *vfx, *mfx, *xarea, *vfy, *mfy, *yarea, *vfz, *mfz, *zarea;
+    double *cellx, *celly, *cellz, *celldx, *celldy, *celldz;
+    double *vertexx, *vertexy, *vertexz, *vertexdx, *vertexdy, *vertexdz;
+} otile;
+
+#define TD(t) 1, (t)->nx, 1, (t)->ny, 1, (t)->nz
+
+static double *zalloc(size_t n)
+{
+    double *a = (double *)calloc(n, sizeof(double));
+    if (!a) {
+        fprintf(stderr, "oracle: out of memory\n");
+        exit(1);
+    }
+    return a;
+}
+
+/* build_field.f90:41-209: allocate and zero every array of a tile */
+static void build_field(otile *t)
+{
+    size_t cx = t->nx + 4, cy = t->ny + 4, cz = t->nz + 4;
+    size_t vx = t->nx + 5, vy = t->ny + 5, vz = t->nz + 5;
+    size_t nc = cx * cy * cz, nv = vx * vy * vz;
+    t->d0 = zalloc(nc); t->d1 = zalloc(nc); t->e0 = zalloc(nc); t->e1 = zalloc(nc);
+    t->p = zalloc(nc); t->q = zalloc(nc); t->c = zalloc(nc); t->vol = zalloc(nc);
+    t->u0 = zalloc(nv); t->u1 = zalloc(nv); t->v0 = zalloc(nv); t->v1 = zalloc(nv);
+    t->w0 = zalloc(nv); t->w1 = zalloc(nv);
+    for (int i = 1; i <= 7; i++) t->w[i] = zalloc(nv);
+    t->vfx = zalloc(vx * cy * cz); t->mfx = zalloc(vx * cy * cz); t->xarea = zalloc(vx * cy * cz);
+    t->vfy = zalloc(cx * vy * cz); t->mfy = zalloc(cx * vy * cz); t->yarea = zalloc(cx * vy * cz);
+    t->vfz = zalloc(cx * cy * vz); t->mfz = zalloc(cx * cy * vz); t->zarea = zalloc(cx * cy * vz);
+    t->cellx = zalloc(cx); t->celly = zalloc(cy); t->cellz = zalloc(cz);
+    t->celldx = zalloc(cx); t->celldy = zalloc(cy); t->celldz = zalloc(cz);
+    t->vertexx = zalloc(vx); t->vertexy = zalloc(vy); t->vertexz = zalloc(vz);
+    t->vertexdx = zalloc(vx); t->vertexdy = zalloc(vy); t->vertexdz = zalloc(vz);
+}
+
+static void free_field(otile *t)
+{
+    double *all[] = {t->d0, t->d1, t->e0, t->e1, t->p, t->q, t->c, t->vol, t->u0, t->u1, t->v0, t->v1, t->w0,
+                     t->w1, t->vfx, t->mfx, t->xarea, t->vfy, t->mfy, t->yarea, t->vfz, t->mfz, t->zarea,
+                     t->cellx, t->celly, t->cellz, t->celldx, t->celldy, t->celldz, t->vertexx, t->vertexy,
+                     t->vertexz, t->vertexdx, t->vertexdy, t->vertexdz};
+    for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) free(all[i]);
+    for (int i = 1; i <= 7; i++) free(t->w[i]);
+}
+
+/* clover.f90:258-447 */
+int oracle_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_chunk, int tile_dims,
+                           int *t_nx, int *t_ny, int *t_nz, int *t_left, int *t_bottom, int *t_back,
+                           int *neighbours)
+{
+    int current_x = 1, current_y = 1, current_z = tiles_per_chunk;
+    double surface = (((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * 2) +
+                     (((1.0 * x_cells) / current_x) * ((1.0 * z_cells) / current_z) * 2) +
+                     (((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z) * 2);
+    double volume = ((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z);
+    double best_metric = surface / volume;
+    int chunk_x = current_x, chunk_y = current_y, chunk_z = current_z;
+    int x_bound = tiles_per_chunk, y_bound = tiles_per_chunk;
+    if (tile_dims == 2) { x_bound = 1; y_bound = tiles_per_chunk; }
+    if (tile_dims == 1) { x_bound = 1; y_bound = 1; }
+    for (int t = 1; t <= x_bound; t++) {
+        if (tiles_per_chunk % t != 0) continue;
+        current_x = t;
+        int div1 = tiles_per_chunk / t;
+        if (y_bound < div1) div1 = y_bound;
+        for (int j = 1; j <= div1; j++) {
+            if (div1 % j != 0) continue;
+            current_y = j;
+            if (tiles_per_chunk % (t * j) != 0) continue;
+            current_z = tiles_per_chunk / (t * j);
+            surface = (((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * 2) +
+                      (((1.0 * x_cells) / current_x) * ((1.0 * z_cells) / current_z) * 2) +
+                      (((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z) * 2);
+            volume = ((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z);
+            double current_metric = surface / volume;
+            if (current_metric < best_metric) {
+                chunk_x = current_x; chunk_y = current_y; chunk_z = current_z;
+                best_metric = current_metric;
+            }
+        }
+    }
+    int delta_x = x_cells / chunk_x, delta_y = y_cells / chunk_y, delta_z = z_cells / chunk_z;
+    int mod_x = x_cells % chunk_x, mod_y = y_cells % chunk_y, mod_z = z_cells % chunk_z;
+    int add_x_prev = 0, add_y_prev = 0, add_z_prev = 0;
+    int tile = 0;
+    for (int cz = 1; cz <= chunk_z; cz++) {
+        for (int cy = 1; cy <= chunk_y; cy++) {
+            for (int cx = 1; cx <= chunk_x; cx++) {
+                int add_x = cx <= mod_x, add_y = cy <= mod_y, add_z = cz <= mod_z;
+                int left = (cx - 1) * delta_x + 1 + add_x_prev;
+                int right = left + delta_x - 1 + add_x;
+                int bottom = (cy - 1) * delta_y + 1 + add_y_prev;
+                int top = bottom + delta_y - 1 + add_y;
+                int back = (cz - 1) * delta_z + 1 + add_z_prev;
+                int front = back + delta_z - 1 + add_z;
+                int id = tile + 1;
+                int *nb = neighbours + 6 * tile;
+                nb[0] = cx == 1 ? -1 : id - 1;
+                nb[1] = cx == chunk_x ? -1 : id + 1;
+                nb[2] = cy == 1 ? -1 : id - chunk_x;
+                nb[3] = cy == chunk_y ? -1 : id + chunk_x;
+                nb[4] = cz == 1 ? -1 : id - chunk_x * chunk_y;
+                nb[5] = cz == chunk_z ? -1 : id + chunk_x * chunk_y;
+                t_nx[tile] = right - left + 1;
+                t_ny[tile] = top - bottom + 1;
+                t_nz[tile] = front - back + 1;
+                t_left[tile] = left;
+                t_bottom[tile] = bottom;
+                t_back[tile] = back;
+                if (cx <= mod_x) add_x_prev++;
+                tile++;
+            }
+            add_x_prev = 0;
+            if (cy <= mod_y) add_y_prev++;
+        }
+        add_x_prev = 0;
+        add_y_prev = 0;
+        if (cz <= mod_z) add_z_prev++;
+    }
+    return tile;
+}
+
+typedef struct {
+    const oracle_deck *deck;
+    const ktab *k;
+    int ntiles;
+    otile *tiles;
+    double dt, dtold, time;
+    int step, advect_x;
+    double g_small, g_big;
+} ostate;
+
+static const int EXT6[6] = {-1, -1, -1, -1, -1, -1}; /* one chunk: every chunk face is external */
+
+static double *const *tile_arrays(otile *t, double **buf)
+{
+    /* update_tile_halo argument order */
+    double *a[19] = {t->d0, t->e0, t->p, t->q, t->c, t->d1, t->e1, t->u0, t->v0, t->w0, t->u1, t->v1, t->w1,
+                     t->vfx, t->vfy, t->vfz, t->mfx, t->mfy, t->mfz};
+    memcpy(buf, a, sizeof(a));
+    return buf;
+}
+
+/* update_tile_halo.f90:47-423: top/bottom for all tiles, then left/right,
+ * then front/back. */
+static void update_tile_halo(ostate *s, const int *fields, int depth)
+{
+    static const int passes[3][2] = {{4, 3}, {1, 2}, {6, 5}};
+    for (int pass = 0; pass < 3; pass++)
+        for (int ti = 0; ti < s->ntiles; ti++)
+            for (int f = 0; f < 2; f++) {
+                int face = passes[pass][f];
+                otile *t = &s->tiles[ti];
+                int n = t->nbr[face - 1];
+                if (n == -1) continue;
+                otile *nt = &s->tiles[n - 1];
+                double *a[19], *b[19];
+                tile_arrays(t, a);
+                tile_arrays(nt, b);
+                s->k->tile_halo(face, TD(t), a, TD(nt), b, fields, depth);
+            }
+}
+
+/* update_halo.f90:27-88 */
+static void update_halo(ostate *s, const int *fields, int depth)
+{
+    update_tile_halo(s, fields, depth);
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        s->k->update_halo(TD(t), EXT6, t->nbr, t->d0, t->e0, t->p, t->q, t->c, t->d1, t->e1, t->u0, t->v0, t->w0,
+                          t->u1, t->v1, t->w1, t->vfx, t->vfy, t->vfz, t->mfx, t->mfy, t->mfz, fields, depth);
+    }
+}
+
+static void set_fields(int *fields, const int *ids, int n)
+{
+    memset(fields, 0, 19 * sizeof(int));
+    for (int i = 0; i < n; i++) fields[ids[i]] = 1;
+}
+
+static void ideal_gas(ostate *s, int predict)
+{
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        if (!predict) s->k->ideal_gas(TD(t), t->d0, t->e0, t->p, t->c);
+        else s->k->ideal_gas(TD(t), t->d1, t->e1, t->p, t->c);
+    }
+}
+
+/* field_summary.f90:27-139 (tile sums added in tile order) */
+static void field_summary(ostate *s, oracle_summary *row)
+{
+    ideal_gas(s, 0);
+    double tv = 0, tm = 0, ti_ = 0, tk = 0, tp = 0;
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        double vol, mass, ie, ke, press;
+        s->k->summary(TD(t), t->vol, t->d0, t->e0, t->p, t->u0, t->v0, t->w0, &vol, &mass, &ie, &ke, &press);
+        tv = tv + vol; tm = tm + mass; ti_ = ti_ + ie; tk = tk + ke; tp = tp + press;
+    }
+    row->step = s->step;
+    row->time = s->time;
+    row->vol = tv;
+    row->mass = tm;
+    row->ie = ti_;
+    row->ke = tk;
+    row->press = tp;
+}
+
+/* timestep.f90:28-128; returns 1 on a small time step */
+static int timestep(ostate *s)
+{
+    const oracle_deck *dk = s->deck;
+    int fields[19];
+    s->dt = s->g_big;
+    ideal_gas(s, 0);
+    static const int f1[] = {F_P, F_E0, F_D0, F_U0, F_V0};
+    set_fields(fields, f1, 5);
+    update_halo(s, fields, 1);
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        s->k->viscosity(TD(t), t->xarea, t->yarea, t->zarea, t->celldx, t->celldy, t->celldz, t->d0, t->p, t->q,
+                        t->u0, t->v0, t->w0);
+    }
+    static const int f2[] = {F_Q};
+    set_fields(fields, f2, 1);
+    update_halo(s, fields, 1);
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        double dtlp, xl, yl, zl;
+        int ctl, jl, kl, ll, small;
+        s->k->calc_dt(TD(t), s->g_small, s->g_big, dk->dtmin, dk->dtc_safe, dk->dtu_safe, dk->dtv_safe,
+                      dk->dtw_safe, dk->dtdiv_safe, t->xarea, t->yarea, t->zarea, t->cellx, t->celly, t->cellz,
+                      t->celldx, t->celldy, t->celldz, t->vol, t->d0, t->e0, t->p, t->q, t->c, t->u0, t->v0, t->w0,
+                      t->w[1], &dtlp, &ctl, &xl, &yl, &zl, &jl, &kl, &ll, &small);
+        if (dtlp <= s->dt) s->dt = dtlp;
+    }
+    double m = s->dt;
+    if (s->dtold * dk->dtrise < m) m = s->dtold * dk->dtrise;
+    if (dk->dtmax < m) m = dk->dtmax;
+    s->dt = m;
+    int small = s->dt < dk->dtmin;
+    s->dtold = s->dt;
+    return small;
+}
+
+/* PdV.f90:26-109 */
+static void pdv(ostate *s, int predict)
+{
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        s->k->pdv(predict, TD(t), s->dt, t->xarea, t->yarea, t->zarea, t->vol, t->d0, t->d1, t->e0, t->e1, t->p,
+                  t->q, t->u0, t->u1, t->v0, t->v1, t->w0, t->w1);
+    }
+    if (predict) {
+        int fields[19];
+        ideal_gas(s, 1);
+        static const int f[] = {F_P};
+        set_fields(fields, f, 1);
+        update_halo(s, fields, 1);
+        for (int ti = 0; ti < s->ntiles; ti++) {
+            otile *t = &s->tiles[ti];
+            s->k->revert(TD(t), t->d0, t->d1, t->e0, t->e1);
+        }
+    }
+}
+
+static void advec_cell(ostate *s, int sweep, int dir)
+{
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        s->k->advec_cell(TD(t), s->advect_x, dir, sweep, t->vertexdx, t->vertexdy, t->vertexdz, t->vol, t->d1, t->e1,
+                         t->mfx, t->vfx, t->mfy, t->vfy, t->mfz, t->vfz, t->w[1], t->w[2], t->w[3], t->w[4], t->w[5],
+                         t->w[6], t->w[7]);
+    }
+}
+
+static void advec_mom(ostate *s, int sweep, int dir)
+{
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        double *vel[3] = {t->u1, t->v1, t->w1};
+        for (int which = 1; which <= 3; which++)
+            s->k->advec_mom(TD(t), vel[which - 1], t->mfx, t->vfx, t->mfy, t->vfy, t->mfz, t->vfz, t->vol, t->d1,
+                            t->w[1], t->w[2], t->w[3], t->w[4], t->w[5], t->w[6], t->w[7], t->celldx, t->celldy,
+                            t->celldz, s->advect_x, which, sweep, dir);
+    }
+}
+
+/* advection.f90:26-181 */
+static void advection(ostate *s)
+{
+    int fields[19];
+    static const int fc[] = {F_E1, F_D1, F_VFX, F_VFY, F_VFZ};
+    static const int fm[] = {F_D1, F_E1, F_U1, F_V1, F_W1, F_MFX, F_MFY, F_MFZ};
+    int dir = s->advect_x ? 1 : 3;
+
+    set_fields(fields, fc, 5);
+    update_halo(s, fields, 2);
+    advec_cell(s, 1, dir);
+    set_fields(fields, fm, 8);
+    update_halo(s, fields, 2);
+    advec_mom(s, 1, dir);
+
+    set_fields(fields, fc, 5);
+    update_halo(s, fields, 2);
+    advec_cell(s, 2, 2);
+    set_fields(fields, fm, 8);
+    update_halo(s, fields, 2);
+    advec_mom(s, 2, 2);
+
+    dir = s->advect_x ? 3 : 1;
+    advec_cell(s, 3, dir);
+    set_fields(fields, fm, 8);
+    update_halo(s, fields, 2);
+    advec_mom(s, 3, dir);
+}
+
+/* The reference has one subroutine per face with the 2 x 19 arrays spelled
+ * out; this adapter expands the oracle's (face, array list) form. */
+typedef void (*fn_ref_tile)(OR_DIMS, double *, double *, double *, double *, double *, double *, double *,
+                            double *, double *, double *, double *, double *, double *, double *, double *,
+                            double *, double *, double *, double *, int, int, int, int, int, int, double *,
+                            double *, double *, double *, double *, double *, double *, double *, double *,
+                            double *, double *, double *, double *, double *, double *, double *, double *,
+                            double *, double *, const int *, int);
+static fn_ref_tile ref_tile_fns[6];
+
+static void ref_tile_adapter(int face, OR_DIMS, double *const *a, int n_xmin, int n_xmax, int n_ymin, int n_ymax,
+                             int n_zmin, int n_zmax, double *const *b, const int *fields, int depth)
+{
+    ref_tile_fns[face - 1](x_min, x_max, y_min, y_max, z_min, z_max, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7],
+                           a[8], a[9], a[10], a[11], a[12], a[13], a[14], a[15], a[16], a[17], a[18], n_xmin, n_xmax,
+                           n_ymin, n_ymax, n_zmin, n_zmax, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8], b[9],
+                           b[10], b[11], b[12], b[13], b[14], b[15], b[16], b[17], b[18], fields, depth);
+}
+
+static int load_ref(const char *path, ktab *k)
+{
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+        fprintf(stderr, "oracle: cannot load %s: %s\n", path, dlerror());
+        return -1;
+    }
+#define SYM(field, name)                                                     \
+    do {                                                                     \
+        *(void **)(&k->field) = dlsym(h, name);                              \
+        if (!k->field) {                                                     \
+            fprintf(stderr, "oracle: %s lacks %s\n", path, name);            \
+            return -1;                                                       \
+        }                                                                    \
+    } while (0)
+    SYM(ideal_gas, "ref_ideal_gas_kernel");
+    SYM(viscosity, "ref_viscosity_kernel");
+    SYM(calc_dt, "ref_calc_dt_kernel");
+    SYM(pdv, "ref_pdv_kernel");
+    SYM(revert, "ref_revert_kernel");
+    SYM(accelerate, "ref_accelerate_kernel");
+    SYM(flux_calc, "ref_flux_calc_kernel");
+    SYM(advec_cell, "ref_advec_cell_kernel");
+    SYM(advec_mom, "ref_advec_mom_kernel");
+    SYM(reset, "ref_reset_field_kernel");
+    SYM(update_halo, "ref_update_halo_kernel");
+    SYM(summary, "ref_field_summary_kernel");
+#undef SYM
+    static const char *tile_names[6] = {"ref_update_tile_halo_left_kernel", "ref_update_tile_halo_right_kernel",
+                                        "ref_update_tile_halo_bottom_kernel", "ref_update_tile_halo_top_kernel",
+                                        "ref_update_tile_halo_back_kernel", "ref_update_tile_halo_front_kernel"};
+    for (int f = 0; f < 6; f++) {
+        *(void **)(&ref_tile_fns[f]) = dlsym(h, tile_names[f]);
+        if (!ref_tile_fns[f]) {
+            fprintf(stderr, "oracle: %s lacks %s\n", path, tile_names[f]);
+            return -1;
+        }
+    }
+    k->tile_halo = ref_tile_adapter;
+    return 0;
+}
+
+int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *rows, int max_rows,
+               double *dt_hist, int max_steps, int *steps_done)
+{
+    ktab k;
+    if (ref_lib) {
+        if (load_ref(ref_lib, &k)) return -1;
+    } else {
+        k.ideal_gas = oracle_ideal_gas_kernel;
+        k.viscosity = oracle_viscosity_kernel;
+        k.calc_dt = oracle_calc_dt_kernel;
+        k.pdv = oracle_PdV_kernel;
+        k.revert = oracle_revert_kernel;
+        k.accelerate = oracle_accelerate_kernel;
+        k.flux_calc = oracle_flux_calc_kernel;
+        k.advec_cell = oracle_advec_cell_kernel;
+        k.advec_mom = oracle_advec_mom_kernel;
+        k.reset = oracle_reset_field_kernel;
+        k.update_halo = oracle_update_halo_kernel;
+        k.tile_halo = oracle_update_tile_halo_face;
+        k.summary = oracle_field_summary_kernel;
+    }
+    ostate s;
+    memset(&s, 0, sizeof(s));
+    s.deck = deck;
+    s.k = &k;
+    s.g_small = (double)1.0e-16f; /* default-real literals, data.f90:30-31 */
+    s.g_big = (double)1.0e+21f;
+
+    int T = deck->tiles_per_chunk;
+    int *nx = calloc(T, sizeof(int)), *ny = calloc(T, sizeof(int)), *nz = calloc(T, sizeof(int));
+    int *lf = calloc(T, sizeof(int)), *bt = calloc(T, sizeof(int)), *bk = calloc(T, sizeof(int));
+    int *nb = calloc(6 * T, sizeof(int));
+    s.ntiles = oracle_decompose_tiles(deck->x_cells, deck->y_cells, deck->z_cells, T, deck->tile_dims, nx, ny, nz,
+                                      lf, bt, bk, nb);
+    s.tiles = calloc(s.ntiles, sizeof(otile));
+
+    /* start.f90:25-144 */
+    s.time = 0.0;
+    s.step = 0;
+    s.dtold = deck->dtinit;
+    s.dt = deck->dtinit;
+    double dx = (deck->xmax - deck->xmin) / (double)(float)deck->x_cells;
+    double dy = (deck->ymax - deck->ymin) / (double)(float)deck->y_cells;
+    double dz = (deck->zmax - deck->zmin) / (double)(float)deck->z_cells;
+    for (int ti = 0; ti < s.ntiles; ti++) {
+        otile *t = &s.tiles[ti];
+        t->nx = nx[ti]; t->ny = ny[ti]; t->nz = nz[ti];
+        t->left = lf[ti]; t->bottom = bt[ti]; t->back = bk[ti];
+        memcpy(t->nbr, nb + 6 * ti, 6 * sizeof(int));
+        build_field(t);
+        /* initialise_chunk.f90:37-44 */
+        double xmin = deck->xmin + dx * (double)(float)(t->left - 1);
+        double ymin = deck->ymin + dy * (double)(float)(t->bottom - 1);
+        double zmin = deck->zmin + dz * (double)(float)(t->back - 1);
+        oracle_initialise_chunk_kernel(TD(t), xmin, ymin, zmin, dx, dy, dz, t->vertexx, t->vertexdx, t->vertexy,
+                                       t->vertexdy, t->vertexz, t->vertexdz, t->cellx, t->celldx, t->celly, t->celldy,
+                                       t->cellz, t->celldz, t->vol, t->xarea, t->yarea, t->zarea);
+        oracle_generate_chunk_kernel(TD(t), t->vertexx, t->vertexy, t->vertexz, t->cellx, t->celly, t->cellz, t->d0,
+                                     t->e0, t->u0, t->v0, t->w0, deck->number_of_states, deck->state_density,
+                                     deck->state_energy, deck->state_xvel, deck->state_yvel, deck->state_zvel,
+                                     deck->state_xmin, deck->state_xmax, deck->state_ymin, deck->state_ymax,
+                                     deck->state_zmin, deck->state_zmax, deck->state_radius, deck->state_geometry);
+    }
+    free(nx); free(ny); free(nz); free(lf); free(bt); free(bk); free(nb);
+    s.advect_x = 1;
+    ideal_gas(&s, 0);
+    int fields[19];
+    static const int fstart[] = {F_D0, F_E0, F_P, F_Q, F_D1, F_E1, F_U0, F_V0, F_W0, F_U1, F_V1, F_W1};
+    set_fields(fields, fstart, 12);
+    update_halo(&s, fields, 2);
+    int nrows = 0;
+    if (nrows < max_rows) field_summary(&s, &rows[nrows++]);
+
+    /* hydro.f90:46-87 */
+    int rc = 0;
+    for (;;) {
+        s.step = s.step + 1;
+        if (timestep(&s)) {
+            fprintf(stderr, "oracle: small timestep at step %d\n", s.step);
+            rc = -2;
+            break;
+        }
+        if (dt_hist && s.step <= max_steps) dt_hist[s.step - 1] = s.dt;
+        pdv(&s, 1);
+        for (int ti = 0; ti < s.ntiles; ti++) {
+            otile *t = &s.tiles[ti];
+            k.accelerate(TD(t), s.dt, t->xarea, t->yarea, t->zarea, t->vol, t->d0, t->p, t->q, t->u0, t->v0, t->w0,
+                         t->u1, t->v1, t->w1);
+        }
+        pdv(&s, 0);
+        for (int ti = 0; ti < s.ntiles; ti++) {
+            otile *t = &s.tiles[ti];
+            k.flux_calc(TD(t), s.dt, t->xarea, t->yarea, t->zarea, t->u0, t->v0, t->w0, t->u1, t->v1, t->w1, t->vfx,
+                        t->vfy, t->vfz);
+        }
+        advection(&s);
+        for (int ti = 0; ti < s.ntiles; ti++) {
+            otile *t = &s.tiles[ti];
+            k.reset(TD(t), t->d0, t->d1, t->e0, t->e1, t->u0, t->u1, t->v0, t->v1, t->w0, t->w1);
+        }
+        s.advect_x = !s.advect_x;
+        s.time = s.time + s.dt;
+        if (deck->summary_frequency != 0 && s.step % deck->summary_frequency == 0)
+            if (nrows < max_rows) field_summary(&s, &rows[nrows++]);
+        if (s.time + s.g_small > deck->end_time || s.step >= deck->end_step) {
+            if (nrows < max_rows) field_summary(&s, &rows[nrows++]);
+            break;
+        }
+    }
+    if (steps_done) *steps_done = s.step;
+    for (int ti = 0; ti < s.ntiles; ti++) free_field(&s.tiles[ti]);
+    free(s.tiles);
+    return rc ? rc : nrows;
+}

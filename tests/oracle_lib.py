@@ -1,0 +1,117 @@
+"""ctypes access to the oracle (oracle/liboracle.so) and to the reference's
+own compiled kernels (oracle/_ref/libclover_ref.so).
+
+Test infrastructure: the checker only.  Nothing in the product imports this.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from cloverleaf3d_ref_amd.deck import Deck
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libclover_ref.so"
+
+MAX_STATES = 16
+
+
+class OracleDeck(ctypes.Structure):
+    _fields_ = [
+        ("x_cells", ctypes.c_int), ("y_cells", ctypes.c_int), ("z_cells", ctypes.c_int),
+        ("xmin", ctypes.c_double), ("ymin", ctypes.c_double), ("zmin", ctypes.c_double),
+        ("xmax", ctypes.c_double), ("ymax", ctypes.c_double), ("zmax", ctypes.c_double),
+        ("dtinit", ctypes.c_double), ("dtmax", ctypes.c_double), ("dtrise", ctypes.c_double),
+        ("dtmin", ctypes.c_double),
+        ("dtc_safe", ctypes.c_double), ("dtu_safe", ctypes.c_double), ("dtv_safe", ctypes.c_double),
+        ("dtw_safe", ctypes.c_double), ("dtdiv_safe", ctypes.c_double),
+        ("end_time", ctypes.c_double),
+        ("end_step", ctypes.c_int),
+        ("summary_frequency", ctypes.c_int),
+        ("tiles_per_chunk", ctypes.c_int),
+        ("tile_dims", ctypes.c_int),
+        ("number_of_states", ctypes.c_int),
+    ] + [(f"state_{n}", ctypes.c_double * MAX_STATES) for n in
+         ("density", "energy", "xvel", "yvel", "zvel", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "radius")] + [
+        ("state_geometry", ctypes.c_int * MAX_STATES),
+    ]
+
+
+class OracleSummary(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int), ("time", ctypes.c_double), ("vol", ctypes.c_double),
+                ("mass", ctypes.c_double), ("press", ctypes.c_double), ("ie", ctypes.c_double),
+                ("ke", ctypes.c_double)]
+
+
+def to_oracle_deck(d: Deck) -> OracleDeck:
+    od = OracleDeck()
+    for name in ("x_cells", "y_cells", "z_cells", "xmin", "ymin", "zmin", "xmax", "ymax", "zmax", "dtinit",
+                 "dtmax", "dtrise", "dtmin", "dtc_safe", "dtu_safe", "dtv_safe", "dtw_safe", "dtdiv_safe",
+                 "end_time", "end_step", "summary_frequency", "tiles_per_chunk", "tile_dims"):
+        setattr(od, name, getattr(d, name))
+    if len(d.states) > MAX_STATES:
+        raise ValueError("too many states for the oracle")
+    od.number_of_states = len(d.states)
+    for i, st in enumerate(d.states):
+        for n in ("density", "energy", "xvel", "yvel", "zvel", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax",
+                  "radius"):
+            getattr(od, f"state_{n}")[i] = getattr(st, n)
+        od.state_geometry[i] = st.geometry
+    return od
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            raise FileNotFoundError(f"{ORACLE_SO} not built (make -C oracle)")
+        _lib = ctypes.CDLL(str(ORACLE_SO))
+        _lib.oracle_run.restype = ctypes.c_int
+        _lib.oracle_run.argtypes = [ctypes.POINTER(OracleDeck), ctypes.c_char_p, ctypes.POINTER(OracleSummary),
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)]
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+_ref = None
+
+
+def ref_lib() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(str(REF_SO))
+    return _ref
+
+
+def run(deck: Deck, use_ref: bool = False, max_rows: int = 4096,
+        max_steps: int = 100000) -> Tuple[List[dict], np.ndarray]:
+    """Run start + hydro on the CPU; returns (summary rows, dt per step)."""
+    od = to_oracle_deck(deck)
+    rows = (OracleSummary * max_rows)()
+    dts = np.zeros(max_steps, dtype=np.float64)
+    done = ctypes.c_int(0)
+    path = str(REF_SO).encode() if use_ref else None
+    n = lib().oracle_run(ctypes.byref(od), path, rows, max_rows,
+                         dts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), max_steps, ctypes.byref(done))
+    if n < 0:
+        raise RuntimeError(f"oracle_run failed ({n})")
+    out = [dict(step=r.step, time=r.time, vol=r.vol, mass=r.mass, press=r.press, ie=r.ie, ke=r.ke)
+           for r in rows[:n]]
+    return out, dts[:min(done.value, max_steps)].copy()
+
+
+def thread_env(n: Optional[int]) -> None:
+    if n is not None:
+        os.environ["OMP_NUM_THREADS"] = str(n)

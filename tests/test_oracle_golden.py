@@ -1,0 +1,97 @@
+"""Pins the oracle against fixed answers that do not need the reference tree:
+
+* the reference's only built-in known answer: test_problem 1 on the default
+  10x2x2 deck ends with kinetic energy 3.64560737191257 (field_summary.f90:
+  119-136; the reference accepts 0.001 %, the oracle is held to 1e-12);
+* the totals the reference program printed for the 64^3 cuboid deck, untiled
+  and with 2x2x2 tiles (tests/golden/config0_summaries.json);
+* sha256 digests of every output array of every reference kernel on seeded
+  inputs (tests/golden/kernel_digests.json);
+* closed forms: the volume total is exactly the domain volume; a single
+  uniform state at rest never moves.
+"""
+from __future__ import annotations
+
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from cloverleaf3d_ref_amd.deck import DEFAULT_DECK, TEST_PROBLEM_1_KE, cuboid_deck, parse_deck
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+CONFIG0 = json.loads((GOLDEN / "config0_summaries.json").read_text())
+
+
+def test_test_problem_1_known_answer():
+    rows, dts = ol.run(parse_deck(DEFAULT_DECK))
+    assert len(dts) == 75
+    assert np.all(dts == 0.04)  # max_timestep caps every step
+    ke = rows[-1]["ke"]
+    assert abs(ke / TEST_PROBLEM_1_KE - 1.0) < 1e-12
+    # the reference's own acceptance test (field_summary.f90:122-133)
+    assert abs(100.0 * (ke / TEST_PROBLEM_1_KE) - 100.0) < 0.001
+
+
+def _cols(r):
+    return [r["step"], r["vol"], r["mass"], r["press"] / r["vol"], r["ie"], r["ke"]]
+
+
+def test_config0_matches_reference_program():
+    rows, dts = ol.run(parse_deck(CONFIG0["deck"]))
+    got = [_cols(r) for r in rows]
+    ref = CONFIG0["reference_program"]
+    assert [g[0] for g in got[:len(ref)]] == [r[0] for r in ref]
+    for g, r in zip(got, ref):
+        for a, b in zip(g[1:], r[1:]):
+            assert a == pytest.approx(b, rel=1e-13, abs=1e-300), (g[0], a, b)
+    assert dts[0] == pytest.approx(0.06, rel=1e-15)  # 0.04 * 1.5, rise-limited
+    # dt sequence equals the reference kernels' bit for bit (min is exact)
+    assert np.array_equal(dts, np.array(CONFIG0["ref_kernels_dt"]))
+
+
+def test_config0_tiled_matches_reference_program():
+    rows, _ = ol.run(parse_deck(cuboid_deck(64, 100, 10, tiles=8)))
+    last = rows[-1]
+    ref = CONFIG0["reference_program_tiled_step100"]
+    for a, b in zip([last["mass"], last["ie"], last["ke"]], ref):
+        assert a == pytest.approx(b, rel=1e-13)
+    # tiling in x changes the answer at the 1e-6 level (a reference property)
+    untiled_ke = CONFIG0["reference_program"][-1][5]
+    assert abs(last["ke"] / untiled_ke - 1) > 1e-7
+
+
+def test_volume_is_exact():
+    rows, _ = ol.run(parse_deck(cuboid_deck(16, 20, 5)))
+    assert all(r["vol"] == 1000.0 for r in rows)
+
+
+def test_rest_state_stays_at_rest():
+    deck = parse_deck("""*clover
+ state 1 density=0.7 energy=1.3
+ x_cells=9 y_cells=7 z_cells=5
+ xmin=0 ymin=0 zmin=0 xmax=9 ymax=7 zmax=5
+ initial_timestep=0.04 end_step=12 end_time=100 summary_frequency=4
+*endclover""")
+    rows, _ = ol.run(deck)
+    for r in rows:
+        assert r["ke"] == 0.0
+        assert r["mass"] == pytest.approx(0.7 * 315.0, rel=1e-14)
+        assert r["ie"] == pytest.approx(0.7 * 1.3 * 315.0, rel=1e-14)
+
+
+def test_kernel_digests():
+    """The oracle reproduces the reference kernels' outputs bit for bit on the
+    digest cases (fixture made by tests/golden/make_golden.py from the
+    reference's compiled kernels)."""
+    import make_golden as mg  # noqa: E402
+    fixture = json.loads((GOLDEN / "kernel_digests.json").read_text())
+    cases = mg.kernel_cases()
+    assert {c[0] for c in cases} == set(fixture["digests"])
+    for case in cases:
+        fn = getattr(ol.lib(), "oracle_" + case[1])
+        fn.restype = None
+        got = mg.run_case(fn, case)
+        assert got == fixture["digests"][case[0]], case[0]
