@@ -1,0 +1,178 @@
+"""Benchmark of the CloverLeaf3D hydro timestep on one MI355X.
+
+Workload: the two-state cuboid deck at 512^3 cells (domain [0,10]^3,
+background rho 0.2 e 1.0, cuboid x in [0,5], y,z in [0,2] at rho 1.0 e 2.5,
+gamma 1.4, reflective walls, dt0 0.04, dt rise 1.5), FP64, one tile, every
+field resident in HBM, field_summary every 25 steps, as the reference's hydro
+loop does it.  A "step" is one pass of hydro.f90:48-72 over the whole mesh.
+Setup (mesh, states, halo prime) is outside the timed region, as it is
+outside the reference's Wall clock (hydro.f90:44, 90).
+
+Prints one JSON line:
+  value        cell-steps/s (cells x timed steps / wall time of the timed steps)
+  roofline     the kernel with the most device time in the timed region,
+               timed live with HIP events on the chunk's stream: algorithmic
+               HBM bytes per launch / average launch time, against 8 TB/s
+  cpu_baseline the reference's own kernels (compiled from its sources with
+               flang -O3 -fopenmp, oracle/_ref) driven through the same
+               hydro loop on this host's cores, on a bounded sample deck
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (kernel-boundary compulsory) traffic of each kernel per launch,
+# in FP64 arrays per cell of its domain: every distinct array it reads or
+# writes, touched once.  Volume kernels read V plus 1-3 volume fluxes
+# depending on the sweep (advec_cell_kernel.f90:97-116, 193-210, 289-306;
+# advec_mom_kernel.f90:95-163): 6, 5 and 4 arrays for sweeps 1, 2, 3, i.e. 5
+# per launch averaged over a step.
+ARRAYS_PER_CELL = {
+    "k_ideal_gas": 4, "k_viscosity": 9, "k_calc_dt": 10, "k_revert": 4, "k_accelerate": 13,
+    "k_flux_calc": 12, "k_cell_vols": 5, "k_cell_flux": 6, "k_cell_update": 12, "k_mom_vols": 5,
+    "k_node_flux": 2, "k_node_mass": 5, "k_mom_flux": 4, "k_mom_update": 5, "k_reset_field": 10,
+    "k_field_summary": 7, "k_pdv<1>": 13, "k_pdv<0>": 16,
+}
+
+
+def kernel_domain_cells(name: str, n: int) -> int:
+    """Cells (or nodes) one launch of `name` covers on an n^3 tile."""
+    if name in ("k_cell_vols", "k_mom_vols"):
+        return (n + 4) ** 3
+    if name in ("k_accelerate", "k_reset_field", "k_flux_calc", "k_mom_flux", "k_mom_update", "k_node_flux",
+                "k_node_mass"):
+        return (n + 1) ** 3
+    return n ** 3
+
+
+def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25):
+    from cloverleaf3d_ref_amd import _lib
+    from cloverleaf3d_ref_amd.chunk import Chunk
+    from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
+
+    deck = parse_deck(cuboid_deck(cells, warmup + steps + 1000, summary_frequency))
+    t0 = time.perf_counter()
+    c = Chunk(deck)
+    setup_s = time.perf_counter() - t0
+    L = _lib.lib()
+    for _ in range(warmup):
+        c.step()
+    _lib.check(L.clover_chunk_set_kernel_timing(c._h, 1), "kernel timing")
+    c.run(max_steps=0)  # synchronise the stream
+    t0 = time.perf_counter()
+    rows = c.run(max_steps=steps)  # summaries every summary_frequency steps, as the hydro loop does
+    wall = time.perf_counter() - t0
+    buf = ctypes.create_string_buffer(1 << 16)
+    _lib.check(L.clover_chunk_kernel_times(c._h, buf, len(buf)), "kernel times")
+    ktimes = {}
+    for line in buf.value.decode().splitlines():
+        name, ms, launches = line.split()
+        ktimes[name] = (float(ms), int(launches))
+    info = dict(setup_s=setup_s, device_bytes=c.device_bytes(), rows=rows, state=c.state())
+    c.close()
+    return wall, ktimes, info
+
+
+def cpu_baseline(cells: int, steps: int, threads: int):
+    """The reference's compiled kernels through the hydro loop on the host
+    (run in a child process so its OpenMP runtime picks up `threads`)."""
+    code = f"""
+import sys, json
+sys.path[:0] = [{str(ROOT)!r}, {str(ROOT / 'tests')!r}]
+import oracle_lib as ol
+from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
+deck = parse_deck(cuboid_deck({cells}, {steps}, 0))
+use_ref = ol.ref_available()
+ol.run(deck, use_ref=use_ref)
+import ctypes
+f = ol.lib().oracle_last_hydro_seconds
+f.restype = ctypes.c_double
+print(json.dumps(dict(seconds=f(), ref=use_ref)))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr[-2000:])
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    rate = cells ** 3 * steps / r["seconds"]
+    return dict(value=rate, unit="cell-steps/s", cores=threads, kind="reference" if r["ref"] else "port",
+                sample=f"cuboid deck {cells}^3, {steps} steps, hydro loop only ({r['seconds']:.2f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=512)
+    ap.add_argument("--cpu-cells", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the hydro path here is single-GPU by design "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup)
+    n = args.cells
+    value = n ** 3 * args.steps / wall
+
+    # dominant kernel by device time
+    name, (ms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
+    per_launch_bytes = ARRAYS_PER_CELL.get(name, 0) * 8 * kernel_domain_cells(name, n)
+    avg_s = ms / launches / 1e3
+    achieved = per_launch_bytes / avg_s / 1e9 if per_launch_bytes else None
+    roofline = dict(bound="hbm", kernel=name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=None,
+                    avg_launch_us=avg_s * 1e6, launches=launches,
+                    bytes_per_launch=per_launch_bytes)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.cpu_cells, args.cpu_steps, threads)
+
+    total_ms = sum(v[0] for v in ktimes.values())
+    out = {
+        "metric": "cell-steps/s at 512^3 FP64 vs FP32-store; HBM GB/s; batched decks/s",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cuboid deck {n}^3 cells, FP64, 1 tile, summary every 25 steps",
+                   "cells": n ** 3, "form": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "step_hbm_gbs_kbc": 1192 * value / 1e9,
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
+                               sorted(ktimes.items(), key=lambda kv: -kv[1][0])},
+        "device_time_fraction": total_ms / (wall * 1e3),
+        "setup_s": info["setup_s"],
+        "device_gib": info["device_bytes"] / 2 ** 30,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

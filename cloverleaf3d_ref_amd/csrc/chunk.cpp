@@ -1,0 +1,833 @@
+// Device-resident driver: start + the hydro loop of the reference with every
+// field of every tile in HBM for the whole run.
+//
+// Phase order follows hydro.f90:48-68, timestep.f90:28-128, PdV.f90:26-109,
+// advection.f90:26-181, update_halo.f90:27-88 (tile exchange in three passes,
+// update_tile_halo.f90:47-423, then the reflective boundary per tile) and
+// field_summary.f90:27-139.  Only dt (8 bytes per step) and the summary sums
+// cross to the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/clover_hip.h"
+#include "kernels.h"
+#include "runtime.h"
+
+namespace clover {
+
+// extra field ids of the resident driver beyond the 19 halo fields
+enum : int { X_VOLUME = NUM_FIELDS, X_XAREA, X_YAREA, X_ZAREA, X_W1, X_W2, X_W3, X_W4, X_W5, X_W6, X_W7, NUM_ARRAYS };
+
+static int class_of(int id)
+{
+    switch (id) {
+    case F_XVEL0: case F_XVEL1: case F_YVEL0: case F_YVEL1: case F_ZVEL0: case F_ZVEL1:
+        return AC_VERT;
+    case F_VOL_FLUX_X: case F_MASS_FLUX_X: case X_XAREA: return AC_XF;
+    case F_VOL_FLUX_Y: case F_MASS_FLUX_Y: case X_YAREA: return AC_YF;
+    case F_VOL_FLUX_Z: case F_MASS_FLUX_Z: case X_ZAREA: return AC_ZF;
+    default: return id >= X_W1 ? AC_VERT : AC_CELL;
+    }
+}
+
+enum Prof : int {
+    P_IDEAL_GAS, P_VISCOSITY, P_TIMESTEP, P_PDV, P_REVERT, P_ACCEL, P_FLUX, P_CELL_ADVEC, P_MOM_ADVEC, P_RESET,
+    P_TILE_HALO, P_SELF_HALO, P_SUMMARY, P_COUNT
+};
+static const char *kProfNames[P_COUNT] = {"ideal_gas", "viscosity", "timestep", "PdV", "revert",
+                                          "acceleration", "fluxes", "cell_advection", "momentum_advection",
+                                          "reset", "tile_halo_exchange", "self_halo_exchange", "summary"};
+
+// Per-kernel device time from HIP events recorded around every launch.
+// Events come from a pool and are only read back when the pool is full or
+// the owner asks, so timing adds no host synchronisation to a step.
+class EventTimer : public KernelTimer {
+  public:
+    struct Stat {
+        double ms = 0.0;
+        long long launches = 0;
+    };
+    ~EventTimer() override
+    {
+        for (auto &e : pool_) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+    }
+    void begin(const char *kernel, hipStream_t st) override
+    {
+        if (used_ == pool_.size()) {
+            if (pool_.size() >= kPool) drain();
+            if (used_ == pool_.size()) {
+                Pair p;
+                hip_check(hipEventCreate(&p.a), "hipEventCreate");
+                hip_check(hipEventCreate(&p.b), "hipEventCreate");
+                pool_.push_back(p);
+            }
+        }
+        Pair &p = pool_[used_];
+        p.name = kernel;
+        hip_check(hipEventRecord(p.a, st), "hipEventRecord");
+    }
+    void end(hipStream_t st) override { hip_check(hipEventRecord(pool_[used_++].b, st), "hipEventRecord"); }
+    void drain()
+    {
+        for (size_t i = 0; i < used_; i++) {
+            hip_check(hipEventSynchronize(pool_[i].b), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, pool_[i].a, pool_[i].b), "hipEventElapsedTime");
+            Stat &s = stats[pool_[i].name];
+            s.ms += ms;
+            s.launches++;
+        }
+        used_ = 0;
+    }
+    void reset()
+    {
+        drain();
+        stats.clear();
+    }
+    std::map<std::string, Stat> stats;
+
+  private:
+    struct Pair {
+        hipEvent_t a = nullptr, b = nullptr;
+        const char *name = nullptr;
+    };
+    static constexpr size_t kPool = 8192;
+    std::vector<Pair> pool_;
+    size_t used_ = 0;
+};
+
+struct TimerScope {
+    TimerScope(EventTimer *t) : prev(t_kernel_timer) { t_kernel_timer = t; }
+    ~TimerScope() { t_kernel_timer = prev; }
+    KernelTimer *prev;
+};
+
+struct Tile {
+    int nx, ny, nz, left, bottom, back;
+    int nbr[6];
+    Lay g;
+    double *f[NUM_ARRAYS];
+    // 1-D geometry, device and host
+    double *cellx, *celly, *cellz, *celldx, *celldy, *celldz, *vertexdx, *vertexdy, *vertexdz;
+    std::vector<double> h_cellx, h_celly, h_cellz;
+};
+
+}  // namespace clover
+
+using namespace clover;
+
+struct clover_chunk {
+    clover_deck deck;
+    int device;
+    hipStream_t st;
+    std::vector<Tile> tiles;
+    double *partial;  // per tile: kMaxPartials * 5
+    double *scalar;   // per tile: 8
+    double *h_scalar; // pinned
+    double dt, dtold, time;
+    int step;
+    int advect_x;
+    double g_small, g_big;
+    long long bytes;
+    bool profiling;
+    hipEvent_t ev0, ev1;
+    double prof_ms[P_COUNT];
+    int prof_calls[P_COUNT];
+    EventTimer *ktimer;  // non-null while per-kernel timing is on
+};
+
+// installs the chunk's kernel timer (if any) for the duration of an API call
+#define KTIMER_SCOPE(c) TimerScope _kts((c)->ktimer)
+
+namespace clover {
+
+// ------------------------------------------------------------ profiling
+struct Phase {
+    clover_chunk *c;
+    int p;
+    Phase(clover_chunk *c_, int p_) : c(c_), p(p_)
+    {
+        if (c->profiling) hip_check(hipEventRecord(c->ev0, c->st), "event");
+    }
+    ~Phase() noexcept(false)
+    {
+        if (!c->profiling) return;
+        hip_check(hipEventRecord(c->ev1, c->st), "event");
+        hip_check(hipEventSynchronize(c->ev1), "event sync");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "elapsed");
+        c->prof_ms[p] += ms;
+        c->prof_calls[p]++;
+    }
+};
+
+// ------------------------------------------------------------ tile decomposition (clover.f90:258-447)
+static void decompose(const clover_deck &d, std::vector<Tile> &tiles)
+{
+    const int T = d.tiles_per_chunk;
+    const int xc = d.x_cells, yc = d.y_cells, zc = d.z_cells;
+    int cur_x = 1, cur_y = 1, cur_z = T;
+    auto metric = [&](int cx, int cy, int cz) {
+        double surface = (((1.0 * xc) / cx) * ((1.0 * yc) / cy) * 2) + (((1.0 * xc) / cx) * ((1.0 * zc) / cz) * 2) +
+                         (((1.0 * yc) / cy) * ((1.0 * zc) / cz) * 2);
+        double volume = ((1.0 * xc) / cx) * ((1.0 * yc) / cy) * ((1.0 * zc) / cz);
+        return surface / volume;
+    };
+    double best = metric(cur_x, cur_y, cur_z);
+    int chunk_x = cur_x, chunk_y = cur_y, chunk_z = cur_z;
+    int x_bound = T, y_bound = T;
+    if (d.tile_dims == 2) { x_bound = 1; y_bound = T; }
+    if (d.tile_dims == 1) { x_bound = 1; y_bound = 1; }
+    for (int t = 1; t <= x_bound; t++) {
+        if (T % t) continue;
+        int div1 = T / t;
+        if (y_bound < div1) div1 = y_bound;
+        for (int j = 1; j <= div1; j++) {
+            if (div1 % j) continue;
+            if (T % (t * j)) continue;
+            const double m = metric(t, j, T / (t * j));
+            if (m < best) { chunk_x = t; chunk_y = j; chunk_z = T / (t * j); best = m; }
+        }
+    }
+    const int dx = xc / chunk_x, dy = yc / chunk_y, dz = zc / chunk_z;
+    const int mx = xc % chunk_x, my = yc % chunk_y, mz = zc % chunk_z;
+    int ax_prev = 0, ay_prev = 0, az_prev = 0;
+    tiles.clear();
+    for (int cz = 1; cz <= chunk_z; cz++) {
+        for (int cy = 1; cy <= chunk_y; cy++) {
+            for (int cx = 1; cx <= chunk_x; cx++) {
+                const int ax = cx <= mx, ay = cy <= my, az = cz <= mz;
+                Tile t{};
+                const int left = (cx - 1) * dx + 1 + ax_prev, right = left + dx - 1 + ax;
+                const int bottom = (cy - 1) * dy + 1 + ay_prev, top = bottom + dy - 1 + ay;
+                const int back = (cz - 1) * dz + 1 + az_prev, front = back + dz - 1 + az;
+                const int id = (int)tiles.size() + 1;
+                t.nbr[0] = cx == 1 ? -1 : id - 1;
+                t.nbr[1] = cx == chunk_x ? -1 : id + 1;
+                t.nbr[2] = cy == 1 ? -1 : id - chunk_x;
+                t.nbr[3] = cy == chunk_y ? -1 : id + chunk_x;
+                t.nbr[4] = cz == 1 ? -1 : id - chunk_x * chunk_y;
+                t.nbr[5] = cz == chunk_z ? -1 : id + chunk_x * chunk_y;
+                t.nx = right - left + 1; t.ny = top - bottom + 1; t.nz = front - back + 1;
+                t.left = left; t.bottom = bottom; t.back = back;
+                tiles.push_back(t);
+                if (cx <= mx) ax_prev++;
+            }
+            ax_prev = 0;
+            if (cy <= my) ay_prev++;
+        }
+        ax_prev = 0;
+        ay_prev = 0;
+        if (cz <= mz) az_prev++;
+    }
+}
+
+// ------------------------------------------------------------ mesh and states on the host
+// initialise_chunk.f90:37-44 + initialise_chunk_kernel.f90:63-178 and
+// generate_chunk_kernel.f90:82-207, into dense Fortran arrays, then uploaded.
+struct HostInit {
+    std::vector<double> vertexx, vertexy, vertexz, vertexdx, vertexdy, vertexdz;
+    std::vector<double> cellx, celly, cellz, celldx, celldy, celldz;
+    std::vector<double> volume, xarea, yarea, zarea, density0, energy0, xvel0, yvel0, zvel0;
+};
+
+static void host_init(const clover_deck &d, const Tile &t, HostInit &h)
+{
+    const int nx = t.nx, ny = t.ny, nz = t.nz;
+    const double dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
+    const double dy = (d.ymax - d.ymin) / (double)(float)d.y_cells;
+    const double dz = (d.zmax - d.zmin) / (double)(float)d.z_cells;
+    const double xmin = d.xmin + dx * (double)(float)(t.left - 1);
+    const double ymin = d.ymin + dy * (double)(float)(t.bottom - 1);
+    const double zmin = d.zmin + dz * (double)(float)(t.back - 1);
+    // 1-D arrays are indexed from min-2 = -1 (offset +2 from index 1)
+    h.vertexx.assign(nx + 5, 0.0); h.vertexdx.assign(nx + 5, 0.0);
+    h.vertexy.assign(ny + 5, 0.0); h.vertexdy.assign(ny + 5, 0.0);
+    h.vertexz.assign(nz + 5, 0.0); h.vertexdz.assign(nz + 5, 0.0);
+    h.cellx.assign(nx + 4, 0.0); h.celldx.assign(nx + 4, 0.0);
+    h.celly.assign(ny + 4, 0.0); h.celldy.assign(ny + 4, 0.0);
+    h.cellz.assign(nz + 4, 0.0); h.celldz.assign(nz + 4, 0.0);
+    for (int j = -1; j <= nx + 3; j++) { h.vertexx[j + 1] = xmin + dx * (double)(float)(j - 1); h.vertexdx[j + 1] = dx; }
+    for (int k = -1; k <= ny + 3; k++) { h.vertexy[k + 1] = ymin + dy * (double)(float)(k - 1); h.vertexdy[k + 1] = dy; }
+    for (int l = -1; l <= nz + 3; l++) { h.vertexz[l + 1] = zmin + dz * (double)(float)(l - 1); h.vertexdz[l + 1] = dz; }
+    for (int j = -1; j <= nx + 2; j++) { h.cellx[j + 1] = 0.5 * (h.vertexx[j + 1] + h.vertexx[j + 2]); h.celldx[j + 1] = dx; }
+    for (int k = -1; k <= ny + 2; k++) { h.celly[k + 1] = 0.5 * (h.vertexy[k + 1] + h.vertexy[k + 2]); h.celldy[k + 1] = dy; }
+    for (int l = -1; l <= nz + 2; l++) { h.cellz[l + 1] = 0.5 * (h.vertexz[l + 1] + h.vertexz[l + 2]); h.celldz[l + 1] = dz; }
+
+    const size_t cx = nx + 4, cy = ny + 4, cz = nz + 4, vx = nx + 5, vy = ny + 5, vz = nz + 5;
+    h.volume.assign(cx * cy * cz, 0.0);
+    h.xarea.assign(vx * cy * cz, 0.0);
+    h.yarea.assign(cx * vy * cz, 0.0);
+    h.zarea.assign(cx * cy * vz, 0.0);
+    h.density0.assign(cx * cy * cz, 0.0);
+    h.energy0.assign(cx * cy * cz, 0.0);
+    h.xvel0.assign(vx * vy * vz, 0.0);
+    h.yvel0.assign(vx * vy * vz, 0.0);
+    h.zvel0.assign(vx * vy * vz, 0.0);
+    auto C = [&](int j, int k, int l) { return (size_t)(j + 1) + cx * ((size_t)(k + 1) + cy * (size_t)(l + 1)); };
+    auto V = [&](int j, int k, int l) { return (size_t)(j + 1) + vx * ((size_t)(k + 1) + vy * (size_t)(l + 1)); };
+    auto XF = [&](int j, int k, int l) { return (size_t)(j + 1) + vx * ((size_t)(k + 1) + cy * (size_t)(l + 1)); };
+    auto YF = [&](int j, int k, int l) { return (size_t)(j + 1) + cx * ((size_t)(k + 1) + vy * (size_t)(l + 1)); };
+    for (int l = -1; l <= nz + 2; l++)
+        for (int k = -1; k <= ny + 2; k++)
+            for (int j = -1; j <= nx + 2; j++) {
+                h.volume[C(j, k, l)] = dx * dy * dz;
+                h.xarea[XF(j, k, l)] = h.celldy[k + 1] * h.celldz[l + 1];
+                h.yarea[YF(j, k, l)] = h.celldx[j + 1] * h.celldz[l + 1];
+                h.zarea[C(j, k, l)] = h.celldx[j + 1] * h.celldy[k + 1];
+            }
+    // generate_chunk_kernel.f90: background state over cells, then states 2..n
+    for (int l = -1; l <= nz + 2; l++)
+        for (int k = -1; k <= ny + 2; k++)
+            for (int j = -1; j <= nx + 2; j++) {
+                h.energy0[C(j, k, l)] = d.state_energy[0];
+                h.density0[C(j, k, l)] = d.state_density[0];
+                h.xvel0[V(j, k, l)] = d.state_xvel[0];
+                h.yvel0[V(j, k, l)] = d.state_yvel[0];
+                h.zvel0[V(j, k, l)] = d.state_zvel[0];
+            }
+    for (int s = 1; s < d.number_of_states; s++) {
+        const double xc = d.state_xmin[s], yc = d.state_ymin[s], zc = d.state_zmin[s];
+        for (int l = -1; l <= nz + 2; l++)
+            for (int k = -1; k <= ny + 2; k++)
+                for (int j = -1; j <= nx + 2; j++) {
+                    bool hit = false;
+                    if (d.state_geometry[s] == 1) {
+                        hit = h.vertexx[j + 2] >= d.state_xmin[s] && h.vertexx[j + 1] < d.state_xmax[s] &&
+                              h.vertexy[k + 2] >= d.state_ymin[s] && h.vertexy[k + 1] < d.state_ymax[s] &&
+                              h.vertexz[l + 2] >= d.state_zmin[s] && h.vertexz[l + 1] < d.state_zmax[s];
+                    } else if (d.state_geometry[s] == 2) {
+                        const double r = std::sqrt((h.cellx[j + 1] - xc) * (h.cellx[j + 1] - xc) +
+                                                   (h.celly[k + 1] - yc) * (h.celly[k + 1] - yc) +
+                                                   (h.cellz[l + 1] - zc) * (h.cellz[l + 1] - zc));
+                        hit = r <= d.state_radius[s];
+                    } else if (d.state_geometry[s] == 3) {
+                        hit = h.vertexx[j + 1] == xc && h.vertexy[k + 1] == yc && h.vertexz[l + 1] == zc;
+                    }
+                    if (!hit) continue;
+                    h.energy0[C(j, k, l)] = d.state_energy[s];
+                    h.density0[C(j, k, l)] = d.state_density[s];
+                    for (int lt = l; lt <= l + 1; lt++)
+                        for (int kt = k; kt <= k + 1; kt++)
+                            for (int jt = j; jt <= j + 1; jt++) {
+                                h.xvel0[V(jt, kt, lt)] = d.state_xvel[s];
+                                h.yvel0[V(jt, kt, lt)] = d.state_yvel[s];
+                                h.zvel0[V(jt, kt, lt)] = d.state_zvel[s];
+                            }
+                }
+    }
+}
+
+static double *dev_alloc(clover_chunk *c, size_t n)
+{
+    double *p = nullptr;
+    hip_check(hipMalloc(&p, n * sizeof(double)), "hipMalloc");
+    hip_check(hipMemsetAsync(p, 0, n * sizeof(double), c->st), "hipMemset");
+    c->bytes += (long long)(n * sizeof(double));
+    return p;
+}
+
+static double *dev_upload_1d(clover_chunk *c, const std::vector<double> &v)
+{
+    double *p = dev_alloc(c, v.size());
+    hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->st), "H2D");
+    return p;
+}
+
+// ------------------------------------------------------------ phases
+static void set_fields(int *fields, std::initializer_list<int> ids)
+{
+    for (int f = 0; f < NUM_FIELDS; f++) fields[f] = 0;
+    for (int id : ids) fields[id] = 1;
+}
+
+static void update_halo(clover_chunk *c, const int *fields, int depth)
+{
+    {
+        Phase ph(c, P_TILE_HALO);
+        // update_tile_halo.f90:47-423: top/bottom for all tiles, then
+        // left/right, then front/back
+        static const int passes[3][2] = {{4, 3}, {1, 2}, {6, 5}};
+        for (int p = 0; p < 3; p++)
+            for (size_t t = 0; t < c->tiles.size(); t++)
+                for (int s = 0; s < 2; s++) {
+                    const int face = passes[p][s];
+                    Tile &tt = c->tiles[t];
+                    const int n = tt.nbr[face - 1];
+                    if (n == -1) continue;
+                    Tile &nt = c->tiles[n - 1];
+                    hip_check(launch_tile_halo(face, tt.g, tt.f, nt.g, nt.f, fields, depth, c->st), "tile_halo");
+                }
+    }
+    Phase ph(c, P_SELF_HALO);
+    for (auto &t : c->tiles) {
+        int ext[6];
+        for (int f = 0; f < 6; f++) ext[f] = t.nbr[f] == -1;  // one chunk: every chunk face is external
+        hip_check(launch_update_halo(t.g, t.f, fields, ext, depth, c->st), "update_halo");
+    }
+}
+
+static void ideal_gas(clover_chunk *c, bool predict)
+{
+    Phase ph(c, P_IDEAL_GAS);
+    for (auto &t : c->tiles) {
+        if (!predict)
+            hip_check(launch_ideal_gas(t.g, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED], c->st),
+                      "ideal_gas");
+        else
+            hip_check(launch_ideal_gas(t.g, t.f[F_DENSITY1], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_SOUNDSPEED], c->st),
+                      "ideal_gas");
+    }
+}
+
+// timestep.f90:28-128.  Returns 1 on a small time step.
+static int timestep(clover_chunk *c)
+{
+    int fields[NUM_FIELDS];
+    const clover_deck &d = c->deck;
+    c->dt = c->g_big;
+    ideal_gas(c, false);
+    set_fields(fields, {F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0});
+    update_halo(c, fields, 1);
+    {
+        Phase ph(c, P_VISCOSITY);
+        for (auto &t : c->tiles)
+            hip_check(launch_viscosity(t.g, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.celldx, t.celldy, t.celldz,
+                                       t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0],
+                                       t.f[F_YVEL0], t.f[F_ZVEL0], c->st),
+                      "viscosity");
+    }
+    set_fields(fields, {F_VISCOSITY});
+    update_halo(c, fields, 1);
+    {
+        Phase ph(c, P_TIMESTEP);
+        const int T = (int)c->tiles.size();
+        for (int ti = 0; ti < T; ti++) {
+            Tile &t = c->tiles[ti];
+            int np = 0;
+            double *part = c->partial + (size_t)ti * kMaxPartials * 5;
+            hip_check(launch_calc_dt_cells(t.g, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
+                                           c->g_small, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.celldx, t.celldy,
+                                           t.celldz, t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_VISCOSITY],
+                                           t.f[F_SOUNDSPEED], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], nullptr, part,
+                                           &np, c->st),
+                      "calc_dt");
+            hip_check(launch_min_final(part, np, c->g_big, c->scalar + 8 * ti, c->st), "calc_dt min");
+        }
+        hip_check(hipMemcpyAsync(c->h_scalar, c->scalar, sizeof(double) * 8 * T, hipMemcpyDeviceToHost, c->st),
+                  "dt D2H");
+        hip_check(hipStreamSynchronize(c->st), "sync");
+        for (int ti = 0; ti < T; ti++) {
+            const double dtlp = c->h_scalar[8 * ti];
+            if (dtlp <= c->dt) c->dt = dtlp;
+        }
+    }
+    double m = c->dt;
+    if (c->dtold * d.dtrise < m) m = c->dtold * d.dtrise;
+    if (d.dtmax < m) m = d.dtmax;
+    c->dt = m;
+    if (c->dt < d.dtmin) return 1;
+    c->dtold = c->dt;
+    return 0;
+}
+
+static void pdv(clover_chunk *c, bool predict)
+{
+    {
+        Phase ph(c, P_PDV);
+        for (auto &t : c->tiles)
+            hip_check(launch_pdv(predict ? 1 : 0, t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA],
+                                 t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_DENSITY1], t.f[F_ENERGY0], t.f[F_ENERGY1],
+                                 t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0],
+                                 t.f[F_YVEL1], t.f[F_ZVEL0], t.f[F_ZVEL1], c->st),
+                      "PdV");
+    }
+    if (predict) {
+        int fields[NUM_FIELDS];
+        ideal_gas(c, true);
+        set_fields(fields, {F_PRESSURE});
+        update_halo(c, fields, 1);
+        Phase ph(c, P_REVERT);
+        for (auto &t : c->tiles)
+            hip_check(launch_revert(t.g, t.f[F_DENSITY0], t.f[F_DENSITY1], t.f[F_ENERGY0], t.f[F_ENERGY1], c->st),
+                      "revert");
+    }
+}
+
+static void advec_cell(clover_chunk *c, int sweep, int dir)
+{
+    Phase ph(c, P_CELL_ADVEC);
+    for (auto &t : c->tiles)
+        hip_check(launch_advec_cell(t.g, c->advect_x, dir, sweep, t.vertexdx, t.vertexdy, t.vertexdz, t.f[X_VOLUME],
+                                    t.f[F_DENSITY1], t.f[F_ENERGY1], t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X],
+                                    t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z],
+                                    t.f[X_W1], t.f[X_W2], t.f[X_W3], t.f[X_W4], t.f[X_W5], t.f[X_W6], t.f[X_W7],
+                                    c->st),
+                  "advec_cell");
+}
+
+static void advec_mom(clover_chunk *c, int sweep, int dir)
+{
+    Phase ph(c, P_MOM_ADVEC);
+    for (auto &t : c->tiles) {
+        double *vel[3] = {t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1]};
+        for (int which = 1; which <= 3; which++)
+            hip_check(launch_advec_mom(t.g, vel[which - 1], t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X],
+                                       t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z],
+                                       t.f[X_VOLUME], t.f[F_DENSITY1], t.f[X_W1], t.f[X_W2], t.f[X_W3], t.f[X_W5],
+                                       t.f[X_W6], t.f[X_W7], t.celldx, t.celldy, t.celldz, c->advect_x, which, sweep,
+                                       dir, c->st),
+                      "advec_mom");
+    }
+}
+
+// advection.f90:26-181
+static void advection(clover_chunk *c)
+{
+    int fc[NUM_FIELDS], fm[NUM_FIELDS];
+    set_fields(fc, {F_ENERGY1, F_DENSITY1, F_VOL_FLUX_X, F_VOL_FLUX_Y, F_VOL_FLUX_Z});
+    set_fields(fm, {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1, F_MASS_FLUX_X, F_MASS_FLUX_Y, F_MASS_FLUX_Z});
+    int dir = c->advect_x ? 1 : 3;
+    update_halo(c, fc, 2);
+    advec_cell(c, 1, dir);
+    update_halo(c, fm, 2);
+    advec_mom(c, 1, dir);
+    update_halo(c, fc, 2);
+    advec_cell(c, 2, 2);
+    update_halo(c, fm, 2);
+    advec_mom(c, 2, 2);
+    dir = c->advect_x ? 3 : 1;
+    advec_cell(c, 3, dir);
+    update_halo(c, fm, 2);
+    advec_mom(c, 3, dir);
+}
+
+static void field_summary(clover_chunk *c, double *out)
+{
+    ideal_gas(c, false);
+    Phase ph(c, P_SUMMARY);
+    const int T = (int)c->tiles.size();
+    for (int ti = 0; ti < T; ti++) {
+        Tile &t = c->tiles[ti];
+        int np = 0;
+        double *part = c->partial + (size_t)ti * kMaxPartials * 5;
+        hip_check(launch_field_summary_partial(t.g, t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE],
+                                               t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], part, &np, c->st),
+                  "field_summary");
+        hip_check(launch_sum5_final(part, np, c->scalar + 8 * ti, c->st), "field_summary sum");
+    }
+    hip_check(hipMemcpyAsync(c->h_scalar, c->scalar, sizeof(double) * 8 * T, hipMemcpyDeviceToHost, c->st), "D2H");
+    hip_check(hipStreamSynchronize(c->st), "sync");
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int ti = 0; ti < T; ti++)
+        for (int q = 0; q < 5; q++) s[q] = s[q] + c->h_scalar[8 * ti + q];
+    for (int q = 0; q < 5; q++) out[q] = s[q];
+}
+
+static void do_step(clover_chunk *c)
+{
+    pdv(c, true);
+    {
+        Phase ph(c, P_ACCEL);
+        for (auto &t : c->tiles)
+            hip_check(launch_accelerate(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[X_VOLUME],
+                                        t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0],
+                                        t.f[F_YVEL0], t.f[F_ZVEL0], t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->st),
+                      "accelerate");
+    }
+    pdv(c, false);
+    {
+        Phase ph(c, P_FLUX);
+        for (auto &t : c->tiles)
+            hip_check(launch_flux_calc(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[F_XVEL0],
+                                       t.f[F_YVEL0], t.f[F_ZVEL0], t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1],
+                                       t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y], t.f[F_VOL_FLUX_Z], c->st),
+                      "flux_calc");
+    }
+    advection(c);
+    {
+        Phase ph(c, P_RESET);
+        for (auto &t : c->tiles)
+            hip_check(launch_reset_field(t.g, t.f[F_DENSITY0], t.f[F_DENSITY1], t.f[F_ENERGY0], t.f[F_ENERGY1],
+                                         t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1], t.f[F_ZVEL0],
+                                         t.f[F_ZVEL1], c->st),
+                      "reset_field");
+    }
+    c->advect_x = !c->advect_x;
+    c->time = c->time + c->dt;
+}
+
+static void destroy(clover_chunk *c)
+{
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto &t : c->tiles) {
+        for (int a = 0; a < NUM_ARRAYS; a++)
+            if (t.f[a]) (void)hipFree(t.f[a]);
+        double *oned[] = {t.cellx, t.celly, t.cellz, t.celldx, t.celldy, t.celldz, t.vertexdx, t.vertexdy, t.vertexdz};
+        for (double *p : oned)
+            if (p) (void)hipFree(p);
+    }
+    if (c->partial) (void)hipFree(c->partial);
+    if (c->scalar) (void)hipFree(c->scalar);
+    if (c->h_scalar) (void)hipHostFree(c->h_scalar);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c->ktimer;
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+}  // namespace clover
+
+extern "C" {
+
+int clover_chunk_create(const clover_deck *deck, int device, clover_chunk **out)
+{
+    clover_chunk *c = nullptr;
+    try {
+        if (!deck || !out) throw Error(E_ARG, "null argument");
+        const clover_deck &d = *deck;
+        if (d.x_cells < 1 || d.y_cells < 1 || d.z_cells < 1) throw Error(E_ARG, "cells must be positive");
+        if (d.number_of_states < 1 || d.number_of_states > CLOVER_MAX_STATES) throw Error(E_ARG, "bad state count");
+        if (d.tiles_per_chunk < 1) throw Error(E_ARG, "tiles_per_chunk must be positive");
+        c = new clover_chunk();  // value-initialised: every scalar member starts at zero
+        c->deck = d;
+        c->device = device;
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&c->ev0), "event");
+        hip_check(hipEventCreate(&c->ev1), "event");
+        c->g_small = (double)1.0e-16f;  // default-real literals (data.f90:30-31)
+        c->g_big = (double)1.0e+21f;
+        decompose(d, c->tiles);
+        const int T = (int)c->tiles.size();
+        c->partial = dev_alloc(c, (size_t)T * kMaxPartials * 5);
+        c->scalar = dev_alloc(c, (size_t)T * 8);
+        hip_check(hipHostMalloc(&c->h_scalar, sizeof(double) * 8 * T, hipHostMallocDefault), "hipHostMalloc");
+        for (auto &t : c->tiles) {
+            t.g = make_lay(1, t.nx, 1, t.ny, 1, t.nz);
+            const int64_t n = lay_elems(t.g);
+            if (n >= (int64_t(1) << 31)) throw Error(E_ARG, "tile too large for 32-bit element offsets");
+            for (int a = 0; a < NUM_ARRAYS; a++) t.f[a] = dev_alloc(c, (size_t)n);
+            HostInit h;
+            host_init(d, t, h);
+            upload_dense(t.g, AC_CELL, h.volume.data(), t.f[X_VOLUME], c->st);
+            upload_dense(t.g, AC_XF, h.xarea.data(), t.f[X_XAREA], c->st);
+            upload_dense(t.g, AC_YF, h.yarea.data(), t.f[X_YAREA], c->st);
+            upload_dense(t.g, AC_ZF, h.zarea.data(), t.f[X_ZAREA], c->st);
+            upload_dense(t.g, AC_CELL, h.density0.data(), t.f[F_DENSITY0], c->st);
+            upload_dense(t.g, AC_CELL, h.energy0.data(), t.f[F_ENERGY0], c->st);
+            upload_dense(t.g, AC_VERT, h.xvel0.data(), t.f[F_XVEL0], c->st);
+            upload_dense(t.g, AC_VERT, h.yvel0.data(), t.f[F_YVEL0], c->st);
+            upload_dense(t.g, AC_VERT, h.zvel0.data(), t.f[F_ZVEL0], c->st);
+            t.cellx = dev_upload_1d(c, h.cellx);
+            t.celly = dev_upload_1d(c, h.celly);
+            t.cellz = dev_upload_1d(c, h.cellz);
+            t.celldx = dev_upload_1d(c, h.celldx);
+            t.celldy = dev_upload_1d(c, h.celldy);
+            t.celldz = dev_upload_1d(c, h.celldz);
+            t.vertexdx = dev_upload_1d(c, h.vertexdx);
+            t.vertexdy = dev_upload_1d(c, h.vertexdy);
+            t.vertexdz = dev_upload_1d(c, h.vertexdz);
+            t.h_cellx = h.cellx;
+            t.h_celly = h.celly;
+            t.h_cellz = h.cellz;
+            hip_check(hipStreamSynchronize(c->st), "sync");  // host buffers die here
+        }
+        // start.f90:58-129
+        c->time = 0.0;
+        c->step = 0;
+        c->dtold = d.dtinit;
+        c->dt = d.dtinit;
+        c->advect_x = 1;
+        ideal_gas(c, false);
+        int fields[NUM_FIELDS];
+        set_fields(fields, {F_DENSITY0, F_ENERGY0, F_PRESSURE, F_VISCOSITY, F_DENSITY1, F_ENERGY1, F_XVEL0, F_YVEL0,
+                            F_ZVEL0, F_XVEL1, F_YVEL1, F_ZVEL1});
+        update_halo(c, fields, 2);
+        hip_check(hipStreamSynchronize(c->st), "sync");
+        *out = c;
+        return 0;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        destroy(c);
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        destroy(c);
+        return E_ARG;
+    }
+}
+
+int clover_chunk_destroy(clover_chunk *c)
+{
+    destroy(c);
+    return 0;
+}
+
+int clover_chunk_step(clover_chunk *c, double *dt)
+{
+    CLOVER_API_BEGIN
+    if (!c) throw Error(E_ARG, "null chunk");
+    KTIMER_SCOPE(c);
+    c->step = c->step + 1;
+    if (timestep(c)) {
+        if (dt) *dt = c->dt;
+        set_last_error("timestep: small timestep");
+        return 1;
+    }
+    do_step(c);
+    if (dt) *dt = c->dt;
+    CLOVER_API_END
+}
+
+int clover_chunk_field_summary(clover_chunk *c, double *out)
+{
+    CLOVER_API_BEGIN
+    if (!c || !out) throw Error(E_ARG, "null argument");
+    KTIMER_SCOPE(c);
+    field_summary(c, out);
+    CLOVER_API_END
+}
+
+int clover_chunk_run(clover_chunk *c, int max_steps, clover_summary *rows, int max_rows, int *nrows)
+{
+    CLOVER_API_BEGIN
+    if (!c) throw Error(E_ARG, "null chunk");
+    KTIMER_SCOPE(c);
+    int n = 0;
+    const clover_deck &d = c->deck;
+    auto summary_row = [&]() {
+        double s[5];
+        field_summary(c, s);
+        if (rows && n < max_rows) {
+            rows[n].step = c->step;
+            rows[n].time = c->time;
+            rows[n].vol = s[0];
+            rows[n].mass = s[1];
+            rows[n].ie = s[2];
+            rows[n].ke = s[3];
+            rows[n].press = s[4];
+        }
+        n++;
+    };
+    for (int it = 0; it < max_steps; it++) {
+        c->step = c->step + 1;
+        if (timestep(c)) {
+            if (nrows) *nrows = n < max_rows ? n : max_rows;
+            set_last_error("timestep: small timestep");
+            return 1;
+        }
+        do_step(c);
+        if (d.summary_frequency != 0 && c->step % d.summary_frequency == 0) summary_row();
+        if (c->time + c->g_small > d.end_time || c->step >= d.end_step) {
+            summary_row();
+            break;
+        }
+    }
+    hip_check(hipStreamSynchronize(c->st), "sync");
+    if (nrows) *nrows = n < max_rows ? n : max_rows;
+    CLOVER_API_END
+}
+
+int clover_chunk_state(const clover_chunk *c, int *step, double *time, double *dt)
+{
+    if (!c) return E_ARG;
+    if (step) *step = c->step;
+    if (time) *time = c->time;
+    if (dt) *dt = c->dt;
+    return 0;
+}
+
+int clover_chunk_tiles(const clover_chunk *c, int *ntiles)
+{
+    if (!c || !ntiles) return E_ARG;
+    *ntiles = (int)c->tiles.size();
+    return 0;
+}
+
+int clover_chunk_tile_info(const clover_chunk *c, int t, int *dims3, int *offs3, int *neighbours6)
+{
+    if (!c || t < 0 || t >= (int)c->tiles.size()) return E_ARG;
+    const Tile &tt = c->tiles[t];
+    if (dims3) { dims3[0] = tt.nx; dims3[1] = tt.ny; dims3[2] = tt.nz; }
+    if (offs3) { offs3[0] = tt.left; offs3[1] = tt.bottom; offs3[2] = tt.back; }
+    if (neighbours6) memcpy(neighbours6, tt.nbr, sizeof(tt.nbr));
+    return 0;
+}
+
+int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host)
+{
+    CLOVER_API_BEGIN
+    if (!c || !host || t < 0 || t >= (int)c->tiles.size()) throw Error(E_ARG, "bad argument");
+    if (field < 0 || field > X_ZAREA) throw Error(E_ARG, "bad field id");
+    Tile &tt = c->tiles[t];
+    download_dense(tt.g, class_of(field), tt.f[field], host, c->st);
+    hip_check(hipStreamSynchronize(c->st), "sync");
+    CLOVER_API_END
+}
+
+int clover_chunk_set_profiling(clover_chunk *c, int on)
+{
+    if (!c) return E_ARG;
+    c->profiling = on != 0;
+    for (int p = 0; p < P_COUNT; p++) { c->prof_ms[p] = 0.0; c->prof_calls[p] = 0; }
+    return 0;
+}
+
+int clover_chunk_profile(const clover_chunk *c, double *ms, int *calls, int n)
+{
+    if (!c) return E_ARG;
+    for (int p = 0; p < n && p < P_COUNT; p++) {
+        if (ms) ms[p] = c->prof_ms[p];
+        if (calls) calls[p] = c->prof_calls[p];
+    }
+    return P_COUNT;
+}
+
+const char *clover_chunk_profile_name(int i) { return (i >= 0 && i < P_COUNT) ? kProfNames[i] : ""; }
+
+long long clover_chunk_device_bytes(const clover_chunk *c) { return c ? c->bytes : 0; }
+
+int clover_chunk_set_kernel_timing(clover_chunk *c, int on)
+{
+    CLOVER_API_BEGIN
+    if (!c) throw Error(E_ARG, "null chunk");
+    if (on) {
+        if (!c->ktimer) c->ktimer = new EventTimer();
+        c->ktimer->reset();
+    } else {
+        delete c->ktimer;
+        c->ktimer = nullptr;
+    }
+    CLOVER_API_END
+}
+
+int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen)
+{
+    CLOVER_API_BEGIN
+    if (!c || !buf || buflen < 1) throw Error(E_ARG, "bad argument");
+    if (!c->ktimer) throw Error(E_STATE, "kernel timing is off");
+    c->ktimer->drain();
+    std::string out;
+    char line[256];
+    for (const auto &kv : c->ktimer->stats) {
+        snprintf(line, sizeof line, "%s %.6f %lld\n", kv.first.c_str(), kv.second.ms, kv.second.launches);
+        out += line;
+    }
+    if ((int)out.size() + 1 > buflen) throw Error(E_ARG, "buffer too small");
+    memcpy(buf, out.c_str(), out.size() + 1);
+    CLOVER_API_END
+}
+
+}  // extern "C"

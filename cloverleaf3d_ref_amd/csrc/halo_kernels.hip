@@ -1,0 +1,239 @@
+// Halo kernels for gfx950: the reflective boundary of external faces
+// (update_halo_kernel.f90:115-1425) and the tile-to-tile copies
+// (update_tile_halo_kernel.f90:31-1993), table driven.
+//
+// One launch handles one axis (both faces when the tile is at least 3 cells
+// thick along it, else one face) for every flagged field: blockIdx.z picks
+// (field, face), threads cover the two tangential axes, each thread walks
+// the 1..depth halo layers.  Exact moves, no arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace clover {
+
+// How a field sits on an axis (see the reflective formulas below).
+enum { K_CELL = 0, K_STAG = 1, K_FTAN = 2 };
+
+struct HaloDesc {
+    unsigned char kind[3];   // per axis x, y, z
+    unsigned char neg[3];    // sign flip on the faces normal to this axis
+    unsigned char ext[6][2]; // per face L,R,B,T,K,F: +1 on the first/second tangential range
+};
+
+// Reflective halo, per field id (data.f90:55-74).  For a face normal to
+// axis a, for d = 1..depth:   halo(W) = sgn * f(R)   with
+//   CELL  low W=1-d R=d     high W=max+d   R=max+1-d   (cell-centred data)
+//   STAG  low W=1-d R=1+d   high W=max+1+d R=max+1-d   (vertex data, face data normal to a)
+//   FTAN  low W=1-d R=1+d   high W=max+d   R=max-d     (face data tangential to a)
+// and tangential ranges min-depth..max+depth (+ext).  Transcribed from
+// update_halo_kernel.f90: cells 115-596, velocities 604-1004, vol/mass_flux_x
+// 1018-1142, _y 1156-1280, _z 1291-1417 (vol_flux_z's bottom face stops at
+// z_max+depth, mass_flux_z's at z_max+1+depth).
+static const HaloDesc kHalo[NUM_FIELDS] = {
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // density0
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // density1
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // energy0
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // energy1
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // pressure
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // viscosity
+    {{0, 0, 0}, {0, 0, 0}, {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // soundspeed
+    {{1, 1, 1}, {1, 0, 0}, {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}},  // xvel0
+    {{1, 1, 1}, {1, 0, 0}, {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}},  // xvel1
+    {{1, 1, 1}, {0, 1, 0}, {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}},  // yvel0
+    {{1, 1, 1}, {0, 1, 0}, {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}},  // yvel1
+    {{1, 1, 1}, {0, 0, 1}, {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}},  // zvel0
+    {{1, 1, 1}, {0, 0, 1}, {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}},  // zvel1
+    {{1, 2, 2}, {1, 0, 0}, {{0, 0}, {0, 0}, {1, 0}, {1, 0}, {0, 0}, {0, 0}}},  // vol_flux_x
+    {{2, 1, 2}, {0, 1, 0}, {{1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // vol_flux_y
+    {{2, 2, 1}, {0, 0, 1}, {{0, 1}, {0, 1}, {0, 0}, {0, 1}, {0, 0}, {0, 0}}},  // vol_flux_z
+    {{1, 2, 2}, {1, 0, 0}, {{0, 0}, {0, 0}, {1, 0}, {1, 0}, {0, 0}, {0, 0}}},  // mass_flux_x
+    {{2, 1, 2}, {0, 1, 0}, {{1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}},  // mass_flux_y
+    {{2, 2, 1}, {0, 0, 1}, {{0, 1}, {0, 1}, {0, 1}, {0, 1}, {0, 0}, {0, 0}}},  // mass_flux_z
+};
+
+// staggered (vertex-like) along x, y, z, per field id
+static const unsigned char kStag[NUM_FIELDS][3] = {
+    {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
+    {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1},
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+
+constexpr int kMaxItems = 2 * NUM_FIELDS;
+
+struct HaloItem {
+    double *dst;
+    const double *src;
+    double sgn;
+    int kind;  // reflective: K_*; tile copy: staggered on the axis (0/1)
+    int hi;    // 0 low face, 1 high face
+    int ea, eb;
+};
+
+struct HaloBatch {
+    HaloItem it[kMaxItems];
+};
+
+constexpr int HX = 64, HY = 4;
+
+__device__ inline int tan_a(int axis) { return axis == 0 ? 1 : 0; }
+__device__ inline int tan_b(int axis) { return axis == 2 ? 1 : 2; }
+
+__global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
+{
+    const HaloItem &it = batch.it[blockIdx.z];
+    const int mn[3] = {g.x_min, g.y_min, g.z_min};
+    const int mx[3] = {g.x_max, g.y_max, g.z_max};
+    const int ta = tan_a(axis), tb = tan_b(axis);
+    const int c = mn[ta] - depth + (int)(blockIdx.x * HX + threadIdx.x);
+    const int b = mn[tb] - depth + (int)(blockIdx.y * HY + threadIdx.y);
+    if (c > mx[ta] + it.ea + depth || b > mx[tb] + it.eb + depth) return;
+    const int amax = mx[axis];
+    for (int d = 1; d <= depth; d++) {
+        int w, r;
+        if (!it.hi) {
+            w = 1 - d;
+            r = it.kind == K_CELL ? d : 1 + d;
+        } else if (it.kind == K_CELL) {
+            w = amax + d; r = amax + 1 - d;
+        } else if (it.kind == K_STAG) {
+            w = amax + 1 + d; r = amax + 1 - d;
+        } else {
+            w = amax + d; r = amax - d;
+        }
+        int iw[3], ir[3];
+        iw[axis] = w; ir[axis] = r;
+        iw[ta] = ir[ta] = c;
+        iw[tb] = ir[tb] = b;
+        it.dst[idx(g, iw[0], iw[1], iw[2])] = it.sgn * it.dst[idx(g, ir[0], ir[1], ir[2])];
+    }
+}
+
+static hipError_t run_batch_reflect(const Lay &g, int axis, int depth, const HaloBatch &b, int n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int mn[3] = {g.x_min, g.y_min, g.z_min};
+    const int mx[3] = {g.x_max, g.y_max, g.z_max};
+    const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
+    const int na = mx[ta] - mn[ta] + 2 + 2 * depth, nb = mx[tb] - mn[tb] + 2 + 2 * depth;
+    if (t_kernel_timer) t_kernel_timer->begin("k_reflect", st);
+    hipLaunchKernelGGL(k_reflect, dim3((na + HX - 1) / HX, (nb + HY - 1) / HY, n), dim3(HX, HY, 1), 0, st, g, axis,
+                       depth, b);
+    if (t_kernel_timer) t_kernel_timer->end(st);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_halo(const Lay &g, double *const *fp, const int *fields, const int *ext, int depth,
+                              hipStream_t st)
+{
+    // The reference's order per field is bottom, top, left, right, back,
+    // front; fields are independent arrays, so all fields' y faces go first,
+    // then x, then z.  The two faces of one axis touch disjoint cells when the
+    // tile is at least 3 cells thick along it and then share a launch.
+    static const int axis_order[3] = {1, 0, 2};
+    const int mn[3] = {g.x_min, g.y_min, g.z_min};
+    const int mx[3] = {g.x_max, g.y_max, g.z_max};
+    for (int ao = 0; ao < 3; ao++) {
+        const int axis = axis_order[ao];
+        const bool together = mx[axis] - mn[axis] + 1 >= 3;
+        HaloBatch b;
+        int n = 0;
+        for (int side = 0; side < 2; side++) {
+            const int face = 2 * axis + side;  // 0..5 = left,right,bottom,top,back,front
+            if (!ext[face]) continue;
+            for (int f = 0; f < NUM_FIELDS; f++) {
+                if (fields[f] != 1 || !fp[f]) continue;
+                const HaloDesc &h = kHalo[f];
+                HaloItem &it = b.it[n++];
+                it.dst = fp[f];
+                it.src = fp[f];
+                it.sgn = h.neg[axis] ? -1.0 : 1.0;
+                it.kind = h.kind[axis];
+                it.hi = side;
+                it.ea = h.ext[face][0];
+                it.eb = h.ext[face][1];
+            }
+            if (!together) {
+                hipError_t e = run_batch_reflect(g, axis, depth, b, n, st);
+                if (e != hipSuccess) return e;
+                n = 0;
+            }
+        }
+        hipError_t e = run_batch_reflect(g, axis, depth, b, n, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Tile copies.  Along the copy axis a field is cell-like or staggered;
+//   low face   f(min-d)   = N(Nmax+1-d)
+//   high face  f(max+d)   = N(Nmin-1+d)   cell-like
+//              f(max+1+d) = N(Nmin+d)     staggered
+// tangential ranges min-depth..max+depth, +1 where the field is staggered.
+__global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
+{
+    const HaloItem &it = batch.it[blockIdx.z];
+    const int mn[3] = {g.x_min, g.y_min, g.z_min};
+    const int mx[3] = {g.x_max, g.y_max, g.z_max};
+    const int nmn[3] = {gn.x_min, gn.y_min, gn.z_min};
+    const int nmx[3] = {gn.x_max, gn.y_max, gn.z_max};
+    const int ta = tan_a(axis), tb = tan_b(axis);
+    const int c = mn[ta] - depth + (int)(blockIdx.x * HX + threadIdx.x);
+    const int b = mn[tb] - depth + (int)(blockIdx.y * HY + threadIdx.y);
+    if (c > mx[ta] + it.ea + depth || b > mx[tb] + it.eb + depth) return;
+    for (int d = 1; d <= depth; d++) {
+        int w, r;
+        if (!it.hi) {
+            w = mn[axis] - d; r = nmx[axis] + 1 - d;
+        } else if (!it.kind) {
+            w = mx[axis] + d; r = nmn[axis] - 1 + d;
+        } else {
+            w = mx[axis] + 1 + d; r = nmn[axis] + d;
+        }
+        int iw[3], ir[3];
+        iw[axis] = w; ir[axis] = r;
+        iw[ta] = ir[ta] = c;
+        iw[tb] = ir[tb] = b;
+        it.dst[idx(g, iw[0], iw[1], iw[2])] = it.src[idx(gn, ir[0], ir[1], ir[2])];
+    }
+}
+
+hipError_t launch_tile_halo(int face, const Lay &g, double *const *fp, const Lay &gn, double *const *np,
+                            const int *fields, int depth, hipStream_t st)
+{
+    const int axis = (face - 1) / 2;
+    const int hi = (face - 1) % 2;
+    const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
+    HaloBatch b;
+    int n = 0;
+    for (int f = 0; f < NUM_FIELDS; f++) {
+        int gate = f, src = f;
+        // quirks of update_tile_halo_kernel.f90: energy1 follows the density1
+        // flag on the left, right, bottom and top faces (113-125, 443-455,
+        // 771-783, 1103-1115); the right face's zvel0 branch copies yvel0
+        // (579-589).
+        if (f == F_ENERGY1 && face <= 4) gate = F_DENSITY1;
+        if (f == F_ZVEL0 && face == 2) src = F_YVEL0;
+        if (fields[gate] != 1) continue;
+        if (!fp[src] || !np[src]) continue;
+        HaloItem &it = b.it[n++];
+        it.dst = fp[src];
+        it.src = np[src];
+        it.sgn = 1.0;
+        it.kind = kStag[src][axis];
+        it.hi = hi;
+        it.ea = kStag[src][ta];
+        it.eb = kStag[src][tb];
+    }
+    if (n == 0) return hipSuccess;
+    const int mn[3] = {g.x_min, g.y_min, g.z_min};
+    const int mx[3] = {g.x_max, g.y_max, g.z_max};
+    const int na = mx[ta] - mn[ta] + 2 + 2 * depth, nb = mx[tb] - mn[tb] + 2 + 2 * depth;
+    if (t_kernel_timer) t_kernel_timer->begin("k_tile_copy", st);
+    hipLaunchKernelGGL(k_tile_copy, dim3((na + HX - 1) / HX, (nb + HY - 1) / HY, n), dim3(HX, HY, 1), 0, st, g, gn,
+                       axis, depth, b);
+    if (t_kernel_timer) t_kernel_timer->end(st);
+    return hipGetLastError();
+}
+
+}  // namespace clover

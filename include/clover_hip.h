@@ -1,0 +1,224 @@
+/*
+ * C ABI of the MI355X (gfx950) CloverLeaf3D hydro kernels.
+ *
+ * Two layers, one shared library (cloverleaf3d_ref_amd/libclover_hip.so):
+ *
+ * 1. Drop-in kernels.  One entry point per reference *_kernel subroutine,
+ *    same arguments in the same order (scalars by value, arrays as the base
+ *    address of the caller's Fortran column-major arrays with lower bound
+ *    min-2, LOGICALs as int, flags by pointer).  A Fortran shim module of the
+ *    same name as the reference module calls these through ISO_C_BINDING.
+ *    Arrays are identified by their host base address: the first time an
+ *    address is seen the library allocates a device copy in HBM and uploads
+ *    it; from then on the device copy is authoritative and no field data
+ *    crosses PCIe, except dt (calc_dt) and the five summary sums
+ *    (field_summary), which are returned synchronously.  clover_hip_fetch()
+ *    copies a device array back into its host array on request.  Calls are
+ *    enqueued in program order on one HIP stream.
+ *
+ * 2. Device-resident driver.  clover_chunk_* runs start + the hydro loop of
+ *    the reference (start.f90, hydro.f90, timestep.f90, PdV.f90,
+ *    advection.f90, update_halo.f90, field_summary.f90) with every field in
+ *    HBM for the whole run.
+ *
+ * Every function returns 0 on success and a negative code on error;
+ * clover_hip_last_error() describes the last error of the calling thread.
+ */
+#ifndef CLOVER_HIP_H
+#define CLOVER_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLOVER_DIMS int x_min, int x_max, int y_min, int y_max, int z_min, int z_max
+
+/* ---------------------------------------------------------------- lifecycle */
+int clover_hip_init(int device);           /* select device, create the stream */
+int clover_hip_finalize(void);             /* free every device array (clover_finalize) */
+int clover_hip_sync(void);                 /* wait for all enqueued work */
+int clover_hip_fetch(const void *host);    /* device copy -> host array */
+int clover_hip_push(const void *host);     /* host array -> device copy */
+int clover_hip_forget(const void *host);   /* drop one device copy */
+int clover_hip_resident_count(void);       /* arrays currently resident */
+const char *clover_hip_last_error(void);
+
+/* ------------------------------------------------------- drop-in kernels */
+
+/* ideal_gas_kernel.f90:27-62 */
+int clover_hip_ideal_gas_kernel(CLOVER_DIMS, double *density, double *energy, double *pressure,
+                                double *soundspeed);
+
+/* viscosity_kernel.f90:32-139 */
+int clover_hip_viscosity_kernel(CLOVER_DIMS, double *xarea, double *yarea, double *zarea, double *celldx,
+                                double *celldy, double *celldz, double *density0, double *pressure,
+                                double *viscosity, double *xvel0, double *yvel0, double *zvel0);
+
+/* calc_dt_kernel.f90:28-183; dt_min (work_array1) receives the per-cell dt */
+int clover_hip_calc_dt_kernel(CLOVER_DIMS, double g_small, double g_big, double dtmin, double dtc_safe,
+                              double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
+                              double *xarea, double *yarea, double *zarea, double *cellx, double *celly,
+                              double *cellz, double *celldx, double *celldy, double *celldz, double *volume,
+                              double *density0, double *energy0, double *pressure, double *viscosity_a,
+                              double *soundspeed, double *xvel0, double *yvel0, double *zvel0, double *dt_min,
+                              double *dt_min_val, int *dtl_control, double *xl_pos, double *yl_pos,
+                              double *zl_pos, int *jldt, int *kldt, int *lldt, int *small);
+
+/* PdV_kernel.f90:32-172 (predict: LOGICAL as int) */
+int clover_hip_PdV_kernel(int predict, CLOVER_DIMS, double dt, double *xarea, double *yarea, double *zarea,
+                          double *volume, double *density0, double *density1, double *energy0, double *energy1,
+                          double *pressure, double *viscosity, double *xvel0, double *xvel1, double *yvel0,
+                          double *yvel1, double *zvel0, double *zvel1);
+
+/* revert_kernel.f90:29-65 */
+int clover_hip_revert_kernel(CLOVER_DIMS, double *density0, double *density1, double *energy0, double *energy1);
+
+/* accelerate_kernel.f90:27-115 */
+int clover_hip_accelerate_kernel(CLOVER_DIMS, double dt, double *xarea, double *yarea, double *zarea,
+                                 double *volume, double *density0, double *pressure, double *viscosity,
+                                 double *xvel0, double *yvel0, double *zvel0, double *xvel1, double *yvel1,
+                                 double *zvel1);
+
+/* flux_calc_kernel.f90:26-95 */
+int clover_hip_flux_calc_kernel(CLOVER_DIMS, double dt, double *xarea, double *yarea, double *zarea,
+                                double *xvel0, double *yvel0, double *zvel0, double *xvel1, double *yvel1,
+                                double *zvel1, double *vol_flux_x, double *vol_flux_y, double *vol_flux_z);
+
+/* advec_cell_kernel.f90:30-383 (advect_x: LOGICAL as int) */
+int clover_hip_advec_cell_kernel(CLOVER_DIMS, int advect_x, int dir, int sweep_number, double *vertexdx,
+                                 double *vertexdy, double *vertexdz, double *volume, double *density1,
+                                 double *energy1, double *mass_flux_x, double *vol_flux_x, double *mass_flux_y,
+                                 double *vol_flux_y, double *mass_flux_z, double *vol_flux_z, double *pre_vol,
+                                 double *post_vol, double *pre_mass, double *post_mass, double *advec_vol,
+                                 double *post_ener, double *ener_flux);
+
+/* advec_mom_kernel.f90:29-428; node_flux, node_mass_post and node_mass_pre
+ * computed by the which_vel=1 call are reused by the which_vel=2,3 calls */
+int clover_hip_advec_mom_kernel(CLOVER_DIMS, double *vel1, double *mass_flux_x, double *vol_flux_x,
+                                double *mass_flux_y, double *vol_flux_y, double *mass_flux_z, double *vol_flux_z,
+                                double *volume, double *density1, double *node_flux, double *node_mass_post,
+                                double *node_mass_pre, double *advec_vel, double *mom_flux, double *pre_vol,
+                                double *post_vol, double *celldx, double *celldy, double *celldz, int advect_x,
+                                int which_vel, int sweep_number, int direction);
+
+/* reset_field_kernel.f90:27-101 */
+int clover_hip_reset_field_kernel(CLOVER_DIMS, double *density0, double *density1, double *energy0,
+                                  double *energy1, double *xvel0, double *xvel1, double *yvel0, double *yvel1,
+                                  double *zvel0, double *zvel1);
+
+/* update_halo_kernel.f90:32-1427; fields[19], neighbours[6] */
+int clover_hip_update_halo_kernel(CLOVER_DIMS, const int *chunk_neighbours, const int *tile_neighbours,
+                                  double *density0, double *energy0, double *pressure, double *viscosity,
+                                  double *soundspeed, double *density1, double *energy1, double *xvel0,
+                                  double *yvel0, double *zvel0, double *xvel1, double *yvel1, double *zvel1,
+                                  double *vol_flux_x, double *vol_flux_y, double *vol_flux_z, double *mass_flux_x,
+                                  double *mass_flux_y, double *mass_flux_z, const int *fields, int depth);
+
+/* update_tile_halo_kernel.f90:31, 361, 690, 1022, 1351, 1674: the tile's
+ * bounds and 19 arrays, the neighbour's bounds and 19 arrays, flags, depth */
+#define CLOVER_TILE_ARGS                                                                                        \
+    CLOVER_DIMS, double *density0, double *energy0, double *pressure, double *viscosity, double *soundspeed,     \
+        double *density1, double *energy1, double *xvel0, double *yvel0, double *zvel0, double *xvel1,          \
+        double *yvel1, double *zvel1, double *vol_flux_x, double *vol_flux_y, double *vol_flux_z,              \
+        double *mass_flux_x, double *mass_flux_y, double *mass_flux_z, int n_xmin, int n_xmax, int n_ymin,      \
+        int n_ymax, int n_zmin, int n_zmax, double *n_density0, double *n_energy0, double *n_pressure,        \
+        double *n_viscosity, double *n_soundspeed, double *n_density1, double *n_energy1, double *n_xvel0,      \
+        double *n_yvel0, double *n_zvel0, double *n_xvel1, double *n_yvel1, double *n_zvel1,                   \
+        double *n_vol_flux_x, double *n_vol_flux_y, double *n_vol_flux_z, double *n_mass_flux_x,               \
+        double *n_mass_flux_y, double *n_mass_flux_z, const int *fields, int depth
+int clover_hip_update_tile_halo_left_kernel(CLOVER_TILE_ARGS);
+int clover_hip_update_tile_halo_right_kernel(CLOVER_TILE_ARGS);
+int clover_hip_update_tile_halo_top_kernel(CLOVER_TILE_ARGS);
+int clover_hip_update_tile_halo_bottom_kernel(CLOVER_TILE_ARGS);
+int clover_hip_update_tile_halo_front_kernel(CLOVER_TILE_ARGS);
+int clover_hip_update_tile_halo_back_kernel(CLOVER_TILE_ARGS);
+
+/* field_summary_kernel.f90:27-81 */
+int clover_hip_field_summary_kernel(CLOVER_DIMS, double *volume, double *density0, double *energy0,
+                                    double *pressure, double *xvel0, double *yvel0, double *zvel0, double *vol,
+                                    double *mass, double *ie, double *ke, double *press);
+
+/* ------------------------------------------------ device-resident driver */
+
+#define CLOVER_MAX_STATES 16
+
+/* A parsed input deck (read_input.f90:24-305), state extents already
+ * shrunk by a hundredth of a cell (read_input.f90:293-303). */
+typedef struct {
+    int x_cells, y_cells, z_cells;
+    double xmin, ymin, zmin, xmax, ymax, zmax;
+    double dtinit, dtmax, dtrise, dtmin;
+    double dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe;
+    double end_time;
+    int end_step;
+    int summary_frequency;
+    int tiles_per_chunk;
+    int tile_dims; /* 1, 2 or 3: 1d/2d/3d_tile_decomposition */
+    int number_of_states;
+    double state_density[CLOVER_MAX_STATES], state_energy[CLOVER_MAX_STATES];
+    double state_xvel[CLOVER_MAX_STATES], state_yvel[CLOVER_MAX_STATES], state_zvel[CLOVER_MAX_STATES];
+    double state_xmin[CLOVER_MAX_STATES], state_xmax[CLOVER_MAX_STATES];
+    double state_ymin[CLOVER_MAX_STATES], state_ymax[CLOVER_MAX_STATES];
+    double state_zmin[CLOVER_MAX_STATES], state_zmax[CLOVER_MAX_STATES];
+    double state_radius[CLOVER_MAX_STATES];
+    int state_geometry[CLOVER_MAX_STATES]; /* 1 cuboid, 2 sphere, 3 point */
+} clover_deck;
+
+typedef struct {
+    int step;
+    double time, vol, mass, press, ie, ke;
+} clover_summary;
+
+typedef struct clover_chunk clover_chunk;
+
+/* Allocates every tile's fields in HBM, builds the mesh and paints the
+ * states (initialise_chunk, generate_chunk), then primes pressure and the
+ * halos as start.f90:103-129 does. */
+int clover_chunk_create(const clover_deck *deck, int device, clover_chunk **out);
+int clover_chunk_destroy(clover_chunk *c);
+
+/* One step of hydro.f90:48-68: timestep, PdV predictor/corrector,
+ * accelerate, flux_calc, advection, reset_field; time += dt.  *dt gets the
+ * step's dt; returns 1 (and does nothing else) on a small time step. */
+int clover_chunk_step(clover_chunk *c, double *dt);
+
+/* field_summary.f90:27-139: ideal_gas, then the five sums (tiles summed in
+ * tile order).  out: vol, mass, ie, ke, press. */
+int clover_chunk_field_summary(clover_chunk *c, double *out);
+
+/* The hydro loop until end_step / end_time (hydro.f90:46-87) or max_steps
+ * more steps, with a summary row every summary_frequency steps and at the
+ * end.  Row 0 (start) is not included; call clover_chunk_field_summary. */
+int clover_chunk_run(clover_chunk *c, int max_steps, clover_summary *rows, int max_rows, int *nrows);
+
+/* step count, simulated time, last dt */
+int clover_chunk_state(const clover_chunk *c, int *step, double *time, double *dt);
+int clover_chunk_tiles(const clover_chunk *c, int *ntiles);
+/* tile t: interior extents, global offsets, neighbours[6] */
+int clover_chunk_tile_info(const clover_chunk *c, int t, int *dims3, int *offs3, int *neighbours6);
+
+/* Copy one field of tile t into a dense Fortran array of the field's own
+ * class.  field: 0..18 the FIELD_* ids minus one (data.f90:55-74); 19 volume,
+ * 20 xarea, 21 yarea, 22 zarea. */
+int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host);
+
+/* Per-phase device time (ms) accumulated with HIP events when profiling is
+ * on: the reference profiler's categories (hydro.f90:147-167). */
+int clover_chunk_set_profiling(clover_chunk *c, int on);
+int clover_chunk_profile(const clover_chunk *c, double *ms, int *calls, int n);
+const char *clover_chunk_profile_name(int i);
+
+/* HBM bytes held by the chunk */
+long long clover_chunk_device_bytes(const clover_chunk *c);
+
+/* Per-kernel device time: with timing on, every kernel launch of the
+ * chunk's calls is bracketed by HIP events on the chunk's stream (no extra
+ * host synchronisation).  clover_chunk_kernel_times writes one line per
+ * kernel, "name total_ms launches", into buf.  Turning timing on resets it. */
+int clover_chunk_set_kernel_timing(clover_chunk *c, int on);
+int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

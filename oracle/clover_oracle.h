@@ -198,6 +198,14 @@ int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *row
 
 /* Tile decomposition (clover.f90:258-447): fills per-tile local extents,
  * global offsets and neighbours (6 per tile, -1 external).  Returns tiles. */
+/* Symbol prefix oracle_run uses to find kernels in ref_lib ("ref_" by
+ * default; "clover_hip_" drives the product's drop-in entry points in the
+ * reference's call order, for the drop-in boundary test). */
+void oracle_set_kernel_prefix(const char *prefix);
+
+/* Wall time of the hydro loop of the last oracle_run (start excluded). */
+double oracle_last_hydro_seconds(void);
+
 int oracle_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_chunk, int tile_dims,
                            int *t_nx, int *t_ny, int *t_nz, int *t_left, int *t_bottom, int *t_back,
                            int *neighbours);

@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef void (*fn_ideal_gas)(OR_DIMS, const double *, const double *, double *, double *);
 typedef void (*fn_viscosity)(OR_DIMS, const double *, const double *, const double *, const double *,
@@ -421,41 +422,51 @@ static void ref_tile_adapter(int face, OR_DIMS, double *const *a, int n_xmin, in
                            b[10], b[11], b[12], b[13], b[14], b[15], b[16], b[17], b[18], fields, depth);
 }
 
-static int load_ref(const char *path, ktab *k)
+/* Loads a kernel table from a shared library whose entry points follow the
+ * reference argument lists: the reference's own kernels (prefix "ref_") or
+ * any other implementation of the same C signatures, e.g. the product's
+ * drop-in ABI (prefix "clover_hip_"), so that a library can be driven in
+ * exactly the reference's call order. */
+static int load_lib(const char *path, const char *prefix, ktab *k)
 {
     void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
     if (!h) {
         fprintf(stderr, "oracle: cannot load %s: %s\n", path, dlerror());
         return -1;
     }
-#define SYM(field, name)                                                     \
+    char name[128];
+#define SYM(field, kname)                                                    \
     do {                                                                     \
+        snprintf(name, sizeof name, "%s%s", prefix, kname);                  \
         *(void **)(&k->field) = dlsym(h, name);                              \
         if (!k->field) {                                                     \
-            fprintf(stderr, "oracle: %s lacks %s\n", path, name);            \
+            fprintf(stderr, "oracle: %s lacks %s\n", path, name);           \
             return -1;                                                       \
         }                                                                    \
     } while (0)
-    SYM(ideal_gas, "ref_ideal_gas_kernel");
-    SYM(viscosity, "ref_viscosity_kernel");
-    SYM(calc_dt, "ref_calc_dt_kernel");
-    SYM(pdv, "ref_pdv_kernel");
-    SYM(revert, "ref_revert_kernel");
-    SYM(accelerate, "ref_accelerate_kernel");
-    SYM(flux_calc, "ref_flux_calc_kernel");
-    SYM(advec_cell, "ref_advec_cell_kernel");
-    SYM(advec_mom, "ref_advec_mom_kernel");
-    SYM(reset, "ref_reset_field_kernel");
-    SYM(update_halo, "ref_update_halo_kernel");
-    SYM(summary, "ref_field_summary_kernel");
+    SYM(ideal_gas, "ideal_gas_kernel");
+    SYM(viscosity, "viscosity_kernel");
+    SYM(calc_dt, "calc_dt_kernel");
+    snprintf(name, sizeof name, "%sPdV_kernel", prefix);
+    *(void **)(&k->pdv) = dlsym(h, name);
+    if (!k->pdv) SYM(pdv, "pdv_kernel");
+    SYM(revert, "revert_kernel");
+    SYM(accelerate, "accelerate_kernel");
+    SYM(flux_calc, "flux_calc_kernel");
+    SYM(advec_cell, "advec_cell_kernel");
+    SYM(advec_mom, "advec_mom_kernel");
+    SYM(reset, "reset_field_kernel");
+    SYM(update_halo, "update_halo_kernel");
+    SYM(summary, "field_summary_kernel");
 #undef SYM
-    static const char *tile_names[6] = {"ref_update_tile_halo_left_kernel", "ref_update_tile_halo_right_kernel",
-                                        "ref_update_tile_halo_bottom_kernel", "ref_update_tile_halo_top_kernel",
-                                        "ref_update_tile_halo_back_kernel", "ref_update_tile_halo_front_kernel"};
+    static const char *tile_names[6] = {"update_tile_halo_left_kernel", "update_tile_halo_right_kernel",
+                                        "update_tile_halo_bottom_kernel", "update_tile_halo_top_kernel",
+                                        "update_tile_halo_back_kernel", "update_tile_halo_front_kernel"};
     for (int f = 0; f < 6; f++) {
-        *(void **)(&ref_tile_fns[f]) = dlsym(h, tile_names[f]);
+        snprintf(name, sizeof name, "%s%s", prefix, tile_names[f]);
+        *(void **)(&ref_tile_fns[f]) = dlsym(h, name);
         if (!ref_tile_fns[f]) {
-            fprintf(stderr, "oracle: %s lacks %s\n", path, tile_names[f]);
+            fprintf(stderr, "oracle: %s lacks %s\n", path, name);
             return -1;
         }
     }
@@ -463,12 +474,23 @@ static int load_ref(const char *path, ktab *k)
     return 0;
 }
 
+static double last_hydro_seconds = 0.0;
+
+double oracle_last_hydro_seconds(void) { return last_hydro_seconds; }
+
+static char kernel_prefix[64] = "ref_";
+
+void oracle_set_kernel_prefix(const char *prefix)
+{
+    snprintf(kernel_prefix, sizeof kernel_prefix, "%s", prefix ? prefix : "ref_");
+}
+
 int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *rows, int max_rows,
                double *dt_hist, int max_steps, int *steps_done)
 {
     ktab k;
     if (ref_lib) {
-        if (load_ref(ref_lib, &k)) return -1;
+        if (load_lib(ref_lib, kernel_prefix, &k)) return -1;
     } else {
         k.ideal_gas = oracle_ideal_gas_kernel;
         k.viscosity = oracle_viscosity_kernel;
@@ -536,8 +558,10 @@ int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *row
     int nrows = 0;
     if (nrows < max_rows) field_summary(&s, &rows[nrows++]);
 
-    /* hydro.f90:46-87 */
+    /* hydro.f90:46-87 (timed like the reference's Wall clock, hydro.f90:44,90) */
     int rc = 0;
+    struct timespec ts0, ts1;
+    clock_gettime(CLOCK_MONOTONIC, &ts0);
     for (;;) {
         s.step = s.step + 1;
         if (timestep(&s)) {
@@ -572,6 +596,8 @@ int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *row
             break;
         }
     }
+    clock_gettime(CLOCK_MONOTONIC, &ts1);
+    last_hydro_seconds = (double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec);
     if (steps_done) *steps_done = s.step;
     for (int ti = 0; ti < s.ntiles; ti++) free_field(&s.tiles[ti]);
     free(s.tiles);

@@ -115,3 +115,27 @@ def run(deck: Deck, use_ref: bool = False, max_rows: int = 4096,
 def thread_env(n: Optional[int]) -> None:
     if n is not None:
         os.environ["OMP_NUM_THREADS"] = str(n)
+
+
+def run_through(deck: Deck, lib_path: str, prefix: str, max_rows: int = 4096,
+                max_steps: int = 100000) -> Tuple[List[dict], np.ndarray]:
+    """The oracle driver (start + hydro loop in the reference's call order)
+    calling the kernels of another library with the reference argument
+    lists, found by symbol prefix (e.g. the product's drop-in ABI)."""
+    L = lib()
+    L.oracle_set_kernel_prefix.argtypes = [ctypes.c_char_p]
+    L.oracle_set_kernel_prefix(prefix.encode())
+    try:
+        od = to_oracle_deck(deck)
+        rows = (OracleSummary * max_rows)()
+        dts = np.zeros(max_steps, dtype=np.float64)
+        done = ctypes.c_int(0)
+        n = L.oracle_run(ctypes.byref(od), str(lib_path).encode(), rows, max_rows,
+                         dts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), max_steps, ctypes.byref(done))
+    finally:
+        L.oracle_set_kernel_prefix(b"ref_")
+    if n < 0:
+        raise RuntimeError(f"oracle_run failed ({n})")
+    out = [dict(step=r.step, time=r.time, vol=r.vol, mass=r.mass, press=r.press, ie=r.ie, ke=r.ke)
+           for r in rows[:n]]
+    return out, dts[:min(done.value, max_steps)].copy()
