@@ -40,10 +40,20 @@ HBM_PEAK_GBS = 8000.0
 # advec_mom_kernel.f90:95-163): 6, 5 and 4 arrays for sweeps 1, 2, 3, i.e. 5
 # per launch averaged over a step.
 ARRAYS_PER_CELL = {
+    # per-call kernels (one per reference loop nest)
     "k_ideal_gas": 4, "k_viscosity": 9, "k_calc_dt": 10, "k_revert": 4, "k_accelerate": 13,
     "k_flux_calc": 12, "k_cell_vols": 5, "k_cell_flux": 6, "k_cell_update": 12, "k_mom_vols": 5,
     "k_node_flux": 2, "k_node_mass": 5, "k_mom_flux": 4, "k_mom_update": 5, "k_reset_field": 10,
     "k_field_summary": 7, "k_pdv<1>": 13, "k_pdv<0>": 16,
+    # fused kernels (geometry as scalars).  Sweep-dependent ones averaged over
+    # the three sweeps of a step: mom_node reads mass_flux, density1 and
+    # 2/1/0 volume fluxes and writes node_flux, node masses post and pre;
+    # cell_flux reads 3/2/1 volume fluxes, density1, energy1 and writes the
+    # mass and energy fluxes; cell_update reads 3/2/1 volume fluxes, the two
+    # fluxes, density1, energy1 and writes density1, energy1.
+    "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accelerate": 9, "k_fast_pdv_correct_flux": 15,
+    "k_fast_cell_flux": 6, "k_fast_cell_update": 8, "k_fast_mom_node": 6, "k_fast_mom_vel": 9,
+    "k_fast_field_summary": 6,
 }
 
 
@@ -52,7 +62,8 @@ def kernel_domain_cells(name: str, n: int) -> int:
     if name in ("k_cell_vols", "k_mom_vols"):
         return (n + 4) ** 3
     if name in ("k_accelerate", "k_reset_field", "k_flux_calc", "k_mom_flux", "k_mom_update", "k_node_flux",
-                "k_node_mass"):
+                "k_node_mass", "k_fast_accelerate", "k_fast_pdv_correct_flux", "k_fast_mom_node",
+                "k_fast_mom_vel"):
         return (n + 1) ** 3
     return n ** 3
 

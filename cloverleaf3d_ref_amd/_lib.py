@@ -75,6 +75,7 @@ def lib() -> ctypes.CDLL:
     L.clover_chunk_device_bytes.restype = ctypes.c_longlong
     L.clover_chunk_device_bytes.argtypes = [vp]
     L.clover_chunk_create.argtypes = [ctypes.POINTER(CloverDeck), ctypes.c_int, ctypes.POINTER(vp)]
+    L.clover_chunk_create_ex.argtypes = [ctypes.POINTER(CloverDeck), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     L.clover_chunk_destroy.argtypes = [vp]
     L.clover_chunk_step.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.clover_chunk_field_summary.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]

@@ -51,17 +51,24 @@ class SmallTimestep(RuntimeError):
     """dt fell below dtmin (timestep.f90:111, 122-124)."""
 
 
+PER_CALL = 1  # CLOVER_CHUNK_PER_CALL
+
+
 class Chunk:
     """One deck resident in HBM.  Construction runs the reference's start
-    sequence (mesh, states, pressure, halo prime)."""
+    sequence (mesh, states, pressure, halo prime).
 
-    def __init__(self, deck: Deck, device: int = 0):
+    per_call=False (default) runs the fused kernels; per_call=True runs one
+    kernel per reference loop nest.  Both produce bit-identical fields."""
+
+    def __init__(self, deck: Deck, device: int = 0, per_call: bool = False):
         self.deck = deck
+        self.per_call = per_call
         self._L = _lib.lib()
         self._cdeck = to_c_deck(deck)
         h = ctypes.c_void_p()
-        _lib.check(self._L.clover_chunk_create(ctypes.byref(self._cdeck), device, ctypes.byref(h)),
-                   "clover_chunk_create")
+        _lib.check(self._L.clover_chunk_create_ex(ctypes.byref(self._cdeck), device, PER_CALL if per_call else 0,
+                                                  ctypes.byref(h)), "clover_chunk_create_ex")
         self._h = h
 
     def close(self) -> None:

@@ -11,11 +11,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/clover_hip.h"
+#include "fast.h"
 #include "kernels.h"
 #include "runtime.h"
 
@@ -143,6 +145,8 @@ struct clover_chunk {
     double prof_ms[P_COUNT];
     int prof_calls[P_COUNT];
     EventTimer *ktimer;  // non-null while per-kernel timing is on
+    bool fast;           // fused kernels (fast.h) instead of the per-call ones
+    Geo geo;             // the uniform mesh, for the fused kernels
 };
 
 // installs the chunk's kernel timer (if any) for the duration of an API call
@@ -394,11 +398,28 @@ static int timestep(clover_chunk *c)
 {
     int fields[NUM_FIELDS];
     const clover_deck &d = c->deck;
+    const int T = (int)c->tiles.size();
     c->dt = c->g_big;
     ideal_gas(c, false);
     set_fields(fields, {F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0});
     update_halo(c, fields, 1);
-    {
+    int nparts[64] = {0};
+    if (c->fast) {
+        // viscosity and calc_dt in one pass: calc_dt reads viscosity only at
+        // the cell itself, which the viscosity halo exchange does not change
+        Phase ph(c, P_VISCOSITY);
+        for (int ti = 0; ti < T; ti++) {
+            Tile &t = c->tiles[ti];
+            hip_check(launch_fast_visc_dt(t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
+                                          c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
+                                          t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
+                                          c->partial + (size_t)ti * kMaxPartials * 5, &nparts[ti % 64], c->st),
+                      "visc_dt");
+            hip_check(launch_min_final(c->partial + (size_t)ti * kMaxPartials * 5, nparts[ti % 64], c->g_big,
+                                       c->scalar + 8 * ti, c->st),
+                      "calc_dt min");
+        }
+    } else {
         Phase ph(c, P_VISCOSITY);
         for (auto &t : c->tiles)
             hip_check(launch_viscosity(t.g, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.celldx, t.celldy, t.celldz,
@@ -410,22 +431,24 @@ static int timestep(clover_chunk *c)
     update_halo(c, fields, 1);
     {
         Phase ph(c, P_TIMESTEP);
-        const int T = (int)c->tiles.size();
-        for (int ti = 0; ti < T; ti++) {
-            Tile &t = c->tiles[ti];
-            int np = 0;
-            double *part = c->partial + (size_t)ti * kMaxPartials * 5;
-            hip_check(launch_calc_dt_cells(t.g, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
-                                           c->g_small, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.celldx, t.celldy,
-                                           t.celldz, t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_VISCOSITY],
-                                           t.f[F_SOUNDSPEED], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], nullptr, part,
-                                           &np, c->st),
-                      "calc_dt");
-            hip_check(launch_min_final(part, np, c->g_big, c->scalar + 8 * ti, c->st), "calc_dt min");
+        if (!c->fast) {
+            for (int ti = 0; ti < T; ti++) {
+                Tile &t = c->tiles[ti];
+                int np = 0;
+                double *part = c->partial + (size_t)ti * kMaxPartials * 5;
+                hip_check(launch_calc_dt_cells(t.g, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
+                                               c->g_small, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.celldx,
+                                               t.celldy, t.celldz, t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_VISCOSITY],
+                                               t.f[F_SOUNDSPEED], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], nullptr,
+                                               part, &np, c->st),
+                          "calc_dt");
+                hip_check(launch_min_final(part, np, c->g_big, c->scalar + 8 * ti, c->st), "calc_dt min");
+            }
         }
         hip_check(hipMemcpyAsync(c->h_scalar, c->scalar, sizeof(double) * 8 * T, hipMemcpyDeviceToHost, c->st),
                   "dt D2H");
         hip_check(hipStreamSynchronize(c->st), "sync");
+        // timestep.f90:93-100: the minimum over tiles, ties to the last tile
         for (int ti = 0; ti < T; ti++) {
             const double dtlp = c->h_scalar[8 * ti];
             if (dtlp <= c->dt) c->dt = dtlp;
@@ -442,6 +465,34 @@ static int timestep(clover_chunk *c)
 
 static void pdv(clover_chunk *c, bool predict)
 {
+    if (c->fast && predict) {
+        // PdV predictor + ideal_gas on the predicted state; revert is a no-op
+        // here because the predicted density/energy are never stored
+        {
+            Phase ph(c, P_PDV);
+            for (auto &t : c->tiles)
+                hip_check(launch_fast_pdv_predict(t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_ENERGY0],
+                                                  t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
+                                                  t.f[F_ZVEL0], c->st),
+                          "PdV predict");
+        }
+        int fields[NUM_FIELDS];
+        set_fields(fields, {F_PRESSURE});
+        update_halo(c, fields, 1);
+        return;
+    }
+    if (c->fast) {
+        // PdV corrector fused with flux_calc, which follows it (hydro.f90:58-60)
+        Phase ph(c, P_PDV);
+        for (auto &t : c->tiles)
+            hip_check(launch_fast_pdv_correct_flux(t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
+                                                   t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
+                                                   t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1],
+                                                   t.f[F_ZVEL0], t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y],
+                                                   t.f[F_VOL_FLUX_Z], c->st),
+                      "PdV correct + flux_calc");
+        return;
+    }
     {
         Phase ph(c, P_PDV);
         for (auto &t : c->tiles)
@@ -466,19 +517,44 @@ static void pdv(clover_chunk *c, bool predict)
 static void advec_cell(clover_chunk *c, int sweep, int dir)
 {
     Phase ph(c, P_CELL_ADVEC);
-    for (auto &t : c->tiles)
-        hip_check(launch_advec_cell(t.g, c->advect_x, dir, sweep, t.vertexdx, t.vertexdy, t.vertexdz, t.f[X_VOLUME],
-                                    t.f[F_DENSITY1], t.f[F_ENERGY1], t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X],
-                                    t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z],
-                                    t.f[X_W1], t.f[X_W2], t.f[X_W3], t.f[X_W4], t.f[X_W5], t.f[X_W6], t.f[X_W7],
-                                    c->st),
-                  "advec_cell");
+    for (auto &t : c->tiles) {
+        if (c->fast)
+            hip_check(launch_fast_advec_cell(t.g, c->geo, c->advect_x, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
+                                             t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y],
+                                             t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W4],
+                                             c->st),
+                      "advec_cell");
+        else
+            hip_check(launch_advec_cell(t.g, c->advect_x, dir, sweep, t.vertexdx, t.vertexdy, t.vertexdz,
+                                        t.f[X_VOLUME], t.f[F_DENSITY1], t.f[F_ENERGY1], t.f[F_MASS_FLUX_X],
+                                        t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z],
+                                        t.f[F_VOL_FLUX_Z], t.f[X_W1], t.f[X_W2], t.f[X_W3], t.f[X_W4], t.f[X_W5],
+                                        t.f[X_W6], t.f[X_W7], c->st),
+                      "advec_cell");
+    }
 }
 
 static void advec_mom(clover_chunk *c, int sweep, int dir)
 {
     Phase ph(c, P_MOM_ADVEC);
     for (auto &t : c->tiles) {
+        if (c->fast) {
+            // the three velocity calls of advec_mom_driver.f90 in one pass; the
+            // new velocities land in the spare arrays, which then take the
+            // velocities' place (only nodes min..max+1 are new; the halos are
+            // refreshed by the next update_halo before anything reads them)
+            const double *vin[3] = {t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1]};
+            double *vout[3] = {t.f[X_W5], t.f[X_W6], t.f[X_W7]};
+            hip_check(launch_fast_advec_mom(t.g, c->geo, c->advect_x, sweep, dir, t.f[F_MASS_FLUX_X],
+                                            t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y],
+                                            t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[F_DENSITY1], t.f[X_W1],
+                                            t.f[X_W2], t.f[X_W3], vin, vout, c->st),
+                      "advec_mom");
+            std::swap(t.f[F_XVEL1], t.f[X_W5]);
+            std::swap(t.f[F_YVEL1], t.f[X_W6]);
+            std::swap(t.f[F_ZVEL1], t.f[X_W7]);
+            continue;
+        }
         double *vel[3] = {t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1]};
         for (int which = 1; which <= 3; which++)
             hip_check(launch_advec_mom(t.g, vel[which - 1], t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X],
@@ -520,9 +596,15 @@ static void field_summary(clover_chunk *c, double *out)
         Tile &t = c->tiles[ti];
         int np = 0;
         double *part = c->partial + (size_t)ti * kMaxPartials * 5;
-        hip_check(launch_field_summary_partial(t.g, t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE],
-                                               t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], part, &np, c->st),
-                  "field_summary");
+        if (c->fast)
+            hip_check(launch_fast_field_summary(t.g, c->geo, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE],
+                                                t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], part, &np, c->st),
+                      "field_summary");
+        else
+            hip_check(launch_field_summary_partial(t.g, t.f[X_VOLUME], t.f[F_DENSITY0], t.f[F_ENERGY0],
+                                                   t.f[F_PRESSURE], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], part,
+                                                   &np, c->st),
+                      "field_summary");
         hip_check(launch_sum5_final(part, np, c->scalar + 8 * ti, c->st), "field_summary sum");
     }
     hip_check(hipMemcpyAsync(c->h_scalar, c->scalar, sizeof(double) * 8 * T, hipMemcpyDeviceToHost, c->st), "D2H");
@@ -538,14 +620,22 @@ static void do_step(clover_chunk *c)
     pdv(c, true);
     {
         Phase ph(c, P_ACCEL);
-        for (auto &t : c->tiles)
-            hip_check(launch_accelerate(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[X_VOLUME],
-                                        t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0],
-                                        t.f[F_YVEL0], t.f[F_ZVEL0], t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->st),
-                      "accelerate");
+        for (auto &t : c->tiles) {
+            if (c->fast)
+                hip_check(launch_fast_accelerate(t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_PRESSURE],
+                                                 t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
+                                                 t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->st),
+                          "accelerate");
+            else
+                hip_check(launch_accelerate(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[X_VOLUME],
+                                            t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0],
+                                            t.f[F_YVEL0], t.f[F_ZVEL0], t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1],
+                                            c->st),
+                          "accelerate");
+        }
     }
     pdv(c, false);
-    {
+    if (!c->fast) {
         Phase ph(c, P_FLUX);
         for (auto &t : c->tiles)
             hip_check(launch_flux_calc(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[F_XVEL0],
@@ -556,11 +646,23 @@ static void do_step(clover_chunk *c)
     advection(c);
     {
         Phase ph(c, P_RESET);
-        for (auto &t : c->tiles)
+        for (auto &t : c->tiles) {
+            if (c->fast) {
+                // reset_field: level 1 becomes level 0 by exchanging the
+                // arrays' roles.  Interiors equal the reference's copy; halo
+                // cells differ but are refreshed before any kernel reads them
+                std::swap(t.f[F_DENSITY0], t.f[F_DENSITY1]);
+                std::swap(t.f[F_ENERGY0], t.f[F_ENERGY1]);
+                std::swap(t.f[F_XVEL0], t.f[F_XVEL1]);
+                std::swap(t.f[F_YVEL0], t.f[F_YVEL1]);
+                std::swap(t.f[F_ZVEL0], t.f[F_ZVEL1]);
+                continue;
+            }
             hip_check(launch_reset_field(t.g, t.f[F_DENSITY0], t.f[F_DENSITY1], t.f[F_ENERGY0], t.f[F_ENERGY1],
                                          t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1], t.f[F_ZVEL0],
                                          t.f[F_ZVEL1], c->st),
                       "reset_field");
+        }
     }
     c->advect_x = !c->advect_x;
     c->time = c->time + c->dt;
@@ -593,6 +695,11 @@ extern "C" {
 
 int clover_chunk_create(const clover_deck *deck, int device, clover_chunk **out)
 {
+    return clover_chunk_create_ex(deck, device, 0, out);
+}
+
+int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clover_chunk **out)
+{
     clover_chunk *c = nullptr;
     try {
         if (!deck || !out) throw Error(E_ARG, "null argument");
@@ -600,9 +707,20 @@ int clover_chunk_create(const clover_deck *deck, int device, clover_chunk **out)
         if (d.x_cells < 1 || d.y_cells < 1 || d.z_cells < 1) throw Error(E_ARG, "cells must be positive");
         if (d.number_of_states < 1 || d.number_of_states > CLOVER_MAX_STATES) throw Error(E_ARG, "bad state count");
         if (d.tiles_per_chunk < 1) throw Error(E_ARG, "tiles_per_chunk must be positive");
+        if (flags & ~CLOVER_CHUNK_PER_CALL) throw Error(E_ARG, "unknown flags");
         c = new clover_chunk();  // value-initialised: every scalar member starts at zero
         c->deck = d;
         c->device = device;
+        c->fast = !(flags & CLOVER_CHUNK_PER_CALL);
+        // the uniform mesh as initialise_chunk.f90:37-39 and
+        // initialise_chunk_kernel.f90:151-178 compute it
+        c->geo.dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
+        c->geo.dy = (d.ymax - d.ymin) / (double)(float)d.y_cells;
+        c->geo.dz = (d.zmax - d.zmin) / (double)(float)d.z_cells;
+        c->geo.vol = c->geo.dx * c->geo.dy * c->geo.dz;
+        c->geo.xa = c->geo.dy * c->geo.dz;
+        c->geo.ya = c->geo.dx * c->geo.dz;
+        c->geo.za = c->geo.dx * c->geo.dy;
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&c->ev0), "event");

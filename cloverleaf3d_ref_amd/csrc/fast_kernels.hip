@@ -1,0 +1,638 @@
+// Fused FP64 kernels of the resident driver's fast path (see fast.h).
+//
+// Each kernel evaluates exactly the expressions of the reference loop nests
+// it replaces, in the same order and without FMA contraction, so its outputs
+// are bit-identical to the per-call kernels'; the savings are in HBM bytes:
+// scalar geometry for the uniform mesh, fused neighbours, dead stores
+// dropped, intermediates recomputed in registers.
+#include <hip/hip_runtime.h>
+
+#include "fast.h"
+#include "launch.h"
+
+namespace clover {
+
+// ---------------------------------------------------------------------------
+// viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
+// viscosity only at the cell itself, so it runs on the value just computed
+// (the halo exchange of viscosity between the two in timestep.f90:80-84
+// does not touch interior cells).  Blocks walk planes l with a stride of
+// gridDim.z so the dt minimum leaves one partial per block.
+__global__ void __launch_bounds__(BX *BY)
+k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
+               double g_small, const double *__restrict__ density0, const double *__restrict__ pressure,
+               const double *__restrict__ soundspeed, double *__restrict__ viscosity,
+               const double *__restrict__ xvel0, const double *__restrict__ yvel0, const double *__restrict__ zvel0,
+               double *__restrict__ partial)
+{
+    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    const bool in = j <= g.x_max && k <= g.y_max;
+    const double mm = dmin(dmin(m.dx, m.dy), m.dz);
+    const double ds = 1.0 / (mm * mm);
+    double dtmin_thread = 1.0e300;
+    for (int l = g.z_min + (int)blockIdx.z; in && l <= g.z_max; l += (int)gridDim.z) {
+        const int i = idx(g, j, k, l);
+#define U(a, b, c) AT(xvel0, a, b, c)
+#define V(a, b, c) AT(yvel0, a, b, c)
+#define W(a, b, c) AT(zvel0, a, b, c)
+        const double ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+        const double ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+        const double ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
+        const double ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
+        const double ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
+        const double ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
+        const double vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
+        const double vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
+        const double vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+        const double vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+        const double vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
+        const double vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
+        const double wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
+        const double wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
+        const double wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
+        const double wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
+        const double wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+        const double wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+        // calc_dt_kernel.f90:124-125 sums the z-face nodes in its own order
+        const double dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
+        const double dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
+#undef U
+#undef V
+#undef W
+        const double div = (m.xa * (ugradx2 - ugradx1) + m.ya * (vgrady2 - vgrady1)) + m.za * (wgradz2 - wgradz1);
+        const double xx = 0.25 * (ugradx2 - ugradx1) / (m.dx);
+        const double yy = 0.25 * (vgrady2 - vgrady1) / (m.dy);
+        const double zz = 0.25 * (wgradz2 - wgradz1) / (m.dz);
+        const double xy = 0.25 * (ugrady2 - ugrady1) / (m.dy) + 0.25 * (vgradx2 - vgradx1) / (m.dx);
+        const double xz = 0.25 * (ugradz2 - ugradz1) / (m.dz) + 0.25 * (wgradx2 - wgradx1) / (m.dx);
+        const double yz = 0.25 * (vgradz2 - vgradz1) / (m.dz) + 0.25 * (wgrady2 - wgrady1) / (m.dy);
+        double pgradx = (AT(pressure, 1, 0, 0) - AT(pressure, -1, 0, 0)) / (m.dx + m.dx);
+        double pgrady = (AT(pressure, 0, 1, 0) - AT(pressure, 0, -1, 0)) / (m.dy + m.dy);
+        double pgradz = (AT(pressure, 0, 0, 1) - AT(pressure, 0, 0, -1)) / (m.dz + m.dz);
+        const double pgradx2 = pgradx * pgradx;
+        const double pgrady2 = pgrady * pgrady;
+        const double pgradz2 = pgradz * pgradz;
+        const double limiter = (xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
+                                xz * pgradx * pgradz + yz * pgrady * pgradz) /
+                               dmax(pgradx2 + pgrady2 + pgradz2, 1.0e-16);
+        const double rho = density0[i];
+        double q;
+        if ((limiter > 0.0) || (div >= 0.0)) {
+            q = 0.0;
+        } else {
+            pgradx = copysign(dmax(1.0e-16, fabs(pgradx)), pgradx);
+            pgrady = copysign(dmax(1.0e-16, fabs(pgrady)), pgrady);
+            pgradz = copysign(dmax(1.0e-16, fabs(pgradz)), pgradz);
+            const double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
+            const double xgrad = fabs(m.dx * pgrad / pgradx);
+            const double ygrad = fabs(m.dy * pgrad / pgrady);
+            const double zgrad = fabs(m.dz * pgrad / pgradz);
+            const double grad = dmin(dmin(xgrad, ygrad), zgrad);
+            const double grad2 = grad * grad;
+            q = 2.0 * rho * grad2 * limiter * limiter;
+        }
+        viscosity[i] = q;
+
+        // calc_dt
+        const double c = soundspeed[i];
+        double cc = c * c;
+        cc = cc + 2.0 * q / rho;
+        double dtct = ds * cc;
+        dtct = dtc_safe * 1.0 / dmax(sqrt(dtct), g_small);
+        const double vol = m.vol;
+        const double du1 = ugradx1 * m.xa, du2 = ugradx2 * m.xa;
+        const double dtut = dtu_safe * 4.0 * vol / dmax(dmax(fabs(du1), fabs(du2)), 1.0e-5 * vol);
+        const double dv1 = vgrady1 * m.ya, dv2 = vgrady2 * m.ya;
+        const double dtvt = dtv_safe * 4.0 * vol / dmax(dmax(fabs(dv1), fabs(dv2)), 1.0e-5 * vol);
+        const double dw1 = dw1s * m.za, dw2 = dw2s * m.za;
+        const double dtwt = dtw_safe * 4.0 * vol / dmax(dmax(fabs(dw1), fabs(dw2)), 1.0e-5 * vol);
+        const double dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
+        const double dtdivt = dtdiv_safe * 4.0 * (vol) / dmax(vol * 1.0e-05, fabs(dv));
+        dtmin_thread = dmin(dtmin_thread, dmin(dmin(dmin(dmin(dtct, dtut), dtvt), dtwt), dtdivt));
+    }
+    __shared__ double red[BX * BY / 64];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const double w = wave_min(dtmin_thread);
+    if ((tid & 63) == 0) red[tid >> 6] = w;
+    __syncthreads();
+    if (tid == 0) {
+        double b = red[0];
+        for (int q = 1; q < BX * BY / 64; q++) b = dmin(b, red[q]);
+        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = b;
+    }
+}
+
+hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, double dtu_safe, double dtv_safe,
+                               double dtw_safe, double dtdiv_safe, double g_small, const double *density0,
+                               const double *pressure, const double *soundspeed, double *viscosity,
+                               const double *xvel0, const double *yvel0, const double *zvel0, double *partial,
+                               int *nparts, hipStream_t st)
+{
+    const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
+    const int bx = (nx + BX - 1) / BX, by = (ny + BY - 1) / BY;
+    int gz = kMaxPartials / (bx * by);
+    if (gz < 1) gz = 1;
+    if (gz > nz) gz = nz;
+    if (bx * by * gz > kMaxPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
+    *nparts = bx * by * gz;
+    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, g, m, dtc_safe,
+                 dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,
+                 yvel0, zvel0, partial);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// PdV predictor (PdV_kernel.f90:78-116) then ideal_gas on the predicted
+// state (ideal_gas_kernel.f90:50-51): only the new pressure is stored.  The
+// predicted density/energy are reverted before anything reads them
+// (PdV.f90:103-107) and the predicted soundspeed is recomputed by the next
+// ideal_gas before calc_dt reads it.  Fortran sums a+b+c+d+a+b+c+d as a left
+// fold, so the 8-term face sums continue from the 4-term partial.
+__global__ void k_fast_pdv_predict(Lay g, Range r, Geo m, double dt, const double *__restrict__ density0,
+                                   const double *__restrict__ energy0, double *__restrict__ pressure,
+                                   const double *__restrict__ viscosity, const double *__restrict__ xvel0,
+                                   const double *__restrict__ yvel0, const double *__restrict__ zvel0)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+#define U(a, b, c) AT(xvel0, a, b, c)
+#define V(a, b, c) AT(yvel0, a, b, c)
+#define W(a, b, c) AT(zvel0, a, b, c)
+    const double sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+    const double sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+    const double sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+    const double stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+    const double sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+    const double sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+#undef U
+#undef V
+#undef W
+    const double left_flux = (m.xa * sl) * 0.125 * dt * 0.5;
+    const double right_flux = (m.xa * sr) * 0.125 * dt * 0.5;
+    const double bottom_flux = (m.ya * sb) * 0.125 * dt * 0.5;
+    const double top_flux = (m.ya * stp) * 0.125 * dt * 0.5;
+    const double back_flux = (m.za * sk) * 0.125 * dt * 0.5;
+    const double front_flux = (m.za * sf) * 0.125 * dt * 0.5;
+    const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+    const double vol = m.vol;
+    const double vol_change_s = vol / (vol + total_flux);
+    const double recip_volume = 1.0 / vol;
+    const double d0 = density0[i];
+    const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
+    const double e1 = energy0[i] - energy_change;
+    const double d1 = d0 * vol_change_s;
+    pressure[i] = (1.4 - 1.0) * d1 * e1;
+}
+
+hipError_t launch_fast_pdv_predict(const Lay &g, const Geo &m, double dt, const double *density0,
+                                   const double *energy0, double *pressure, const double *viscosity,
+                                   const double *xvel0, const double *yvel0, const double *zvel0, hipStream_t st)
+{
+    Range r{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
+    LAUNCH(k_fast_pdv_predict, r, st, g, r, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// accelerate_kernel.f90:61-109 with scalar areas and volume
+__global__ void k_fast_accelerate(Lay g, Range r, Geo m, double dt, const double *__restrict__ density0,
+                                  const double *__restrict__ pressure, const double *__restrict__ viscosity,
+                                  const double *__restrict__ xvel0, const double *__restrict__ yvel0,
+                                  const double *__restrict__ zvel0, double *__restrict__ xvel1,
+                                  double *__restrict__ yvel1, double *__restrict__ zvel1)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const double vol = m.vol;
+#define RV(a, b, c) (AT(density0, a, b, c) * vol)
+    const double nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
+                               RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
+                              0.125;
+#undef RV
+    const double stepbymass_s = 0.25 * dt / nodal_mass;
+    const double xa = m.xa, ya = m.ya, za = m.za;
+#define P(a, b, c) AT(pressure, a, b, c)
+#define Q(a, b, c) AT(viscosity, a, b, c)
+    const double xvel1_s = AT(xvel0, 0, 0, 0) - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
+                                                                xa * (P(0, -1, 0) - P(-1, -1, 0)) +
+                                                                xa * (P(0, 0, -1) - P(-1, 0, -1)) +
+                                                                xa * (P(0, -1, -1) - P(-1, -1, -1)));
+    const double yvel1_s = AT(yvel0, 0, 0, 0) - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
+                                                                ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
+                                                                ya * (P(0, 0, -1) - P(0, -1, -1)) +
+                                                                ya * (P(-1, 0, -1) - P(-1, -1, -1)));
+    const double zvel1_s = AT(zvel0, 0, 0, 0) - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
+                                                                za * (P(0, -1, 0) - P(0, -1, -1)) +
+                                                                za * (P(-1, 0, 0) - P(-1, 0, -1)) +
+                                                                za * (P(-1, -1, 0) - P(-1, -1, -1)));
+    xvel1[i] = xvel1_s - stepbymass_s * (xa * (Q(0, 0, 0) - Q(-1, 0, 0)) + xa * (Q(0, -1, 0) - Q(-1, -1, 0)) +
+                                         xa * (Q(0, 0, -1) - Q(-1, 0, -1)) + xa * (Q(0, -1, -1) - Q(-1, -1, -1)));
+    yvel1[i] = yvel1_s - stepbymass_s * (ya * (Q(0, 0, 0) - Q(0, -1, 0)) + ya * (Q(-1, 0, 0) - Q(-1, -1, 0)) +
+                                         ya * (Q(0, 0, -1) - Q(0, -1, -1)) + ya * (Q(-1, 0, -1) - Q(-1, -1, -1)));
+    zvel1[i] = zvel1_s - stepbymass_s * (za * (Q(0, 0, 0) - Q(0, 0, -1)) + za * (Q(0, -1, 0) - Q(0, -1, -1)) +
+                                         za * (Q(-1, 0, 0) - Q(-1, 0, -1)) + za * (Q(-1, -1, 0) - Q(-1, -1, -1)));
+#undef P
+#undef Q
+}
+
+hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const double *density0,
+                                  const double *pressure, const double *viscosity, const double *xvel0,
+                                  const double *yvel0, const double *zvel0, double *xvel1, double *yvel1,
+                                  double *zvel1, hipStream_t st)
+{
+    Range r{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
+    LAUNCH(k_fast_accelerate, r, st, g, r, m, dt, density0, pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1,
+           zvel1);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// PdV corrector (PdV_kernel.f90:126-164) on interior cells and flux_calc
+// (flux_calc_kernel.f90:56-90) on its face ranges, over the node range that
+// covers both.  The corrector's left/bottom face sums equal flux_calc's x/y
+// sums term for term; the z sums differ (flux_calc_kernel.f90:86 repeats
+// zvel0(j+1,k,l)) and are formed separately.
+__global__ void k_fast_pdv_correct_flux(Lay g, Range r, Geo m, double dt, const double *__restrict__ density0,
+                                        double *__restrict__ density1, const double *__restrict__ energy0,
+                                        double *__restrict__ energy1, const double *__restrict__ pressure,
+                                        const double *__restrict__ viscosity, const double *__restrict__ xvel0,
+                                        const double *__restrict__ xvel1, const double *__restrict__ yvel0,
+                                        const double *__restrict__ yvel1, const double *__restrict__ zvel0,
+                                        const double *__restrict__ zvel1, double *__restrict__ vol_flux_x,
+                                        double *__restrict__ vol_flux_y, double *__restrict__ vol_flux_z)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const bool jin = j <= g.x_max, kin = k <= g.y_max, lin = l <= g.z_max;
+#define U0(a, b, c) AT(xvel0, a, b, c)
+#define U1(a, b, c) AT(xvel1, a, b, c)
+#define V0(a, b, c) AT(yvel0, a, b, c)
+#define V1(a, b, c) AT(yvel1, a, b, c)
+#define W0(a, b, c) AT(zvel0, a, b, c)
+#define W1(a, b, c) AT(zvel1, a, b, c)
+    const double c0 = 0.125 * dt;
+    double sl = 0.0, sb = 0.0;
+    if (kin && lin) {
+        sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
+        vol_flux_x[i] = c0 * m.xa * sl;
+    }
+    if (jin && lin) {
+        sb = V0(0, 0, 0) + V0(1, 0, 0) + V0(0, 0, 1) + V0(1, 0, 1) + V1(0, 0, 0) + V1(1, 0, 0) + V1(0, 0, 1) + V1(1, 0, 1);
+        vol_flux_y[i] = c0 * m.ya * sb;
+    }
+    if (jin && kin)
+        vol_flux_z[i] = c0 * m.za *
+                        (W0(0, 0, 0) + W0(1, 0, 0) + W0(1, 0, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) +
+                         W1(1, 1, 0));
+    if (!(jin && kin && lin)) return;
+    const double sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
+    const double stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
+    const double sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
+    const double sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
+#undef U0
+#undef U1
+#undef V0
+#undef V1
+#undef W0
+#undef W1
+    const double left_flux = (m.xa * sl) * 0.125 * dt;
+    const double right_flux = (m.xa * sr) * 0.125 * dt;
+    const double bottom_flux = (m.ya * sb) * 0.125 * dt;
+    const double top_flux = (m.ya * stp) * 0.125 * dt;
+    const double back_flux = (m.za * sk) * 0.125 * dt;
+    const double front_flux = (m.za * sf) * 0.125 * dt;
+    const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+    const double vol = m.vol;
+    const double vol_change_s = vol / (vol + total_flux);
+    const double recip_volume = 1.0 / vol;
+    const double d0 = density0[i];
+    const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
+    energy1[i] = energy0[i] - energy_change;
+    density1[i] = d0 * vol_change_s;
+}
+
+hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, const double *density0,
+                                        double *density1, const double *energy0, double *energy1,
+                                        const double *pressure, const double *viscosity, const double *xvel0,
+                                        const double *xvel1, const double *yvel0, const double *yvel1,
+                                        const double *zvel0, const double *zvel1, double *vol_flux_x,
+                                        double *vol_flux_y, double *vol_flux_z, hipStream_t st)
+{
+    Range r{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
+    LAUNCH(k_fast_pdv_correct_flux, r, st, g, r, m, dt, density0, density1, energy0, energy1, pressure, viscosity,
+           xvel0, xvel1, yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// advec_cell, one sweep.  pre_vol of any cell is recomputed from the volume
+// fluxes (advec_cell_kernel.f90:97-116, 193-210, 289-306); `mode` as in the
+// per-call kernel: 1 x-sweep 1, 2 x-sweep 3, 3 y advect_x, 4 y not advect_x,
+// 5 z-sweep 1, 6 z-sweep 3.
+__device__ inline double cell_pre_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
+                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
+{
+    switch (mode) {
+    case 1:
+    case 5: return V0 + (vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i]);
+    case 2: return V0 + vfx[i + 1] - vfx[i];
+    case 3: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
+    case 4: return V0 + vfy[i + g.ld] - vfy[i] + vfx[i + 1] - vfx[i];
+    default: return V0 + vfz[i + g.plane] - vfz[i];
+    }
+}
+
+__device__ inline double cell_limiter(double sigma, double sigma3, double sigma4, double diffuw, double diffdw)
+{
+    const double one_by_six = 1.0 / 6.0;
+    if (diffuw * diffdw > 0.0)
+        return (1.0 - sigma) * copysign(1.0, diffdw) *
+               dmin(dmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
+    return 0.0;
+}
+
+__global__ void k_fast_cell_flux(Lay g, Range r, Geo m, int dir, int mode, const double *__restrict__ vfx,
+                                 const double *__restrict__ vfy, const double *__restrict__ vfz,
+                                 const double *__restrict__ density1, const double *__restrict__ energy1,
+                                 double *__restrict__ mass_flux, double *__restrict__ ener_flux)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
+    const int c = dir == 1 ? j : dir == 2 ? k : l;
+    const int cmax = dir == 1 ? g.x_max : dir == 2 ? g.y_max : g.z_max;
+    const double vd = dir == 1 ? m.dx : dir == 2 ? m.dy : m.dz;
+    const double *__restrict__ vol_flux = dir == 1 ? vfx : dir == 2 ? vfy : vfz;
+    const double vf = vol_flux[i];
+    int upwind, donor, downwind;
+    if (vf > 0.0) {
+        upwind = c - 2; donor = c - 1; downwind = c;
+    } else {
+        upwind = c + 1 < cmax + 2 ? c + 1 : cmax + 2; donor = c; downwind = c - 1;
+    }
+    const int iu = i + (upwind - c) * s, id = i + (donor - c) * s, iw = i + (downwind - c) * s;
+    const double pv = cell_pre_vol(mode, g, id, m.vol, vfx, vfy, vfz);
+    const double sigmat = fabs(vf) / pv;
+    const double sigma3 = (1.0 + sigmat) * (vd / vd);  // vertexd*(j)/vertexd*(dif) on the uniform mesh
+    const double sigma4 = 2.0 - sigmat;
+    const double rd = density1[id];
+    double diffuw = rd - density1[iu];
+    double diffdw = density1[iw] - rd;
+    double limiter = cell_limiter(sigmat, sigma3, sigma4, diffuw, diffdw);
+    const double mf = vf * (rd + limiter);
+    mass_flux[i] = mf;
+    const double sigmam = fabs(mf) / (rd * pv);
+    const double ed = energy1[id];
+    diffuw = ed - energy1[iu];
+    diffdw = energy1[iw] - ed;
+    limiter = cell_limiter(sigmam, sigma3, sigma4, diffuw, diffdw);
+    ener_flux[i] = mf * (ed + limiter);
+}
+
+__global__ void k_fast_cell_update(Lay g, Range r, Geo m, int dir, int mode, const double *__restrict__ vfx,
+                                   const double *__restrict__ vfy, const double *__restrict__ vfz,
+                                   const double *__restrict__ mass_flux, const double *__restrict__ ener_flux,
+                                   double *__restrict__ density1, double *__restrict__ energy1)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
+    const double *__restrict__ vol_flux = dir == 1 ? vfx : dir == 2 ? vfy : vfz;
+    const double pv = cell_pre_vol(mode, g, i, m.vol, vfx, vfy, vfz);
+    const double pm = density1[i] * pv;
+    const double pom = pm + mass_flux[i] - mass_flux[i + s];
+    const double pe = (energy1[i] * pm + ener_flux[i] - ener_flux[i + s]) / pom;
+    const double av = pv + vol_flux[i] - vol_flux[i + s];
+    density1[i] = pom / av;
+    energy1[i] = pe;
+}
+
+hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
+                                  double *density1, double *energy1, double *mass_flux_x, const double *vol_flux_x,
+                                  double *mass_flux_y, const double *vol_flux_y, double *mass_flux_z,
+                                  const double *vol_flux_z, double *ener_flux, hipStream_t st)
+{
+    int mode = 0;
+    if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
+    else if (dir == 2) mode = sweep_number == 2 ? (advect_x ? 3 : 4) : 0;
+    else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
+    if (!mode) return hipErrorInvalidValue;  // the resident driver issues only the reference's sweeps
+    Range fr;
+    double *mf;
+    if (dir == 1) {
+        fr = Range{g.x_min, g.x_max + 2, g.y_min, g.y_max, g.z_min, g.z_max};
+        mf = mass_flux_x;
+    } else if (dir == 2) {
+        fr = Range{g.x_min, g.x_max, g.y_min, g.y_max + 2, g.z_min, g.z_max + 2};
+        mf = mass_flux_y;
+    } else {
+        fr = Range{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max + 2};
+        mf = mass_flux_z;
+    }
+    LAUNCH(k_fast_cell_flux, fr, st, g, fr, m, dir, mode, vol_flux_x, vol_flux_y, vol_flux_z, density1, energy1, mf,
+           ener_flux);
+    Range in{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
+    LAUNCH(k_fast_cell_update, in, st, g, in, m, dir, mode, vol_flux_x, vol_flux_y, vol_flux_z, mf, ener_flux,
+           density1, energy1);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// advec_mom, one sweep, three velocities.
+//
+// Node kernel: node_flux at the node and its lower neighbour along the sweep,
+// node_mass_post from the eight surrounding cells with post_vol recomputed
+// from the volume fluxes (advec_mom_kernel.f90:95-163: mode 1 sweep1/x,
+// 2 sweep1/z, 3 sweep2/advect_x, 4 sweep2/not, 5-6 sweep3), node_mass_pre.
+__device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
+                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
+{
+    switch (mode) {
+    case 1: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
+    case 2: return V0 + vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i];
+    case 3: return V0 + vfz[i + g.plane] - vfz[i];
+    case 4: return V0 + vfx[i + 1] - vfx[i];
+    default: return V0;
+    }
+}
+
+__device__ inline double node_flux_at(int dir, const Lay &g, int i, const double *__restrict__ mf)
+{
+    if (dir == 1)
+        return 0.125 * (AT(mf, 0, -1, 0) + AT(mf, 0, 0, 0) + AT(mf, 1, -1, 0) + AT(mf, 1, 0, 0) + AT(mf, 0, -1, -1) +
+                        AT(mf, 0, 0, -1) + AT(mf, 1, -1, -1) + AT(mf, 1, 0, -1));
+    if (dir == 2)
+        return 0.125 * (AT(mf, -1, 0, 0) + AT(mf, 0, 0, 0) + AT(mf, -1, 1, 0) + AT(mf, 0, 1, 0) + AT(mf, -1, 0, -1) +
+                        AT(mf, 0, 0, -1) + AT(mf, -1, 1, -1) + AT(mf, 0, 1, -1));
+    return 0.125 * (AT(mf, -1, 0, 0) + AT(mf, 0, 0, 0) + AT(mf, -1, 0, 1) + AT(mf, 0, 0, 1) + AT(mf, -1, -1, 0) +
+                    AT(mf, 0, -1, 0) + AT(mf, -1, -1, 1) + AT(mf, 0, -1, 1));
+}
+
+__global__ void k_fast_mom_node(Lay g, Range r, Geo m, int dir, int mode, const double *__restrict__ mf,
+                                const double *__restrict__ vfx, const double *__restrict__ vfy,
+                                const double *__restrict__ vfz, const double *__restrict__ density1,
+                                double *__restrict__ node_flux, double *__restrict__ node_mass_post,
+                                double *__restrict__ node_mass_pre)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
+    const double nf = node_flux_at(dir, g, i, mf);
+    const double nf_lo = node_flux_at(dir, g, i - s, mf);
+    const double V0 = m.vol;
+#define NMP(a, b, c)                                                                                        \
+    (AT(density1, a, b, c) *                                                                                \
+     mom_post_vol(mode, g, i + (a) + (b) * g.ld + (c) * g.plane, V0, vfx, vfy, vfz))
+    const double nmp = 0.125 * (NMP(0, -1, 0) + NMP(0, 0, 0) + NMP(-1, -1, 0) + NMP(-1, 0, 0) + NMP(0, -1, -1) +
+                                NMP(0, 0, -1) + NMP(-1, -1, -1) + NMP(-1, 0, -1));
+#undef NMP
+    node_flux[i] = nf;
+    node_mass_post[i] = nmp;
+    node_mass_pre[i] = nmp - nf_lo + nf;
+}
+
+// Velocity kernel: each node forms the momentum fluxes of its two faces along
+// the sweep (advec_mom_kernel.f90:213-249 and twins) for all three velocity
+// components and writes the updated velocities to separate arrays.
+__device__ inline double mom_flux_face(int f, int s, double d, const double *__restrict__ node_flux,
+                                       const double *__restrict__ node_mass_pre, const double *__restrict__ vel)
+{
+    const double nf = node_flux[f];
+    int iu, id, iw;
+    if (nf < 0.0) {
+        iu = f + 2 * s; id = f + s; iw = f;
+    } else {
+        iu = f - s; id = f; iw = f + s;
+    }
+    const double sigma = fabs(nf) / (node_mass_pre[id]);
+    const double width = d;
+    const double vdonor = vel[id];
+    const double vdiffuw = vdonor - vel[iu];
+    const double vdiffdw = vel[iw] - vdonor;
+    double limiter = 0.0;
+    if (vdiffuw * vdiffdw > 0.0) {
+        const double auw = fabs(vdiffuw);
+        const double adw = fabs(vdiffdw);
+        double wind = 1.0;
+        if (vdiffdw <= 0.0) wind = -1.0;
+        limiter = wind * dmin(dmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / d) / 6.0, auw), adw);
+    }
+    const double advec_vel_s = vdonor + (1.0 - sigma) * limiter;
+    return advec_vel_s * nf;
+}
+
+struct Vel3 {
+    const double *in[3];
+    double *out[3];
+};
+
+__global__ void k_fast_mom_vel(Lay g, Range r, Geo m, int dir, const double *__restrict__ node_flux,
+                               const double *__restrict__ node_mass_post, const double *__restrict__ node_mass_pre,
+                               Vel3 v)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
+    const double d = dir == 1 ? m.dx : dir == 2 ? m.dy : m.dz;
+    const double nmpre = node_mass_pre[i], nmpost = node_mass_post[i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double *__restrict__ vel = v.in[c];
+        const double mf_lo = mom_flux_face(i - s, s, d, node_flux, node_mass_pre, vel);
+        const double mf_hi = mom_flux_face(i, s, d, node_flux, node_mass_pre, vel);
+        v.out[c][i] = (vel[i] * nmpre + mf_lo - mf_hi) / nmpost;
+    }
+}
+
+hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int sweep_number, int direction,
+                                 const double *mass_flux_x, const double *vol_flux_x, const double *mass_flux_y,
+                                 const double *vol_flux_y, const double *mass_flux_z, const double *vol_flux_z,
+                                 const double *density1, double *node_flux, double *node_mass_post,
+                                 double *node_mass_pre, const double *const *vel_in, double *const *vel_out,
+                                 hipStream_t st)
+{
+    int mode = 0;
+    if (sweep_number == 1 && direction == 1) mode = 1;
+    else if (sweep_number == 1 && direction == 3) mode = 2;
+    else if (sweep_number == 2 && advect_x) mode = 3;
+    else if (sweep_number == 2 && !advect_x) mode = 4;
+    else if (sweep_number == 3 && direction == 1) mode = 5;
+    else if (sweep_number == 3 && direction == 3) mode = 6;
+    if (!mode || direction < 1 || direction > 3) return hipErrorInvalidValue;
+    const double *mf = direction == 1 ? mass_flux_x : direction == 2 ? mass_flux_y : mass_flux_z;
+    Range rn;
+    if (direction == 1) rn = Range{g.x_min - 1, g.x_max + 2, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
+    else if (direction == 2) rn = Range{g.x_min, g.x_max + 1, g.y_min - 1, g.y_max + 2, g.z_min, g.z_max + 1};
+    else rn = Range{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min - 1, g.z_max + 2};
+    LAUNCH(k_fast_mom_node, rn, st, g, rn, m, direction, mode, mf, vol_flux_x, vol_flux_y, vol_flux_z, density1,
+           node_flux, node_mass_post, node_mass_pre);
+    Range upd{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
+    Vel3 v;
+    for (int c = 0; c < 3; c++) {
+        v.in[c] = vel_in[c];
+        v.out[c] = vel_out[c];
+    }
+    LAUNCH(k_fast_mom_vel, upd, st, g, upd, m, direction, node_flux, node_mass_post, node_mass_pre, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// field_summary_kernel.f90:51-78 with the scalar cell volume; same
+// reduction tree as the per-call kernel.
+__global__ void __launch_bounds__(RED_THREADS)
+k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const double *__restrict__ density0,
+                     const double *__restrict__ energy0, const double *__restrict__ pressure,
+                     const double *__restrict__ xvel0, const double *__restrict__ yvel0,
+                     const double *__restrict__ zvel0, double *__restrict__ partial)
+{
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < ncells; c += gridDim.x * RED_THREADS) {
+        const int j = g.x_min + c % nj;
+        const int k = g.y_min + (c / nj) % nk;
+        const int l = g.z_min + c / (nj * nk);
+        const int i = idx(g, j, k, l);
+        double vsqrd = 0.0;
+        for (int lv = 0; lv <= 1; lv++)
+            for (int kv = 0; kv <= 1; kv++)
+                for (int jv = 0; jv <= 1; jv++) {
+                    const double u = AT(xvel0, jv, kv, lv), v = AT(yvel0, jv, kv, lv), w = AT(zvel0, jv, kv, lv);
+                    vsqrd = vsqrd + 0.125 * (u * u + v * v + w * w);
+                }
+        const double cell_vol = m.vol;
+        const double cell_mass = cell_vol * density0[i];
+        s[0] += cell_vol;
+        s[1] += cell_mass;
+        s[2] += cell_mass * energy0[i];
+        s[3] += cell_mass * 0.5 * vsqrd;
+        s[4] += cell_vol * pressure[i];
+    }
+    __shared__ double red[5][RED_THREADS / 64];
+    for (int q = 0; q < 5; q++) {
+        const double w = wave_sum(s[q]);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double b = 0.0;
+        for (int w = 0; w < RED_THREADS / 64; w++) b += red[threadIdx.x][w];
+        partial[5 * blockIdx.x + threadIdx.x] = b;
+    }
+}
+
+hipError_t launch_fast_field_summary(const Lay &g, const Geo &m, const double *density0, const double *energy0,
+                                     const double *pressure, const double *xvel0, const double *yvel0,
+                                     const double *zvel0, double *partial, int *nparts, hipStream_t st)
+{
+    const int nj = g.x_max - g.x_min + 1, nk = g.y_max - g.y_min + 1, nl = g.z_max - g.z_min + 1;
+    const int ncells = nj * nk * nl;
+    int nb = (ncells + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+    if (nb < 1) nb = 1;
+    if (nb > kMaxPartials) nb = kMaxPartials;
+    *nparts = nb;
+    TIMED_LAUNCH("k_fast_field_summary", st, k_fast_field_summary, dim3(nb), dim3(RED_THREADS), 0, st, g, m, nj, nk,
+                 ncells, density0, energy0, pressure, xvel0, yvel0, zvel0, partial);
+    return hipGetLastError();
+}
+
+}  // namespace clover

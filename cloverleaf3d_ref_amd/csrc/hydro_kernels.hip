@@ -11,47 +11,11 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "launch.h"
 
 namespace clover {
 
-struct Range {
-    int j0, j1, k0, k1, l0, l1;  // inclusive
-};
-
-constexpr int BX = 64, BY = 4;
-
-static inline dim3 grid_of(const Range &r)
-{
-    return dim3((unsigned)((r.j1 - r.j0 + 1 + BX - 1) / BX), (unsigned)((r.k1 - r.k0 + 1 + BY - 1) / BY),
-                (unsigned)(r.l1 - r.l0 + 1));
-}
-
-#define RANGE_THREAD(r)                                      \
-    const int j = (r).j0 + (int)(blockIdx.x * BX + threadIdx.x); \
-    const int k = (r).k0 + (int)(blockIdx.y * BY + threadIdx.y); \
-    const int l = (r).l0 + (int)blockIdx.z;                  \
-    if (j > (r).j1 || k > (r).k1) return;
-
 thread_local KernelTimer *t_kernel_timer = nullptr;
-
-#define TIMED_LAUNCH(name, st, ...)                           \
-    do {                                                      \
-        if (t_kernel_timer) t_kernel_timer->begin(name, st);  \
-        hipLaunchKernelGGL(__VA_ARGS__);                      \
-        if (t_kernel_timer) t_kernel_timer->end(st);          \
-    } while (0)
-
-#define LAUNCH(kern, r, st, ...)                                                                \
-    do {                                                                                        \
-        if ((r).j1 >= (r).j0 && (r).k1 >= (r).k0 && (r).l1 >= (r).l0)                           \
-            TIMED_LAUNCH(#kern, st, kern, grid_of(r), dim3(BX, BY, 1), 0, st, __VA_ARGS__);      \
-    } while (0)
-
-// neighbour access relative to the thread's offset i
-#define AT(p, dj, dk, dl) (p)[i + (dj) + (dk) * g.ld + (dl) * g.plane]
-
-__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
-__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 
 // ---------------------------------------------------------------------------
 // ideal_gas_kernel.f90:47-58
@@ -171,20 +135,6 @@ hipError_t launch_viscosity(const Lay &g, const double *xarea, const double *yar
 // calc_dt_kernel.f90:102-149.  Per-cell dt, then a deterministic two-pass
 // minimum: each block reduces its cells (wavefront shuffle, then LDS) into
 // partial[block]; a one-block kernel reduces the partials in a fixed order.
-constexpr int RED_THREADS = 256;
-
-__device__ inline double wave_min(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ inline double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 __global__ void __launch_bounds__(RED_THREADS)
 k_calc_dt(Lay g, int nj, int nk, int ncells, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe,
           double dtdiv_safe, double g_small, const double *__restrict__ xarea, const double *__restrict__ yarea,

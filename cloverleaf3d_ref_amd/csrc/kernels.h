@@ -94,7 +94,7 @@ hipError_t launch_field_summary_partial(const Lay &g, const double *volume, cons
                                         hipStream_t st);
 hipError_t launch_sum5_final(const double *partial, int n, double *out, hipStream_t st);
 
-constexpr int kMaxPartials = 2048;
+constexpr int kMaxPartials = 16384;
 
 // Optional per-launch timing: while a timer is installed on the calling
 // thread, every launcher brackets each kernel with HIP events on its stream

@@ -1,0 +1,66 @@
+// Launch geometry shared by the stencil kernels.
+//
+// A 64 x 4 workgroup covers 64 consecutive j (one wavefront per row, so every
+// array access of a wave is one coalesced 512-byte segment) by 4 rows k, at
+// one plane l per grid z index.  Ranges are inclusive, in the reference's
+// tile-local indices.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace clover {
+
+struct Range {
+    int j0, j1, k0, k1, l0, l1;  // inclusive
+};
+
+constexpr int BX = 64, BY = 4;
+
+static inline dim3 grid_of(const Range &r)
+{
+    return dim3((unsigned)((r.j1 - r.j0 + 1 + BX - 1) / BX), (unsigned)((r.k1 - r.k0 + 1 + BY - 1) / BY),
+                (unsigned)(r.l1 - r.l0 + 1));
+}
+
+#define RANGE_THREAD(r)                                              \
+    const int j = (r).j0 + (int)(blockIdx.x * BX + threadIdx.x);     \
+    const int k = (r).k0 + (int)(blockIdx.y * BY + threadIdx.y);     \
+    const int l = (r).l0 + (int)blockIdx.z;                          \
+    if (j > (r).j1 || k > (r).k1) return;
+
+#define TIMED_LAUNCH(name, st, ...)                           \
+    do {                                                      \
+        if (t_kernel_timer) t_kernel_timer->begin(name, st);  \
+        hipLaunchKernelGGL(__VA_ARGS__);                      \
+        if (t_kernel_timer) t_kernel_timer->end(st);          \
+    } while (0)
+
+#define LAUNCH(kern, r, st, ...)                                                                \
+    do {                                                                                        \
+        if ((r).j1 >= (r).j0 && (r).k1 >= (r).k0 && (r).l1 >= (r).l0)                           \
+            TIMED_LAUNCH(#kern, st, kern, grid_of(r), dim3(BX, BY, 1), 0, st, __VA_ARGS__);      \
+    } while (0)
+
+// neighbour access relative to the thread's offset i
+#define AT(p, dj, dk, dl) (p)[i + (dj) + (dk) * g.ld + (dl) * g.plane]
+
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+
+constexpr int RED_THREADS = 256;
+
+__device__ inline double wave_min(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace clover

@@ -177,6 +177,15 @@ typedef struct clover_chunk clover_chunk;
 int clover_chunk_create(const clover_deck *deck, int device, clover_chunk **out);
 int clover_chunk_destroy(clover_chunk *c);
 
+/* As clover_chunk_create, with flags.  By default the chunk runs the fused
+ * kernels, which write bit-identical values with fewer HBM passes (uniform
+ * mesh geometry as scalars, neighbouring loop nests fused, values the
+ * reference never reads not stored, reset_field as an exchange of the level
+ * 0/1 arrays); CLOVER_CHUNK_PER_CALL runs one kernel per reference loop nest
+ * instead, every intermediate stored as the reference stores it. */
+#define CLOVER_CHUNK_PER_CALL 1
+int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clover_chunk **out);
+
 /* One step of hydro.f90:48-68: timestep, PdV predictor/corrector,
  * accelerate, flux_calc, advection, reset_field; time += dt.  *dt gets the
  * step's dt; returns 1 (and does nothing else) on a small time step. */

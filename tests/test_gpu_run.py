@@ -25,9 +25,10 @@ pytestmark = pytest.mark.gpu
 GOLDEN = json.loads((pathlib.Path(__file__).resolve().parent / "golden" / "config0_summaries.json").read_text())
 
 
-def _run(deck):
-    print(f"[gpu] chunk {deck.x_cells}x{deck.y_cells}x{deck.z_cells} tiles={deck.tiles_per_chunk}", flush=True)
-    with Chunk(deck) as c:
+def _run(deck, per_call=False):
+    print(f"[gpu] chunk {deck.x_cells}x{deck.y_cells}x{deck.z_cells} tiles={deck.tiles_per_chunk} "
+          f"{'per-call' if per_call else 'fused'}", flush=True)
+    with Chunk(deck, per_call=per_call) as c:
         rows = [c.field_summary()]
         dts = []
         for _ in range(deck.end_step):
@@ -41,9 +42,10 @@ def _run(deck):
     return rows, np.array(dts)
 
 
-def test_config0_against_reference_program():
+@pytest.mark.parametrize("per_call", [False, True])
+def test_config0_against_reference_program(per_call):
     deck = parse_deck(GOLDEN["deck"])
-    rows, dts = _run(deck)
+    rows, dts = _run(deck, per_call)
     ref = GOLDEN["reference_program"]
     # step 100 is both a summary step and the last step: printed twice
     assert [r["step"] for r in rows] == [r[0] for r in ref] + [100]
@@ -66,9 +68,10 @@ def test_config0_run_api_matches_step_api():
             assert a[k] == b[k]
 
 
-def test_config0_tiled_against_reference_program():
+@pytest.mark.parametrize("per_call", [False, True])
+def test_config0_tiled_against_reference_program(per_call):
     deck = parse_deck(cuboid_deck(64, 100, 10, tiles=8))
-    rows, _ = _run(deck)
+    rows, _ = _run(deck, per_call)
     last = rows[-1]
     assert last["step"] == 100
     for a, b in zip([last["mass"], last["ie"], last["ke"]], GOLDEN["reference_program_tiled_step100"]):
@@ -77,8 +80,9 @@ def test_config0_tiled_against_reference_program():
         assert got["ke"] == pytest.approx(want[5], rel=1e-12, abs=1e-300)
 
 
-def test_test_problem_1():
-    rows, dts = _run(parse_deck(DEFAULT_DECK))
+@pytest.mark.parametrize("per_call", [False, True])
+def test_test_problem_1(per_call):
+    rows, dts = _run(parse_deck(DEFAULT_DECK), per_call)
     assert len(dts) == 75 and rows[-1]["step"] == 75
     assert abs(rows[-1]["ke"] / TEST_PROBLEM_1_KE - 1) < 1e-12
 
@@ -95,15 +99,45 @@ ODD_DECK = """*clover
 *endclover"""
 
 
+@pytest.mark.parametrize("per_call", [False, True])
 @pytest.mark.parametrize("nx,ny,nz,tiles,td", [(23, 17, 13, 1, "3d"), (23, 17, 13, 6, "3d"), (19, 9, 8, 3, "1d"),
                                               (19, 9, 8, 4, "2d")])
-def test_odd_decks_against_oracle(nx, ny, nz, tiles, td):
+def test_odd_decks_against_oracle(nx, ny, nz, tiles, td, per_call):
     deck = parse_deck(ODD_DECK.format(nx=nx, ny=ny, nz=nz, steps=30, tiles=tiles, td=td))
     want, want_dt = ol.run(deck)
-    rows, dts = _run(deck)
+    rows, dts = _run(deck, per_call)
     assert np.array_equal(dts, want_dt)
     assert len(rows) == len(want)
     for g, w in zip(rows, want):
         assert g["step"] == w["step"]
         for k in ("vol", "mass", "ie", "ke", "press"):
             assert g[k] == pytest.approx(w[k], rel=1e-12, abs=1e-300), (g["step"], k)
+
+
+# level-1 arrays are not compared: the fused path's reset_field exchanges the
+# level 0/1 arrays instead of copying 1 into 0, so after a step level 1 holds
+# the previous level 0 (never read before being rewritten)
+INTERIOR_FIELDS = ["density0", "energy0", "pressure", "viscosity", "xvel0", "yvel0", "zvel0",
+                   "vol_flux_x", "vol_flux_y", "vol_flux_z", "mass_flux_x", "mass_flux_y", "mass_flux_z"]
+
+
+@pytest.mark.parametrize("nx,ny,nz,tiles", [(29, 21, 17, 1), (29, 21, 17, 4), (3, 5, 4, 1)])
+def test_fused_equals_per_call(nx, ny, nz, tiles):
+    """The fused kernels write exactly the per-call kernels' values: equal
+    totals (same reduction tree) and bit-identical interiors after 12 steps."""
+    deck = parse_deck(ODD_DECK.format(nx=nx, ny=ny, nz=nz, steps=12, tiles=tiles, td="3d"))
+    out = {}
+    for per_call in (False, True):
+        rows, dts = _run(deck, per_call)
+        with Chunk(deck, per_call=per_call) as c:
+            for _ in range(12):
+                c.step()
+            fields = {(t, n): c.fetch(n, t) for t in range(c.ntiles) for n in INTERIOR_FIELDS}
+        out[per_call] = (rows, dts, fields)
+    (r0, d0, f0), (r1, d1, f1) = out[False], out[True]
+    assert np.array_equal(d0, d1)
+    assert r0 == r1
+    for key in f0:
+        a, b = f0[key], f1[key]
+        inner = tuple(slice(2, s - 2) for s in a.shape)  # interior and the +1 face/node planes
+        assert np.array_equal(a[inner].view(np.uint64), b[inner].view(np.uint64)), key
