@@ -96,6 +96,21 @@ def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25):
     return wall, ktimes, info
 
 
+def calibration_copy(mib: int) -> None:
+    """One plain device-to-device copy of `mib` MiB: it reads and writes
+    exactly that many bytes, so its counter readings calibrate the HBM
+    traffic counters for the hydro kernels profiled in the same run."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def cpu_baseline(cells: int, steps: int, threads: int):
     """The reference's compiled kernels through the hydro loop on the host
     (run in a child process so its OpenMP runtime picks up `threads`)."""
@@ -132,12 +147,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the hydro path here is single-GPU by design "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_mib:
+        calibration_copy(args.calib_copy_mib)
     wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup)
     n = args.cells
     value = n ** 3 * args.steps / wall

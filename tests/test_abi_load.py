@@ -3,6 +3,7 @@ declares (no compute calls: this runs without a GPU)."""
 from __future__ import annotations
 
 import ctypes
+import pathlib
 import subprocess
 
 import pytest
@@ -56,3 +57,52 @@ def test_deck_struct_matches_header():
             members.append(m.strip().split("[")[0])
     assert members == [f[0] for f in _lib.CloverDeck._fields_]
     assert ctypes.sizeof(_lib.CloverDeck) > 0
+
+
+# The reference's kernel modules and the subroutines the drivers call
+# (ideal_gas_kernel.f90:23, viscosity_kernel.f90:28, calc_dt_kernel.f90:24,
+# PdV_kernel.f90:28, revert_kernel.f90:25, accelerate_kernel.f90:23,
+# flux_calc_kernel.f90:22, advec_cell_kernel.f90:26, advec_mom_kernel.f90:25,
+# reset_field_kernel.f90:23, update_halo_kernel.f90:28,
+# update_tile_halo_kernel.f90:25, field_summary_kernel.f90:23).
+REFERENCE_MODULES = {
+    "ideal_gas_kernel_module": ["ideal_gas_kernel"],
+    "viscosity_kernel_module": ["viscosity_kernel"],
+    "calc_dt_kernel_module": ["calc_dt_kernel"],
+    "pdv_kernel_module": ["pdv_kernel"],
+    "revert_kernel_module": ["revert_kernel"],
+    "accelerate_kernel_module": ["accelerate_kernel"],
+    "flux_calc_kernel_module": ["flux_calc_kernel"],
+    "advec_cell_kernel_module": ["advec_cell_kernel"],
+    "advec_mom_kernel_mod": ["advec_mom_kernel"],
+    "reset_field_kernel_module": ["reset_field_kernel"],
+    "update_halo_kernel_module": ["update_halo_kernel"],
+    "update_tile_halo_kernel_module": [f"update_tile_halo_{f}_kernel" for f in
+                                       ("left", "right", "top", "bottom", "front", "back")],
+    "field_summary_kernel_module": ["field_summary_kernel"],
+}
+
+
+def test_fortran_shims_cover_every_reference_module():
+    import re
+    src = (_lib.PKG / "fortran" / "clover_hip_shims.f90").read_text().lower()
+    modules = {}
+    for m in re.finditer(r"^module (\w+)\s*$(.*?)^end module \1", src, flags=re.M | re.S):
+        modules[m.group(1)] = re.findall(r"^\s*subroutine (\w+)\(", m.group(2), flags=re.M)
+    for mod, subs in REFERENCE_MODULES.items():
+        assert mod in modules, mod
+        assert sorted(subs) == sorted(modules[mod]), mod
+    # every C name the shims bind is declared in the header and exported
+    bound = set(re.findall(r"name='(clover_hip_\w+)'", src))
+    assert {f"clover_hip_{k.lower()}" for k in ks.SPECS} <= bound
+    declared = {n.lower() for n in _lib.header_functions()}
+    assert bound <= declared
+
+
+def test_fortran_shims_compile():
+    import shutil
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    if not pathlib.Path(flang).exists():
+        pytest.skip("no flang in this image")
+    subprocess.run(["make", "-C", str(_lib.PKG / "fortran")], check=True, capture_output=True)
+    assert (_lib.PKG / "_build" / "fortran" / "clover_hip_shims.o").exists()
