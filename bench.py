@@ -32,6 +32,10 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of each kernel at the bench workload, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a plain copy of
+# known size (python -m cloverleaf3d_ref_amd.profiling traffic ...)
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_cuboid512_fp64.json"
 
 # Algorithmic (kernel-boundary compulsory) traffic of each kernel per launch,
 # in FP64 arrays per cell of its domain: every distinct array it reads or
@@ -46,13 +50,14 @@ ARRAYS_PER_CELL = {
     "k_node_flux": 2, "k_node_mass": 5, "k_mom_flux": 4, "k_mom_update": 5, "k_reset_field": 10,
     "k_field_summary": 7, "k_pdv<1>": 13, "k_pdv<0>": 16,
     # fused kernels (geometry as scalars).  Sweep-dependent ones averaged over
-    # the three sweeps of a step: mom_node reads mass_flux, density1 and
-    # 2/1/0 volume fluxes and writes node_flux, node masses post and pre;
-    # cell_flux reads 3/2/1 volume fluxes, density1, energy1 and writes the
-    # mass and energy fluxes; cell_update reads 3/2/1 volume fluxes, the two
-    # fluxes, density1, energy1 and writes density1, energy1.
+    # the sweeps they run in: cell_flux reads 3/2/1 volume fluxes, density1,
+    # energy1 and writes the mass and energy fluxes; cell_update reads 3/2/1
+    # volume fluxes, the two fluxes, density1, energy1 and writes density1,
+    # energy1; mom_x/y/z read the sweep's mass flux, density1, 2/1/0 volume
+    # fluxes and three velocities and write three velocities (x and z run
+    # in sweeps 1 and 3, y always in sweep 2 with one volume flux).
     "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accelerate": 9, "k_fast_pdv_correct_flux": 15,
-    "k_fast_cell_flux": 6, "k_fast_cell_update": 8, "k_fast_mom_node": 6, "k_fast_mom_vel": 9,
+    "k_fast_cell_flux": 6, "k_fast_cell_update": 8, "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
     "k_fast_field_summary": 6,
 }
 
@@ -62,8 +67,8 @@ def kernel_domain_cells(name: str, n: int) -> int:
     if name in ("k_cell_vols", "k_mom_vols"):
         return (n + 4) ** 3
     if name in ("k_accelerate", "k_reset_field", "k_flux_calc", "k_mom_flux", "k_mom_update", "k_node_flux",
-                "k_node_mass", "k_fast_accelerate", "k_fast_pdv_correct_flux", "k_fast_mom_node",
-                "k_fast_mom_vel"):
+                "k_node_mass", "k_fast_accelerate", "k_fast_pdv_correct_flux", "k_fast_mom_x", "k_fast_mom_y",
+                "k_fast_mom_z"):
         return (n + 1) ** 3
     return n ** 3
 
@@ -167,8 +172,12 @@ def main():
     per_launch_bytes = ARRAYS_PER_CELL.get(name, 0) * 8 * kernel_domain_cells(name, n)
     avg_s = ms / launches / 1e3
     achieved = per_launch_bytes / avg_s / 1e9 if per_launch_bytes else None
+    traffic = None
+    if n == 512 and TRAFFIC_FILE.exists():
+        measured = json.loads(TRAFFIC_FILE.read_text())["kernels"].get(name)
+        traffic = measured["hbm_bytes"] if measured else None
     roofline = dict(bound="hbm", kernel=name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=None,
+                    frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=traffic,
                     avg_launch_us=avg_s * 1e6, launches=launches,
                     bytes_per_launch=per_launch_bytes)
 

@@ -547,8 +547,8 @@ static void advec_mom(clover_chunk *c, int sweep, int dir)
             double *vout[3] = {t.f[X_W5], t.f[X_W6], t.f[X_W7]};
             hip_check(launch_fast_advec_mom(t.g, c->geo, c->advect_x, sweep, dir, t.f[F_MASS_FLUX_X],
                                             t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y],
-                                            t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[F_DENSITY1], t.f[X_W1],
-                                            t.f[X_W2], t.f[X_W3], vin, vout, c->st),
+                                            t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[F_DENSITY1], vin, vout,
+                                            c->st),
                       "advec_mom");
             std::swap(t.f[F_XVEL1], t.f[X_W5]);
             std::swap(t.f[F_YVEL1], t.f[X_W6]);

@@ -64,14 +64,14 @@ hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int 
 
 // one advec_mom sweep for all three velocities (advec_mom_driver.f90 calls
 // the kernel for xvel1, yvel1, zvel1 with the node quantities of the first
-// call): node flux and masses into scratch, then the new velocities into
-// vel_out[3] (they read their neighbours' old values, so they cannot be
-// updated in place).  Only nodes min..max+1 of vel_out are written.
+// call), node flux and masses kept on chip (mom_kernels.hip).  The new
+// velocities go to vel_out[3]: they read their neighbours' old values, so
+// they cannot be updated in place.  Only nodes min..max+1 of vel_out are
+// written.
 hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int sweep_number, int direction,
                                  const double *mass_flux_x, const double *vol_flux_x, const double *mass_flux_y,
                                  const double *vol_flux_y, const double *mass_flux_z, const double *vol_flux_z,
-                                 const double *density1, double *node_flux, double *node_mass_post,
-                                 double *node_mass_pre, const double *const *vel_in, double *const *vel_out,
+                                 const double *density1, const double *const *vel_in, double *const *vel_out,
                                  hipStream_t st);
 
 hipError_t launch_fast_field_summary(const Lay &g, const Geo &m, const double *density0, const double *energy0,

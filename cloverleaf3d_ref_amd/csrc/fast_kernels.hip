@@ -439,145 +439,6 @@ hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int 
 }
 
 // ---------------------------------------------------------------------------
-// advec_mom, one sweep, three velocities.
-//
-// Node kernel: node_flux at the node and its lower neighbour along the sweep,
-// node_mass_post from the eight surrounding cells with post_vol recomputed
-// from the volume fluxes (advec_mom_kernel.f90:95-163: mode 1 sweep1/x,
-// 2 sweep1/z, 3 sweep2/advect_x, 4 sweep2/not, 5-6 sweep3), node_mass_pre.
-__device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
-                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
-{
-    switch (mode) {
-    case 1: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
-    case 2: return V0 + vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i];
-    case 3: return V0 + vfz[i + g.plane] - vfz[i];
-    case 4: return V0 + vfx[i + 1] - vfx[i];
-    default: return V0;
-    }
-}
-
-__device__ inline double node_flux_at(int dir, const Lay &g, int i, const double *__restrict__ mf)
-{
-    if (dir == 1)
-        return 0.125 * (AT(mf, 0, -1, 0) + AT(mf, 0, 0, 0) + AT(mf, 1, -1, 0) + AT(mf, 1, 0, 0) + AT(mf, 0, -1, -1) +
-                        AT(mf, 0, 0, -1) + AT(mf, 1, -1, -1) + AT(mf, 1, 0, -1));
-    if (dir == 2)
-        return 0.125 * (AT(mf, -1, 0, 0) + AT(mf, 0, 0, 0) + AT(mf, -1, 1, 0) + AT(mf, 0, 1, 0) + AT(mf, -1, 0, -1) +
-                        AT(mf, 0, 0, -1) + AT(mf, -1, 1, -1) + AT(mf, 0, 1, -1));
-    return 0.125 * (AT(mf, -1, 0, 0) + AT(mf, 0, 0, 0) + AT(mf, -1, 0, 1) + AT(mf, 0, 0, 1) + AT(mf, -1, -1, 0) +
-                    AT(mf, 0, -1, 0) + AT(mf, -1, -1, 1) + AT(mf, 0, -1, 1));
-}
-
-__global__ void k_fast_mom_node(Lay g, Range r, Geo m, int dir, int mode, const double *__restrict__ mf,
-                                const double *__restrict__ vfx, const double *__restrict__ vfy,
-                                const double *__restrict__ vfz, const double *__restrict__ density1,
-                                double *__restrict__ node_flux, double *__restrict__ node_mass_post,
-                                double *__restrict__ node_mass_pre)
-{
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
-    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
-    const double nf = node_flux_at(dir, g, i, mf);
-    const double nf_lo = node_flux_at(dir, g, i - s, mf);
-    const double V0 = m.vol;
-#define NMP(a, b, c)                                                                                        \
-    (AT(density1, a, b, c) *                                                                                \
-     mom_post_vol(mode, g, i + (a) + (b) * g.ld + (c) * g.plane, V0, vfx, vfy, vfz))
-    const double nmp = 0.125 * (NMP(0, -1, 0) + NMP(0, 0, 0) + NMP(-1, -1, 0) + NMP(-1, 0, 0) + NMP(0, -1, -1) +
-                                NMP(0, 0, -1) + NMP(-1, -1, -1) + NMP(-1, 0, -1));
-#undef NMP
-    node_flux[i] = nf;
-    node_mass_post[i] = nmp;
-    node_mass_pre[i] = nmp - nf_lo + nf;
-}
-
-// Velocity kernel: each node forms the momentum fluxes of its two faces along
-// the sweep (advec_mom_kernel.f90:213-249 and twins) for all three velocity
-// components and writes the updated velocities to separate arrays.
-__device__ inline double mom_flux_face(int f, int s, double d, const double *__restrict__ node_flux,
-                                       const double *__restrict__ node_mass_pre, const double *__restrict__ vel)
-{
-    const double nf = node_flux[f];
-    int iu, id, iw;
-    if (nf < 0.0) {
-        iu = f + 2 * s; id = f + s; iw = f;
-    } else {
-        iu = f - s; id = f; iw = f + s;
-    }
-    const double sigma = fabs(nf) / (node_mass_pre[id]);
-    const double width = d;
-    const double vdonor = vel[id];
-    const double vdiffuw = vdonor - vel[iu];
-    const double vdiffdw = vel[iw] - vdonor;
-    double limiter = 0.0;
-    if (vdiffuw * vdiffdw > 0.0) {
-        const double auw = fabs(vdiffuw);
-        const double adw = fabs(vdiffdw);
-        double wind = 1.0;
-        if (vdiffdw <= 0.0) wind = -1.0;
-        limiter = wind * dmin(dmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / d) / 6.0, auw), adw);
-    }
-    const double advec_vel_s = vdonor + (1.0 - sigma) * limiter;
-    return advec_vel_s * nf;
-}
-
-struct Vel3 {
-    const double *in[3];
-    double *out[3];
-};
-
-__global__ void k_fast_mom_vel(Lay g, Range r, Geo m, int dir, const double *__restrict__ node_flux,
-                               const double *__restrict__ node_mass_post, const double *__restrict__ node_mass_pre,
-                               Vel3 v)
-{
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
-    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
-    const double d = dir == 1 ? m.dx : dir == 2 ? m.dy : m.dz;
-    const double nmpre = node_mass_pre[i], nmpost = node_mass_post[i];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double *__restrict__ vel = v.in[c];
-        const double mf_lo = mom_flux_face(i - s, s, d, node_flux, node_mass_pre, vel);
-        const double mf_hi = mom_flux_face(i, s, d, node_flux, node_mass_pre, vel);
-        v.out[c][i] = (vel[i] * nmpre + mf_lo - mf_hi) / nmpost;
-    }
-}
-
-hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int sweep_number, int direction,
-                                 const double *mass_flux_x, const double *vol_flux_x, const double *mass_flux_y,
-                                 const double *vol_flux_y, const double *mass_flux_z, const double *vol_flux_z,
-                                 const double *density1, double *node_flux, double *node_mass_post,
-                                 double *node_mass_pre, const double *const *vel_in, double *const *vel_out,
-                                 hipStream_t st)
-{
-    int mode = 0;
-    if (sweep_number == 1 && direction == 1) mode = 1;
-    else if (sweep_number == 1 && direction == 3) mode = 2;
-    else if (sweep_number == 2 && advect_x) mode = 3;
-    else if (sweep_number == 2 && !advect_x) mode = 4;
-    else if (sweep_number == 3 && direction == 1) mode = 5;
-    else if (sweep_number == 3 && direction == 3) mode = 6;
-    if (!mode || direction < 1 || direction > 3) return hipErrorInvalidValue;
-    const double *mf = direction == 1 ? mass_flux_x : direction == 2 ? mass_flux_y : mass_flux_z;
-    Range rn;
-    if (direction == 1) rn = Range{g.x_min - 1, g.x_max + 2, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
-    else if (direction == 2) rn = Range{g.x_min, g.x_max + 1, g.y_min - 1, g.y_max + 2, g.z_min, g.z_max + 1};
-    else rn = Range{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min - 1, g.z_max + 2};
-    LAUNCH(k_fast_mom_node, rn, st, g, rn, m, direction, mode, mf, vol_flux_x, vol_flux_y, vol_flux_z, density1,
-           node_flux, node_mass_post, node_mass_pre);
-    Range upd{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
-    Vel3 v;
-    for (int c = 0; c < 3; c++) {
-        v.in[c] = vel_in[c];
-        v.out[c] = vel_out[c];
-    }
-    LAUNCH(k_fast_mom_vel, upd, st, g, upd, m, direction, node_flux, node_mass_post, node_mass_pre, v);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
 // field_summary_kernel.f90:51-78 with the scalar cell volume; same
 // reduction tree as the per-call kernel.
 __global__ void __launch_bounds__(RED_THREADS)

@@ -1,0 +1,339 @@
+// advec_mom (advec_mom_kernel.f90:29-428) for one sweep and all three
+// velocity components, as one kernel that keeps every intermediate on chip.
+//
+// Per sweep along axis s the reference forms, node by node,
+//   node_flux     nf(n)  = 0.125 * (8 mass fluxes around n)
+//   node_mass_post  mp(n) = 0.125 * (8 x density1 * post_vol of the cells around n)
+//   node_mass_pre   mq(n) = mp(n) - nf(n-s) + nf(n)
+//   mom_flux      F(f)   = advec_vel_s(f) * nf(f)      (face f between n and n+s)
+//   vel(n) <- (vel(n) * mq(n) + F(n-s) - F(n)) / mp(n)
+// and needs, for node n, nf at n-2s..n+s, mp at n-s..n+s and vel at n-2s..n+2s.
+// The per-call path stores nf, mp, mq and mom_flux in work arrays; here
+// every value is formed once, in registers, with the same expression and
+// operand order (so the results are bit-identical), and shared with the
+// neighbours that need it:
+//   * lanes of a wavefront lie along x (coalesced rows); neighbours along x
+//     come from lane shuffles;
+//   * the waves of a workgroup lie along the second axis; a neighbour row is
+//     read from LDS (double-buffered, one barrier per plane).  Wave 0 is the
+//     halo row and stores nothing;
+//   * the workgroup marches along the third axis, keeping the previous
+//     planes' values in registers.
+// The y and z sweeps march along the sweep axis itself, so the +-2 window
+// along s is a register pipeline; the x sweep marches along z and takes its
+// +-2 window from the lanes (60 of 64 lanes store).
+//
+// Each face's momentum flux is formed once (the per-call kernel, like the
+// reference, forms mom_flux once per face too) and each input element is
+// read about once, instead of the 8 density/volume reads per node mass and
+// 16 mass-flux reads per pair of node fluxes of a plain per-node kernel.
+#include <hip/hip_runtime.h>
+
+#include "fast.h"
+#include "launch.h"
+
+namespace clover {
+
+constexpr int MW = 8;       // waves per workgroup (wave 0 is the halo row)
+constexpr int MCHUNK = 64;  // march positions per workgroup
+
+struct Vel3 {
+    const double *in[3];
+    double *out[3];
+};
+
+// post_vol of advec_mom_kernel.f90:95-163 for the sweep `mode`:
+// 1 sweep 1 along x, 2 sweep 1 along z, 3 sweep 2 with advect_x,
+// 4 sweep 2 without, 5 and 6 sweep 3
+__device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
+                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
+{
+    switch (mode) {
+    case 1: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
+    case 2: return V0 + vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i];
+    case 3: return V0 + vfz[i + g.plane] - vfz[i];
+    case 4: return V0 + vfx[i + 1] - vfx[i];
+    default: return V0;
+    }
+}
+
+// momentum flux through the face between nodes n and n+s
+// (advec_mom_kernel.f90:213-249 and its y/z twins); vm1..vp2 are the
+// velocity at n-s, n, n+s, n+2s; sigma is |nf| / node_mass_pre(donor)
+__device__ inline double mom_face_flux(double nf, double sigma, double d, double vm1, double v0, double vp1,
+                                       double vp2)
+{
+    double upwind, donor, downwind;
+    if (nf < 0.0) {
+        upwind = vp2; donor = vp1; downwind = v0;
+    } else {
+        upwind = vm1; donor = v0; downwind = vp1;
+    }
+    const double width = d;
+    const double vdiffuw = donor - upwind;
+    const double vdiffdw = downwind - donor;
+    double limiter = 0.0;
+    if (vdiffuw * vdiffdw > 0.0) {
+        const double auw = fabs(vdiffuw);
+        const double adw = fabs(vdiffdw);
+        double wind = 1.0;
+        if (vdiffdw <= 0.0) wind = -1.0;
+        limiter = wind * dmin(dmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / d) / 6.0, auw), adw);
+    }
+    const double advec_vel_s = donor + (1.0 - sigma) * limiter;
+    return advec_vel_s * nf;
+}
+
+__device__ inline double face_sigma(double nf, double mq_lo, double mq_hi)
+{
+    return fabs(nf) / (nf < 0.0 ? mq_hi : mq_lo);
+}
+
+// A value and its neighbours at -1 along the lane axis (or +1, see
+// exchange_x) and -1 along the wave axis.
+struct Quad {
+    double c, x, w, xw;
+};
+
+// Publishes a and p of every thread of the workgroup and returns each
+// thread's neighbours: a at lane+dxa, p at lane+dxp, both also one wave
+// down.  Lanes and waves without such a neighbour get their own values
+// (used only by threads that store nothing).
+template <int DXA, int DXP>
+__device__ inline void exchange(double (*buf)[MW][64], int lane, int w, double a, double p, Quad &A, Quad &P)
+{
+    buf[0][w][lane] = a;
+    buf[1][w][lane] = p;
+    __syncthreads();
+    const int la = min(max(lane + DXA, 0), 63), lp = min(max(lane + DXP, 0), 63), wd = w > 0 ? w - 1 : 0;
+    A.c = a;
+    A.x = buf[0][w][la];
+    A.w = buf[0][wd][lane];
+    A.xw = buf[0][wd][la];
+    P.c = p;
+    P.x = buf[1][w][lp];
+    P.w = buf[1][wd][lane];
+    P.xw = buf[1][wd][lp];
+}
+
+// ---------------------------------------------------------------------------
+// y and z sweeps: lanes along x (lane 0 is the j-1 halo, 63 lanes store),
+// waves along the other axis, march along the sweep axis.
+//
+// In (x, m = sweep axis, w = wave axis) offsets both sweeps have the same
+// node flux (advec_mom_kernel.f90:260-265 for y, 345-350 for z):
+//   nf(m) = 0.125*(f(-1,0,0)+f(0,0,0)+f(-1,1,0)+f(0,1,0)+f(-1,0,-1)+f(0,0,-1)+f(-1,1,-1)+f(0,1,-1))
+// and node_mass_post (advec_mom_kernel.f90:267-275, 352-360) sums the cells
+// (dj,dk,dl) = (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1),
+// which is a different order in (x, m, w) for the two sweeps.
+template <int DIR>
+__device__ inline double march_node_flux(const Quad &f0, const Quad &f1)
+{
+    return 0.125 * (f0.x + f0.c + f1.x + f1.c + f0.xw + f0.w + f1.xw + f1.w);
+}
+
+template <int DIR>
+__device__ inline double march_node_mass(const Quad &pm, const Quad &p0)  // cell planes m-1, m
+{
+    if (DIR == 2)  // k = m, l = w
+        return 0.125 * (pm.c + p0.c + pm.x + p0.x + pm.w + p0.w + pm.xw + p0.xw);
+    // DIR == 3: k = w, l = m
+    return 0.125 * (p0.w + p0.c + p0.xw + p0.x + pm.w + pm.c + pm.xw + pm.x);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(64 * MW)
+k_fast_mom_march(Lay g, Geo geo, int mode, const double *__restrict__ mf, const double *__restrict__ vfx,
+                 const double *__restrict__ vfy, const double *__restrict__ vfz,
+                 const double *__restrict__ density1, Vel3 v)
+{
+    __shared__ double xbuf[2][2][MW][64];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int sM = DIR == 3 ? g.plane : g.ld;
+    const int wmin = DIR == 3 ? g.y_min : g.z_min, wmax = DIR == 3 ? g.y_max : g.z_max;
+    const int mmin = DIR == 3 ? g.z_min : g.y_min, mmax = DIR == 3 ? g.z_max : g.y_max;
+    const double d = DIR == 3 ? geo.dz : geo.dy;
+    const double V0 = geo.vol;
+
+    const int j = g.x_min - 1 + (int)blockIdx.x * 63 + lane;
+    const int wp = wmin - 1 + (int)blockIdx.y * (MW - 1) + w;
+    const int m0 = mmin + (int)blockIdx.z * MCHUNK;
+    const int m1 = min(m0 + MCHUNK - 1, mmax + 1);
+    // j >= x_min-1 and wp >= wmin-1 always; upper limits of the arrays read:
+    // cells and the sweep's mass flux end at +2 across the sweep, vertices at +3
+    const bool ok_cell = j <= g.x_max + 2 && wp <= wmax + 2;
+    const bool store = lane >= 1 && w >= 1 && j <= g.x_max + 1 && wp <= wmax + 1;
+    const int i0 = DIR == 3 ? idx(g, j, wp, mmin) : idx(g, j, mmin, wp);  // offset at march position mmin
+#define OFF(q) (i0 + ((q) - mmin) * sM)
+
+    auto plane = [&](int q, Quad &F, Quad &P, int &b) {
+        const int i = OFF(q);
+        const double a = ok_cell ? mf[i] : 0.0;  // q <= mmax+3: inside the sweep's face extent
+        const double p = (ok_cell && q <= mmax + 2) ? density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
+        exchange<-1, -1>(xbuf[b], lane, w, a, p, F, P);
+        b ^= 1;
+    };
+    auto vel_at = [&](int c, int q) { return store ? v.in[c][OFF(q)] : 0.0; };
+
+    int b = 0;
+    Quad Fa, Pa, Fb, Pb;
+    // prologue: nf(m0-2..m0), node masses at m0-1, m0, m0+1, velocities m0-2..m0+1
+    plane(m0 - 2, Fa, Pa, b);
+    plane(m0 - 1, Fb, Pb, b);
+    const double nf_m2 = march_node_flux<DIR>(Fa, Fb);
+    const double mp_m1 = march_node_mass<DIR>(Pa, Pb);
+    plane(m0, Fa, Pa, b);
+    double nf_lo = march_node_flux<DIR>(Fb, Fa);  // nf(m-1)
+    double mp0 = march_node_mass<DIR>(Pb, Pa);    // mp(m)
+    const double mq_m1 = mp_m1 - nf_m2 + nf_lo;
+    plane(m0 + 1, Fb, Pb, b);
+    double nf0 = march_node_flux<DIR>(Fa, Fb);     // nf(m)
+    double mp1 = march_node_mass<DIR>(Pa, Pb);     // mp(m+1)
+    double mq0 = mp0 - nf_lo + nf0;                // mq(m)
+    // Fb, Pb hold plane m+1
+
+    double vm1[3], v0[3], vp1[3], flo[3];
+    {
+        const double sig = face_sigma(nf_lo, mq_m1, mq0);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double vm2 = vel_at(c, m0 - 2);
+            vm1[c] = vel_at(c, m0 - 1);
+            v0[c] = vel_at(c, m0);
+            vp1[c] = vel_at(c, m0 + 1);
+            flo[c] = mom_face_flux(nf_lo, sig, d, vm2, vm1[c], v0[c], vp1[c]);  // F(m0-1)
+        }
+    }
+
+    for (int m = m0; m <= m1; m++) {
+        Quad Fc, Pc;
+        plane(m + 2, Fc, Pc, b);
+        const double nf1 = march_node_flux<DIR>(Fb, Fc);  // nf(m+1)
+        const double mp2 = march_node_mass<DIR>(Pb, Pc);  // mp(m+2), garbage at mmax+3 (never used)
+        const double mq1 = mp1 - nf0 + nf1;              // mq(m+1)
+        const double sig = face_sigma(nf0, mq0, mq1);
+        const int i = OFF(m);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double vp2 = vel_at(c, m + 2);
+            const double fhi = mom_face_flux(nf0, sig, d, vm1[c], v0[c], vp1[c], vp2);  // F(m)
+            if (store) v.out[c][i] = (v0[c] * mq0 + flo[c] - fhi) / mp0;
+            flo[c] = fhi;
+            vm1[c] = v0[c];
+            v0[c] = vp1[c];
+            vp1[c] = vp2;
+        }
+        nf0 = nf1;
+        mp0 = mp1;
+        mp1 = mp2;
+        mq0 = mq1;
+        Fb = Fc;
+        Pb = Pc;
+    }
+#undef OFF
+}
+
+// ---------------------------------------------------------------------------
+// x sweep: lanes along x with lanes 0-1 and 62-63 as halo (60 store), waves
+// along y, march along z.  advec_mom_kernel.f90:172-206:
+//   nf(j)  = 0.125*(f(0,-1,0)+f(0,0,0)+f(1,-1,0)+f(1,0,0)+f(0,-1,-1)+f(0,0,-1)+f(1,-1,-1)+f(1,0,-1))
+//   mp(j)  = cells (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1)
+__device__ inline double shfl_d(double x, int src) { return __shfl(x, src, 64); }
+
+__global__ void __launch_bounds__(64 * MW)
+k_fast_mom_x(Lay g, Geo geo, int mode, const double *__restrict__ mf, const double *__restrict__ vfx,
+             const double *__restrict__ vfy, const double *__restrict__ vfz, const double *__restrict__ density1,
+             Vel3 v)
+{
+    __shared__ double xbuf[2][2][MW][64];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const double d = geo.dx;
+    const double V0 = geo.vol;
+    const int j = g.x_min - 2 + (int)blockIdx.x * 60 + lane;
+    const int k = g.y_min - 1 + (int)blockIdx.y * (MW - 1) + w;
+    const int l0 = g.z_min + (int)blockIdx.z * MCHUNK;
+    const int l1 = min(l0 + MCHUNK - 1, g.z_max + 1);
+    // j >= x_min-2 and k >= y_min-1 always.  mass_flux_x ends at x_max+3 /
+    // y_max+2, cells at x_max+2 / y_max+2, vertices at x_max+3 / y_max+3.
+    const bool ok_face = j <= g.x_max + 3 && k <= g.y_max + 2;
+    const bool ok_cell = j <= g.x_max + 2 && k <= g.y_max + 2;
+    const bool ok_vert = j <= g.x_max + 3 && k <= g.y_max + 3;
+    const bool store = lane >= 2 && lane <= 61 && w >= 1 && j <= g.x_max + 1 && k <= g.y_max + 1;
+    const int i0 = idx(g, j, k, g.z_min);
+#define OFF(q) (i0 + ((q) - g.z_min) * g.plane)
+
+    auto plane = [&](int q, Quad &F, Quad &P, int &b) {
+        const int i = OFF(q);
+        const double a = ok_face ? mf[i] : 0.0;
+        const double p = ok_cell ? density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
+        exchange<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
+        b ^= 1;
+    };
+
+    int b = 0;
+    Quad Fp, Pp;  // plane l-1
+    plane(l0 - 1, Fp, Pp, b);
+    const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63, lp2 = lane < 62 ? lane + 2 : 63;
+    for (int l = l0; l <= l1; l++) {
+        Quad F, P;
+        plane(l, F, P, b);
+        // F.x = f(+1,0), F.w = f(0,-1), F.xw = f(+1,-1); P.x = cell(-1,0), ...
+        const double nf = 0.125 * (F.w + F.c + F.xw + F.x + Fp.w + Fp.c + Fp.xw + Fp.x);
+        const double mp = 0.125 * (P.w + P.c + P.xw + P.x + Pp.w + Pp.c + Pp.xw + Pp.x);
+        const double nf_lo = shfl_d(nf, lm);
+        const double mq = mp - nf_lo + nf;
+        const double mq_hi = shfl_d(mq, lp);
+        const double sig = face_sigma(nf, mq, mq_hi);
+        const int i = OFF(l);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double v0 = ok_vert ? v.in[c][i] : 0.0;
+            const double vm1 = shfl_d(v0, lm), vp1 = shfl_d(v0, lp), vp2 = shfl_d(v0, lp2);
+            const double fhi = mom_face_flux(nf, sig, d, vm1, v0, vp1, vp2);  // face j | j+1
+            const double flo = shfl_d(fhi, lm);                                // face j-1 | j
+            if (store) v.out[c][i] = (v0 * mq + flo - fhi) / mp;
+        }
+        Fp = F;
+        Pp = P;
+    }
+#undef OFF
+}
+
+hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int sweep_number, int direction,
+                                 const double *mass_flux_x, const double *vol_flux_x, const double *mass_flux_y,
+                                 const double *vol_flux_y, const double *mass_flux_z, const double *vol_flux_z,
+                                 const double *density1, const double *const *vel_in, double *const *vel_out,
+                                 hipStream_t st)
+{
+    int mode = 0;
+    if (sweep_number == 1 && direction == 1) mode = 1;
+    else if (sweep_number == 1 && direction == 3) mode = 2;
+    else if (sweep_number == 2 && advect_x) mode = 3;
+    else if (sweep_number == 2 && !advect_x) mode = 4;
+    else if (sweep_number == 3 && direction == 1) mode = 5;
+    else if (sweep_number == 3 && direction == 3) mode = 6;
+    if (!mode || direction < 1 || direction > 3) return hipErrorInvalidValue;
+    Vel3 v;
+    for (int c = 0; c < 3; c++) {
+        v.in[c] = vel_in[c];
+        v.out[c] = vel_out[c];
+    }
+    const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
+    const dim3 blk(64, MW, 1);
+    if (direction == 1) {
+        const dim3 grid((nxn + 59) / 60, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
+        TIMED_LAUNCH("k_fast_mom_x", st, k_fast_mom_x, grid, blk, 0, st, g, m, mode, mass_flux_x, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, v);
+    } else if (direction == 2) {
+        const dim3 grid((nxn + 62) / 63, (nzn + MW - 2) / (MW - 1), (nyn + MCHUNK - 1) / MCHUNK);
+        TIMED_LAUNCH("k_fast_mom_y", st, k_fast_mom_march<2>, grid, blk, 0, st, g, m, mode, mass_flux_y, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, v);
+    } else {
+        const dim3 grid((nxn + 62) / 63, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
+        TIMED_LAUNCH("k_fast_mom_z", st, k_fast_mom_march<3>, grid, blk, 0, st, g, m, mode, mass_flux_z, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, v);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace clover
