@@ -50,14 +50,14 @@ ARRAYS_PER_CELL = {
     "k_node_flux": 2, "k_node_mass": 5, "k_mom_flux": 4, "k_mom_update": 5, "k_reset_field": 10,
     "k_field_summary": 7, "k_pdv<1>": 13, "k_pdv<0>": 16,
     # fused kernels (geometry as scalars).  Sweep-dependent ones averaged over
-    # the sweeps they run in: cell_flux reads 3/2/1 volume fluxes, density1,
-    # energy1 and writes the mass and energy fluxes; cell_update reads 3/2/1
-    # volume fluxes, the two fluxes, density1, energy1 and writes density1,
-    # energy1; mom_x/y/z read the sweep's mass flux, density1, 2/1/0 volume
-    # fluxes and three velocities and write three velocities (x and z run
-    # in sweeps 1 and 3, y always in sweep 2 with one volume flux).
+    # the sweeps they run in: cell_x/y/z read 3/2/1 volume fluxes (sweep
+    # 1/2/3), density1 and energy1 and write the sweep's mass flux, density1
+    # and energy1; mom_x/y/z read the sweep's mass flux, density1, 2/1/0
+    # volume fluxes and three velocities and write three velocities.  x and
+    # z run in sweeps 1 and 3, y always in sweep 2.
     "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accelerate": 9, "k_fast_pdv_correct_flux": 15,
-    "k_fast_cell_flux": 6, "k_fast_cell_update": 8, "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
+    "k_fast_cell_x": 7, "k_fast_cell_y": 7, "k_fast_cell_z": 7,
+    "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
     "k_fast_field_summary": 6,
 }
 

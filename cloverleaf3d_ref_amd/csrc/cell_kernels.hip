@@ -1,0 +1,235 @@
+// advec_cell (advec_cell_kernel.f90:30-383) for one sweep as one kernel:
+// the mass and energy fluxes of every face along the sweep and the cell
+// update that consumes them, with the energy flux and pre_vol kept in
+// registers.  The fluxes of a face depend only on cells along the sweep
+// (upwind, donor, downwind), so
+//   * the x sweep works on rows: lanes along x, each lane forms the flux of
+//     its face from its neighbours' values (lane shuffles) and updates its
+//     cell from its own and the next lane's flux (60 of 64 lanes store);
+//   * the y and z sweeps march each (x, cross) column along the sweep
+//     axis with the upwind..downwind window in registers.
+// The reference computes every flux with the old density1/energy1 before
+// it updates any cell, so the new values go to separate arrays (rho_out,
+// e_out) which then take the place of density1/energy1; only interior cells
+// of those are written (the halo update that follows every advec_cell
+// sweep in advection.f90 refreshes their halos before anything reads them).
+// Every value is formed with the expression and operand order of the
+// per-call kernels, so the results are bit-identical.
+#include <hip/hip_runtime.h>
+
+#include "fast.h"
+#include "launch.h"
+
+namespace clover {
+
+constexpr int CROWS = 4;   // rows (or columns) per workgroup
+constexpr int CCHUNK = 64; // faces per march chunk
+
+// pre_vol (advec_cell_kernel.f90:97-116, 193-210, 289-306) for `mode`:
+// 1 x sweep 1, 2 x sweep 3, 3 y sweep with advect_x, 4 y without,
+// 5 z sweep 1, 6 z sweep 3
+__device__ inline double cell_pre_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
+                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
+{
+    switch (mode) {
+    case 1:
+    case 5: return V0 + (vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i]);
+    case 2: return V0 + vfx[i + 1] - vfx[i];
+    case 3: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
+    case 4: return V0 + vfy[i + g.ld] - vfy[i] + vfx[i + 1] - vfx[i];
+    default: return V0 + vfz[i + g.plane] - vfz[i];
+    }
+}
+
+__device__ inline double cell_limiter(double sigma, double sigma3, double sigma4, double diffuw, double diffdw)
+{
+    const double one_by_six = 1.0 / 6.0;
+    if (diffuw * diffdw > 0.0)
+        return (1.0 - sigma) * copysign(1.0, diffdw) *
+               dmin(dmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
+    return 0.0;
+}
+
+// Mass and energy flux through one face (advec_cell_kernel.f90:123-186 and
+// twins): ru/rd/rw, eu/ed/ew the density and energy of the upwind, donor and
+// downwind cells, pv the donor's pre_vol, vd the vertex spacing.
+__device__ inline void cell_face_flux(double vf, double pv, double vd, double ru, double rd, double rw, double eu,
+                                      double ed, double ew, double &mf, double &ef)
+{
+    const double sigmat = fabs(vf) / pv;
+    const double sigma3 = (1.0 + sigmat) * (vd / vd);  // vertexd*(j)/vertexd*(dif) on the uniform mesh
+    const double sigma4 = 2.0 - sigmat;
+    double limiter = cell_limiter(sigmat, sigma3, sigma4, rd - ru, rw - rd);
+    mf = vf * (rd + limiter);
+    const double sigmam = fabs(mf) / (rd * pv);
+    limiter = cell_limiter(sigmam, sigma3, sigma4, ed - eu, ew - ed);
+    ef = mf * (ed + limiter);
+}
+
+// cell update (advec_cell_kernel.f90:188-191 and twins)
+__device__ inline void cell_update(double rho, double e, double pv, double mf_lo, double mf_hi, double ef_lo,
+                                   double ef_hi, double vf_lo, double vf_hi, double &rho_new, double &e_new)
+{
+    const double pm = rho * pv;
+    const double pom = pm + mf_lo - mf_hi;
+    const double pe = (e * pm + ef_lo - ef_hi) / pom;
+    const double av = pv + vf_lo - vf_hi;
+    rho_new = pom / av;
+    e_new = pe;
+}
+
+__device__ inline double shfl_c(double x, int src) { return __shfl(x, src, 64); }
+
+// ---------------------------------------------------------------------------
+// x sweep: faces j in [x_min, x_max+2], rows k, l interior.  Lane i holds
+// cell/face j = jt - 2 + i.
+__global__ void __launch_bounds__(64 * CROWS)
+k_fast_cell_x(Lay g, Geo m, int mode, const double *__restrict__ vfx, const double *__restrict__ vfy,
+              const double *__restrict__ vfz, const double *__restrict__ density1,
+              const double *__restrict__ energy1, double *__restrict__ mass_flux_x, double *__restrict__ rho_out,
+              double *__restrict__ e_out)
+{
+    const int lane = threadIdx.x;
+    const int ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
+    const int row = (int)blockIdx.y * CROWS + (int)threadIdx.y;
+    if (row >= ny * nz) return;  // whole wave: no shuffle partner is lost
+    const int k = g.y_min + row % ny, l = g.z_min + row / ny;
+    const int j = g.x_min - 2 + (int)blockIdx.x * 60 + lane;
+    const int i = idx(g, j, k, l);
+    const bool ok_cell = j <= g.x_max + 2;  // j >= x_min-2 always
+    const double rho = ok_cell ? density1[i] : 0.0;
+    const double e = ok_cell ? energy1[i] : 0.0;
+    const double pv = ok_cell ? cell_pre_vol(mode, g, i, m.vol, vfx, vfy, vfz) : 0.0;
+    const double vf = j <= g.x_max + 3 ? vfx[i] : 0.0;
+    const int lm2 = max(lane - 2, 0), lm1 = max(lane - 1, 0), lp1 = min(lane + 1, 63);
+    const double rm2 = shfl_c(rho, lm2), rm1 = shfl_c(rho, lm1), rp1 = shfl_c(rho, lp1);
+    const double em2 = shfl_c(e, lm2), em1 = shfl_c(e, lm1), ep1 = shfl_c(e, lp1);
+    const double pvm1 = shfl_c(pv, lm1);
+    // face j: upwind/donor/downwind (advec_cell_kernel.f90:123-132); the
+    // upwind index is clamped at x_max+2
+    double mf, ef;
+    if (vf > 0.0) {
+        cell_face_flux(vf, pvm1, m.dx, rm2, rm1, rho, em2, em1, e, mf, ef);
+    } else {
+        const bool clamp = j + 1 > g.x_max + 2;
+        cell_face_flux(vf, pv, m.dx, clamp ? rho : rp1, rho, rm1, clamp ? e : ep1, e, em1, mf, ef);
+    }
+    const double mf_hi = shfl_c(mf, lp1), ef_hi = shfl_c(ef, lp1), vf_hi = shfl_c(vf, lp1);
+    if (lane < 2 || lane > 61) return;
+    if (j <= g.x_max + 2) mass_flux_x[i] = mf;
+    if (j <= g.x_max) {
+        double rn, en;
+        cell_update(rho, e, pv, mf, mf_hi, ef, ef_hi, vf, vf_hi, rn, en);
+        rho_out[i] = rn;
+        e_out[i] = en;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y and z sweeps: lanes along x (interior j), threadIdx.y along the cross
+// axis, march along the sweep axis over faces [smin, smax+2] in chunks.
+// Faces of the y sweep run over l in [z_min, z_max+2] (advec_cell_kernel.f90:
+// 215-217); the cells updated are interior.
+template <int DIR>
+__global__ void __launch_bounds__(64 * CROWS)
+k_fast_cell_march(Lay g, Geo m, int mode, const double *__restrict__ vfx, const double *__restrict__ vfy,
+                  const double *__restrict__ vfz, const double *__restrict__ density1,
+                  const double *__restrict__ energy1, double *__restrict__ mass_flux, double *__restrict__ rho_out,
+                  double *__restrict__ e_out)
+{
+    const int j = g.x_min + (int)(blockIdx.x * 64 + threadIdx.x);
+    const int cr = (DIR == 3 ? g.y_min : g.z_min) + (int)(blockIdx.y * CROWS + threadIdx.y);
+    const int crmax = DIR == 3 ? g.y_max : g.z_max + 2;  // last cross position with faces
+    if (j > g.x_max || cr > crmax) return;                 // no cross-lane traffic in this kernel
+    const bool cross_in = DIR == 3 || cr <= g.z_max;       // updates only on interior cells
+    const int s = DIR == 3 ? g.plane : g.ld;
+    const int smin = DIR == 3 ? g.z_min : g.y_min, smax = DIR == 3 ? g.z_max : g.y_max;
+    const double vd = DIR == 3 ? m.dz : m.dy;
+    const double *__restrict__ vfs = DIR == 3 ? vfz : vfy;
+    const int f0 = smin + (int)blockIdx.z * CCHUNK;
+    const int f1 = min(f0 + CCHUNK - 1, smax + 2);
+    const int ib = DIR == 3 ? idx(g, j, cr, smin) : idx(g, j, smin, cr);  // offset at sweep position smin
+#define OFF(q) (ib + ((q) - smin) * s)
+    auto pre = [&](int q) { return cell_pre_vol(mode, g, OFF(q), m.vol, vfx, vfy, vfz); };
+
+    // window of old values at f-2, f-1, f, f+1 and pre_vol at f-1, f
+    double r_m2 = density1[OFF(f0 - 2)], r_m1 = density1[OFF(f0 - 1)], r_0 = density1[OFF(f0)];
+    double e_m2 = energy1[OFF(f0 - 2)], e_m1 = energy1[OFF(f0 - 1)], e_0 = energy1[OFF(f0)];
+    double pv_m1 = pre(f0 - 1);
+    // flux of face f0-1, needed when cell f0-1 is updated here (f0 > smin)
+    double mf_lo = 0.0, ef_lo = 0.0, vf_lo = 0.0;
+    if (f0 > smin) {
+        vf_lo = vfs[OFF(f0 - 1)];
+        const double r_m3 = density1[OFF(f0 - 3)], e_m3 = energy1[OFF(f0 - 3)];
+        if (vf_lo > 0.0)
+            cell_face_flux(vf_lo, pre(f0 - 2), vd, r_m3, r_m2, r_m1, e_m3, e_m2, e_m1, mf_lo, ef_lo);
+        else
+            cell_face_flux(vf_lo, pv_m1, vd, r_0, r_m1, r_m2, e_0, e_m1, e_m2, mf_lo, ef_lo);  // f0 <= smax+1
+    }
+    for (int f = f0; f <= f1; f++) {
+        const int i = OFF(f);
+        const double vf = vfs[i];
+        const double pv_0 = pre(f);
+        // upwind for vf <= 0 is min(f+1, smax+2)
+        double r_p1 = r_0, e_p1 = e_0;
+        if (f + 1 <= smax + 2) {
+            r_p1 = density1[OFF(f + 1)];
+            e_p1 = energy1[OFF(f + 1)];
+        }
+        double mf, ef;
+        if (vf > 0.0)
+            cell_face_flux(vf, pv_m1, vd, r_m2, r_m1, r_0, e_m2, e_m1, e_0, mf, ef);
+        else
+            cell_face_flux(vf, pv_0, vd, r_p1, r_0, r_m1, e_p1, e_0, e_m1, mf, ef);
+        mass_flux[i] = mf;
+        const int c = f - 1;  // cell between faces f-1 and f
+        if (cross_in && c >= smin && c <= smax) {
+            double rn, en;
+            cell_update(r_m1, e_m1, pv_m1, mf_lo, mf, ef_lo, ef, vf_lo, vf, rn, en);
+            rho_out[OFF(c)] = rn;
+            e_out[OFF(c)] = en;
+        }
+        mf_lo = mf;
+        ef_lo = ef;
+        vf_lo = vf;
+        r_m2 = r_m1;
+        r_m1 = r_0;
+        r_0 = r_p1;
+        e_m2 = e_m1;
+        e_m1 = e_0;
+        e_0 = e_p1;
+        pv_m1 = pv_0;
+    }
+#undef OFF
+}
+
+hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
+                                  const double *density1, const double *energy1, double *mass_flux_x,
+                                  const double *vol_flux_x, double *mass_flux_y, const double *vol_flux_y,
+                                  double *mass_flux_z, const double *vol_flux_z, double *rho_out, double *e_out,
+                                  hipStream_t st)
+{
+    int mode = 0;
+    if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
+    else if (dir == 2) mode = sweep_number == 2 ? (advect_x ? 3 : 4) : 0;
+    else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
+    if (!mode) return hipErrorInvalidValue;  // the resident driver issues only the reference's sweeps
+    const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
+    const dim3 blk(64, CROWS, 1);
+    if (dir == 1) {
+        const dim3 grid((nx + 2 + 59) / 60, (ny * nz + CROWS - 1) / CROWS, 1);
+        TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x, grid, blk, 0, st, g, m, mode, vol_flux_x, vol_flux_y,
+                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out);
+    } else if (dir == 2) {
+        const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, (ny + 2 + CCHUNK - 1) / CCHUNK);
+        TIMED_LAUNCH("k_fast_cell_y", st, k_fast_cell_march<2>, grid, blk, 0, st, g, m, mode, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_y, rho_out, e_out);
+    } else {
+        const dim3 grid((nx + 63) / 64, (ny + CROWS - 1) / CROWS, (nz + 2 + CCHUNK - 1) / CCHUNK);
+        TIMED_LAUNCH("k_fast_cell_z", st, k_fast_cell_march<3>, grid, blk, 0, st, g, m, mode, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_z, rho_out, e_out);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace clover

@@ -518,13 +518,18 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
 {
     Phase ph(c, P_CELL_ADVEC);
     for (auto &t : c->tiles) {
-        if (c->fast)
+        if (c->fast) {
+            // new density1/energy1 land in spare arrays, which then take their
+            // place (interior cells only; the halo update that follows every
+            // advec_cell sweep refreshes the halos before anything reads them)
             hip_check(launch_fast_advec_cell(t.g, c->geo, c->advect_x, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
                                              t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y],
-                                             t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W4],
-                                             c->st),
+                                             t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W1],
+                                             t.f[X_W2], c->st),
                       "advec_cell");
-        else
+            std::swap(t.f[F_DENSITY1], t.f[X_W1]);
+            std::swap(t.f[F_ENERGY1], t.f[X_W2]);
+        } else
             hip_check(launch_advec_cell(t.g, c->advect_x, dir, sweep, t.vertexdx, t.vertexdy, t.vertexdz,
                                         t.f[X_VOLUME], t.f[F_DENSITY1], t.f[F_ENERGY1], t.f[F_MASS_FLUX_X],
                                         t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z],

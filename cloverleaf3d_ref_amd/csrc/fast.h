@@ -55,12 +55,14 @@ hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, c
                                         const double *zvel0, const double *zvel1, double *vol_flux_x,
                                         double *vol_flux_y, double *vol_flux_z, hipStream_t st);
 
-// one advec_cell sweep: fluxes (pre_vol recomputed from the volume fluxes),
-// then the cell update.  ener_flux is scratch.
+// one advec_cell sweep, fluxes and cell update in one kernel
+// (cell_kernels.hip): mass_flux_<dir> is written over the reference's face
+// range, the new density1/energy1 of interior cells go to rho_out/e_out.
 hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
-                                  double *density1, double *energy1, double *mass_flux_x, const double *vol_flux_x,
-                                  double *mass_flux_y, const double *vol_flux_y, double *mass_flux_z,
-                                  const double *vol_flux_z, double *ener_flux, hipStream_t st);
+                                  const double *density1, const double *energy1, double *mass_flux_x,
+                                  const double *vol_flux_x, double *mass_flux_y, const double *vol_flux_y,
+                                  double *mass_flux_z, const double *vol_flux_z, double *rho_out, double *e_out,
+                                  hipStream_t st);
 
 // one advec_mom sweep for all three velocities (advec_mom_driver.f90 calls
 // the kernel for xvel1, yvel1, zvel1 with the node quantities of the first

@@ -326,119 +326,6 @@ hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, c
 }
 
 // ---------------------------------------------------------------------------
-// advec_cell, one sweep.  pre_vol of any cell is recomputed from the volume
-// fluxes (advec_cell_kernel.f90:97-116, 193-210, 289-306); `mode` as in the
-// per-call kernel: 1 x-sweep 1, 2 x-sweep 3, 3 y advect_x, 4 y not advect_x,
-// 5 z-sweep 1, 6 z-sweep 3.
-__device__ inline double cell_pre_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
-                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
-{
-    switch (mode) {
-    case 1:
-    case 5: return V0 + (vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i]);
-    case 2: return V0 + vfx[i + 1] - vfx[i];
-    case 3: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
-    case 4: return V0 + vfy[i + g.ld] - vfy[i] + vfx[i + 1] - vfx[i];
-    default: return V0 + vfz[i + g.plane] - vfz[i];
-    }
-}
-
-__device__ inline double cell_limiter(double sigma, double sigma3, double sigma4, double diffuw, double diffdw)
-{
-    const double one_by_six = 1.0 / 6.0;
-    if (diffuw * diffdw > 0.0)
-        return (1.0 - sigma) * copysign(1.0, diffdw) *
-               dmin(dmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
-    return 0.0;
-}
-
-__global__ void k_fast_cell_flux(Lay g, Range r, Geo m, int dir, int mode, const double *__restrict__ vfx,
-                                 const double *__restrict__ vfy, const double *__restrict__ vfz,
-                                 const double *__restrict__ density1, const double *__restrict__ energy1,
-                                 double *__restrict__ mass_flux, double *__restrict__ ener_flux)
-{
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
-    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
-    const int c = dir == 1 ? j : dir == 2 ? k : l;
-    const int cmax = dir == 1 ? g.x_max : dir == 2 ? g.y_max : g.z_max;
-    const double vd = dir == 1 ? m.dx : dir == 2 ? m.dy : m.dz;
-    const double *__restrict__ vol_flux = dir == 1 ? vfx : dir == 2 ? vfy : vfz;
-    const double vf = vol_flux[i];
-    int upwind, donor, downwind;
-    if (vf > 0.0) {
-        upwind = c - 2; donor = c - 1; downwind = c;
-    } else {
-        upwind = c + 1 < cmax + 2 ? c + 1 : cmax + 2; donor = c; downwind = c - 1;
-    }
-    const int iu = i + (upwind - c) * s, id = i + (donor - c) * s, iw = i + (downwind - c) * s;
-    const double pv = cell_pre_vol(mode, g, id, m.vol, vfx, vfy, vfz);
-    const double sigmat = fabs(vf) / pv;
-    const double sigma3 = (1.0 + sigmat) * (vd / vd);  // vertexd*(j)/vertexd*(dif) on the uniform mesh
-    const double sigma4 = 2.0 - sigmat;
-    const double rd = density1[id];
-    double diffuw = rd - density1[iu];
-    double diffdw = density1[iw] - rd;
-    double limiter = cell_limiter(sigmat, sigma3, sigma4, diffuw, diffdw);
-    const double mf = vf * (rd + limiter);
-    mass_flux[i] = mf;
-    const double sigmam = fabs(mf) / (rd * pv);
-    const double ed = energy1[id];
-    diffuw = ed - energy1[iu];
-    diffdw = energy1[iw] - ed;
-    limiter = cell_limiter(sigmam, sigma3, sigma4, diffuw, diffdw);
-    ener_flux[i] = mf * (ed + limiter);
-}
-
-__global__ void k_fast_cell_update(Lay g, Range r, Geo m, int dir, int mode, const double *__restrict__ vfx,
-                                   const double *__restrict__ vfy, const double *__restrict__ vfz,
-                                   const double *__restrict__ mass_flux, const double *__restrict__ ener_flux,
-                                   double *__restrict__ density1, double *__restrict__ energy1)
-{
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
-    const int s = dir == 1 ? 1 : dir == 2 ? g.ld : g.plane;
-    const double *__restrict__ vol_flux = dir == 1 ? vfx : dir == 2 ? vfy : vfz;
-    const double pv = cell_pre_vol(mode, g, i, m.vol, vfx, vfy, vfz);
-    const double pm = density1[i] * pv;
-    const double pom = pm + mass_flux[i] - mass_flux[i + s];
-    const double pe = (energy1[i] * pm + ener_flux[i] - ener_flux[i + s]) / pom;
-    const double av = pv + vol_flux[i] - vol_flux[i + s];
-    density1[i] = pom / av;
-    energy1[i] = pe;
-}
-
-hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
-                                  double *density1, double *energy1, double *mass_flux_x, const double *vol_flux_x,
-                                  double *mass_flux_y, const double *vol_flux_y, double *mass_flux_z,
-                                  const double *vol_flux_z, double *ener_flux, hipStream_t st)
-{
-    int mode = 0;
-    if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
-    else if (dir == 2) mode = sweep_number == 2 ? (advect_x ? 3 : 4) : 0;
-    else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
-    if (!mode) return hipErrorInvalidValue;  // the resident driver issues only the reference's sweeps
-    Range fr;
-    double *mf;
-    if (dir == 1) {
-        fr = Range{g.x_min, g.x_max + 2, g.y_min, g.y_max, g.z_min, g.z_max};
-        mf = mass_flux_x;
-    } else if (dir == 2) {
-        fr = Range{g.x_min, g.x_max, g.y_min, g.y_max + 2, g.z_min, g.z_max + 2};
-        mf = mass_flux_y;
-    } else {
-        fr = Range{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max + 2};
-        mf = mass_flux_z;
-    }
-    LAUNCH(k_fast_cell_flux, fr, st, g, fr, m, dir, mode, vol_flux_x, vol_flux_y, vol_flux_z, density1, energy1, mf,
-           ener_flux);
-    Range in{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
-    LAUNCH(k_fast_cell_update, in, st, g, in, m, dir, mode, vol_flux_x, vol_flux_y, vol_flux_z, mf, ener_flux,
-           density1, energy1);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
 // field_summary_kernel.f90:51-78 with the scalar cell volume; same
 // reduction tree as the per-call kernel.
 __global__ void __launch_bounds__(RED_THREADS)
