@@ -5,6 +5,12 @@
 // are bit-identical to the per-call kernels'; the savings are in HBM bytes:
 // scalar geometry for the uniform mesh, fused neighbours, dead stores
 // dropped, intermediates recomputed in registers.
+//
+// The stencils here read two neighbouring planes (l and l+1 for cells and
+// faces, l-1 and l for nodes).  Each thread owns one (j, k) column and
+// marches up a chunk of planes, keeping the lower plane's 2x2 corner values
+// in registers, so every plane is read from memory once per column instead
+// of once per plane that uses it.
 #include <hip/hip_runtime.h>
 
 #include "fast.h"
@@ -12,12 +18,43 @@
 
 namespace clover {
 
+constexpr int ZCH = 64;  // planes per marching workgroup
+
+// 2x2 corner values of an array around (j, k) in one plane:
+// v[a + 2b] = p(j+a, k+b) (a, b in {0, 1})
+struct Quad4 {
+    double v[4];
+};
+
+__device__ inline Quad4 quad_up(const double *__restrict__ p, int i, int ld)
+{
+    return Quad4{{p[i], p[i + 1], p[i + ld], p[i + 1 + ld]}};
+}
+
+// v[(a+1) + 2(b+1)] = p(j+a, k+b) (a, b in {-1, 0})
+__device__ inline Quad4 quad_down(const double *__restrict__ p, int i, int ld)
+{
+    return Quad4{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
+}
+
+// the column's planes [l0, l1] of the chunk this workgroup marches
+__device__ inline void chunk_planes(int lstart, int lend, int &l0, int &l1)
+{
+    const int nch = (int)gridDim.z;
+    const int n = lend - lstart + 1;
+    const int per = (n + nch - 1) / nch;
+    l0 = lstart + (int)blockIdx.z * per;
+    l1 = min(l0 + per - 1, lend);
+}
+
+static inline int chunks_for(int nplanes) { return (nplanes + ZCH - 1) / ZCH; }
+
 // ---------------------------------------------------------------------------
 // viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
 // viscosity only at the cell itself, so it runs on the value just computed
 // (the halo exchange of viscosity between the two in timestep.f90:80-84
-// does not touch interior cells).  Blocks walk planes l with a stride of
-// gridDim.z so the dt minimum leaves one partial per block.
+// does not touch interior cells).  Each workgroup leaves one partial of the
+// dt minimum.
 __global__ void __launch_bounds__(BX *BY)
 k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const double *__restrict__ density0, const double *__restrict__ pressure,
@@ -27,89 +64,103 @@ k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, 
 {
     const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
     const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
-    const bool in = j <= g.x_max && k <= g.y_max;
+    int l0, l1;
+    chunk_planes(g.z_min, g.z_max, l0, l1);
+    const bool in = j <= g.x_max && k <= g.y_max && l0 <= l1;  // every thread reaches the reduction
     const double mm = dmin(dmin(m.dx, m.dy), m.dz);
     const double ds = 1.0 / (mm * mm);
     double dtmin_thread = 1.0e300;
-    for (int l = g.z_min + (int)blockIdx.z; in && l <= g.z_max; l += (int)gridDim.z) {
-        const int i = idx(g, j, k, l);
-#define U(a, b, c) AT(xvel0, a, b, c)
-#define V(a, b, c) AT(yvel0, a, b, c)
-#define W(a, b, c) AT(zvel0, a, b, c)
-        const double ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
-        const double ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
-        const double ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
-        const double ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
-        const double ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
-        const double ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
-        const double vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
-        const double vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
-        const double vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
-        const double vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
-        const double vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
-        const double vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
-        const double wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
-        const double wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
-        const double wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
-        const double wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
-        const double wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
-        const double wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
-        // calc_dt_kernel.f90:124-125 sums the z-face nodes in its own order
-        const double dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
-        const double dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
+    if (in) {
+        int i = idx(g, j, k, l0);
+        Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
+        double p_lo = pressure[i - g.plane], p_c = pressure[i];
+        for (int l = l0; l <= l1; l++, i += g.plane) {
+            const Quad4 uu = quad_up(xvel0, i + g.plane, g.ld), vu = quad_up(yvel0, i + g.plane, g.ld),
+                        wu = quad_up(zvel0, i + g.plane, g.ld);
+            const double p_hi = pressure[i + g.plane];
+#define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
+#define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
+#define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
+            const double ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+            const double ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+            const double ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
+            const double ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
+            const double ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
+            const double ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
+            const double vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
+            const double vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
+            const double vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+            const double vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+            const double vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
+            const double vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
+            const double wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
+            const double wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
+            const double wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
+            const double wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
+            const double wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+            const double wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+            // calc_dt_kernel.f90:124-125 sums the z-face nodes in its own order
+            const double dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
+            const double dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
 #undef U
 #undef V
 #undef W
-        const double div = (m.xa * (ugradx2 - ugradx1) + m.ya * (vgrady2 - vgrady1)) + m.za * (wgradz2 - wgradz1);
-        const double xx = 0.25 * (ugradx2 - ugradx1) / (m.dx);
-        const double yy = 0.25 * (vgrady2 - vgrady1) / (m.dy);
-        const double zz = 0.25 * (wgradz2 - wgradz1) / (m.dz);
-        const double xy = 0.25 * (ugrady2 - ugrady1) / (m.dy) + 0.25 * (vgradx2 - vgradx1) / (m.dx);
-        const double xz = 0.25 * (ugradz2 - ugradz1) / (m.dz) + 0.25 * (wgradx2 - wgradx1) / (m.dx);
-        const double yz = 0.25 * (vgradz2 - vgradz1) / (m.dz) + 0.25 * (wgrady2 - wgrady1) / (m.dy);
-        double pgradx = (AT(pressure, 1, 0, 0) - AT(pressure, -1, 0, 0)) / (m.dx + m.dx);
-        double pgrady = (AT(pressure, 0, 1, 0) - AT(pressure, 0, -1, 0)) / (m.dy + m.dy);
-        double pgradz = (AT(pressure, 0, 0, 1) - AT(pressure, 0, 0, -1)) / (m.dz + m.dz);
-        const double pgradx2 = pgradx * pgradx;
-        const double pgrady2 = pgrady * pgrady;
-        const double pgradz2 = pgradz * pgradz;
-        const double limiter = (xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
-                                xz * pgradx * pgradz + yz * pgrady * pgradz) /
-                               dmax(pgradx2 + pgrady2 + pgradz2, 1.0e-16);
-        const double rho = density0[i];
-        double q;
-        if ((limiter > 0.0) || (div >= 0.0)) {
-            q = 0.0;
-        } else {
-            pgradx = copysign(dmax(1.0e-16, fabs(pgradx)), pgradx);
-            pgrady = copysign(dmax(1.0e-16, fabs(pgrady)), pgrady);
-            pgradz = copysign(dmax(1.0e-16, fabs(pgradz)), pgradz);
-            const double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
-            const double xgrad = fabs(m.dx * pgrad / pgradx);
-            const double ygrad = fabs(m.dy * pgrad / pgrady);
-            const double zgrad = fabs(m.dz * pgrad / pgradz);
-            const double grad = dmin(dmin(xgrad, ygrad), zgrad);
-            const double grad2 = grad * grad;
-            q = 2.0 * rho * grad2 * limiter * limiter;
-        }
-        viscosity[i] = q;
+            const double div = (m.xa * (ugradx2 - ugradx1) + m.ya * (vgrady2 - vgrady1)) + m.za * (wgradz2 - wgradz1);
+            const double xx = 0.25 * (ugradx2 - ugradx1) / (m.dx);
+            const double yy = 0.25 * (vgrady2 - vgrady1) / (m.dy);
+            const double zz = 0.25 * (wgradz2 - wgradz1) / (m.dz);
+            const double xy = 0.25 * (ugrady2 - ugrady1) / (m.dy) + 0.25 * (vgradx2 - vgradx1) / (m.dx);
+            const double xz = 0.25 * (ugradz2 - ugradz1) / (m.dz) + 0.25 * (wgradx2 - wgradx1) / (m.dx);
+            const double yz = 0.25 * (vgradz2 - vgradz1) / (m.dz) + 0.25 * (wgrady2 - wgrady1) / (m.dy);
+            double pgradx = (pressure[i + 1] - pressure[i - 1]) / (m.dx + m.dx);
+            double pgrady = (pressure[i + g.ld] - pressure[i - g.ld]) / (m.dy + m.dy);
+            double pgradz = (p_hi - p_lo) / (m.dz + m.dz);
+            const double pgradx2 = pgradx * pgradx;
+            const double pgrady2 = pgrady * pgrady;
+            const double pgradz2 = pgradz * pgradz;
+            const double limiter = (xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
+                                    xz * pgradx * pgradz + yz * pgrady * pgradz) /
+                                   dmax(pgradx2 + pgrady2 + pgradz2, 1.0e-16);
+            const double rho = density0[i];
+            double q;
+            if ((limiter > 0.0) || (div >= 0.0)) {
+                q = 0.0;
+            } else {
+                pgradx = copysign(dmax(1.0e-16, fabs(pgradx)), pgradx);
+                pgrady = copysign(dmax(1.0e-16, fabs(pgrady)), pgrady);
+                pgradz = copysign(dmax(1.0e-16, fabs(pgradz)), pgradz);
+                const double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
+                const double xgrad = fabs(m.dx * pgrad / pgradx);
+                const double ygrad = fabs(m.dy * pgrad / pgrady);
+                const double zgrad = fabs(m.dz * pgrad / pgradz);
+                const double grad = dmin(dmin(xgrad, ygrad), zgrad);
+                const double grad2 = grad * grad;
+                q = 2.0 * rho * grad2 * limiter * limiter;
+            }
+            viscosity[i] = q;
 
-        // calc_dt
-        const double c = soundspeed[i];
-        double cc = c * c;
-        cc = cc + 2.0 * q / rho;
-        double dtct = ds * cc;
-        dtct = dtc_safe * 1.0 / dmax(sqrt(dtct), g_small);
-        const double vol = m.vol;
-        const double du1 = ugradx1 * m.xa, du2 = ugradx2 * m.xa;
-        const double dtut = dtu_safe * 4.0 * vol / dmax(dmax(fabs(du1), fabs(du2)), 1.0e-5 * vol);
-        const double dv1 = vgrady1 * m.ya, dv2 = vgrady2 * m.ya;
-        const double dtvt = dtv_safe * 4.0 * vol / dmax(dmax(fabs(dv1), fabs(dv2)), 1.0e-5 * vol);
-        const double dw1 = dw1s * m.za, dw2 = dw2s * m.za;
-        const double dtwt = dtw_safe * 4.0 * vol / dmax(dmax(fabs(dw1), fabs(dw2)), 1.0e-5 * vol);
-        const double dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
-        const double dtdivt = dtdiv_safe * 4.0 * (vol) / dmax(vol * 1.0e-05, fabs(dv));
-        dtmin_thread = dmin(dtmin_thread, dmin(dmin(dmin(dmin(dtct, dtut), dtvt), dtwt), dtdivt));
+            // calc_dt
+            const double c = soundspeed[i];
+            double cc = c * c;
+            cc = cc + 2.0 * q / rho;
+            double dtct = ds * cc;
+            dtct = dtc_safe * 1.0 / dmax(sqrt(dtct), g_small);
+            const double vol = m.vol;
+            const double du1 = ugradx1 * m.xa, du2 = ugradx2 * m.xa;
+            const double dtut = dtu_safe * 4.0 * vol / dmax(dmax(fabs(du1), fabs(du2)), 1.0e-5 * vol);
+            const double dv1 = vgrady1 * m.ya, dv2 = vgrady2 * m.ya;
+            const double dtvt = dtv_safe * 4.0 * vol / dmax(dmax(fabs(dv1), fabs(dv2)), 1.0e-5 * vol);
+            const double dw1 = dw1s * m.za, dw2 = dw2s * m.za;
+            const double dtwt = dtw_safe * 4.0 * vol / dmax(dmax(fabs(dw1), fabs(dw2)), 1.0e-5 * vol);
+            const double dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
+            const double dtdivt = dtdiv_safe * 4.0 * (vol) / dmax(vol * 1.0e-05, fabs(dv));
+            dtmin_thread = dmin(dtmin_thread, dmin(dmin(dmin(dmin(dtct, dtut), dtvt), dtwt), dtdivt));
+            ul = uu;
+            vl = vu;
+            wl = wu;
+            p_lo = p_c;
+            p_c = p_hi;
+        }
     }
     __shared__ double red[BX * BY / 64];
     const int tid = threadIdx.y * BX + threadIdx.x;
@@ -131,10 +182,9 @@ hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, doub
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const int bx = (nx + BX - 1) / BX, by = (ny + BY - 1) / BY;
-    int gz = kMaxPartials / (bx * by);
-    if (gz < 1) gz = 1;
-    if (gz > nz) gz = nz;
-    if (bx * by * gz > kMaxPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
+    if (bx * by > kMaxPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
+    int gz = chunks_for(nz);
+    if (bx * by * gz > kMaxPartials) gz = kMaxPartials / (bx * by);
     *nparts = bx * by * gz;
     TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, g, m, dtc_safe,
                  dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,
@@ -149,91 +199,123 @@ hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, doub
 // (PdV.f90:103-107) and the predicted soundspeed is recomputed by the next
 // ideal_gas before calc_dt reads it.  Fortran sums a+b+c+d+a+b+c+d as a left
 // fold, so the 8-term face sums continue from the 4-term partial.
-__global__ void k_fast_pdv_predict(Lay g, Range r, Geo m, double dt, const double *__restrict__ density0,
-                                   const double *__restrict__ energy0, double *__restrict__ pressure,
-                                   const double *__restrict__ viscosity, const double *__restrict__ xvel0,
-                                   const double *__restrict__ yvel0, const double *__restrict__ zvel0)
+__global__ void __launch_bounds__(BX *BY)
+k_fast_pdv_predict(Lay g, Geo m, double dt, const double *__restrict__ density0, const double *__restrict__ energy0,
+                   double *__restrict__ pressure, const double *__restrict__ viscosity,
+                   const double *__restrict__ xvel0, const double *__restrict__ yvel0,
+                   const double *__restrict__ zvel0)
 {
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
-#define U(a, b, c) AT(xvel0, a, b, c)
-#define V(a, b, c) AT(yvel0, a, b, c)
-#define W(a, b, c) AT(zvel0, a, b, c)
-    const double sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
-    const double sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
-    const double sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
-    const double stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
-    const double sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
-    const double sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    if (j > g.x_max || k > g.y_max) return;
+    int l0, l1;
+    chunk_planes(g.z_min, g.z_max, l0, l1);
+    if (l0 > l1) return;
+    int i = idx(g, j, k, l0);
+    Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
+    for (int l = l0; l <= l1; l++, i += g.plane) {
+        const Quad4 uu = quad_up(xvel0, i + g.plane, g.ld), vu = quad_up(yvel0, i + g.plane, g.ld),
+                    wu = quad_up(zvel0, i + g.plane, g.ld);
+#define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
+#define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
+#define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
+        const double sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+        const double sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+        const double sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+        const double stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+        const double sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+        const double sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
 #undef U
 #undef V
 #undef W
-    const double left_flux = (m.xa * sl) * 0.125 * dt * 0.5;
-    const double right_flux = (m.xa * sr) * 0.125 * dt * 0.5;
-    const double bottom_flux = (m.ya * sb) * 0.125 * dt * 0.5;
-    const double top_flux = (m.ya * stp) * 0.125 * dt * 0.5;
-    const double back_flux = (m.za * sk) * 0.125 * dt * 0.5;
-    const double front_flux = (m.za * sf) * 0.125 * dt * 0.5;
-    const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
-    const double vol = m.vol;
-    const double vol_change_s = vol / (vol + total_flux);
-    const double recip_volume = 1.0 / vol;
-    const double d0 = density0[i];
-    const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
-    const double e1 = energy0[i] - energy_change;
-    const double d1 = d0 * vol_change_s;
-    pressure[i] = (1.4 - 1.0) * d1 * e1;
+        const double left_flux = (m.xa * sl) * 0.125 * dt * 0.5;
+        const double right_flux = (m.xa * sr) * 0.125 * dt * 0.5;
+        const double bottom_flux = (m.ya * sb) * 0.125 * dt * 0.5;
+        const double top_flux = (m.ya * stp) * 0.125 * dt * 0.5;
+        const double back_flux = (m.za * sk) * 0.125 * dt * 0.5;
+        const double front_flux = (m.za * sf) * 0.125 * dt * 0.5;
+        const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+        const double vol = m.vol;
+        const double vol_change_s = vol / (vol + total_flux);
+        const double recip_volume = 1.0 / vol;
+        const double d0 = density0[i];
+        const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
+        const double e1 = energy0[i] - energy_change;
+        const double d1 = d0 * vol_change_s;
+        pressure[i] = (1.4 - 1.0) * d1 * e1;
+        ul = uu;
+        vl = vu;
+        wl = wu;
+    }
 }
 
 hipError_t launch_fast_pdv_predict(const Lay &g, const Geo &m, double dt, const double *density0,
                                    const double *energy0, double *pressure, const double *viscosity,
                                    const double *xvel0, const double *yvel0, const double *zvel0, hipStream_t st)
 {
-    Range r{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
-    LAUNCH(k_fast_pdv_predict, r, st, g, r, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
+    const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
+    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
+    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
+                 energy0, pressure, viscosity, xvel0, yvel0, zvel0);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// accelerate_kernel.f90:61-109 with scalar areas and volume
-__global__ void k_fast_accelerate(Lay g, Range r, Geo m, double dt, const double *__restrict__ density0,
-                                  const double *__restrict__ pressure, const double *__restrict__ viscosity,
-                                  const double *__restrict__ xvel0, const double *__restrict__ yvel0,
-                                  const double *__restrict__ zvel0, double *__restrict__ xvel1,
-                                  double *__restrict__ yvel1, double *__restrict__ zvel1)
+// accelerate_kernel.f90:61-109 with scalar areas and volume.  Node (j,k,l)
+// reads the cells (j-1..j, k-1..k, l-1..l); the march keeps plane l-1.
+__global__ void __launch_bounds__(BX *BY)
+k_fast_accelerate(Lay g, Geo m, double dt, const double *__restrict__ density0, const double *__restrict__ pressure,
+                  const double *__restrict__ viscosity, const double *__restrict__ xvel0,
+                  const double *__restrict__ yvel0, const double *__restrict__ zvel0, double *__restrict__ xvel1,
+                  double *__restrict__ yvel1, double *__restrict__ zvel1)
 {
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
+    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    if (j > g.x_max + 1 || k > g.y_max + 1) return;
+    int l0, l1;
+    chunk_planes(g.z_min, g.z_max + 1, l0, l1);
+    if (l0 > l1) return;
+    int i = idx(g, j, k, l0);
+    Quad4 dl = quad_down(density0, i - g.plane, g.ld), pl = quad_down(pressure, i - g.plane, g.ld),
+          ql = quad_down(viscosity, i - g.plane, g.ld);
     const double vol = m.vol;
-#define RV(a, b, c) (AT(density0, a, b, c) * vol)
-    const double nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
-                               RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
-                              0.125;
-#undef RV
-    const double stepbymass_s = 0.25 * dt / nodal_mass;
     const double xa = m.xa, ya = m.ya, za = m.za;
-#define P(a, b, c) AT(pressure, a, b, c)
-#define Q(a, b, c) AT(viscosity, a, b, c)
-    const double xvel1_s = AT(xvel0, 0, 0, 0) - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
-                                                                xa * (P(0, -1, 0) - P(-1, -1, 0)) +
-                                                                xa * (P(0, 0, -1) - P(-1, 0, -1)) +
-                                                                xa * (P(0, -1, -1) - P(-1, -1, -1)));
-    const double yvel1_s = AT(yvel0, 0, 0, 0) - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
-                                                                ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
-                                                                ya * (P(0, 0, -1) - P(0, -1, -1)) +
-                                                                ya * (P(-1, 0, -1) - P(-1, -1, -1)));
-    const double zvel1_s = AT(zvel0, 0, 0, 0) - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
-                                                                za * (P(0, -1, 0) - P(0, -1, -1)) +
-                                                                za * (P(-1, 0, 0) - P(-1, 0, -1)) +
-                                                                za * (P(-1, -1, 0) - P(-1, -1, -1)));
-    xvel1[i] = xvel1_s - stepbymass_s * (xa * (Q(0, 0, 0) - Q(-1, 0, 0)) + xa * (Q(0, -1, 0) - Q(-1, -1, 0)) +
-                                         xa * (Q(0, 0, -1) - Q(-1, 0, -1)) + xa * (Q(0, -1, -1) - Q(-1, -1, -1)));
-    yvel1[i] = yvel1_s - stepbymass_s * (ya * (Q(0, 0, 0) - Q(0, -1, 0)) + ya * (Q(-1, 0, 0) - Q(-1, -1, 0)) +
-                                         ya * (Q(0, 0, -1) - Q(0, -1, -1)) + ya * (Q(-1, 0, -1) - Q(-1, -1, -1)));
-    zvel1[i] = zvel1_s - stepbymass_s * (za * (Q(0, 0, 0) - Q(0, 0, -1)) + za * (Q(0, -1, 0) - Q(0, -1, -1)) +
-                                         za * (Q(-1, 0, 0) - Q(-1, 0, -1)) + za * (Q(-1, -1, 0) - Q(-1, -1, -1)));
+    for (int l = l0; l <= l1; l++, i += g.plane) {
+        const Quad4 du = quad_down(density0, i, g.ld), pu = quad_down(pressure, i, g.ld),
+                    qu = quad_down(viscosity, i, g.ld);
+        // (a, b, c) in {-1, 0}^3; c = 0 is plane l, c = -1 plane l-1
+#define RV(a, b, c) (((c) ? dl : du).v[((a) + 1) + 2 * ((b) + 1)] * vol)
+#define P(a, b, c) ((c) ? pl : pu).v[((a) + 1) + 2 * ((b) + 1)]
+#define Q(a, b, c) ((c) ? ql : qu).v[((a) + 1) + 2 * ((b) + 1)]
+        const double nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
+                                   RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
+                                  0.125;
+        const double stepbymass_s = 0.25 * dt / nodal_mass;
+        const double xvel1_s = xvel0[i] - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
+                                                          xa * (P(0, -1, 0) - P(-1, -1, 0)) +
+                                                          xa * (P(0, 0, -1) - P(-1, 0, -1)) +
+                                                          xa * (P(0, -1, -1) - P(-1, -1, -1)));
+        const double yvel1_s = yvel0[i] - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
+                                                          ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
+                                                          ya * (P(0, 0, -1) - P(0, -1, -1)) +
+                                                          ya * (P(-1, 0, -1) - P(-1, -1, -1)));
+        const double zvel1_s = zvel0[i] - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
+                                                          za * (P(0, -1, 0) - P(0, -1, -1)) +
+                                                          za * (P(-1, 0, 0) - P(-1, 0, -1)) +
+                                                          za * (P(-1, -1, 0) - P(-1, -1, -1)));
+        xvel1[i] = xvel1_s - stepbymass_s * (xa * (Q(0, 0, 0) - Q(-1, 0, 0)) + xa * (Q(0, -1, 0) - Q(-1, -1, 0)) +
+                                             xa * (Q(0, 0, -1) - Q(-1, 0, -1)) + xa * (Q(0, -1, -1) - Q(-1, -1, -1)));
+        yvel1[i] = yvel1_s - stepbymass_s * (ya * (Q(0, 0, 0) - Q(0, -1, 0)) + ya * (Q(-1, 0, 0) - Q(-1, -1, 0)) +
+                                             ya * (Q(0, 0, -1) - Q(0, -1, -1)) + ya * (Q(-1, 0, -1) - Q(-1, -1, -1)));
+        zvel1[i] = zvel1_s - stepbymass_s * (za * (Q(0, 0, 0) - Q(0, 0, -1)) + za * (Q(0, -1, 0) - Q(0, -1, -1)) +
+                                             za * (Q(-1, 0, 0) - Q(-1, 0, -1)) + za * (Q(-1, -1, 0) - Q(-1, -1, -1)));
+#undef RV
 #undef P
 #undef Q
+        dl = du;
+        pl = pu;
+        ql = qu;
+    }
 }
 
 hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const double *density0,
@@ -241,9 +323,10 @@ hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const d
                                   const double *yvel0, const double *zvel0, double *xvel1, double *yvel1,
                                   double *zvel1, hipStream_t st)
 {
-    Range r{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
-    LAUNCH(k_fast_accelerate, r, st, g, r, m, dt, density0, pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1,
-           zvel1);
+    const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
+    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
+    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
+                 pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
     return hipGetLastError();
 }
 
@@ -253,63 +336,92 @@ hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const d
 // covers both.  The corrector's left/bottom face sums equal flux_calc's x/y
 // sums term for term; the z sums differ (flux_calc_kernel.f90:86 repeats
 // zvel0(j+1,k,l)) and are formed separately.
-__global__ void k_fast_pdv_correct_flux(Lay g, Range r, Geo m, double dt, const double *__restrict__ density0,
-                                        double *__restrict__ density1, const double *__restrict__ energy0,
-                                        double *__restrict__ energy1, const double *__restrict__ pressure,
-                                        const double *__restrict__ viscosity, const double *__restrict__ xvel0,
-                                        const double *__restrict__ xvel1, const double *__restrict__ yvel0,
-                                        const double *__restrict__ yvel1, const double *__restrict__ zvel0,
-                                        const double *__restrict__ zvel1, double *__restrict__ vol_flux_x,
-                                        double *__restrict__ vol_flux_y, double *__restrict__ vol_flux_z)
+__global__ void __launch_bounds__(BX *BY)
+k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const double *__restrict__ density0, double *__restrict__ density1,
+                        const double *__restrict__ energy0, double *__restrict__ energy1,
+                        const double *__restrict__ pressure, const double *__restrict__ viscosity,
+                        const double *__restrict__ xvel0, const double *__restrict__ xvel1,
+                        const double *__restrict__ yvel0, const double *__restrict__ yvel1,
+                        const double *__restrict__ zvel0, const double *__restrict__ zvel1,
+                        double *__restrict__ vol_flux_x, double *__restrict__ vol_flux_y,
+                        double *__restrict__ vol_flux_z)
 {
-    RANGE_THREAD(r);
-    const int i = idx(g, j, k, l);
-    const bool jin = j <= g.x_max, kin = k <= g.y_max, lin = l <= g.z_max;
-#define U0(a, b, c) AT(xvel0, a, b, c)
-#define U1(a, b, c) AT(xvel1, a, b, c)
-#define V0(a, b, c) AT(yvel0, a, b, c)
-#define V1(a, b, c) AT(yvel1, a, b, c)
-#define W0(a, b, c) AT(zvel0, a, b, c)
-#define W1(a, b, c) AT(zvel1, a, b, c)
+    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    if (j > g.x_max + 1 || k > g.y_max + 1) return;
+    int l0, l1;
+    chunk_planes(g.z_min, g.z_max + 1, l0, l1);
+    if (l0 > l1) return;
+    const bool jin = j <= g.x_max, kin = k <= g.y_max;
+    int i = idx(g, j, k, l0);
+    Quad4 u0l = quad_up(xvel0, i, g.ld), u1l = quad_up(xvel1, i, g.ld), v0l = quad_up(yvel0, i, g.ld),
+          v1l = quad_up(yvel1, i, g.ld), w0l = quad_up(zvel0, i, g.ld), w1l = quad_up(zvel1, i, g.ld);
     const double c0 = 0.125 * dt;
-    double sl = 0.0, sb = 0.0;
-    if (kin && lin) {
-        sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
-        vol_flux_x[i] = c0 * m.xa * sl;
-    }
-    if (jin && lin) {
-        sb = V0(0, 0, 0) + V0(1, 0, 0) + V0(0, 0, 1) + V0(1, 0, 1) + V1(0, 0, 0) + V1(1, 0, 0) + V1(0, 0, 1) + V1(1, 0, 1);
-        vol_flux_y[i] = c0 * m.ya * sb;
-    }
-    if (jin && kin)
-        vol_flux_z[i] = c0 * m.za *
-                        (W0(0, 0, 0) + W0(1, 0, 0) + W0(1, 0, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) +
-                         W1(1, 1, 0));
-    if (!(jin && kin && lin)) return;
-    const double sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
-    const double stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
-    const double sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
-    const double sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
+    for (int l = l0; l <= l1; l++, i += g.plane) {
+        const bool lin = l <= g.z_max;
+        // the plane above is needed only by cells and x/y faces (l <= z_max)
+        Quad4 u0u = u0l, u1u = u1l, v0u = v0l, v1u = v1l, w0u = w0l, w1u = w1l;
+        if (lin) {
+            const int iu = i + g.plane;
+            u0u = quad_up(xvel0, iu, g.ld);
+            u1u = quad_up(xvel1, iu, g.ld);
+            v0u = quad_up(yvel0, iu, g.ld);
+            v1u = quad_up(yvel1, iu, g.ld);
+            w0u = quad_up(zvel0, iu, g.ld);
+            w1u = quad_up(zvel1, iu, g.ld);
+        }
+#define U0(a, b, c) ((c) ? u0u : u0l).v[(a) + 2 * (b)]
+#define U1(a, b, c) ((c) ? u1u : u1l).v[(a) + 2 * (b)]
+#define V0(a, b, c) ((c) ? v0u : v0l).v[(a) + 2 * (b)]
+#define V1(a, b, c) ((c) ? v1u : v1l).v[(a) + 2 * (b)]
+#define W0(a, b, c) ((c) ? w0u : w0l).v[(a) + 2 * (b)]
+#define W1(a, b, c) ((c) ? w1u : w1l).v[(a) + 2 * (b)]
+        double sl = 0.0, sb = 0.0;
+        if (kin && lin) {
+            sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
+            vol_flux_x[i] = c0 * m.xa * sl;
+        }
+        if (jin && lin) {
+            sb = V0(0, 0, 0) + V0(1, 0, 0) + V0(0, 0, 1) + V0(1, 0, 1) + V1(0, 0, 0) + V1(1, 0, 0) + V1(0, 0, 1) + V1(1, 0, 1);
+            vol_flux_y[i] = c0 * m.ya * sb;
+        }
+        if (jin && kin)
+            vol_flux_z[i] = c0 * m.za *
+                            (W0(0, 0, 0) + W0(1, 0, 0) + W0(1, 0, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) +
+                             W1(1, 1, 0));
+        if (jin && kin && lin) {
+            const double sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
+            const double stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
+            const double sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
+            const double sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
+            const double left_flux = (m.xa * sl) * 0.125 * dt;
+            const double right_flux = (m.xa * sr) * 0.125 * dt;
+            const double bottom_flux = (m.ya * sb) * 0.125 * dt;
+            const double top_flux = (m.ya * stp) * 0.125 * dt;
+            const double back_flux = (m.za * sk) * 0.125 * dt;
+            const double front_flux = (m.za * sf) * 0.125 * dt;
+            const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+            const double vol = m.vol;
+            const double vol_change_s = vol / (vol + total_flux);
+            const double recip_volume = 1.0 / vol;
+            const double d0 = density0[i];
+            const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
+            energy1[i] = energy0[i] - energy_change;
+            density1[i] = d0 * vol_change_s;
+        }
 #undef U0
 #undef U1
 #undef V0
 #undef V1
 #undef W0
 #undef W1
-    const double left_flux = (m.xa * sl) * 0.125 * dt;
-    const double right_flux = (m.xa * sr) * 0.125 * dt;
-    const double bottom_flux = (m.ya * sb) * 0.125 * dt;
-    const double top_flux = (m.ya * stp) * 0.125 * dt;
-    const double back_flux = (m.za * sk) * 0.125 * dt;
-    const double front_flux = (m.za * sf) * 0.125 * dt;
-    const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
-    const double vol = m.vol;
-    const double vol_change_s = vol / (vol + total_flux);
-    const double recip_volume = 1.0 / vol;
-    const double d0 = density0[i];
-    const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
-    energy1[i] = energy0[i] - energy_change;
-    density1[i] = d0 * vol_change_s;
+        u0l = u0u;
+        u1l = u1u;
+        v0l = v0u;
+        v1l = v1u;
+        w0l = w0u;
+        w1l = w1u;
+    }
 }
 
 hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, const double *density0,
@@ -319,9 +431,11 @@ hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, c
                                         const double *zvel0, const double *zvel1, double *vol_flux_x,
                                         double *vol_flux_y, double *vol_flux_z, hipStream_t st)
 {
-    Range r{g.x_min, g.x_max + 1, g.y_min, g.y_max + 1, g.z_min, g.z_max + 1};
-    LAUNCH(k_fast_pdv_correct_flux, r, st, g, r, m, dt, density0, density1, energy0, energy1, pressure, viscosity,
-           xvel0, xvel1, yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
+    const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
+    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
+    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux, grid, dim3(BX, BY, 1), 0, st, g, m, dt,
+                 density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1,
+                 vol_flux_x, vol_flux_y, vol_flux_z);
     return hipGetLastError();
 }
 
