@@ -58,7 +58,7 @@ ARRAYS_PER_CELL = {
     "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accelerate": 9, "k_fast_pdv_correct_flux": 15,
     "k_fast_cell_x": 7, "k_fast_cell_y": 7, "k_fast_cell_z": 7,
     "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
-    "k_fast_field_summary": 6,
+    "k_fast_field_summary": 6, "k_fast_ideal_gas": 4,
 }
 
 
@@ -73,14 +73,14 @@ def kernel_domain_cells(name: str, n: int) -> int:
     return n ** 3
 
 
-def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25):
+def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, fp32: bool = False):
     from cloverleaf3d_ref_amd import _lib
     from cloverleaf3d_ref_amd.chunk import Chunk
     from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
 
     deck = parse_deck(cuboid_deck(cells, warmup + steps + 1000, summary_frequency))
     t0 = time.perf_counter()
-    c = Chunk(deck)
+    c = Chunk(deck, fp32=fp32)
     setup_s = time.perf_counter() - t0
     L = _lib.lib()
     for _ in range(warmup):
@@ -96,9 +96,26 @@ def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25):
     for line in buf.value.decode().splitlines():
         name, ms, launches = line.split()
         ktimes[name] = (float(ms), int(launches))
-    info = dict(setup_s=setup_s, device_bytes=c.device_bytes(), rows=rows, state=c.state())
+    final = c.field_summary()  # after the timed region
+    info = dict(setup_s=setup_s, device_bytes=c.device_bytes(), rows=rows, state=c.state(), final=final)
     c.close()
     return wall, ktimes, info
+
+
+def dominant_roofline(ktimes, n: int, bytes_per_value: int, traffic_file=None):
+    """The kernel with the most device time: algorithmic bytes per launch
+    over its average launch time (HIP events on the chunk's stream)."""
+    name, (ms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
+    per_launch_bytes = ARRAYS_PER_CELL.get(name, 0) * bytes_per_value * kernel_domain_cells(name, n)
+    avg_s = ms / launches / 1e3
+    achieved = per_launch_bytes / avg_s / 1e9 if per_launch_bytes else None
+    traffic = None
+    if traffic_file is not None and traffic_file.exists():
+        measured = json.loads(traffic_file.read_text())["kernels"].get(name)
+        traffic = measured["hbm_bytes"] if measured else None
+    return dict(bound="hbm", kernel=name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=traffic,
+                avg_launch_us=avg_s * 1e6, launches=launches, bytes_per_launch=per_launch_bytes)
 
 
 def calibration_copy(mib: int) -> None:
@@ -152,6 +169,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-store run")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -166,20 +184,21 @@ def main():
     wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup)
     n = args.cells
     value = n ** 3 * args.steps / wall
+    roofline = dominant_roofline(ktimes, n, 8, TRAFFIC_FILE if n == 512 else None)
 
-    # dominant kernel by device time
-    name, (ms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
-    per_launch_bytes = ARRAYS_PER_CELL.get(name, 0) * 8 * kernel_domain_cells(name, n)
-    avg_s = ms / launches / 1e3
-    achieved = per_launch_bytes / avg_s / 1e9 if per_launch_bytes else None
-    traffic = None
-    if n == 512 and TRAFFIC_FILE.exists():
-        measured = json.loads(TRAFFIC_FILE.read_text())["kernels"].get(name)
-        traffic = measured["hbm_bytes"] if measured else None
-    roofline = dict(bound="hbm", kernel=name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=traffic,
-                    avg_launch_us=avg_s * 1e6, launches=launches,
-                    bytes_per_launch=per_launch_bytes)
+    fp32 = None
+    if not args.no_fp32:
+        wall32, ktimes32, info32 = gpu_bench(args.cells, args.steps, args.warmup, fp32=True)
+        value32 = n ** 3 * args.steps / wall32
+        f64, f32 = info["final"], info32["final"]
+        fp32 = dict(value=value32, unit="cell-steps/s", ms_per_step=wall32 / args.steps * 1e3,
+                    speedup_vs_fp64=value32 / value, dtype="f32 storage, f64 arithmetic",
+                    rel_err_vs_fp64={k: abs(f32[k] / f64[k] - 1) for k in ("vol", "mass", "ie", "ke", "press")},
+                    after_steps=f32["step"],
+                    roofline=dominant_roofline(ktimes32, n, 4),
+                    kernel_ms_per_step={k: round(v[0] / args.steps, 4) for k, v in
+                                        sorted(ktimes32.items(), key=lambda kv: -kv[1][0])},
+                    device_gib=info32["device_bytes"] / 2 ** 30)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -204,6 +223,7 @@ def main():
                    "cells": n ** 3, "form": "fp64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp32_store": fp32,
         "step_hbm_gbs_kbc": 1192 * value / 1e9,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
                                sorted(ktimes.items(), key=lambda kv: -kv[1][0])},

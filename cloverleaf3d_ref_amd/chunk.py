@@ -52,6 +52,7 @@ class SmallTimestep(RuntimeError):
 
 
 PER_CALL = 1  # CLOVER_CHUNK_PER_CALL
+FP32 = 2  # CLOVER_CHUNK_FP32
 
 
 class Chunk:
@@ -59,16 +60,20 @@ class Chunk:
     sequence (mesh, states, pressure, halo prime).
 
     per_call=False (default) runs the fused kernels; per_call=True runs one
-    kernel per reference loop nest.  Both produce bit-identical fields."""
+    kernel per reference loop nest.  Both produce bit-identical fields.
+    fp32=True holds the fields in FP32 (the FP32-store form, fused kernels
+    only); fetch() still returns FP64 arrays."""
 
-    def __init__(self, deck: Deck, device: int = 0, per_call: bool = False):
+    def __init__(self, deck: Deck, device: int = 0, per_call: bool = False, fp32: bool = False):
         self.deck = deck
         self.per_call = per_call
+        self.fp32 = fp32
         self._L = _lib.lib()
         self._cdeck = to_c_deck(deck)
         h = ctypes.c_void_p()
-        _lib.check(self._L.clover_chunk_create_ex(ctypes.byref(self._cdeck), device, PER_CALL if per_call else 0,
-                                                  ctypes.byref(h)), "clover_chunk_create_ex")
+        flags = (PER_CALL if per_call else 0) | (FP32 if fp32 else 0)
+        _lib.check(self._L.clover_chunk_create_ex(ctypes.byref(self._cdeck), device, flags, ctypes.byref(h)),
+                   "clover_chunk_create_ex")
         self._h = h
 
     def close(self) -> None:

@@ -32,34 +32,44 @@ int64_t dense_elems(const Lay &g, int cls)
     return (int64_t)ex * ey * ez;
 }
 
-void upload_dense(const Lay &g, int cls, const double *host, double *dev, hipStream_t st)
+void upload_dense_any(const Lay &g, int cls, const void *host, void *dev, size_t es, hipStream_t st)
 {
     int ex, ey, ez;
     class_extents(cls, g.x_max - g.x_min + 1, g.y_max - g.y_min + 1, g.z_max - g.z_min + 1, ex, ey, ez);
     if (class_is_1d(cls)) {
-        hip_check(hipMemcpyAsync(dev, host, sizeof(double) * ex, hipMemcpyHostToDevice, st), "upload 1-D");
+        hip_check(hipMemcpyAsync(dev, host, es * ex, hipMemcpyHostToDevice, st), "upload 1-D");
         return;
     }
     for (int l = 0; l < ez; l++)
-        hip_check(hipMemcpy2DAsync(dev + g.base + (size_t)g.plane * l, sizeof(double) * g.ld,
-                                   host + (size_t)ex * ey * l, sizeof(double) * ex, sizeof(double) * ex, ey,
+        hip_check(hipMemcpy2DAsync((char *)dev + es * (g.base + (size_t)g.plane * l), es * g.ld,
+                                   (const char *)host + es * ((size_t)ex * ey * l), es * ex, es * ex, ey,
                                    hipMemcpyHostToDevice, st),
                   "upload plane");
 }
 
-void download_dense(const Lay &g, int cls, const double *dev, double *host, hipStream_t st)
+void download_dense_any(const Lay &g, int cls, const void *dev, void *host, size_t es, hipStream_t st)
 {
     int ex, ey, ez;
     class_extents(cls, g.x_max - g.x_min + 1, g.y_max - g.y_min + 1, g.z_max - g.z_min + 1, ex, ey, ez);
     if (class_is_1d(cls)) {
-        hip_check(hipMemcpyAsync(host, dev, sizeof(double) * ex, hipMemcpyDeviceToHost, st), "download 1-D");
+        hip_check(hipMemcpyAsync(host, dev, es * ex, hipMemcpyDeviceToHost, st), "download 1-D");
         return;
     }
     for (int l = 0; l < ez; l++)
-        hip_check(hipMemcpy2DAsync(host + (size_t)ex * ey * l, sizeof(double) * ex,
-                                   dev + g.base + (size_t)g.plane * l, sizeof(double) * g.ld, sizeof(double) * ex,
-                                   ey, hipMemcpyDeviceToHost, st),
+        hip_check(hipMemcpy2DAsync((char *)host + es * ((size_t)ex * ey * l), es * ex,
+                                   (const char *)dev + es * (g.base + (size_t)g.plane * l), es * g.ld, es * ex, ey,
+                                   hipMemcpyDeviceToHost, st),
                   "download plane");
+}
+
+void upload_dense(const Lay &g, int cls, const double *host, double *dev, hipStream_t st)
+{
+    upload_dense_any(g, cls, host, dev, sizeof(double), st);
+}
+
+void download_dense(const Lay &g, int cls, const double *dev, double *host, hipStream_t st)
+{
+    download_dense_any(g, cls, dev, host, sizeof(double), st);
 }
 
 namespace {

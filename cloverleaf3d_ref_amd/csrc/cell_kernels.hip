@@ -28,17 +28,20 @@ constexpr int CCHUNK = 64; // faces per march chunk
 // pre_vol (advec_cell_kernel.f90:97-116, 193-210, 289-306) for `mode`:
 // 1 x sweep 1, 2 x sweep 3, 3 y sweep with advect_x, 4 y without,
 // 5 z sweep 1, 6 z sweep 3
-__device__ inline double cell_pre_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
-                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
+template <typename S>
+__device__ inline double cell_pre_vol(int mode, const Lay &g, int i, double V0, const S *__restrict__ vfx,
+                                      const S *__restrict__ vfy, const S *__restrict__ vfz)
 {
+#define D(x) ((double)(x))
     switch (mode) {
     case 1:
-    case 5: return V0 + (vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i]);
-    case 2: return V0 + vfx[i + 1] - vfx[i];
-    case 3: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
-    case 4: return V0 + vfy[i + g.ld] - vfy[i] + vfx[i + 1] - vfx[i];
-    default: return V0 + vfz[i + g.plane] - vfz[i];
+    case 5: return V0 + (D(vfx[i + 1]) - D(vfx[i]) + D(vfy[i + g.ld]) - D(vfy[i]) + D(vfz[i + g.plane]) - D(vfz[i]));
+    case 2: return V0 + D(vfx[i + 1]) - D(vfx[i]);
+    case 3: return V0 + D(vfy[i + g.ld]) - D(vfy[i]) + D(vfz[i + g.plane]) - D(vfz[i]);
+    case 4: return V0 + D(vfy[i + g.ld]) - D(vfy[i]) + D(vfx[i + 1]) - D(vfx[i]);
+    default: return V0 + D(vfz[i + g.plane]) - D(vfz[i]);
     }
+#undef D
 }
 
 __device__ inline double cell_limiter(double sigma, double sigma3, double sigma4, double diffuw, double diffdw)
@@ -83,11 +86,12 @@ __device__ inline double shfl_c(double x, int src) { return __shfl(x, src, 64); 
 // ---------------------------------------------------------------------------
 // x sweep: faces j in [x_min, x_max+2], rows k, l interior.  Lane i holds
 // cell/face j = jt - 2 + i.
+template <typename S>
 __global__ void __launch_bounds__(64 * CROWS)
-k_fast_cell_x(Lay g, Geo m, int mode, const double *__restrict__ vfx, const double *__restrict__ vfy,
-              const double *__restrict__ vfz, const double *__restrict__ density1,
-              const double *__restrict__ energy1, double *__restrict__ mass_flux_x, double *__restrict__ rho_out,
-              double *__restrict__ e_out)
+k_fast_cell_x(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+              const S *__restrict__ vfz, const S *__restrict__ density1,
+              const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
+              S *__restrict__ e_out)
 {
     const int lane = threadIdx.x;
     const int ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
@@ -130,12 +134,12 @@ k_fast_cell_x(Lay g, Geo m, int mode, const double *__restrict__ vfx, const doub
 // axis, march along the sweep axis over faces [smin, smax+2] in chunks.
 // Faces of the y sweep run over l in [z_min, z_max+2] (advec_cell_kernel.f90:
 // 215-217); the cells updated are interior.
-template <int DIR>
+template <int DIR, typename S>
 __global__ void __launch_bounds__(64 * CROWS)
-k_fast_cell_march(Lay g, Geo m, int mode, const double *__restrict__ vfx, const double *__restrict__ vfy,
-                  const double *__restrict__ vfz, const double *__restrict__ density1,
-                  const double *__restrict__ energy1, double *__restrict__ mass_flux, double *__restrict__ rho_out,
-                  double *__restrict__ e_out)
+k_fast_cell_march(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+                  const S *__restrict__ vfz, const S *__restrict__ density1,
+                  const S *__restrict__ energy1, S *__restrict__ mass_flux, S *__restrict__ rho_out,
+                  S *__restrict__ e_out)
 {
     const int j = g.x_min + (int)(blockIdx.x * 64 + threadIdx.x);
     const int cr = (DIR == 3 ? g.y_min : g.z_min) + (int)(blockIdx.y * CROWS + threadIdx.y);
@@ -145,7 +149,7 @@ k_fast_cell_march(Lay g, Geo m, int mode, const double *__restrict__ vfx, const 
     const int s = DIR == 3 ? g.plane : g.ld;
     const int smin = DIR == 3 ? g.z_min : g.y_min, smax = DIR == 3 ? g.z_max : g.y_max;
     const double vd = DIR == 3 ? m.dz : m.dy;
-    const double *__restrict__ vfs = DIR == 3 ? vfz : vfy;
+    const S *__restrict__ vfs = DIR == 3 ? vfz : vfy;
     const int f0 = smin + (int)blockIdx.z * CCHUNK;
     const int f1 = min(f0 + CCHUNK - 1, smax + 2);
     const int ib = DIR == 3 ? idx(g, j, cr, smin) : idx(g, j, smin, cr);  // offset at sweep position smin
@@ -203,10 +207,34 @@ k_fast_cell_march(Lay g, Geo m, int mode, const double *__restrict__ vfx, const 
 #undef OFF
 }
 
-hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
-                                  const double *density1, const double *energy1, double *mass_flux_x,
-                                  const double *vol_flux_x, double *mass_flux_y, const double *vol_flux_y,
-                                  double *mass_flux_z, const double *vol_flux_z, double *rho_out, double *e_out,
+template <typename S>
+static hipError_t fast_advec_cell_t(const Lay &g, const Geo &m, int mode, int dir, const S *density1,
+                                    const S *energy1, S *mass_flux_x, const S *vol_flux_x, S *mass_flux_y,
+                                    const S *vol_flux_y, S *mass_flux_z, const S *vol_flux_z, S *rho_out, S *e_out,
+                                    hipStream_t st)
+{
+    const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
+    const dim3 blk(64, CROWS, 1);
+    if (dir == 1) {
+        const dim3 grid((nx + 2 + 59) / 60, (ny * nz + CROWS - 1) / CROWS, 1);
+        TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x<S>, grid, blk, 0, st, g, m, mode, vol_flux_x, vol_flux_y,
+                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out);
+    } else if (dir == 2) {
+        const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, (ny + 2 + CCHUNK - 1) / CCHUNK);
+        TIMED_LAUNCH("k_fast_cell_y", st, (k_fast_cell_march<2, S>), grid, blk, 0, st, g, m, mode, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_y, rho_out, e_out);
+    } else {
+        const dim3 grid((nx + 63) / 64, (ny + CROWS - 1) / CROWS, (nz + 2 + CCHUNK - 1) / CCHUNK);
+        TIMED_LAUNCH("k_fast_cell_z", st, (k_fast_cell_march<3, S>), grid, blk, 0, st, g, m, mode, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_z, rho_out, e_out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
+                                  const void *density1, const void *energy1, void *mass_flux_x,
+                                  const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
+                                  void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
                                   hipStream_t st)
 {
     int mode = 0;
@@ -214,22 +242,14 @@ hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int 
     else if (dir == 2) mode = sweep_number == 2 ? (advect_x ? 3 : 4) : 0;
     else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
     if (!mode) return hipErrorInvalidValue;  // the resident driver issues only the reference's sweeps
-    const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
-    const dim3 blk(64, CROWS, 1);
-    if (dir == 1) {
-        const dim3 grid((nx + 2 + 59) / 60, (ny * nz + CROWS - 1) / CROWS, 1);
-        TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x, grid, blk, 0, st, g, m, mode, vol_flux_x, vol_flux_y,
-                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out);
-    } else if (dir == 2) {
-        const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, (ny + 2 + CCHUNK - 1) / CCHUNK);
-        TIMED_LAUNCH("k_fast_cell_y", st, k_fast_cell_march<2>, grid, blk, 0, st, g, m, mode, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_y, rho_out, e_out);
-    } else {
-        const dim3 grid((nx + 63) / 64, (ny + CROWS - 1) / CROWS, (nz + 2 + CCHUNK - 1) / CCHUNK);
-        TIMED_LAUNCH("k_fast_cell_z", st, k_fast_cell_march<3>, grid, blk, 0, st, g, m, mode, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_z, rho_out, e_out);
-    }
-    return hipGetLastError();
+#define CALL(T)                                                                                                 \
+    fast_advec_cell_t<T>(g, m, mode, dir, static_cast<const T *>(density1), static_cast<const T *>(energy1),     \
+                         static_cast<T *>(mass_flux_x), static_cast<const T *>(vol_flux_x),                      \
+                         static_cast<T *>(mass_flux_y), static_cast<const T *>(vol_flux_y),                      \
+                         static_cast<T *>(mass_flux_z), static_cast<const T *>(vol_flux_z),                      \
+                         static_cast<T *>(rho_out), static_cast<T *>(e_out), st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
 }
 
 }  // namespace clover

@@ -146,6 +146,7 @@ struct clover_chunk {
     int prof_calls[P_COUNT];
     EventTimer *ktimer;  // non-null while per-kernel timing is on
     bool fast;           // fused kernels (fast.h) instead of the per-call ones
+    Store store;         // element type of the field arrays (ST_F32 only with the fused kernels)
     Geo geo;             // the uniform mesh, for the fused kernels
 };
 
@@ -331,13 +332,27 @@ static void host_init(const clover_deck &d, const Tile &t, HostInit &h)
     }
 }
 
-static double *dev_alloc(clover_chunk *c, size_t n)
+static double *dev_alloc_bytes(clover_chunk *c, size_t bytes)
 {
     double *p = nullptr;
-    hip_check(hipMalloc(&p, n * sizeof(double)), "hipMalloc");
-    hip_check(hipMemsetAsync(p, 0, n * sizeof(double), c->st), "hipMemset");
-    c->bytes += (long long)(n * sizeof(double));
+    hip_check(hipMalloc(&p, bytes), "hipMalloc");
+    hip_check(hipMemsetAsync(p, 0, bytes, c->st), "hipMemset");
+    c->bytes += (long long)bytes;
     return p;
+}
+
+static double *dev_alloc(clover_chunk *c, size_t n) { return dev_alloc_bytes(c, n * sizeof(double)); }
+
+// a 3-D array of the chunk's element type, filled from a dense FP64 host array
+static void upload_field(clover_chunk *c, const Lay &g, int cls, const std::vector<double> &h, double *dev)
+{
+    if (c->store == ST_F64) {
+        upload_dense(g, cls, h.data(), dev, c->st);
+        return;
+    }
+    std::vector<float> f(h.begin(), h.end());  // round to nearest
+    upload_dense_any(g, cls, f.data(), dev, sizeof(float), c->st);
+    hip_check(hipStreamSynchronize(c->st), "sync");  // f dies here
 }
 
 static double *dev_upload_1d(clover_chunk *c, const std::vector<double> &v)
@@ -369,14 +384,22 @@ static void update_halo(clover_chunk *c, const int *fields, int depth)
                     const int n = tt.nbr[face - 1];
                     if (n == -1) continue;
                     Tile &nt = c->tiles[n - 1];
-                    hip_check(launch_tile_halo(face, tt.g, tt.f, nt.g, nt.f, fields, depth, c->st), "tile_halo");
+                    void *fp[NUM_FIELDS], *np[NUM_FIELDS];
+                    for (int f = 0; f < NUM_FIELDS; f++) {
+                        fp[f] = tt.f[f];
+                        np[f] = nt.f[f];
+                    }
+                    hip_check(launch_tile_halo_store(c->store, face, tt.g, fp, nt.g, np, fields, depth, c->st),
+                              "tile_halo");
                 }
     }
     Phase ph(c, P_SELF_HALO);
     for (auto &t : c->tiles) {
         int ext[6];
         for (int f = 0; f < 6; f++) ext[f] = t.nbr[f] == -1;  // one chunk: every chunk face is external
-        hip_check(launch_update_halo(t.g, t.f, fields, ext, depth, c->st), "update_halo");
+        void *fp[NUM_FIELDS];
+        for (int f = 0; f < NUM_FIELDS; f++) fp[f] = t.f[f];
+        hip_check(launch_update_halo_store(c->store, t.g, fp, fields, ext, depth, c->st), "update_halo");
     }
 }
 
@@ -384,7 +407,11 @@ static void ideal_gas(clover_chunk *c, bool predict)
 {
     Phase ph(c, P_IDEAL_GAS);
     for (auto &t : c->tiles) {
-        if (!predict)
+        if (c->fast && !predict)
+            hip_check(launch_fast_ideal_gas(c->store, t.g, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE],
+                                            t.f[F_SOUNDSPEED], c->st),
+                      "ideal_gas");
+        else if (!predict)
             hip_check(launch_ideal_gas(t.g, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED], c->st),
                       "ideal_gas");
         else
@@ -410,7 +437,7 @@ static int timestep(clover_chunk *c)
         Phase ph(c, P_VISCOSITY);
         for (int ti = 0; ti < T; ti++) {
             Tile &t = c->tiles[ti];
-            hip_check(launch_fast_visc_dt(t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
+            hip_check(launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
                                           c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
                                           t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
                                           c->partial + (size_t)ti * kMaxPartials * 5, &nparts[ti % 64], c->st),
@@ -471,7 +498,7 @@ static void pdv(clover_chunk *c, bool predict)
         {
             Phase ph(c, P_PDV);
             for (auto &t : c->tiles)
-                hip_check(launch_fast_pdv_predict(t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_ENERGY0],
+                hip_check(launch_fast_pdv_predict(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_ENERGY0],
                                                   t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
                                                   t.f[F_ZVEL0], c->st),
                           "PdV predict");
@@ -485,7 +512,7 @@ static void pdv(clover_chunk *c, bool predict)
         // PdV corrector fused with flux_calc, which follows it (hydro.f90:58-60)
         Phase ph(c, P_PDV);
         for (auto &t : c->tiles)
-            hip_check(launch_fast_pdv_correct_flux(t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
+            hip_check(launch_fast_pdv_correct_flux(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
                                                    t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
                                                    t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1],
                                                    t.f[F_ZVEL0], t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y],
@@ -522,7 +549,7 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
             // new density1/energy1 land in spare arrays, which then take their
             // place (interior cells only; the halo update that follows every
             // advec_cell sweep refreshes the halos before anything reads them)
-            hip_check(launch_fast_advec_cell(t.g, c->geo, c->advect_x, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
+            hip_check(launch_fast_advec_cell(c->store, t.g, c->geo, c->advect_x, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
                                              t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y],
                                              t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W1],
                                              t.f[X_W2], c->st),
@@ -548,9 +575,9 @@ static void advec_mom(clover_chunk *c, int sweep, int dir)
             // new velocities land in the spare arrays, which then take the
             // velocities' place (only nodes min..max+1 are new; the halos are
             // refreshed by the next update_halo before anything reads them)
-            const double *vin[3] = {t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1]};
-            double *vout[3] = {t.f[X_W5], t.f[X_W6], t.f[X_W7]};
-            hip_check(launch_fast_advec_mom(t.g, c->geo, c->advect_x, sweep, dir, t.f[F_MASS_FLUX_X],
+            const void *vin[3] = {t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1]};
+            void *vout[3] = {t.f[X_W5], t.f[X_W6], t.f[X_W7]};
+            hip_check(launch_fast_advec_mom(c->store, t.g, c->geo, c->advect_x, sweep, dir, t.f[F_MASS_FLUX_X],
                                             t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y],
                                             t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[F_DENSITY1], vin, vout,
                                             c->st),
@@ -602,7 +629,7 @@ static void field_summary(clover_chunk *c, double *out)
         int np = 0;
         double *part = c->partial + (size_t)ti * kMaxPartials * 5;
         if (c->fast)
-            hip_check(launch_fast_field_summary(t.g, c->geo, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE],
+            hip_check(launch_fast_field_summary(c->store, t.g, c->geo, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[F_PRESSURE],
                                                 t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], part, &np, c->st),
                       "field_summary");
         else
@@ -627,7 +654,7 @@ static void do_step(clover_chunk *c)
         Phase ph(c, P_ACCEL);
         for (auto &t : c->tiles) {
             if (c->fast)
-                hip_check(launch_fast_accelerate(t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_PRESSURE],
+                hip_check(launch_fast_accelerate(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_PRESSURE],
                                                  t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
                                                  t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->st),
                           "accelerate");
@@ -712,11 +739,14 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         if (d.x_cells < 1 || d.y_cells < 1 || d.z_cells < 1) throw Error(E_ARG, "cells must be positive");
         if (d.number_of_states < 1 || d.number_of_states > CLOVER_MAX_STATES) throw Error(E_ARG, "bad state count");
         if (d.tiles_per_chunk < 1) throw Error(E_ARG, "tiles_per_chunk must be positive");
-        if (flags & ~CLOVER_CHUNK_PER_CALL) throw Error(E_ARG, "unknown flags");
+        if (flags & ~(CLOVER_CHUNK_PER_CALL | CLOVER_CHUNK_FP32)) throw Error(E_ARG, "unknown flags");
+        if ((flags & CLOVER_CHUNK_PER_CALL) && (flags & CLOVER_CHUNK_FP32))
+            throw Error(E_ARG, "the FP32-store form runs the fused kernels only");
         c = new clover_chunk();  // value-initialised: every scalar member starts at zero
         c->deck = d;
         c->device = device;
         c->fast = !(flags & CLOVER_CHUNK_PER_CALL);
+        c->store = (flags & CLOVER_CHUNK_FP32) ? ST_F32 : ST_F64;
         // the uniform mesh as initialise_chunk.f90:37-39 and
         // initialise_chunk_kernel.f90:151-178 compute it
         c->geo.dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
@@ -741,18 +771,18 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
             t.g = make_lay(1, t.nx, 1, t.ny, 1, t.nz);
             const int64_t n = lay_elems(t.g);
             if (n >= (int64_t(1) << 31)) throw Error(E_ARG, "tile too large for 32-bit element offsets");
-            for (int a = 0; a < NUM_ARRAYS; a++) t.f[a] = dev_alloc(c, (size_t)n);
+            for (int a = 0; a < NUM_ARRAYS; a++) t.f[a] = dev_alloc_bytes(c, (size_t)n * store_bytes(c->store));
             HostInit h;
             host_init(d, t, h);
-            upload_dense(t.g, AC_CELL, h.volume.data(), t.f[X_VOLUME], c->st);
-            upload_dense(t.g, AC_XF, h.xarea.data(), t.f[X_XAREA], c->st);
-            upload_dense(t.g, AC_YF, h.yarea.data(), t.f[X_YAREA], c->st);
-            upload_dense(t.g, AC_ZF, h.zarea.data(), t.f[X_ZAREA], c->st);
-            upload_dense(t.g, AC_CELL, h.density0.data(), t.f[F_DENSITY0], c->st);
-            upload_dense(t.g, AC_CELL, h.energy0.data(), t.f[F_ENERGY0], c->st);
-            upload_dense(t.g, AC_VERT, h.xvel0.data(), t.f[F_XVEL0], c->st);
-            upload_dense(t.g, AC_VERT, h.yvel0.data(), t.f[F_YVEL0], c->st);
-            upload_dense(t.g, AC_VERT, h.zvel0.data(), t.f[F_ZVEL0], c->st);
+            upload_field(c, t.g, AC_CELL, h.volume, t.f[X_VOLUME]);
+            upload_field(c, t.g, AC_XF, h.xarea, t.f[X_XAREA]);
+            upload_field(c, t.g, AC_YF, h.yarea, t.f[X_YAREA]);
+            upload_field(c, t.g, AC_ZF, h.zarea, t.f[X_ZAREA]);
+            upload_field(c, t.g, AC_CELL, h.density0, t.f[F_DENSITY0]);
+            upload_field(c, t.g, AC_CELL, h.energy0, t.f[F_ENERGY0]);
+            upload_field(c, t.g, AC_VERT, h.xvel0, t.f[F_XVEL0]);
+            upload_field(c, t.g, AC_VERT, h.yvel0, t.f[F_YVEL0]);
+            upload_field(c, t.g, AC_VERT, h.zvel0, t.f[F_ZVEL0]);
             t.cellx = dev_upload_1d(c, h.cellx);
             t.celly = dev_upload_1d(c, h.celly);
             t.cellz = dev_upload_1d(c, h.cellz);
@@ -895,8 +925,15 @@ int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host)
     if (!c || !host || t < 0 || t >= (int)c->tiles.size()) throw Error(E_ARG, "bad argument");
     if (field < 0 || field > X_ZAREA) throw Error(E_ARG, "bad field id");
     Tile &tt = c->tiles[t];
-    download_dense(tt.g, class_of(field), tt.f[field], host, c->st);
-    hip_check(hipStreamSynchronize(c->st), "sync");
+    if (c->store == ST_F64) {
+        download_dense(tt.g, class_of(field), tt.f[field], host, c->st);
+        hip_check(hipStreamSynchronize(c->st), "sync");
+    } else {
+        std::vector<float> f((size_t)dense_elems(tt.g, class_of(field)));
+        download_dense_any(tt.g, class_of(field), tt.f[field], f.data(), sizeof(float), c->st);
+        hip_check(hipStreamSynchronize(c->st), "sync");
+        for (size_t q = 0; q < f.size(); q++) host[q] = f[q];
+    }
     CLOVER_API_END
 }
 

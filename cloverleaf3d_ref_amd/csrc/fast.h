@@ -12,6 +12,13 @@
 //    anything reads them, PdV.f90:103-107 + hydro.f90:58; the predictor's
 //    soundspeed is recomputed before calc_dt reads it);
 //  * intermediates are recomputed where that is cheaper than storing them.
+//
+// Storage.  Each launcher takes the element type of the field arrays:
+// ST_F64 (the drop-in form) or ST_F32 (the FP32-store form: fields held in
+// FP32, every value converted to FP64 when loaded and rounded to FP32 when
+// stored, so all arithmetic, calc_dt's minimum and the advection's mass and
+// energy accumulation run in FP64).  Array arguments are untyped pointers
+// to arrays of that element type.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +27,10 @@
 
 namespace clover {
 
+enum Store : int { ST_F64 = 0, ST_F32 = 1 };
+
+inline size_t store_bytes(Store s) { return s == ST_F32 ? sizeof(float) : sizeof(double); }
+
 // uniform mesh, as initialise_chunk_kernel.f90 computes it
 struct Geo {
     double dx, dy, dz;   // celld*, vertexd*
@@ -27,41 +38,43 @@ struct Geo {
     double xa, ya, za;   // celldy*celldz, celldx*celldz, celldx*celldy
 };
 
+// ideal_gas_kernel.f90:45-60 on interior cells
+hipError_t launch_fast_ideal_gas(Store s, const Lay &g, const void *density, const void *energy, void *pressure,
+                                 void *soundspeed, hipStream_t st);
+
 // viscosity_kernel + calc_dt_kernel on the same interior cells: writes
-// viscosity, and a block-partial minimum of the cell dt into partial[];
-// *nparts receives the number of partials (<= kMaxPartials).
-hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, double dtu_safe, double dtv_safe,
-                               double dtw_safe, double dtdiv_safe, double g_small, const double *density0,
-                               const double *pressure, const double *soundspeed, double *viscosity,
-                               const double *xvel0, const double *yvel0, const double *zvel0, double *partial,
-                               int *nparts, hipStream_t st);
+// viscosity, and a block-partial minimum (FP64) of the cell dt into
+// partial[]; *nparts receives the number of partials (<= kMaxPartials).
+hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_safe, double dtu_safe,
+                               double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
+                               const void *density0, const void *pressure, const void *soundspeed, void *viscosity,
+                               const void *xvel0, const void *yvel0, const void *zvel0, double *partial, int *nparts,
+                               hipStream_t st);
 
 // PdV predictor (PdV_kernel.f90:76-117) followed by ideal_gas on the
 // predicted density/energy (ideal_gas.f90:53-62): writes pressure only.
-hipError_t launch_fast_pdv_predict(const Lay &g, const Geo &m, double dt, const double *density0,
-                                   const double *energy0, double *pressure, const double *viscosity,
-                                   const double *xvel0, const double *yvel0, const double *zvel0, hipStream_t st);
+hipError_t launch_fast_pdv_predict(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                   const void *energy0, void *pressure, const void *viscosity, const void *xvel0,
+                                   const void *yvel0, const void *zvel0, hipStream_t st);
 
-hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const double *density0,
-                                  const double *pressure, const double *viscosity, const double *xvel0,
-                                  const double *yvel0, const double *zvel0, double *xvel1, double *yvel1,
-                                  double *zvel1, hipStream_t st);
+hipError_t launch_fast_accelerate(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                  const void *pressure, const void *viscosity, const void *xvel0, const void *yvel0,
+                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, hipStream_t st);
 
 // PdV corrector (PdV_kernel.f90:124-165) and flux_calc (flux_calc_kernel.f90)
-hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, const double *density0,
-                                        double *density1, const double *energy0, double *energy1,
-                                        const double *pressure, const double *viscosity, const double *xvel0,
-                                        const double *xvel1, const double *yvel0, const double *yvel1,
-                                        const double *zvel0, const double *zvel1, double *vol_flux_x,
-                                        double *vol_flux_y, double *vol_flux_z, hipStream_t st);
+hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                        void *density1, const void *energy0, void *energy1, const void *pressure,
+                                        const void *viscosity, const void *xvel0, const void *xvel1,
+                                        const void *yvel0, const void *yvel1, const void *zvel0, const void *zvel1,
+                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, hipStream_t st);
 
 // one advec_cell sweep, fluxes and cell update in one kernel
 // (cell_kernels.hip): mass_flux_<dir> is written over the reference's face
 // range, the new density1/energy1 of interior cells go to rho_out/e_out.
-hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
-                                  const double *density1, const double *energy1, double *mass_flux_x,
-                                  const double *vol_flux_x, double *mass_flux_y, const double *vol_flux_y,
-                                  double *mass_flux_z, const double *vol_flux_z, double *rho_out, double *e_out,
+hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
+                                  const void *density1, const void *energy1, void *mass_flux_x,
+                                  const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
+                                  void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
                                   hipStream_t st);
 
 // one advec_mom sweep for all three velocities (advec_mom_driver.f90 calls
@@ -70,14 +83,25 @@ hipError_t launch_fast_advec_cell(const Lay &g, const Geo &m, int advect_x, int 
 // velocities go to vel_out[3]: they read their neighbours' old values, so
 // they cannot be updated in place.  Only nodes min..max+1 of vel_out are
 // written.
-hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int sweep_number, int direction,
-                                 const double *mass_flux_x, const double *vol_flux_x, const double *mass_flux_y,
-                                 const double *vol_flux_y, const double *mass_flux_z, const double *vol_flux_z,
-                                 const double *density1, const double *const *vel_in, double *const *vel_out,
-                                 hipStream_t st);
+hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect_x, int sweep_number,
+                                 int direction, const void *mass_flux_x, const void *vol_flux_x,
+                                 const void *mass_flux_y, const void *vol_flux_y, const void *mass_flux_z,
+                                 const void *vol_flux_z, const void *density1, const void *const *vel_in,
+                                 void *const *vel_out, hipStream_t st);
 
-hipError_t launch_fast_field_summary(const Lay &g, const Geo &m, const double *density0, const double *energy0,
-                                     const double *pressure, const double *xvel0, const double *yvel0,
-                                     const double *zvel0, double *partial, int *nparts, hipStream_t st);
+// field_summary_kernel.f90:51-78: five FP64 partials per block.  The FP64
+// form sums as the per-call kernel does (same tree); the FP32 form
+// accumulates each thread's cells with compensated (Neumaier) sums.
+hipError_t launch_fast_field_summary(Store s, const Lay &g, const Geo &m, const void *density0, const void *energy0,
+                                     const void *pressure, const void *xvel0, const void *yvel0, const void *zvel0,
+                                     double *partial, int *nparts, hipStream_t st);
+
+// Reflective halo and tile copies on arrays of either element type
+// (halo_kernels.hip; same tables and order as launch_update_halo and
+// launch_tile_halo of kernels.h).
+hipError_t launch_update_halo_store(Store s, const Lay &g, void *const *field_ptrs, const int *fields,
+                                    const int *ext, int depth, hipStream_t st);
+hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *field_ptrs, const Lay &gn,
+                                  void *const *nbr_ptrs, const int *fields, int depth, hipStream_t st);
 
 }  // namespace clover

@@ -26,15 +26,17 @@ struct Quad4 {
     double v[4];
 };
 
-__device__ inline Quad4 quad_up(const double *__restrict__ p, int i, int ld)
+template <typename S>
+__device__ inline Quad4 quad_up(const S *__restrict__ p, int i, int ld)
 {
-    return Quad4{{p[i], p[i + 1], p[i + ld], p[i + 1 + ld]}};
+    return Quad4{{(double)p[i], (double)p[i + 1], (double)p[i + ld], (double)p[i + 1 + ld]}};
 }
 
 // v[(a+1) + 2(b+1)] = p(j+a, k+b) (a, b in {-1, 0})
-__device__ inline Quad4 quad_down(const double *__restrict__ p, int i, int ld)
+template <typename S>
+__device__ inline Quad4 quad_down(const S *__restrict__ p, int i, int ld)
 {
-    return Quad4{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
+    return Quad4{{(double)p[i - 1 - ld], (double)p[i - ld], (double)p[i - 1], (double)p[i]}};
 }
 
 // the column's planes [l0, l1] of the chunk this workgroup marches
@@ -55,11 +57,12 @@ static inline int chunks_for(int nplanes) { return (nplanes + ZCH - 1) / ZCH; }
 // (the halo exchange of viscosity between the two in timestep.f90:80-84
 // does not touch interior cells).  Each workgroup leaves one partial of the
 // dt minimum.
+template <typename S>
 __global__ void __launch_bounds__(BX *BY)
 k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
-               double g_small, const double *__restrict__ density0, const double *__restrict__ pressure,
-               const double *__restrict__ soundspeed, double *__restrict__ viscosity,
-               const double *__restrict__ xvel0, const double *__restrict__ yvel0, const double *__restrict__ zvel0,
+               double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
+               const S *__restrict__ soundspeed, S *__restrict__ viscosity,
+               const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
                double *__restrict__ partial)
 {
     const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
@@ -73,11 +76,11 @@ k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, 
     if (in) {
         int i = idx(g, j, k, l0);
         Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
-        double p_lo = pressure[i - g.plane], p_c = pressure[i];
+        double p_lo = (double)pressure[i - g.plane], p_c = (double)pressure[i];
         for (int l = l0; l <= l1; l++, i += g.plane) {
             const Quad4 uu = quad_up(xvel0, i + g.plane, g.ld), vu = quad_up(yvel0, i + g.plane, g.ld),
                         wu = quad_up(zvel0, i + g.plane, g.ld);
-            const double p_hi = pressure[i + g.plane];
+            const double p_hi = (double)pressure[i + g.plane];
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
@@ -112,8 +115,8 @@ k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, 
             const double xy = 0.25 * (ugrady2 - ugrady1) / (m.dy) + 0.25 * (vgradx2 - vgradx1) / (m.dx);
             const double xz = 0.25 * (ugradz2 - ugradz1) / (m.dz) + 0.25 * (wgradx2 - wgradx1) / (m.dx);
             const double yz = 0.25 * (vgradz2 - vgradz1) / (m.dz) + 0.25 * (wgrady2 - wgrady1) / (m.dy);
-            double pgradx = (pressure[i + 1] - pressure[i - 1]) / (m.dx + m.dx);
-            double pgrady = (pressure[i + g.ld] - pressure[i - g.ld]) / (m.dy + m.dy);
+            double pgradx = ((double)pressure[i + 1] - (double)pressure[i - 1]) / (m.dx + m.dx);
+            double pgrady = ((double)pressure[i + g.ld] - (double)pressure[i - g.ld]) / (m.dy + m.dy);
             double pgradz = (p_hi - p_lo) / (m.dz + m.dz);
             const double pgradx2 = pgradx * pgradx;
             const double pgrady2 = pgrady * pgrady;
@@ -174,10 +177,11 @@ k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, 
     }
 }
 
-hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, double dtu_safe, double dtv_safe,
-                               double dtw_safe, double dtdiv_safe, double g_small, const double *density0,
-                               const double *pressure, const double *soundspeed, double *viscosity,
-                               const double *xvel0, const double *yvel0, const double *zvel0, double *partial,
+template <typename S>
+static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, double dtu_safe, double dtv_safe,
+                               double dtw_safe, double dtdiv_safe, double g_small, const S *density0,
+                               const S *pressure, const S *soundspeed, S *viscosity,
+                               const S *xvel0, const S *yvel0, const S *zvel0, double *partial,
                                int *nparts, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
@@ -186,7 +190,7 @@ hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, doub
     int gz = chunks_for(nz);
     if (bx * by * gz > kMaxPartials) gz = kMaxPartials / (bx * by);
     *nparts = bx * by * gz;
-    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, g, m, dtc_safe,
+    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, g, m, dtc_safe,
                  dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,
                  yvel0, zvel0, partial);
     return hipGetLastError();
@@ -199,11 +203,12 @@ hipError_t launch_fast_visc_dt(const Lay &g, const Geo &m, double dtc_safe, doub
 // (PdV.f90:103-107) and the predicted soundspeed is recomputed by the next
 // ideal_gas before calc_dt reads it.  Fortran sums a+b+c+d+a+b+c+d as a left
 // fold, so the 8-term face sums continue from the 4-term partial.
+template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-k_fast_pdv_predict(Lay g, Geo m, double dt, const double *__restrict__ density0, const double *__restrict__ energy0,
-                   double *__restrict__ pressure, const double *__restrict__ viscosity,
-                   const double *__restrict__ xvel0, const double *__restrict__ yvel0,
-                   const double *__restrict__ zvel0)
+k_fast_pdv_predict(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
+                   S *__restrict__ pressure, const S *__restrict__ viscosity,
+                   const S *__restrict__ xvel0, const S *__restrict__ yvel0,
+                   const S *__restrict__ zvel0)
 {
     const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
     const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
@@ -249,13 +254,14 @@ k_fast_pdv_predict(Lay g, Geo m, double dt, const double *__restrict__ density0,
     }
 }
 
-hipError_t launch_fast_pdv_predict(const Lay &g, const Geo &m, double dt, const double *density0,
-                                   const double *energy0, double *pressure, const double *viscosity,
-                                   const double *xvel0, const double *yvel0, const double *zvel0, hipStream_t st)
+template <typename S>
+static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, const S *density0,
+                                   const S *energy0, S *pressure, const S *viscosity,
+                                   const S *xvel0, const S *yvel0, const S *zvel0, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
-    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
+    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict<S>, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
                  energy0, pressure, viscosity, xvel0, yvel0, zvel0);
     return hipGetLastError();
 }
@@ -263,11 +269,12 @@ hipError_t launch_fast_pdv_predict(const Lay &g, const Geo &m, double dt, const 
 // ---------------------------------------------------------------------------
 // accelerate_kernel.f90:61-109 with scalar areas and volume.  Node (j,k,l)
 // reads the cells (j-1..j, k-1..k, l-1..l); the march keeps plane l-1.
+template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-k_fast_accelerate(Lay g, Geo m, double dt, const double *__restrict__ density0, const double *__restrict__ pressure,
-                  const double *__restrict__ viscosity, const double *__restrict__ xvel0,
-                  const double *__restrict__ yvel0, const double *__restrict__ zvel0, double *__restrict__ xvel1,
-                  double *__restrict__ yvel1, double *__restrict__ zvel1)
+k_fast_accelerate(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
+                  const S *__restrict__ viscosity, const S *__restrict__ xvel0,
+                  const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
+                  S *__restrict__ yvel1, S *__restrict__ zvel1)
 {
     const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
     const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
@@ -318,14 +325,15 @@ k_fast_accelerate(Lay g, Geo m, double dt, const double *__restrict__ density0, 
     }
 }
 
-hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const double *density0,
-                                  const double *pressure, const double *viscosity, const double *xvel0,
-                                  const double *yvel0, const double *zvel0, double *xvel1, double *yvel1,
-                                  double *zvel1, hipStream_t st)
+template <typename S>
+static hipError_t fast_accelerate_t(const Lay &g, const Geo &m, double dt, const S *density0,
+                                  const S *pressure, const S *viscosity, const S *xvel0,
+                                  const S *yvel0, const S *zvel0, S *xvel1, S *yvel1,
+                                  S *zvel1, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
     const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
-    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
+    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate<S>, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
                  pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
     return hipGetLastError();
 }
@@ -336,15 +344,16 @@ hipError_t launch_fast_accelerate(const Lay &g, const Geo &m, double dt, const d
 // covers both.  The corrector's left/bottom face sums equal flux_calc's x/y
 // sums term for term; the z sums differ (flux_calc_kernel.f90:86 repeats
 // zvel0(j+1,k,l)) and are formed separately.
+template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const double *__restrict__ density0, double *__restrict__ density1,
-                        const double *__restrict__ energy0, double *__restrict__ energy1,
-                        const double *__restrict__ pressure, const double *__restrict__ viscosity,
-                        const double *__restrict__ xvel0, const double *__restrict__ xvel1,
-                        const double *__restrict__ yvel0, const double *__restrict__ yvel1,
-                        const double *__restrict__ zvel0, const double *__restrict__ zvel1,
-                        double *__restrict__ vol_flux_x, double *__restrict__ vol_flux_y,
-                        double *__restrict__ vol_flux_z)
+k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
+                        const S *__restrict__ energy0, S *__restrict__ energy1,
+                        const S *__restrict__ pressure, const S *__restrict__ viscosity,
+                        const S *__restrict__ xvel0, const S *__restrict__ xvel1,
+                        const S *__restrict__ yvel0, const S *__restrict__ yvel1,
+                        const S *__restrict__ zvel0, const S *__restrict__ zvel1,
+                        S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
+                        S *__restrict__ vol_flux_z)
 {
     const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
     const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
@@ -424,16 +433,17 @@ k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const double *__restrict__ dens
     }
 }
 
-hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, const double *density0,
-                                        double *density1, const double *energy0, double *energy1,
-                                        const double *pressure, const double *viscosity, const double *xvel0,
-                                        const double *xvel1, const double *yvel0, const double *yvel1,
-                                        const double *zvel0, const double *zvel1, double *vol_flux_x,
-                                        double *vol_flux_y, double *vol_flux_z, hipStream_t st)
+template <typename S>
+static hipError_t fast_pdv_correct_flux_t(const Lay &g, const Geo &m, double dt, const S *density0,
+                                        S *density1, const S *energy0, S *energy1,
+                                        const S *pressure, const S *viscosity, const S *xvel0,
+                                        const S *xvel1, const S *yvel0, const S *yvel1,
+                                        const S *zvel0, const S *zvel1, S *vol_flux_x,
+                                        S *vol_flux_y, S *vol_flux_z, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
     const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
-    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux, grid, dim3(BX, BY, 1), 0, st, g, m, dt,
+    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux<S>, grid, dim3(BX, BY, 1), 0, st, g, m, dt,
                  density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1,
                  vol_flux_x, vol_flux_y, vol_flux_z);
     return hipGetLastError();
@@ -442,13 +452,25 @@ hipError_t launch_fast_pdv_correct_flux(const Lay &g, const Geo &m, double dt, c
 // ---------------------------------------------------------------------------
 // field_summary_kernel.f90:51-78 with the scalar cell volume; same
 // reduction tree as the per-call kernel.
+template <typename S, bool COMP>
 __global__ void __launch_bounds__(RED_THREADS)
-k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const double *__restrict__ density0,
-                     const double *__restrict__ energy0, const double *__restrict__ pressure,
-                     const double *__restrict__ xvel0, const double *__restrict__ yvel0,
-                     const double *__restrict__ zvel0, double *__restrict__ partial)
+k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const S *__restrict__ density0,
+                     const S *__restrict__ energy0, const S *__restrict__ pressure,
+                     const S *__restrict__ xvel0, const S *__restrict__ yvel0,
+                     const S *__restrict__ zvel0, double *__restrict__ partial)
 {
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double comp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // Neumaier compensation (COMP only)
+    auto add = [&](int q, double x) {
+        if (!COMP) {
+            s[q] += x;
+            return;
+        }
+        const double t = s[q] + x;
+        if (fabs(s[q]) >= fabs(x)) comp[q] += (s[q] - t) + x;
+        else comp[q] += (x - t) + s[q];
+        s[q] = t;
+    };
     for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < ncells; c += gridDim.x * RED_THREADS) {
         const int j = g.x_min + c % nj;
         const int k = g.y_min + (c / nj) % nk;
@@ -463,12 +485,14 @@ k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const double *__r
                 }
         const double cell_vol = m.vol;
         const double cell_mass = cell_vol * density0[i];
-        s[0] += cell_vol;
-        s[1] += cell_mass;
-        s[2] += cell_mass * energy0[i];
-        s[3] += cell_mass * 0.5 * vsqrd;
-        s[4] += cell_vol * pressure[i];
+        add(0, cell_vol);
+        add(1, cell_mass);
+        add(2, cell_mass * energy0[i]);
+        add(3, cell_mass * 0.5 * vsqrd);
+        add(4, cell_vol * pressure[i]);
     }
+    if (COMP)
+        for (int q = 0; q < 5; q++) s[q] += comp[q];
     __shared__ double red[5][RED_THREADS / 64];
     for (int q = 0; q < 5; q++) {
         const double w = wave_sum(s[q]);
@@ -482,9 +506,10 @@ k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const double *__r
     }
 }
 
-hipError_t launch_fast_field_summary(const Lay &g, const Geo &m, const double *density0, const double *energy0,
-                                     const double *pressure, const double *xvel0, const double *yvel0,
-                                     const double *zvel0, double *partial, int *nparts, hipStream_t st)
+template <typename S>
+static hipError_t fast_field_summary_t(const Lay &g, const Geo &m, const S *density0, const S *energy0,
+                                     const S *pressure, const S *xvel0, const S *yvel0,
+                                     const S *zvel0, double *partial, int *nparts, hipStream_t st)
 {
     const int nj = g.x_max - g.x_min + 1, nk = g.y_max - g.y_min + 1, nl = g.z_max - g.z_min + 1;
     const int ncells = nj * nk * nl;
@@ -492,9 +517,114 @@ hipError_t launch_fast_field_summary(const Lay &g, const Geo &m, const double *d
     if (nb < 1) nb = 1;
     if (nb > kMaxPartials) nb = kMaxPartials;
     *nparts = nb;
-    TIMED_LAUNCH("k_fast_field_summary", st, k_fast_field_summary, dim3(nb), dim3(RED_THREADS), 0, st, g, m, nj, nk,
+    TIMED_LAUNCH("k_fast_field_summary", st, (k_fast_field_summary<S, (sizeof(S) < sizeof(double))>), dim3(nb), dim3(RED_THREADS), 0, st, g, m, nj, nk,
                  ncells, density0, energy0, pressure, xvel0, yvel0, zvel0, partial);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// ideal_gas_kernel.f90:45-60 (same expressions as the per-call kernel)
+template <typename S>
+__global__ void k_fast_ideal_gas(Lay g, Range r, const S *__restrict__ density, const S *__restrict__ energy,
+                                 S *__restrict__ pressure, S *__restrict__ soundspeed)
+{
+    RANGE_THREAD(r);
+    const int i = idx(g, j, k, l);
+    const double d = density[i];
+    const double v = 1.0 / d;
+    const double p = (1.4 - 1.0) * d * (double)energy[i];
+    pressure[i] = p;
+    const double pressurebyenergy = (1.4 - 1.0) * d;
+    const double pressurebyvolume = -d * p;
+    const double sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
+    soundspeed[i] = sqrt(sound_speed_squared);
+}
+
+template <typename S>
+static hipError_t fast_ideal_gas_t(const Lay &g, const S *density, const S *energy, S *pressure, S *soundspeed,
+                                   hipStream_t st)
+{
+    Range r{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
+    TIMED_LAUNCH("k_fast_ideal_gas", st, k_fast_ideal_gas<S>, grid_of(r), dim3(BX, BY, 1), 0, st, g, r, density,
+                 energy, pressure, soundspeed);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// storage dispatch
+#define CP(T, p) static_cast<const T *>(p)
+#define MP(T, p) static_cast<T *>(p)
+
+hipError_t launch_fast_ideal_gas(Store s, const Lay &g, const void *density, const void *energy, void *pressure,
+                                 void *soundspeed, hipStream_t st)
+{
+#define CALL(T) fast_ideal_gas_t<T>(g, CP(T, density), CP(T, energy), MP(T, pressure), MP(T, soundspeed), st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
+}
+
+hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_safe, double dtu_safe,
+                               double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
+                               const void *density0, const void *pressure, const void *soundspeed, void *viscosity,
+                               const void *xvel0, const void *yvel0, const void *zvel0, double *partial, int *nparts,
+                               hipStream_t st)
+{
+#define CALL(T)                                                                                                 \
+    fast_visc_dt_t<T>(g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, CP(T, density0),      \
+                      CP(T, pressure), CP(T, soundspeed), MP(T, viscosity), CP(T, xvel0), CP(T, yvel0),         \
+                      CP(T, zvel0), partial, nparts, st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
+}
+
+hipError_t launch_fast_pdv_predict(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                   const void *energy0, void *pressure, const void *viscosity, const void *xvel0,
+                                   const void *yvel0, const void *zvel0, hipStream_t st)
+{
+#define CALL(T)                                                                                                 \
+    fast_pdv_predict_t<T>(g, m, dt, CP(T, density0), CP(T, energy0), MP(T, pressure), CP(T, viscosity),         \
+                          CP(T, xvel0), CP(T, yvel0), CP(T, zvel0), st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
+}
+
+hipError_t launch_fast_accelerate(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                  const void *pressure, const void *viscosity, const void *xvel0, const void *yvel0,
+                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, hipStream_t st)
+{
+#define CALL(T)                                                                                                 \
+    fast_accelerate_t<T>(g, m, dt, CP(T, density0), CP(T, pressure), CP(T, viscosity), CP(T, xvel0),            \
+                         CP(T, yvel0), CP(T, zvel0), MP(T, xvel1), MP(T, yvel1), MP(T, zvel1), st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
+}
+
+hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                        void *density1, const void *energy0, void *energy1, const void *pressure,
+                                        const void *viscosity, const void *xvel0, const void *xvel1,
+                                        const void *yvel0, const void *yvel1, const void *zvel0, const void *zvel1,
+                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, hipStream_t st)
+{
+#define CALL(T)                                                                                                 \
+    fast_pdv_correct_flux_t<T>(g, m, dt, CP(T, density0), MP(T, density1), CP(T, energy0), MP(T, energy1),      \
+                               CP(T, pressure), CP(T, viscosity), CP(T, xvel0), CP(T, xvel1), CP(T, yvel0),      \
+                               CP(T, yvel1), CP(T, zvel0), CP(T, zvel1), MP(T, vol_flux_x), MP(T, vol_flux_y),   \
+                               MP(T, vol_flux_z), st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
+}
+
+hipError_t launch_fast_field_summary(Store s, const Lay &g, const Geo &m, const void *density0, const void *energy0,
+                                     const void *pressure, const void *xvel0, const void *yvel0, const void *zvel0,
+                                     double *partial, int *nparts, hipStream_t st)
+{
+#define CALL(T)                                                                                                 \
+    fast_field_summary_t<T>(g, m, CP(T, density0), CP(T, energy0), CP(T, pressure), CP(T, xvel0), CP(T, yvel0), \
+                            CP(T, zvel0), partial, nparts, st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
+}
+#undef CP
+#undef MP
 
 }  // namespace clover

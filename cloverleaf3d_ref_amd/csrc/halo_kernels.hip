@@ -8,6 +8,7 @@
 // the 1..depth halo layers.  Exact moves, no arithmetic.
 #include <hip/hip_runtime.h>
 
+#include "fast.h"
 #include "kernels.h"
 
 namespace clover {
@@ -61,8 +62,8 @@ static const unsigned char kStag[NUM_FIELDS][3] = {
 constexpr int kMaxItems = 2 * NUM_FIELDS;
 
 struct HaloItem {
-    double *dst;
-    const double *src;
+    void *dst;  // arrays of the launch's element type
+    const void *src;
     double sgn;
     int kind;  // reflective: K_*; tile copy: staggered on the axis (0/1)
     int hi;    // 0 low face, 1 high face
@@ -78,9 +79,11 @@ constexpr int HX = 64, HY = 4;
 __device__ inline int tan_a(int axis) { return axis == 0 ? 1 : 0; }
 __device__ inline int tan_b(int axis) { return axis == 2 ? 1 : 2; }
 
+template <typename S>
 __global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
 {
     const HaloItem &it = batch.it[blockIdx.z];
+    S *const dst = static_cast<S *>(it.dst);
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
     const int ta = tan_a(axis), tb = tan_b(axis);
@@ -104,11 +107,12 @@ __global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
         iw[axis] = w; ir[axis] = r;
         iw[ta] = ir[ta] = c;
         iw[tb] = ir[tb] = b;
-        it.dst[idx(g, iw[0], iw[1], iw[2])] = it.sgn * it.dst[idx(g, ir[0], ir[1], ir[2])];
+        dst[idx(g, iw[0], iw[1], iw[2])] = (S)(it.sgn * (double)dst[idx(g, ir[0], ir[1], ir[2])]);
     }
 }
 
-static hipError_t run_batch_reflect(const Lay &g, int axis, int depth, const HaloBatch &b, int n, hipStream_t st)
+static hipError_t run_batch_reflect(Store s, const Lay &g, int axis, int depth, const HaloBatch &b, int n,
+                                    hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
@@ -116,14 +120,15 @@ static hipError_t run_batch_reflect(const Lay &g, int axis, int depth, const Hal
     const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
     const int na = mx[ta] - mn[ta] + 2 + 2 * depth, nb = mx[tb] - mn[tb] + 2 + 2 * depth;
     if (t_kernel_timer) t_kernel_timer->begin("k_reflect", st);
-    hipLaunchKernelGGL(k_reflect, dim3((na + HX - 1) / HX, (nb + HY - 1) / HY, n), dim3(HX, HY, 1), 0, st, g, axis,
-                       depth, b);
+    const dim3 grid((na + HX - 1) / HX, (nb + HY - 1) / HY, n);
+    if (s == ST_F32) hipLaunchKernelGGL(k_reflect<float>, grid, dim3(HX, HY, 1), 0, st, g, axis, depth, b);
+    else hipLaunchKernelGGL(k_reflect<double>, grid, dim3(HX, HY, 1), 0, st, g, axis, depth, b);
     if (t_kernel_timer) t_kernel_timer->end(st);
     return hipGetLastError();
 }
 
-hipError_t launch_update_halo(const Lay &g, double *const *fp, const int *fields, const int *ext, int depth,
-                              hipStream_t st)
+hipError_t launch_update_halo_store(Store s, const Lay &g, void *const *fp, const int *fields, const int *ext,
+                                    int depth, hipStream_t st)
 {
     // The reference's order per field is bottom, top, left, right, back,
     // front; fields are independent arrays, so all fields' y faces go first,
@@ -153,12 +158,12 @@ hipError_t launch_update_halo(const Lay &g, double *const *fp, const int *fields
                 it.eb = h.ext[face][1];
             }
             if (!together) {
-                hipError_t e = run_batch_reflect(g, axis, depth, b, n, st);
+                hipError_t e = run_batch_reflect(s, g, axis, depth, b, n, st);
                 if (e != hipSuccess) return e;
                 n = 0;
             }
         }
-        hipError_t e = run_batch_reflect(g, axis, depth, b, n, st);
+        hipError_t e = run_batch_reflect(s, g, axis, depth, b, n, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -170,9 +175,12 @@ hipError_t launch_update_halo(const Lay &g, double *const *fp, const int *fields
 //   high face  f(max+d)   = N(Nmin-1+d)   cell-like
 //              f(max+1+d) = N(Nmin+d)     staggered
 // tangential ranges min-depth..max+depth, +1 where the field is staggered.
+template <typename S>
 __global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
 {
     const HaloItem &it = batch.it[blockIdx.z];
+    S *const dst = static_cast<S *>(it.dst);
+    const S *const src = static_cast<const S *>(it.src);
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
     const int nmn[3] = {gn.x_min, gn.y_min, gn.z_min};
@@ -194,12 +202,12 @@ __global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
         iw[axis] = w; ir[axis] = r;
         iw[ta] = ir[ta] = c;
         iw[tb] = ir[tb] = b;
-        it.dst[idx(g, iw[0], iw[1], iw[2])] = it.src[idx(gn, ir[0], ir[1], ir[2])];
+        dst[idx(g, iw[0], iw[1], iw[2])] = src[idx(gn, ir[0], ir[1], ir[2])];
     }
 }
 
-hipError_t launch_tile_halo(int face, const Lay &g, double *const *fp, const Lay &gn, double *const *np,
-                            const int *fields, int depth, hipStream_t st)
+hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *fp, const Lay &gn, void *const *np,
+                                  const int *fields, int depth, hipStream_t st)
 {
     const int axis = (face - 1) / 2;
     const int hi = (face - 1) % 2;
@@ -230,10 +238,31 @@ hipError_t launch_tile_halo(int face, const Lay &g, double *const *fp, const Lay
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
     const int na = mx[ta] - mn[ta] + 2 + 2 * depth, nb = mx[tb] - mn[tb] + 2 + 2 * depth;
     if (t_kernel_timer) t_kernel_timer->begin("k_tile_copy", st);
-    hipLaunchKernelGGL(k_tile_copy, dim3((na + HX - 1) / HX, (nb + HY - 1) / HY, n), dim3(HX, HY, 1), 0, st, g, gn,
-                       axis, depth, b);
+    const dim3 grid((na + HX - 1) / HX, (nb + HY - 1) / HY, n);
+    if (s == ST_F32) hipLaunchKernelGGL(k_tile_copy<float>, grid, dim3(HX, HY, 1), 0, st, g, gn, axis, depth, b);
+    else hipLaunchKernelGGL(k_tile_copy<double>, grid, dim3(HX, HY, 1), 0, st, g, gn, axis, depth, b);
     if (t_kernel_timer) t_kernel_timer->end(st);
     return hipGetLastError();
+}
+
+// FP64 entry points used by the drop-in ABI and the per-call driver
+hipError_t launch_update_halo(const Lay &g, double *const *fp, const int *fields, const int *ext, int depth,
+                              hipStream_t st)
+{
+    void *p[NUM_FIELDS];
+    for (int f = 0; f < NUM_FIELDS; f++) p[f] = fp[f];
+    return launch_update_halo_store(ST_F64, g, p, fields, ext, depth, st);
+}
+
+hipError_t launch_tile_halo(int face, const Lay &g, double *const *fp, const Lay &gn, double *const *np,
+                            const int *fields, int depth, hipStream_t st)
+{
+    void *p[NUM_FIELDS], *q[NUM_FIELDS];
+    for (int f = 0; f < NUM_FIELDS; f++) {
+        p[f] = fp[f];
+        q[f] = np[f];
+    }
+    return launch_tile_halo_store(ST_F64, face, g, p, gn, q, fields, depth, st);
 }
 
 }  // namespace clover

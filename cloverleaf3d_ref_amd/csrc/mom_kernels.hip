@@ -37,24 +37,28 @@ namespace clover {
 constexpr int MW = 8;       // waves per workgroup (wave 0 is the halo row)
 constexpr int MCHUNK = 64;  // march positions per workgroup
 
+template <typename S>
 struct Vel3 {
-    const double *in[3];
-    double *out[3];
+    const S *in[3];
+    S *out[3];
 };
 
 // post_vol of advec_mom_kernel.f90:95-163 for the sweep `mode`:
 // 1 sweep 1 along x, 2 sweep 1 along z, 3 sweep 2 with advect_x,
 // 4 sweep 2 without, 5 and 6 sweep 3
-__device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, const double *__restrict__ vfx,
-                                      const double *__restrict__ vfy, const double *__restrict__ vfz)
+template <typename S>
+__device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, const S *__restrict__ vfx,
+                                      const S *__restrict__ vfy, const S *__restrict__ vfz)
 {
+#define D(x) ((double)(x))
     switch (mode) {
-    case 1: return V0 + vfy[i + g.ld] - vfy[i] + vfz[i + g.plane] - vfz[i];
-    case 2: return V0 + vfx[i + 1] - vfx[i] + vfy[i + g.ld] - vfy[i];
-    case 3: return V0 + vfz[i + g.plane] - vfz[i];
-    case 4: return V0 + vfx[i + 1] - vfx[i];
+    case 1: return V0 + D(vfy[i + g.ld]) - D(vfy[i]) + D(vfz[i + g.plane]) - D(vfz[i]);
+    case 2: return V0 + D(vfx[i + 1]) - D(vfx[i]) + D(vfy[i + g.ld]) - D(vfy[i]);
+    case 3: return V0 + D(vfz[i + g.plane]) - D(vfz[i]);
+    case 4: return V0 + D(vfx[i + 1]) - D(vfx[i]);
     default: return V0;
     }
+#undef D
 }
 
 // momentum flux through the face between nodes n and n+s
@@ -141,11 +145,11 @@ __device__ inline double march_node_mass(const Quad &pm, const Quad &p0)  // cel
     return 0.125 * (p0.w + p0.c + p0.xw + p0.x + pm.w + pm.c + pm.xw + pm.x);
 }
 
-template <int DIR>
+template <int DIR, typename S>
 __global__ void __launch_bounds__(64 * MW)
-k_fast_mom_march(Lay g, Geo geo, int mode, const double *__restrict__ mf, const double *__restrict__ vfx,
-                 const double *__restrict__ vfy, const double *__restrict__ vfz,
-                 const double *__restrict__ density1, Vel3 v)
+k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+                 const S *__restrict__ vfy, const S *__restrict__ vfz,
+                 const S *__restrict__ density1, Vel3<S> v)
 {
     __shared__ double xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
@@ -168,12 +172,12 @@ k_fast_mom_march(Lay g, Geo geo, int mode, const double *__restrict__ mf, const 
 
     auto plane = [&](int q, Quad &F, Quad &P, int &b) {
         const int i = OFF(q);
-        const double a = ok_cell ? mf[i] : 0.0;  // q <= mmax+3: inside the sweep's face extent
-        const double p = (ok_cell && q <= mmax + 2) ? density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
+        const double a = ok_cell ? (double)mf[i] : 0.0;  // q <= mmax+3: inside the sweep's face extent
+        const double p = (ok_cell && q <= mmax + 2) ? (double)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
         exchange<-1, -1>(xbuf[b], lane, w, a, p, F, P);
         b ^= 1;
     };
-    auto vel_at = [&](int c, int q) { return store ? v.in[c][OFF(q)] : 0.0; };
+    auto vel_at = [&](int c, int q) { return store ? (double)v.in[c][OFF(q)] : 0.0; };
 
     int b = 0;
     Quad Fa, Pa, Fb, Pb;
@@ -240,10 +244,11 @@ k_fast_mom_march(Lay g, Geo geo, int mode, const double *__restrict__ mf, const 
 //   mp(j)  = cells (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1)
 __device__ inline double shfl_d(double x, int src) { return __shfl(x, src, 64); }
 
+template <typename S>
 __global__ void __launch_bounds__(64 * MW)
-k_fast_mom_x(Lay g, Geo geo, int mode, const double *__restrict__ mf, const double *__restrict__ vfx,
-             const double *__restrict__ vfy, const double *__restrict__ vfz, const double *__restrict__ density1,
-             Vel3 v)
+k_fast_mom_x(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+             const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
+             Vel3<S> v)
 {
     __shared__ double xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
@@ -264,8 +269,8 @@ k_fast_mom_x(Lay g, Geo geo, int mode, const double *__restrict__ mf, const doub
 
     auto plane = [&](int q, Quad &F, Quad &P, int &b) {
         const int i = OFF(q);
-        const double a = ok_face ? mf[i] : 0.0;
-        const double p = ok_cell ? density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
+        const double a = ok_face ? (double)mf[i] : 0.0;
+        const double p = ok_cell ? (double)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
         exchange<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
         b ^= 1;
     };
@@ -287,7 +292,7 @@ k_fast_mom_x(Lay g, Geo geo, int mode, const double *__restrict__ mf, const doub
         const int i = OFF(l);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double v0 = ok_vert ? v.in[c][i] : 0.0;
+            const double v0 = ok_vert ? (double)v.in[c][i] : 0.0;
             const double vm1 = shfl_d(v0, lm), vp1 = shfl_d(v0, lp), vp2 = shfl_d(v0, lp2);
             const double fhi = mom_face_flux(nf, sig, d, vm1, v0, vp1, vp2);  // face j | j+1
             const double flo = shfl_d(fhi, lm);                                // face j-1 | j
@@ -299,11 +304,40 @@ k_fast_mom_x(Lay g, Geo geo, int mode, const double *__restrict__ mf, const doub
 #undef OFF
 }
 
-hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int sweep_number, int direction,
-                                 const double *mass_flux_x, const double *vol_flux_x, const double *mass_flux_y,
-                                 const double *vol_flux_y, const double *mass_flux_z, const double *vol_flux_z,
-                                 const double *density1, const double *const *vel_in, double *const *vel_out,
-                                 hipStream_t st)
+template <typename S>
+static hipError_t fast_advec_mom_t(const Lay &g, const Geo &m, int mode, int direction, const S *mass_flux_x,
+                                   const S *vol_flux_x, const S *mass_flux_y, const S *vol_flux_y,
+                                   const S *mass_flux_z, const S *vol_flux_z, const S *density1,
+                                   const void *const *vel_in, void *const *vel_out, hipStream_t st)
+{
+    Vel3<S> v;
+    for (int c = 0; c < 3; c++) {
+        v.in[c] = static_cast<const S *>(vel_in[c]);
+        v.out[c] = static_cast<S *>(vel_out[c]);
+    }
+    const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
+    const dim3 blk(64, MW, 1);
+    if (direction == 1) {
+        const dim3 grid((nxn + 59) / 60, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
+        TIMED_LAUNCH("k_fast_mom_x", st, k_fast_mom_x<S>, grid, blk, 0, st, g, m, mode, mass_flux_x, vol_flux_x,
+                     vol_flux_y, vol_flux_z, density1, v);
+    } else if (direction == 2) {
+        const dim3 grid((nxn + 62) / 63, (nzn + MW - 2) / (MW - 1), (nyn + MCHUNK - 1) / MCHUNK);
+        TIMED_LAUNCH("k_fast_mom_y", st, (k_fast_mom_march<2, S>), grid, blk, 0, st, g, m, mode, mass_flux_y,
+                     vol_flux_x, vol_flux_y, vol_flux_z, density1, v);
+    } else {
+        const dim3 grid((nxn + 62) / 63, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
+        TIMED_LAUNCH("k_fast_mom_z", st, (k_fast_mom_march<3, S>), grid, blk, 0, st, g, m, mode, mass_flux_z,
+                     vol_flux_x, vol_flux_y, vol_flux_z, density1, v);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect_x, int sweep_number,
+                                 int direction, const void *mass_flux_x, const void *vol_flux_x,
+                                 const void *mass_flux_y, const void *vol_flux_y, const void *mass_flux_z,
+                                 const void *vol_flux_z, const void *density1, const void *const *vel_in,
+                                 void *const *vel_out, hipStream_t st)
 {
     int mode = 0;
     if (sweep_number == 1 && direction == 1) mode = 1;
@@ -313,27 +347,13 @@ hipError_t launch_fast_advec_mom(const Lay &g, const Geo &m, int advect_x, int s
     else if (sweep_number == 3 && direction == 1) mode = 5;
     else if (sweep_number == 3 && direction == 3) mode = 6;
     if (!mode || direction < 1 || direction > 3) return hipErrorInvalidValue;
-    Vel3 v;
-    for (int c = 0; c < 3; c++) {
-        v.in[c] = vel_in[c];
-        v.out[c] = vel_out[c];
-    }
-    const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
-    const dim3 blk(64, MW, 1);
-    if (direction == 1) {
-        const dim3 grid((nxn + 59) / 60, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
-        TIMED_LAUNCH("k_fast_mom_x", st, k_fast_mom_x, grid, blk, 0, st, g, m, mode, mass_flux_x, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, v);
-    } else if (direction == 2) {
-        const dim3 grid((nxn + 62) / 63, (nzn + MW - 2) / (MW - 1), (nyn + MCHUNK - 1) / MCHUNK);
-        TIMED_LAUNCH("k_fast_mom_y", st, k_fast_mom_march<2>, grid, blk, 0, st, g, m, mode, mass_flux_y, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, v);
-    } else {
-        const dim3 grid((nxn + 62) / 63, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
-        TIMED_LAUNCH("k_fast_mom_z", st, k_fast_mom_march<3>, grid, blk, 0, st, g, m, mode, mass_flux_z, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, v);
-    }
-    return hipGetLastError();
+#define CALL(T)                                                                                                 \
+    fast_advec_mom_t<T>(g, m, mode, direction, static_cast<const T *>(mass_flux_x),                             \
+                        static_cast<const T *>(vol_flux_x), static_cast<const T *>(mass_flux_y),                \
+                        static_cast<const T *>(vol_flux_y), static_cast<const T *>(mass_flux_z),                \
+                        static_cast<const T *>(vol_flux_z), static_cast<const T *>(density1), vel_in, vel_out, st)
+    return s == ST_F32 ? CALL(float) : CALL(double);
+#undef CALL
 }
 
 }  // namespace clover

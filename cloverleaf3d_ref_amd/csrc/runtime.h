@@ -51,6 +51,9 @@ inline void hip_check(hipError_t e, const char *what)
 // layout g.  1-D classes are plain contiguous copies.
 void upload_dense(const Lay &g, int cls, const double *host, double *dev, hipStream_t st);
 void download_dense(const Lay &g, int cls, const double *dev, double *host, hipStream_t st);
+// the same for arrays of es-byte elements
+void upload_dense_any(const Lay &g, int cls, const void *host, void *dev, size_t es, hipStream_t st);
+void download_dense_any(const Lay &g, int cls, const void *dev, void *host, size_t es, hipStream_t st);
 
 // element count of the dense host array of class cls
 int64_t dense_elems(const Lay &g, int cls);

@@ -184,6 +184,12 @@ int clover_chunk_destroy(clover_chunk *c);
  * 0/1 arrays); CLOVER_CHUNK_PER_CALL runs one kernel per reference loop nest
  * instead, every intermediate stored as the reference stores it. */
 #define CLOVER_CHUNK_PER_CALL 1
+/* The FP32-store form: every field array held in FP32 (half the HBM bytes),
+ * values converted to FP64 on load, so the arithmetic, calc_dt's minimum,
+ * the advection's mass and energy accumulation and the field_summary sums
+ * (compensated) run in FP64.  Results differ from the FP64 form by the
+ * rounding of the stored fields.  Fused kernels only. */
+#define CLOVER_CHUNK_FP32 2
 int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clover_chunk **out);
 
 /* One step of hydro.f90:48-68: timestep, PdV predictor/corrector,

@@ -1,0 +1,87 @@
+"""The FP32-store form: fields held in FP32, arithmetic in FP64.
+
+Its results differ from the FP64 form only through the rounding of the
+stored fields, so every summary total stays within 1e-5 relative of the
+FP64 oracle (the reference program's own totals for the 64^3 deck, the
+oracle driver for odd tiled decks), the fields are exactly representable in
+FP32, and the start state (volume, mass) is the FP64 one rounded once.
+"""
+from __future__ import annotations
+
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from cloverleaf3d_ref_amd.chunk import Chunk
+from cloverleaf3d_ref_amd.deck import DEFAULT_DECK, TEST_PROBLEM_1_KE, cuboid_deck, parse_deck
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.loads((pathlib.Path(__file__).resolve().parent / "golden" / "config0_summaries.json").read_text())
+TOL = 1e-5
+
+
+def _rows(deck, fp32):
+    print(f"[gpu] chunk {deck.x_cells}x{deck.y_cells}x{deck.z_cells} tiles={deck.tiles_per_chunk} "
+          f"{'fp32' if fp32 else 'fp64'}", flush=True)
+    with Chunk(deck, fp32=fp32) as c:
+        rows = [c.field_summary()] + c.run()
+    return rows
+
+
+def test_fp32_config0_against_reference_program():
+    rows = _rows(parse_deck(GOLDEN["deck"]), True)
+    ref = GOLDEN["reference_program"]
+    assert [r["step"] for r in rows] == [r[0] for r in ref] + [100]
+    for got, want in zip(rows, ref):
+        vals = [got["vol"], got["mass"], got["press"] / got["vol"], got["ie"], got["ke"]]
+        for name, a, b in zip(("vol", "mass", "press", "ie", "ke"), vals, want[1:]):
+            if b == 0.0:
+                assert abs(a) < 1e-12
+            else:
+                assert abs(a / b - 1) < TOL, (got["step"], name, a, b)
+
+
+def test_fp32_test_problem_1():
+    rows = _rows(parse_deck(DEFAULT_DECK), True)
+    assert rows[-1]["step"] == 75
+    assert abs(rows[-1]["ke"] / TEST_PROBLEM_1_KE - 1) < TOL
+
+
+@pytest.mark.parametrize("tiles", [1, 4])
+def test_fp32_odd_deck_against_oracle(tiles):
+    from test_gpu_run import ODD_DECK
+    deck = parse_deck(ODD_DECK.format(nx=23, ny=17, nz=13, steps=30, tiles=tiles, td="3d"))
+    want, _ = ol.run(deck)
+    got = _rows(deck, True)
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        for k in ("vol", "mass", "ie", "ke", "press"):
+            assert abs(g[k] / w[k] - 1) < TOL, (g["step"], k, g[k], w[k])
+
+
+def test_fp32_fields_are_single_precision_values():
+    deck = parse_deck(cuboid_deck(24, 10, 0))
+    with Chunk(deck, fp32=True) as c32, Chunk(deck) as c64:
+        for _ in range(10):
+            c32.step()
+            c64.step()
+        for name in ("density0", "energy0", "xvel0", "mass_flux_x"):
+            a, b = c32.fetch(name), c64.fetch(name)
+            assert np.array_equal(a, a.astype(np.float32).astype(np.float64)), name
+            inner = tuple(slice(2, s - 2) for s in a.shape)
+            scale = np.abs(b[inner]).max()
+            assert np.abs(a[inner] - b[inner]).max() <= 1e-5 * scale, name
+
+
+def test_fp32_start_state_is_rounded_fp64():
+    deck = parse_deck(cuboid_deck(32, 1, 0))
+    with Chunk(deck, fp32=True) as c32, Chunk(deck) as c64:
+        for name in ("density0", "energy0", "volume"):
+            a, b = c32.fetch(name), c64.fetch(name)
+            assert np.array_equal(a, b.astype(np.float32).astype(np.float64)), name
+        s32, s64 = c32.field_summary(), c64.field_summary()
+        assert abs(s32["mass"] / s64["mass"] - 1) < 1e-7
