@@ -90,6 +90,16 @@ def lib() -> ctypes.CDLL:
     L.clover_chunk_set_profiling.argtypes = [vp, ctypes.c_int]
     L.clover_chunk_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                        ctypes.c_int]
+    L.clover_ensemble_create.argtypes = [ctypes.POINTER(CloverDeck), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(vp)]
+    L.clover_ensemble_destroy.argtypes = [vp]
+    L.clover_ensemble_run.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.clover_ensemble_summaries.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.clover_ensemble_state.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    L.clover_ensemble_fetch_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.clover_ensemble_device_bytes.restype = ctypes.c_longlong
+    L.clover_ensemble_device_bytes.argtypes = [vp]
     L.clover_hip_fetch.argtypes = [vp]
     L.clover_hip_push.argtypes = [vp]
     L.clover_hip_forget.argtypes = [vp]

@@ -1,0 +1,95 @@
+// Batched launches over many independent decks (the ensemble driver).
+//
+// Every deck is one tile with its own extents, mesh spacing, dt and time.
+// One launch of a batched kernel covers all decks: deck d owns the
+// workgroups start[d] .. start[d+1]-1 of the launch (a prefix table built on
+// the host from the same per-deck grid the single-deck launcher uses), a
+// workgroup finds its deck by binary search and runs the single-deck kernel
+// body on that deck's block.  Array pointers, dt and time are read from the
+// device-side deck table, so a whole step needs no host involvement and can
+// be captured in a graph; the level and sweep exchanges of the fast path are
+// done on the device by exchanging the table's pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "driver.h"
+#include "fast.h"
+
+namespace clover {
+
+struct DeckDev {
+    Lay g;
+    Geo m;
+    void *f[NUM_ARRAYS];  // field arrays, element type of the ensemble
+    double dt, dtold, time;
+    double dtmax, dtrise, dtmin;
+    double dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe;
+    int small;  // set when dt fell below dtmin (timestep.f90:122-124)
+};
+
+// A batched launch: total workgroups and the device prefix table (nd+1).
+struct BatchGrid {
+    const int *start;
+    int total;
+};
+
+// Workgroups one deck needs in each batched kernel (same shapes as the
+// single-deck launches).
+enum BatchKernel : int {
+    BK_IDEAL_GAS, BK_VISC_DT, BK_PDV_PREDICT, BK_ACCELERATE, BK_PDV_CORRECT, BK_SUMMARY,
+    BK_CELL_X, BK_CELL_Y, BK_CELL_Z, BK_MOM_X, BK_MOM_Y, BK_MOM_Z, BK_COUNT
+};
+int batch_blocks(BatchKernel k, const Lay &g);
+// reflective halo along `axis` for `nitems` (field, face) items
+int batch_blocks_reflect(const Lay &g, int axis, int depth, int nitems);
+
+__device__ inline int deck_of(const int *__restrict__ start, int nd, int b)
+{
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Launchers.  decks: device table; grid from the matching prefix table.
+hipError_t batch_ideal_gas(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
+// writes one dt partial per workgroup at partial[workgroup]
+hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double g_small,
+                         double *partial, hipStream_t st);
+hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
+hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
+hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
+// five partials per workgroup at partial[5*workgroup]
+hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double *partial,
+                               hipStream_t st);
+// advec_cell sweep `dir`; new density1/energy1 into the arrays X_W1/X_W2
+hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
+                            int sweep_number, hipStream_t st);
+// advec_mom sweep `dir`; new velocities into X_W5..X_W7
+hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
+                           int sweep_number, hipStream_t st);
+// reflective halo of the flagged fields along one axis, both faces
+hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,
+                         const int *fields, hipStream_t st);
+
+// Per-deck steps (one workgroup per deck).
+// timestep.f90:86-126 on the device: dt = min over the deck's partials,
+// then min(dt, dtold*dtrise, dtmax); small flag; dtold = dt.
+hipError_t batch_deck_dt(DeckDev *decks, int nd, const int *visc_start, const double *partial, double g_big,
+                         hipStream_t st);
+// the five sums of each deck's summary partials, as the single-deck final pass
+hipError_t batch_deck_sum5(int nd, const int *sum_start, const double *partial, double *out, hipStream_t st);
+// exchange up to 5 pairs of array slots in every deck's table; add dt to
+// time when add_time
+struct SlotSwap {
+    int n;
+    int a[5], b[5];
+    int add_time;
+};
+hipError_t batch_swap(DeckDev *decks, int nd, const SlotSwap &sw, hipStream_t st);
+
+}  // namespace clover

@@ -17,6 +17,7 @@
 // per-call kernels, so the results are bit-identical.
 #include <hip/hip_runtime.h>
 
+#include "batch.h"
 #include "fast.h"
 #include "launch.h"
 
@@ -87,18 +88,17 @@ __device__ inline double shfl_c(double x, int src) { return __shfl(x, src, 64); 
 // x sweep: faces j in [x_min, x_max+2], rows k, l interior.  Lane i holds
 // cell/face j = jt - 2 + i.
 template <typename S>
-__global__ void __launch_bounds__(64 * CROWS)
-k_fast_cell_x(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+__device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
               const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
               S *__restrict__ e_out)
 {
     const int lane = threadIdx.x;
     const int ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
-    const int row = (int)blockIdx.y * CROWS + (int)threadIdx.y;
+    const int row = bk.y * CROWS + (int)threadIdx.y;
     if (row >= ny * nz) return;  // whole wave: no shuffle partner is lost
     const int k = g.y_min + row % ny, l = g.z_min + row / ny;
-    const int j = g.x_min - 2 + (int)blockIdx.x * 60 + lane;
+    const int j = g.x_min - 2 + bk.x * 60 + lane;
     const int i = idx(g, j, k, l);
     const bool ok_cell = j <= g.x_max + 2;  // j >= x_min-2 always
     const double rho = ok_cell ? density1[i] : 0.0;
@@ -129,20 +129,28 @@ k_fast_cell_x(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__rest
     }
 }
 
+template <typename S>
+__global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+              const S *__restrict__ vfz, const S *__restrict__ density1,
+              const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
+              S *__restrict__ e_out)
+{
+    cell_x_body<S>(hw_blk(), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out);
+}
+
 // ---------------------------------------------------------------------------
 // y and z sweeps: lanes along x (interior j), threadIdx.y along the cross
 // axis, march along the sweep axis over faces [smin, smax+2] in chunks.
 // Faces of the y sweep run over l in [z_min, z_max+2] (advec_cell_kernel.f90:
 // 215-217); the cells updated are interior.
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * CROWS)
-k_fast_cell_march(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+__device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
                   const S *__restrict__ vfz, const S *__restrict__ density1,
                   const S *__restrict__ energy1, S *__restrict__ mass_flux, S *__restrict__ rho_out,
                   S *__restrict__ e_out)
 {
-    const int j = g.x_min + (int)(blockIdx.x * 64 + threadIdx.x);
-    const int cr = (DIR == 3 ? g.y_min : g.z_min) + (int)(blockIdx.y * CROWS + threadIdx.y);
+    const int j = g.x_min + (int)(bk.x * 64 + threadIdx.x);
+    const int cr = (DIR == 3 ? g.y_min : g.z_min) + (int)(bk.y * CROWS + threadIdx.y);
     const int crmax = DIR == 3 ? g.y_max : g.z_max + 2;  // last cross position with faces
     if (j > g.x_max || cr > crmax) return;                 // no cross-lane traffic in this kernel
     const bool cross_in = DIR == 3 || cr <= g.z_max;       // updates only on interior cells
@@ -150,7 +158,7 @@ k_fast_cell_march(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__
     const int smin = DIR == 3 ? g.z_min : g.y_min, smax = DIR == 3 ? g.z_max : g.y_max;
     const double vd = DIR == 3 ? m.dz : m.dy;
     const S *__restrict__ vfs = DIR == 3 ? vfz : vfy;
-    const int f0 = smin + (int)blockIdx.z * CCHUNK;
+    const int f0 = smin + bk.z * CCHUNK;
     const int f1 = min(f0 + CCHUNK - 1, smax + 2);
     const int ib = DIR == 3 ? idx(g, j, cr, smin) : idx(g, j, smin, cr);  // offset at sweep position smin
 #define OFF(q) (ib + ((q) - smin) * s)
@@ -207,6 +215,15 @@ k_fast_cell_march(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__
 #undef OFF
 }
 
+template <int DIR, typename S>
+__global__ void __launch_bounds__(64 * CROWS) k_fast_cell_march(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+                  const S *__restrict__ vfz, const S *__restrict__ density1,
+                  const S *__restrict__ energy1, S *__restrict__ mass_flux, S *__restrict__ rho_out,
+                  S *__restrict__ e_out)
+{
+    cell_march_body<DIR, S>(hw_blk(), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux, rho_out, e_out);
+}
+
 template <typename S>
 static hipError_t fast_advec_cell_t(const Lay &g, const Geo &m, int mode, int dir, const S *density1,
                                     const S *energy1, S *mass_flux_x, const S *vol_flux_x, S *mass_flux_y,
@@ -250,6 +267,70 @@ hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advec
                          static_cast<T *>(rho_out), static_cast<T *>(e_out), st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
+}
+
+// ---------------------------------------------------------------------------
+// Batched over many decks (batch.h): new density1/energy1 go to X_W1/X_W2.
+__host__ __device__ inline void dims_cell(int dir, const Lay &g, int &gx, int &gy, int &gz)
+{
+    const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
+    if (dir == 1) {
+        gx = (nx + 2 + 59) / 60; gy = (ny * nz + CROWS - 1) / CROWS; gz = 1;
+    } else if (dir == 2) {
+        gx = (nx + 63) / 64; gy = (nz + 2 + CROWS - 1) / CROWS; gz = (ny + 2 + CCHUNK - 1) / CCHUNK;
+    } else {
+        gx = (nx + 63) / 64; gy = (ny + CROWS - 1) / CROWS; gz = (nz + 2 + CCHUNK - 1) / CCHUNK;
+    }
+}
+
+int batch_blocks_cell(int dir, const Lay &g)
+{
+    int gx, gy, gz;
+    dims_cell(dir, g, gx, gy, gz);
+    return gx * gy * gz;
+}
+
+template <int DIR, typename S>
+__global__ void __launch_bounds__(64 * CROWS)
+kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode)
+{
+    const int d = deck_of(start, nd, (int)blockIdx.x);
+    const DeckDev &D = decks[d];
+    int gx, gy, gz;
+    dims_cell(DIR, D.g, gx, gy, gz);
+    const int lb = (int)blockIdx.x - start[d];
+    const Blk bk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
+#define CF(id) static_cast<const S *>(D.f[id])
+#define MF(id) static_cast<S *>(D.f[id])
+    if constexpr (DIR == 1)
+        cell_x_body<S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z), CF(F_DENSITY1),
+                       CF(F_ENERGY1), MF(F_MASS_FLUX_X), MF(X_W1), MF(X_W2));
+    else
+        cell_march_body<DIR, S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z),
+                                CF(F_DENSITY1), CF(F_ENERGY1), MF(DIR == 2 ? F_MASS_FLUX_Y : F_MASS_FLUX_Z),
+                                MF(X_W1), MF(X_W2));
+#undef CF
+#undef MF
+}
+
+hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
+                            int sweep_number, hipStream_t st)
+{
+    int mode = 0;
+    if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
+    else if (dir == 2) mode = sweep_number == 2 ? (advect_x ? 3 : 4) : 0;
+    else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
+    if (!mode) return hipErrorInvalidValue;
+    if (bg.total == 0) return hipSuccess;
+    const dim3 grid(bg.total), blk(64, CROWS);
+#define L(D_, T_) TIMED_LAUNCH("kb_cell", st, (kb_cell<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode)
+    if (s == ST_F32) {
+        if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float);
+    } else {
+        if (dir == 1) L(1, double); else if (dir == 2) L(2, double); else L(3, double);
+    }
+#undef L
+    return hipGetLastError();
 }
 
 }  // namespace clover

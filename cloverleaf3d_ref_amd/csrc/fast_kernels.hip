@@ -13,6 +13,7 @@
 // of once per plane that uses it.
 #include <hip/hip_runtime.h>
 
+#include "batch.h"
 #include "fast.h"
 #include "launch.h"
 
@@ -40,16 +41,15 @@ __device__ inline Quad4 quad_down(const S *__restrict__ p, int i, int ld)
 }
 
 // the column's planes [l0, l1] of the chunk this workgroup marches
-__device__ inline void chunk_planes(int lstart, int lend, int &l0, int &l1)
+__device__ inline void chunk_planes(const Blk &b, int lstart, int lend, int &l0, int &l1)
 {
-    const int nch = (int)gridDim.z;
     const int n = lend - lstart + 1;
-    const int per = (n + nch - 1) / nch;
-    l0 = lstart + (int)blockIdx.z * per;
+    const int per = (n + b.gz - 1) / b.gz;
+    l0 = lstart + b.z * per;
     l1 = min(l0 + per - 1, lend);
 }
 
-static inline int chunks_for(int nplanes) { return (nplanes + ZCH - 1) / ZCH; }
+__host__ __device__ inline int chunks_for(int nplanes) { return (nplanes + ZCH - 1) / ZCH; }
 
 // ---------------------------------------------------------------------------
 // viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
@@ -58,17 +58,16 @@ static inline int chunks_for(int nplanes) { return (nplanes + ZCH - 1) / ZCH; }
 // does not touch interior cells).  Each workgroup leaves one partial of the
 // dt minimum.
 template <typename S>
-__global__ void __launch_bounds__(BX *BY)
-k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
+__device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
                const S *__restrict__ soundspeed, S *__restrict__ viscosity,
                const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
                double *__restrict__ partial)
 {
-    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     int l0, l1;
-    chunk_planes(g.z_min, g.z_max, l0, l1);
+    chunk_planes(bk, g.z_min, g.z_max, l0, l1);
     const bool in = j <= g.x_max && k <= g.y_max && l0 <= l1;  // every thread reaches the reduction
     const double mm = dmin(dmin(m.dx, m.dy), m.dz);
     const double ds = 1.0 / (mm * mm);
@@ -173,8 +172,18 @@ k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, 
     if (tid == 0) {
         double b = red[0];
         for (int q = 1; q < BX * BY / 64; q++) b = dmin(b, red[q]);
-        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = b;
+        partial[(bk.z * bk.gy + bk.y) * bk.gx + bk.x] = b;
     }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BX *BY) k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
+               double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
+               const S *__restrict__ soundspeed, S *__restrict__ viscosity,
+               const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
+               double *__restrict__ partial)
+{
+    visc_dt_body<S>(hw_blk(), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0, yvel0, zvel0, partial);
 }
 
 template <typename S>
@@ -204,17 +213,16 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
 // ideal_gas before calc_dt reads it.  Fortran sums a+b+c+d+a+b+c+d as a left
 // fold, so the 8-term face sums continue from the 4-term partial.
 template <typename S>
-__global__ void __launch_bounds__(BX *BY)
-k_fast_pdv_predict(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
+__device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
                    S *__restrict__ pressure, const S *__restrict__ viscosity,
                    const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                    const S *__restrict__ zvel0)
 {
-    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     if (j > g.x_max || k > g.y_max) return;
     int l0, l1;
-    chunk_planes(g.z_min, g.z_max, l0, l1);
+    chunk_planes(bk, g.z_min, g.z_max, l0, l1);
     if (l0 > l1) return;
     int i = idx(g, j, k, l0);
     Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
@@ -255,6 +263,15 @@ k_fast_pdv_predict(Lay g, Geo m, double dt, const S *__restrict__ density0, cons
 }
 
 template <typename S>
+__global__ void __launch_bounds__(BX *BY) k_fast_pdv_predict(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
+                   S *__restrict__ pressure, const S *__restrict__ viscosity,
+                   const S *__restrict__ xvel0, const S *__restrict__ yvel0,
+                   const S *__restrict__ zvel0)
+{
+    pdv_predict_body<S>(hw_blk(), g, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
+}
+
+template <typename S>
 static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, const S *density0,
                                    const S *energy0, S *pressure, const S *viscosity,
                                    const S *xvel0, const S *yvel0, const S *zvel0, hipStream_t st)
@@ -270,17 +287,16 @@ static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, cons
 // accelerate_kernel.f90:61-109 with scalar areas and volume.  Node (j,k,l)
 // reads the cells (j-1..j, k-1..k, l-1..l); the march keeps plane l-1.
 template <typename S>
-__global__ void __launch_bounds__(BX *BY)
-k_fast_accelerate(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
+__device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
                   const S *__restrict__ viscosity, const S *__restrict__ xvel0,
                   const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
                   S *__restrict__ yvel1, S *__restrict__ zvel1)
 {
-    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     if (j > g.x_max + 1 || k > g.y_max + 1) return;
     int l0, l1;
-    chunk_planes(g.z_min, g.z_max + 1, l0, l1);
+    chunk_planes(bk, g.z_min, g.z_max + 1, l0, l1);
     if (l0 > l1) return;
     int i = idx(g, j, k, l0);
     Quad4 dl = quad_down(density0, i - g.plane, g.ld), pl = quad_down(pressure, i - g.plane, g.ld),
@@ -326,6 +342,15 @@ k_fast_accelerate(Lay g, Geo m, double dt, const S *__restrict__ density0, const
 }
 
 template <typename S>
+__global__ void __launch_bounds__(BX *BY) k_fast_accelerate(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
+                  const S *__restrict__ viscosity, const S *__restrict__ xvel0,
+                  const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
+                  S *__restrict__ yvel1, S *__restrict__ zvel1)
+{
+    accelerate_body<S>(hw_blk(), g, m, dt, density0, pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
+}
+
+template <typename S>
 static hipError_t fast_accelerate_t(const Lay &g, const Geo &m, double dt, const S *density0,
                                   const S *pressure, const S *viscosity, const S *xvel0,
                                   const S *yvel0, const S *zvel0, S *xvel1, S *yvel1,
@@ -345,8 +370,7 @@ static hipError_t fast_accelerate_t(const Lay &g, const Geo &m, double dt, const
 // sums term for term; the z sums differ (flux_calc_kernel.f90:86 repeats
 // zvel0(j+1,k,l)) and are formed separately.
 template <typename S>
-__global__ void __launch_bounds__(BX *BY)
-k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
+__device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
                         const S *__restrict__ energy0, S *__restrict__ energy1,
                         const S *__restrict__ pressure, const S *__restrict__ viscosity,
                         const S *__restrict__ xvel0, const S *__restrict__ xvel1,
@@ -355,11 +379,11 @@ k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const S *__restrict__ density0,
                         S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
                         S *__restrict__ vol_flux_z)
 {
-    const int j = g.x_min + (int)(blockIdx.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(blockIdx.y * BY + threadIdx.y);
+    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
+    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     if (j > g.x_max + 1 || k > g.y_max + 1) return;
     int l0, l1;
-    chunk_planes(g.z_min, g.z_max + 1, l0, l1);
+    chunk_planes(bk, g.z_min, g.z_max + 1, l0, l1);
     if (l0 > l1) return;
     const bool jin = j <= g.x_max, kin = k <= g.y_max;
     int i = idx(g, j, k, l0);
@@ -434,6 +458,19 @@ k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const S *__restrict__ density0,
 }
 
 template <typename S>
+__global__ void __launch_bounds__(BX *BY) k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
+                        const S *__restrict__ energy0, S *__restrict__ energy1,
+                        const S *__restrict__ pressure, const S *__restrict__ viscosity,
+                        const S *__restrict__ xvel0, const S *__restrict__ xvel1,
+                        const S *__restrict__ yvel0, const S *__restrict__ yvel1,
+                        const S *__restrict__ zvel0, const S *__restrict__ zvel1,
+                        S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
+                        S *__restrict__ vol_flux_z)
+{
+    pdv_correct_flux_body<S>(hw_blk(), g, m, dt, density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
+}
+
+template <typename S>
 static hipError_t fast_pdv_correct_flux_t(const Lay &g, const Geo &m, double dt, const S *density0,
                                         S *density1, const S *energy0, S *energy1,
                                         const S *pressure, const S *viscosity, const S *xvel0,
@@ -453,8 +490,7 @@ static hipError_t fast_pdv_correct_flux_t(const Lay &g, const Geo &m, double dt,
 // field_summary_kernel.f90:51-78 with the scalar cell volume; same
 // reduction tree as the per-call kernel.
 template <typename S, bool COMP>
-__global__ void __launch_bounds__(RED_THREADS)
-k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const S *__restrict__ density0,
+__device__ __forceinline__ void field_summary_body(const Blk &bk, Lay g, Geo m, int nj, int nk, int ncells, const S *__restrict__ density0,
                      const S *__restrict__ energy0, const S *__restrict__ pressure,
                      const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                      const S *__restrict__ zvel0, double *__restrict__ partial)
@@ -471,7 +507,7 @@ k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const S *__restri
         else comp[q] += (x - t) + s[q];
         s[q] = t;
     };
-    for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < ncells; c += gridDim.x * RED_THREADS) {
+    for (int c = bk.x * RED_THREADS + threadIdx.x; c < ncells; c += bk.gx * RED_THREADS) {
         const int j = g.x_min + c % nj;
         const int k = g.y_min + (c / nj) % nk;
         const int l = g.z_min + c / (nj * nk);
@@ -502,8 +538,17 @@ k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const S *__restri
     if (threadIdx.x < 5) {
         double b = 0.0;
         for (int w = 0; w < RED_THREADS / 64; w++) b += red[threadIdx.x][w];
-        partial[5 * blockIdx.x + threadIdx.x] = b;
+        partial[5 * bk.x + threadIdx.x] = b;
     }
+}
+
+template <typename S, bool COMP>
+__global__ void __launch_bounds__(RED_THREADS) k_fast_field_summary(Lay g, Geo m, int nj, int nk, int ncells, const S *__restrict__ density0,
+                     const S *__restrict__ energy0, const S *__restrict__ pressure,
+                     const S *__restrict__ xvel0, const S *__restrict__ yvel0,
+                     const S *__restrict__ zvel0, double *__restrict__ partial)
+{
+    field_summary_body<S, COMP>(hw_blk(), g, m, nj, nk, ncells, density0, energy0, pressure, xvel0, yvel0, zvel0, partial);
 }
 
 template <typename S>
@@ -525,10 +570,13 @@ static hipError_t fast_field_summary_t(const Lay &g, const Geo &m, const S *dens
 // ---------------------------------------------------------------------------
 // ideal_gas_kernel.f90:45-60 (same expressions as the per-call kernel)
 template <typename S>
-__global__ void k_fast_ideal_gas(Lay g, Range r, const S *__restrict__ density, const S *__restrict__ energy,
+__device__ __forceinline__ void ideal_gas_body(const Blk &bk, Lay g, Range r, const S *__restrict__ density, const S *__restrict__ energy,
                                  S *__restrict__ pressure, S *__restrict__ soundspeed)
 {
-    RANGE_THREAD(r);
+    const int j = r.j0 + bk.x * BX + (int)threadIdx.x;
+    const int k = r.k0 + bk.y * BY + (int)threadIdx.y;
+    const int l = r.l0 + bk.z;
+    if (j > r.j1 || k > r.k1) return;
     const int i = idx(g, j, k, l);
     const double d = density[i];
     const double v = 1.0 / d;
@@ -538,6 +586,13 @@ __global__ void k_fast_ideal_gas(Lay g, Range r, const S *__restrict__ density, 
     const double pressurebyvolume = -d * p;
     const double sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
     soundspeed[i] = sqrt(sound_speed_squared);
+}
+
+template <typename S>
+__global__ void k_fast_ideal_gas(Lay g, Range r, const S *__restrict__ density, const S *__restrict__ energy,
+                                 S *__restrict__ pressure, S *__restrict__ soundspeed)
+{
+    ideal_gas_body<S>(hw_blk(), g, r, density, energy, pressure, soundspeed);
 }
 
 template <typename S>
@@ -626,5 +681,199 @@ hipError_t launch_fast_field_summary(Store s, const Lay &g, const Geo &m, const 
 }
 #undef CP
 #undef MP
+
+// ---------------------------------------------------------------------------
+// Batched launches over many decks (batch.h): each deck's blocks run the
+// bodies above with the deck's extents, mesh, dt and arrays.
+__host__ __device__ inline void dims_ideal(const Lay &g, int &gx, int &gy, int &gz)
+{
+    gx = (g.x_max - g.x_min + BX) / BX;
+    gy = (g.y_max - g.y_min + BY) / BY;
+    gz = g.z_max - g.z_min + 1;
+}
+
+__host__ __device__ inline void dims_cells(const Lay &g, int &gx, int &gy, int &gz)
+{
+    gx = (g.x_max - g.x_min + BX) / BX;
+    gy = (g.y_max - g.y_min + BY) / BY;
+    gz = chunks_for(g.z_max - g.z_min + 1);
+}
+
+__host__ __device__ inline void dims_nodes(const Lay &g, int &gx, int &gy, int &gz)
+{
+    gx = (g.x_max - g.x_min + 1 + BX) / BX;
+    gy = (g.y_max - g.y_min + 1 + BY) / BY;
+    gz = chunks_for(g.z_max - g.z_min + 2);
+}
+
+__host__ __device__ inline int summary_blocks(const Lay &g)
+{
+    const int ncells = (g.x_max - g.x_min + 1) * (g.y_max - g.y_min + 1) * (g.z_max - g.z_min + 1);
+    int nb = (ncells + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+    if (nb < 1) nb = 1;
+    if (nb > kMaxPartials) nb = kMaxPartials;
+    return nb;
+}
+
+__device__ inline Blk local_blk(int lb, int gx, int gy, int gz)
+{
+    return Blk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
+}
+
+#define DECK_BLOCK(dimsfn)                                     \
+    const int d = deck_of(start, nd, (int)blockIdx.x);         \
+    const DeckDev &D = decks[d];                               \
+    int gx, gy, gz;                                            \
+    dimsfn(D.g, gx, gy, gz);                                   \
+    const Blk bk = local_blk((int)blockIdx.x - start[d], gx, gy, gz);
+#define CF(id) static_cast<const S *>(D.f[id])
+#define MF(id) static_cast<S *>(D.f[id])
+
+template <typename S>
+__global__ void __launch_bounds__(BX *BY)
+kb_ideal_gas(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+{
+    DECK_BLOCK(dims_ideal)
+    const Lay g = D.g;
+    const Range r{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
+    ideal_gas_body<S>(bk, g, r, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), MF(F_SOUNDSPEED));
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BX *BY)
+kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, double g_small,
+           double *__restrict__ partial)
+{
+    DECK_BLOCK(dims_cells)
+    visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
+                    CF(F_DENSITY0), CF(F_PRESSURE), CF(F_SOUNDSPEED), MF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0),
+                    CF(F_ZVEL0), partial + start[d]);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BX *BY)
+kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+{
+    DECK_BLOCK(dims_cells)
+    pdv_predict_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), CF(F_VISCOSITY),
+                        CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BX *BY)
+kb_accelerate(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+{
+    DECK_BLOCK(dims_nodes)
+    accelerate_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0),
+                       CF(F_YVEL0), CF(F_ZVEL0), MF(F_XVEL1), MF(F_YVEL1), MF(F_ZVEL1));
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BX *BY)
+kb_pdv_correct_flux(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+{
+    DECK_BLOCK(dims_nodes)
+    pdv_correct_flux_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), MF(F_DENSITY1), CF(F_ENERGY0), MF(F_ENERGY1),
+                             CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0), CF(F_XVEL1), CF(F_YVEL0), CF(F_YVEL1),
+                             CF(F_ZVEL0), CF(F_ZVEL1), MF(F_VOL_FLUX_X), MF(F_VOL_FLUX_Y), MF(F_VOL_FLUX_Z));
+}
+
+__host__ __device__ inline void dims_summary(const Lay &g, int &gx, int &gy, int &gz)
+{
+    gx = summary_blocks(g);
+    gy = gz = 1;
+}
+
+template <typename S, bool COMP>
+__global__ void __launch_bounds__(RED_THREADS)
+kb_field_summary(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd,
+                 double *__restrict__ partial)
+{
+    DECK_BLOCK(dims_summary)
+    const Lay g = D.g;
+    const int nj = g.x_max - g.x_min + 1, nk = g.y_max - g.y_min + 1, nl = g.z_max - g.z_min + 1;
+    field_summary_body<S, COMP>(bk, g, D.m, nj, nk, nj * nk * nl, CF(F_DENSITY0), CF(F_ENERGY0), CF(F_PRESSURE),
+                                CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0), partial + 5 * start[d]);
+}
+#undef DECK_BLOCK
+#undef CF
+#undef MF
+
+int batch_blocks_mom(int dir, const Lay &g);
+int batch_blocks_cell(int dir, const Lay &g);
+
+int batch_blocks(BatchKernel k, const Lay &g)
+{
+    int gx = 1, gy = 1, gz = 1;
+    switch (k) {
+    case BK_IDEAL_GAS: dims_ideal(g, gx, gy, gz); break;
+    case BK_VISC_DT:
+    case BK_PDV_PREDICT: dims_cells(g, gx, gy, gz); break;
+    case BK_ACCELERATE:
+    case BK_PDV_CORRECT: dims_nodes(g, gx, gy, gz); break;
+    case BK_SUMMARY: dims_summary(g, gx, gy, gz); break;
+    case BK_CELL_X: return batch_blocks_cell(1, g);
+    case BK_CELL_Y: return batch_blocks_cell(2, g);
+    case BK_CELL_Z: return batch_blocks_cell(3, g);
+    case BK_MOM_X: return batch_blocks_mom(1, g);
+    case BK_MOM_Y: return batch_blocks_mom(2, g);
+    case BK_MOM_Z: return batch_blocks_mom(3, g);
+    default: return 0;
+    }
+    return gx * gy * gz;
+}
+
+#define BLAUNCH(name, kern, block, ...)                                                                       \
+    do {                                                                                                     \
+        if (bg.total > 0) TIMED_LAUNCH(name, st, kern, dim3(bg.total), block, 0, st, __VA_ARGS__);           \
+    } while (0)
+
+hipError_t batch_ideal_gas(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+{
+    if (s == ST_F32) BLAUNCH("kb_ideal_gas", kb_ideal_gas<float>, dim3(BX, BY), decks, bg.start, nd);
+    else BLAUNCH("kb_ideal_gas", kb_ideal_gas<double>, dim3(BX, BY), decks, bg.start, nd);
+    return hipGetLastError();
+}
+
+hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double g_small,
+                         double *partial, hipStream_t st)
+{
+    if (s == ST_F32) BLAUNCH("kb_visc_dt", kb_visc_dt<float>, dim3(BX, BY), decks, bg.start, nd, g_small, partial);
+    else BLAUNCH("kb_visc_dt", kb_visc_dt<double>, dim3(BX, BY), decks, bg.start, nd, g_small, partial);
+    return hipGetLastError();
+}
+
+hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+{
+    if (s == ST_F32) BLAUNCH("kb_pdv_predict", kb_pdv_predict<float>, dim3(BX, BY), decks, bg.start, nd);
+    else BLAUNCH("kb_pdv_predict", kb_pdv_predict<double>, dim3(BX, BY), decks, bg.start, nd);
+    return hipGetLastError();
+}
+
+hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+{
+    if (s == ST_F32) BLAUNCH("kb_accelerate", kb_accelerate<float>, dim3(BX, BY), decks, bg.start, nd);
+    else BLAUNCH("kb_accelerate", kb_accelerate<double>, dim3(BX, BY), decks, bg.start, nd);
+    return hipGetLastError();
+}
+
+hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+{
+    if (s == ST_F32) BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<float>, dim3(BX, BY), decks, bg.start, nd);
+    else BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<double>, dim3(BX, BY), decks, bg.start, nd);
+    return hipGetLastError();
+}
+
+hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double *partial,
+                               hipStream_t st)
+{
+    if (s == ST_F32)
+        BLAUNCH("kb_field_summary", (kb_field_summary<float, true>), dim3(RED_THREADS), decks, bg.start, nd, partial);
+    else
+        BLAUNCH("kb_field_summary", (kb_field_summary<double, false>), dim3(RED_THREADS), decks, bg.start, nd,
+                partial);
+    return hipGetLastError();
+}
+#undef BLAUNCH
 
 }  // namespace clover

@@ -8,6 +8,7 @@
 // the 1..depth halo layers.  Exact moves, no arithmetic.
 #include <hip/hip_runtime.h>
 
+#include "batch.h"
 #include "fast.h"
 #include "kernels.h"
 
@@ -79,26 +80,26 @@ constexpr int HX = 64, HY = 4;
 __device__ inline int tan_a(int axis) { return axis == 0 ? 1 : 0; }
 __device__ inline int tan_b(int axis) { return axis == 2 ? 1 : 2; }
 
+// reflective halo of one (field, face) item over the tangential block (bx, by)
 template <typename S>
-__global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
+__device__ inline void reflect_body(const Lay &g, int axis, int depth, S *dst, double sgn, int kind, int hi, int ea,
+                                    int eb, int bx, int by)
 {
-    const HaloItem &it = batch.it[blockIdx.z];
-    S *const dst = static_cast<S *>(it.dst);
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
     const int ta = tan_a(axis), tb = tan_b(axis);
-    const int c = mn[ta] - depth + (int)(blockIdx.x * HX + threadIdx.x);
-    const int b = mn[tb] - depth + (int)(blockIdx.y * HY + threadIdx.y);
-    if (c > mx[ta] + it.ea + depth || b > mx[tb] + it.eb + depth) return;
+    const int c = mn[ta] - depth + (bx * HX + (int)threadIdx.x);
+    const int b = mn[tb] - depth + (by * HY + (int)threadIdx.y);
+    if (c > mx[ta] + ea + depth || b > mx[tb] + eb + depth) return;
     const int amax = mx[axis];
     for (int d = 1; d <= depth; d++) {
         int w, r;
-        if (!it.hi) {
+        if (!hi) {
             w = 1 - d;
-            r = it.kind == K_CELL ? d : 1 + d;
-        } else if (it.kind == K_CELL) {
+            r = kind == K_CELL ? d : 1 + d;
+        } else if (kind == K_CELL) {
             w = amax + d; r = amax + 1 - d;
-        } else if (it.kind == K_STAG) {
+        } else if (kind == K_STAG) {
             w = amax + 1 + d; r = amax + 1 - d;
         } else {
             w = amax + d; r = amax - d;
@@ -107,8 +108,16 @@ __global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
         iw[axis] = w; ir[axis] = r;
         iw[ta] = ir[ta] = c;
         iw[tb] = ir[tb] = b;
-        dst[idx(g, iw[0], iw[1], iw[2])] = (S)(it.sgn * (double)dst[idx(g, ir[0], ir[1], ir[2])]);
+        dst[idx(g, iw[0], iw[1], iw[2])] = (S)(sgn * (double)dst[idx(g, ir[0], ir[1], ir[2])]);
     }
+}
+
+template <typename S>
+__global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
+{
+    const HaloItem &it = batch.it[blockIdx.z];
+    reflect_body<S>(g, axis, depth, static_cast<S *>(it.dst), it.sgn, it.kind, it.hi, it.ea, it.eb, (int)blockIdx.x,
+                    (int)blockIdx.y);
 }
 
 static hipError_t run_batch_reflect(Store s, const Lay &g, int axis, int depth, const HaloBatch &b, int n,
@@ -241,6 +250,78 @@ hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *
     const dim3 grid((na + HX - 1) / HX, (nb + HY - 1) / HY, n);
     if (s == ST_F32) hipLaunchKernelGGL(k_tile_copy<float>, grid, dim3(HX, HY, 1), 0, st, g, gn, axis, depth, b);
     else hipLaunchKernelGGL(k_tile_copy<double>, grid, dim3(HX, HY, 1), 0, st, g, gn, axis, depth, b);
+    if (t_kernel_timer) t_kernel_timer->end(st);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched reflective halo over many decks (batch.h): both faces of one axis
+// for the flagged fields; every deck is at least 3 cells thick along every
+// axis, so the two faces touch disjoint cells and share the launch.
+struct BItem {
+    int field, hi, kind, ea, eb;
+    double sgn;
+};
+struct BItems {
+    int n;
+    BItem it[kMaxItems];
+};
+
+__host__ __device__ inline void dims_reflect(const Lay &g, int axis, int depth, int &ga, int &gb)
+{
+    const int mn[3] = {g.x_min, g.y_min, g.z_min};
+    const int mx[3] = {g.x_max, g.y_max, g.z_max};
+    const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
+    const int na = mx[ta] - mn[ta] + 2 + 2 * depth, nb = mx[tb] - mn[tb] + 2 + 2 * depth;
+    ga = (na + HX - 1) / HX;
+    gb = (nb + HY - 1) / HY;
+}
+
+int batch_blocks_reflect(const Lay &g, int axis, int depth, int nitems)
+{
+    int ga, gb;
+    dims_reflect(g, axis, depth, ga, gb);
+    return nitems * ga * gb;
+}
+
+template <typename S>
+__global__ void kb_reflect(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int axis,
+                           int depth, BItems items)
+{
+    const int d = deck_of(start, nd, (int)blockIdx.x);
+    const DeckDev &D = decks[d];
+    int ga, gb;
+    dims_reflect(D.g, axis, depth, ga, gb);
+    const int lb = (int)blockIdx.x - start[d];
+    const int item = lb / (ga * gb), r = lb % (ga * gb);
+    const BItem &it = items.it[item];
+    reflect_body<S>(D.g, axis, depth, static_cast<S *>(D.f[it.field]), it.sgn, it.kind, it.hi, it.ea, it.eb, r % ga,
+                    r / ga);
+}
+
+hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,
+                         const int *fields, hipStream_t st)
+{
+    BItems b;
+    b.n = 0;
+    for (int side = 0; side < 2; side++) {
+        const int face = 2 * axis + side;
+        for (int f = 0; f < NUM_FIELDS; f++) {
+            if (fields[f] != 1) continue;
+            const HaloDesc &h = kHalo[f];
+            BItem &it = b.it[b.n++];
+            it.field = f;
+            it.hi = side;
+            it.kind = h.kind[axis];
+            it.ea = h.ext[face][0];
+            it.eb = h.ext[face][1];
+            it.sgn = h.neg[axis] ? -1.0 : 1.0;
+        }
+    }
+    if (b.n == 0 || bg.total == 0) return hipSuccess;
+    if (t_kernel_timer) t_kernel_timer->begin("kb_reflect", st);
+    if (s == ST_F32) hipLaunchKernelGGL(kb_reflect<float>, dim3(bg.total), dim3(HX, HY), 0, st, decks, bg.start, nd, axis, depth, b);
+    else hipLaunchKernelGGL(kb_reflect<double>, dim3(bg.total), dim3(HX, HY), 0, st, decks, bg.start, nd, axis, depth, b);
     if (t_kernel_timer) t_kernel_timer->end(st);
     return hipGetLastError();
 }

@@ -30,6 +30,18 @@ static inline dim3 grid_of(const Range &r)
     const int l = (r).l0 + (int)blockIdx.z;                          \
     if (j > (r).j1 || k > (r).k1) return;
 
+// Block coordinates of a kernel body: the hardware's for a launch over one
+// tile, or a block of one deck's share of a launch over many decks.
+struct Blk {
+    int x, y, z;     // block index
+    int gx, gy, gz;  // grid dimensions
+};
+
+__device__ inline Blk hw_blk()
+{
+    return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+
 #define TIMED_LAUNCH(name, st, ...)                           \
     do {                                                      \
         if (t_kernel_timer) t_kernel_timer->begin(name, st);  \

@@ -29,6 +29,7 @@
 // 16 mass-flux reads per pair of node fluxes of a plain per-node kernel.
 #include <hip/hip_runtime.h>
 
+#include "batch.h"
 #include "fast.h"
 #include "launch.h"
 
@@ -146,8 +147,7 @@ __device__ inline double march_node_mass(const Quad &pm, const Quad &p0)  // cel
 }
 
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * MW)
-k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+__device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
@@ -159,9 +159,9 @@ k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__
     const double d = DIR == 3 ? geo.dz : geo.dy;
     const double V0 = geo.vol;
 
-    const int j = g.x_min - 1 + (int)blockIdx.x * 63 + lane;
-    const int wp = wmin - 1 + (int)blockIdx.y * (MW - 1) + w;
-    const int m0 = mmin + (int)blockIdx.z * MCHUNK;
+    const int j = g.x_min - 1 + bk.x * 63 + lane;
+    const int wp = wmin - 1 + bk.y * (MW - 1) + w;
+    const int m0 = mmin + bk.z * MCHUNK;
     const int m1 = min(m0 + MCHUNK - 1, mmax + 1);
     // j >= x_min-1 and wp >= wmin-1 always; upper limits of the arrays read:
     // cells and the sweep's mass flux end at +2 across the sweep, vertices at +3
@@ -237,6 +237,14 @@ k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__
 #undef OFF
 }
 
+template <int DIR, typename S>
+__global__ void __launch_bounds__(64 * MW) k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+                 const S *__restrict__ vfy, const S *__restrict__ vfz,
+                 const S *__restrict__ density1, Vel3<S> v)
+{
+    mom_march_body<DIR, S>(hw_blk(), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+}
+
 // ---------------------------------------------------------------------------
 // x sweep: lanes along x with lanes 0-1 and 62-63 as halo (60 store), waves
 // along y, march along z.  advec_mom_kernel.f90:172-206:
@@ -245,8 +253,7 @@ k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__
 __device__ inline double shfl_d(double x, int src) { return __shfl(x, src, 64); }
 
 template <typename S>
-__global__ void __launch_bounds__(64 * MW)
-k_fast_mom_x(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+__device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
              Vel3<S> v)
 {
@@ -254,9 +261,9 @@ k_fast_mom_x(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__rest
     const int lane = threadIdx.x, w = threadIdx.y;
     const double d = geo.dx;
     const double V0 = geo.vol;
-    const int j = g.x_min - 2 + (int)blockIdx.x * 60 + lane;
-    const int k = g.y_min - 1 + (int)blockIdx.y * (MW - 1) + w;
-    const int l0 = g.z_min + (int)blockIdx.z * MCHUNK;
+    const int j = g.x_min - 2 + bk.x * 60 + lane;
+    const int k = g.y_min - 1 + bk.y * (MW - 1) + w;
+    const int l0 = g.z_min + bk.z * MCHUNK;
     const int l1 = min(l0 + MCHUNK - 1, g.z_max + 1);
     // j >= x_min-2 and k >= y_min-1 always.  mass_flux_x ends at x_max+3 /
     // y_max+2, cells at x_max+2 / y_max+2, vertices at x_max+3 / y_max+3.
@@ -302,6 +309,14 @@ k_fast_mom_x(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__rest
         Pp = P;
     }
 #undef OFF
+}
+
+template <typename S>
+__global__ void __launch_bounds__(64 * MW) k_fast_mom_x(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+             const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
+             Vel3<S> v)
+{
+    mom_x_body<S>(hw_blk(), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 template <typename S>
@@ -354,6 +369,79 @@ hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect
                         static_cast<const T *>(vol_flux_z), static_cast<const T *>(density1), vel_in, vel_out, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
+}
+
+// ---------------------------------------------------------------------------
+// Batched over many decks (batch.h): the new velocities go to X_W5..X_W7.
+__host__ __device__ inline void dims_mom(int dir, const Lay &g, int &gx, int &gy, int &gz)
+{
+    const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
+    if (dir == 1) {
+        gx = (nxn + 59) / 60; gy = (nyn + MW - 2) / (MW - 1); gz = (nzn + MCHUNK - 1) / MCHUNK;
+    } else if (dir == 2) {
+        gx = (nxn + 62) / 63; gy = (nzn + MW - 2) / (MW - 1); gz = (nyn + MCHUNK - 1) / MCHUNK;
+    } else {
+        gx = (nxn + 62) / 63; gy = (nyn + MW - 2) / (MW - 1); gz = (nzn + MCHUNK - 1) / MCHUNK;
+    }
+}
+
+int batch_blocks_mom(int dir, const Lay &g)
+{
+    int gx, gy, gz;
+    dims_mom(dir, g, gx, gy, gz);
+    return gx * gy * gz;
+}
+
+template <int DIR, typename S>
+__global__ void __launch_bounds__(64 * MW)
+kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode)
+{
+    const int d = deck_of(start, nd, (int)blockIdx.x);
+    const DeckDev &D = decks[d];
+    int gx, gy, gz;
+    dims_mom(DIR, D.g, gx, gy, gz);
+    const int lb = (int)blockIdx.x - start[d];
+    const Blk bk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
+    Vel3<S> v;
+    v.in[0] = static_cast<const S *>(D.f[F_XVEL1]);
+    v.in[1] = static_cast<const S *>(D.f[F_YVEL1]);
+    v.in[2] = static_cast<const S *>(D.f[F_ZVEL1]);
+    v.out[0] = static_cast<S *>(D.f[X_W5]);
+    v.out[1] = static_cast<S *>(D.f[X_W6]);
+    v.out[2] = static_cast<S *>(D.f[X_W7]);
+    const S *vfx = static_cast<const S *>(D.f[F_VOL_FLUX_X]);
+    const S *vfy = static_cast<const S *>(D.f[F_VOL_FLUX_Y]);
+    const S *vfz = static_cast<const S *>(D.f[F_VOL_FLUX_Z]);
+    const S *rho = static_cast<const S *>(D.f[F_DENSITY1]);
+    if constexpr (DIR == 1)
+        mom_x_body<S>(bk, D.g, D.m, mode, static_cast<const S *>(D.f[F_MASS_FLUX_X]), vfx, vfy, vfz, rho, v);
+    else
+        mom_march_body<DIR, S>(bk, D.g, D.m, mode,
+                               static_cast<const S *>(D.f[DIR == 2 ? F_MASS_FLUX_Y : F_MASS_FLUX_Z]), vfx, vfy,
+                               vfz, rho, v);
+}
+
+hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
+                           int sweep_number, hipStream_t st)
+{
+    int mode = 0;
+    if (sweep_number == 1 && dir == 1) mode = 1;
+    else if (sweep_number == 1 && dir == 3) mode = 2;
+    else if (sweep_number == 2 && advect_x) mode = 3;
+    else if (sweep_number == 2 && !advect_x) mode = 4;
+    else if (sweep_number == 3 && dir == 1) mode = 5;
+    else if (sweep_number == 3 && dir == 3) mode = 6;
+    if (!mode || dir < 1 || dir > 3) return hipErrorInvalidValue;
+    if (bg.total == 0) return hipSuccess;
+    const dim3 grid(bg.total), blk(64, MW);
+#define L(D_, T_) TIMED_LAUNCH("kb_mom", st, (kb_mom<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode)
+    if (s == ST_F32) {
+        if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float);
+    } else {
+        if (dir == 1) L(1, double); else if (dir == 2) L(2, double); else L(3, double);
+    }
+#undef L
+    return hipGetLastError();
 }
 
 }  // namespace clover

@@ -245,3 +245,72 @@ def cuboid_deck(cells: int, end_step: int, summary_frequency: int, tiles: int = 
     if tiles > 1:
         text += " 3d_tile_decomposition\n"
     return text + "*endclover\n"
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014): the ensemble generator's PRNG."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = seed & self.MASK
+
+    def next(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        """Uniform on [lo, hi) from the top 53 bits."""
+        return lo + (hi - lo) * ((self.next() >> 11) * 2.0 ** -53)
+
+
+ENSEMBLE_SEED = 0x434C4F5645523344  # fixed seed of the batched workload
+ENSEMBLE_EDGES = tuple(range(16, 97, 8))  # 16, 24, ..., 96
+
+
+def ensemble_deck_texts(n: int, seed: int = ENSEMBLE_SEED, end_step: int = 200,
+                        edges=ENSEMBLE_EDGES) -> List[str]:
+    """Input decks of the batched ensemble: per deck an edge drawn uniformly
+    from `edges` on every axis, background density U[0.1,0.5] and energy
+    U[0.5,1.5], one cuboid of density U[0.5,2.0] and energy U[1.5,5.0]
+    covering U[10%,40%] of each axis at a uniform position, on the domain
+    [0,10]^3 with the cuboid deck's time stepping (dt0 0.04, rise 1.5)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        cells = [edges[int(rng.uniform() * len(edges))] for _ in range(3)]
+        rho_b, e_b = rng.uniform(0.1, 0.5), rng.uniform(0.5, 1.5)
+        rho_c, e_c = rng.uniform(0.5, 2.0), rng.uniform(1.5, 5.0)
+        box = []
+        for _ in range(3):
+            length = 10.0 * rng.uniform(0.1, 0.4)
+            lo = rng.uniform(0.0, 10.0 - length)
+            box += [lo, lo + length]
+        out.append(f"""*clover
+ state 1 density={rho_b!r} energy={e_b!r}
+ state 2 density={rho_c!r} energy={e_c!r} geometry=cuboid xmin={box[0]!r} xmax={box[1]!r} ymin={box[2]!r} ymax={box[3]!r} zmin={box[4]!r} zmax={box[5]!r}
+ x_cells={cells[0]}
+ y_cells={cells[1]}
+ z_cells={cells[2]}
+ xmin=0.0
+ ymin=0.0
+ zmin=0.0
+ xmax=10.0
+ ymax=10.0
+ zmax=10.0
+ initial_timestep=0.04
+ timestep_rise=1.5
+ end_time=1000.0
+ end_step={end_step}
+ summary_frequency=0
+ tiles_per_chunk 1
+*endclover
+""")
+    return out
+
+
+def ensemble_decks(n: int, seed: int = ENSEMBLE_SEED, end_step: int = 200, edges=ENSEMBLE_EDGES) -> List[Deck]:
+    return [parse_deck(t) for t in ensemble_deck_texts(n, seed, end_step, edges)]

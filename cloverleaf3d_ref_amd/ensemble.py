@@ -1,0 +1,74 @@
+"""Many independent decks advanced together on one device (clover_ensemble_*).
+
+Each deck is one tile with its own mesh, states, dt and time; every kernel
+launch covers all decks, dt is a per-deck minimum formed on the device, and
+the step of each advect_x parity is captured in a graph and replayed.  The
+fields evolve exactly as a :class:`~cloverleaf3d_ref_amd.chunk.Chunk` running
+each deck alone with the fused kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+from . import _lib
+from .chunk import FIELD_NAMES, FP32, _CLASS, _extents, to_c_deck
+from .deck import Deck
+
+
+class Ensemble:
+    def __init__(self, decks: Sequence[Deck], device: int = 0, fp32: bool = False):
+        self.decks = list(decks)
+        self._L = _lib.lib()
+        arr = (_lib.CloverDeck * len(self.decks))(*[to_c_deck(d) for d in self.decks])
+        self._arr = arr
+        h = ctypes.c_void_p()
+        _lib.check(self._L.clover_ensemble_create(arr, len(self.decks), device, FP32 if fp32 else 0, ctypes.byref(h)),
+                   "clover_ensemble_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            self._L.clover_ensemble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, steps: int, graphs: bool = True) -> None:
+        _lib.check(self._L.clover_ensemble_run(self._h, steps, 1 if graphs else 0), "clover_ensemble_run")
+
+    def summaries(self) -> List[Dict[str, float]]:
+        out = (ctypes.c_double * (5 * len(self.decks)))()
+        _lib.check(self._L.clover_ensemble_summaries(self._h, out), "clover_ensemble_summaries")
+        return [dict(vol=out[5 * d], mass=out[5 * d + 1], ie=out[5 * d + 2], ke=out[5 * d + 3],
+                     press=out[5 * d + 4]) for d in range(len(self.decks))]
+
+    def state(self):
+        n = len(self.decks)
+        step = ctypes.c_int()
+        time, dt = (ctypes.c_double * n)(), (ctypes.c_double * n)()
+        small = (ctypes.c_int * n)()
+        _lib.check(self._L.clover_ensemble_state(self._h, ctypes.byref(step), time, dt, small), "clover_ensemble_state")
+        return step.value, list(time), list(dt), list(small)
+
+    def fetch(self, deck: int, name: str) -> np.ndarray:
+        d = self.decks[deck]
+        a = np.zeros(_extents(_CLASS[name], d.x_cells, d.y_cells, d.z_cells), dtype=np.float64, order="F")
+        _lib.check(self._L.clover_ensemble_fetch_field(self._h, deck, FIELD_NAMES.index(name),
+                                                       ctypes.c_void_p(a.ctypes.data)), "clover_ensemble_fetch_field")
+        return a
+
+    def device_bytes(self) -> int:
+        return int(self._L.clover_ensemble_device_bytes(self._h))

@@ -233,6 +233,29 @@ long long clover_chunk_device_bytes(const clover_chunk *c);
 int clover_chunk_set_kernel_timing(clover_chunk *c, int on);
 int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
 
+/* ------------------------------------------------------------ ensembles */
+
+/* Many independent decks advanced together on one device: each deck is one
+ * tile with its own mesh, states, dt and time (a per-deck minimum of the
+ * calc_dt cells on the device), and every kernel launch covers all decks.
+ * The fields evolve exactly as clover_chunk_* evolves each deck alone with
+ * the fused kernels.  Decks need tiles_per_chunk 1 and at least 3 cells
+ * along every axis.  flags: 0 or CLOVER_CHUNK_FP32. */
+typedef struct clover_ensemble clover_ensemble;
+int clover_ensemble_create(const clover_deck *decks, int ndecks, int device, int flags, clover_ensemble **out);
+int clover_ensemble_destroy(clover_ensemble *e);
+/* Advance every deck `steps` steps (hydro.f90:48-68, no summaries); with
+ * use_graphs the step of each advect_x parity is captured in a graph once
+ * and replayed. */
+int clover_ensemble_run(clover_ensemble *e, int steps, int use_graphs);
+/* field_summary of every deck: out[5*d + (vol, mass, ie, ke, press)] */
+int clover_ensemble_summaries(clover_ensemble *e, double *out);
+/* steps taken; per deck: simulated time, last dt, small-dt flag */
+int clover_ensemble_state(clover_ensemble *e, int *step, double *time, double *dt, int *small);
+/* one field (0..18) of deck d as a dense Fortran array of its class */
+int clover_ensemble_fetch_field(clover_ensemble *e, int deck, int field, double *host);
+long long clover_ensemble_device_bytes(const clover_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

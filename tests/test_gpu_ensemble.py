@@ -1,0 +1,96 @@
+"""The ensemble driver against each deck run alone.
+
+Every deck of an ensemble evolves exactly as a Chunk running it alone with
+the fused kernels: after the same number of steps its summary totals, time
+and dt are bit-identical, in FP64 and in the FP32-store form, whether the
+steps run eagerly or as replays of the captured step graphs.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cloverleaf3d_ref_amd.chunk import Chunk
+from cloverleaf3d_ref_amd.deck import ensemble_decks, parse_deck
+from cloverleaf3d_ref_amd.ensemble import Ensemble
+
+pytestmark = pytest.mark.gpu
+
+UNEVEN = """*clover
+ state 1 density={rb} energy={eb}
+ state 2 density={rc} energy={ec} xvel={u} geometry=cuboid xmin=1.5 xmax=4.0 ymin=2.0 ymax=6.5 zmin=0.5 zmax=3.0
+ state 3 density=0.8 energy=2.0 zvel=-0.1 geometry=sphere xmin=6.0 ymin=3.0 zmin=4.0 radius=1.6
+ x_cells={nx} y_cells={ny} z_cells={nz}
+ xmin=0 ymin=0 zmin=0 xmax=9 ymax=7 zmax=6
+ initial_timestep=0.02 timestep_rise=1.3 end_time=100 end_step=100 summary_frequency=0
+*endclover"""
+
+
+def _uneven():
+    shapes = [(3, 3, 3), (23, 17, 13), (8, 31, 5), (70, 9, 11), (16, 16, 70), (5, 66, 4)]
+    out = []
+    for q, (nx, ny, nz) in enumerate(shapes):
+        out.append(parse_deck(UNEVEN.format(nx=nx, ny=ny, nz=nz, rb=0.2 + 0.05 * q, eb=1.0 + 0.1 * q,
+                                            rc=1.0 + 0.2 * q, ec=2.5 - 0.1 * q, u=0.05 * q)))
+    return out
+
+
+def _alone(deck, steps, fp32):
+    with Chunk(deck, fp32=fp32) as c:
+        for _ in range(steps):
+            c.step()
+        step, time, dt = c.state()
+        s = c.field_summary()
+    return s, time, dt
+
+
+def _check(decks, steps, fp32, graphs, split=None):
+    print(f"[gpu] ensemble of {len(decks)} decks, {steps} steps, fp32={fp32} graphs={graphs}", flush=True)
+    with Ensemble(decks, fp32=fp32) as e:
+        if split:
+            e.run(split, graphs=graphs)
+            e.run(steps - split, graphs=graphs)
+        else:
+            e.run(steps, graphs=graphs)
+        sums = e.summaries()
+        step, times, dts, small = e.state()
+    assert step == steps and not any(small)
+    for d, deck in enumerate(decks):
+        want, time, dt = _alone(deck, steps, fp32)
+        assert times[d] == time and dts[d] == dt, d
+        for k in ("vol", "mass", "ie", "ke", "press"):
+            assert sums[d][k] == want[k], (d, k, sums[d][k], want[k])
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_uneven_decks_fp64(graphs):
+    _check(_uneven(), 14, False, graphs, split=5)
+
+
+def test_uneven_decks_fp32_graphs():
+    _check(_uneven(), 14, True, True, split=9)
+
+
+def test_generated_decks():
+    decks = ensemble_decks(6, end_step=200, edges=(16, 24, 32))
+    _check(decks, 12, False, True)
+
+
+def test_graph_replays_equal_eager_fields():
+    decks = _uneven()[:3]
+    with Ensemble(decks) as a, Ensemble(decks) as b:
+        a.run(9, graphs=True)
+        b.run(9, graphs=False)
+        for d in range(len(decks)):
+            for name in ("density0", "energy0", "xvel0", "zvel0", "mass_flux_y"):
+                assert np.array_equal(a.fetch(d, name), b.fetch(d, name)), (d, name)
+
+
+def test_rejects_tiled_and_thin_decks():
+    from cloverleaf3d_ref_amd._lib import CloverError
+    tiled = parse_deck(UNEVEN.format(nx=8, ny=8, nz=8, rb=0.2, eb=1.0, rc=1.0, ec=2.5, u=0.0).replace(
+        "*endclover", " tiles_per_chunk 2\n*endclover"))
+    thin = parse_deck(UNEVEN.format(nx=8, ny=2, nz=8, rb=0.2, eb=1.0, rc=1.0, ec=2.5, u=0.0))
+    for bad in (tiled, thin):
+        with pytest.raises(CloverError):
+            Ensemble([bad])
