@@ -73,12 +73,12 @@ def kernel_domain_cells(name: str, n: int) -> int:
     return n ** 3
 
 
-def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, fp32: bool = False):
+def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, fp32: bool = False, tiles: int = 1):
     from cloverleaf3d_ref_amd import _lib
     from cloverleaf3d_ref_amd.chunk import Chunk
     from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
 
-    deck = parse_deck(cuboid_deck(cells, warmup + steps + 1000, summary_frequency))
+    deck = parse_deck(cuboid_deck(cells, warmup + steps + 1000, summary_frequency, tiles=tiles))
     t0 = time.perf_counter()
     c = Chunk(deck, fp32=fp32)
     setup_s = time.perf_counter() - t0
@@ -116,6 +116,58 @@ def dominant_roofline(ktimes, n: int, bytes_per_value: int, traffic_file=None):
     return dict(bound="hbm", kernel=name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=(achieved / HBM_PEAK_GBS) if achieved else None, traffic=traffic,
                 avg_launch_us=avg_s * 1e6, launches=launches, bytes_per_launch=per_launch_bytes)
+
+
+def ensemble_bench(ndecks: int, steps: int, fp32: bool = False, warmup: int = 2):
+    """The batched form: all decks advanced together (graph replays), then
+    each deck's final summary.  Setup (mesh and states) and the warmup steps
+    that capture the graph of each advect_x parity are outside the timed
+    region, which covers `steps` steps of every deck."""
+    from cloverleaf3d_ref_amd.deck import ensemble_decks
+    from cloverleaf3d_ref_amd.ensemble import Ensemble
+
+    decks = ensemble_decks(ndecks, end_step=warmup + steps)
+    t0 = time.perf_counter()
+    e = Ensemble(decks, fp32=fp32)
+    setup_s = time.perf_counter() - t0
+    e.run(warmup)
+    t0 = time.perf_counter()
+    e.run(steps)
+    sums = e.summaries()
+    wall = time.perf_counter() - t0
+    _, _, _, small = e.state()
+    out = dict(wall_s=wall, setup_s=setup_s, device_gib=e.device_bytes() / 2 ** 30, small=sum(small), sums=sums)
+    e.close()
+    return out, decks
+
+
+def ensemble_errors(sums32, sums64):
+    """Relative difference of every deck's totals, FP32 vs FP64: median,
+    99th percentile and maximum per total."""
+    import numpy as np
+    out = {}
+    for k in ("mass", "ie", "ke", "press"):
+        e = np.array([abs(a[k] / b[k] - 1) for a, b in zip(sums32, sums64)])
+        out[k] = dict(median=float(np.median(e)), p99=float(np.percentile(e, 99)), max=float(e.max()))
+    return out
+
+
+def one_at_a_time(decks, steps: int) -> float:
+    """The same decks, each run alone by the plain FP64 form (one deck's
+    hydro loop after another, dt back to the host every step as the
+    reference driver does); only the steps and final summaries are timed."""
+    from cloverleaf3d_ref_amd.chunk import Chunk
+
+    total = 0.0
+    for d in decks:
+        with Chunk(d) as c:
+            c.run(max_steps=0)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                c.step()
+            c.field_summary()
+            total += time.perf_counter() - t0
+    return total
 
 
 def calibration_copy(mib: int) -> None:
@@ -170,6 +222,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-store run")
+    ap.add_argument("--decks", type=int, default=512, help="decks of the batched run (0 skips it)")
+    ap.add_argument("--tiled-cells", type=int, default=640, help="edge of the tiled run (0 skips it)")
+    ap.add_argument("--tiled-steps", type=int, default=50)
+    ap.add_argument("--tiles", type=int, default=8)
+    ap.add_argument("--deck-steps", type=int, default=200)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -200,6 +257,40 @@ def main():
                                         sorted(ktimes32.items(), key=lambda kv: -kv[1][0])},
                     device_gib=info32["device_bytes"] / 2 ** 30)
 
+    tiled = None
+    if args.tiled_cells:
+        nt = args.tiled_cells
+        w64, k64, i64 = gpu_bench(nt, args.tiled_steps, 2, tiles=args.tiles)
+        w32, k32, i32 = gpu_bench(nt, args.tiled_steps, 2, tiles=args.tiles, fp32=True)
+        f64, f32 = i64["final"], i32["final"]
+        tiled = dict(workload=f"cuboid deck {nt}^3 cells as {args.tiles} tiles (3d decomposition), summary every 25",
+                     steps=args.tiled_steps,
+                     value=nt ** 3 * args.tiled_steps / w64, unit="cell-steps/s", ms_per_step=w64 / args.tiled_steps * 1e3,
+                     fp32_value=nt ** 3 * args.tiled_steps / w32, fp32_ms_per_step=w32 / args.tiled_steps * 1e3,
+                     fp32_rel_err_vs_fp64={k: abs(f32[k] / f64[k] - 1) for k in ("mass", "ie", "ke", "press")},
+                     tile_halo_ms_per_step=round(k64.get("k_tile_copy", (0.0, 0))[0] / args.tiled_steps, 4),
+                     device_gib=i64["device_bytes"] / 2 ** 30, setup_s=i64["setup_s"])
+
+    batched = None
+    if args.decks:
+        b64, decks = ensemble_bench(args.decks, args.deck_steps)
+        b32, _ = ensemble_bench(args.decks, args.deck_steps, fp32=True)
+        alone = one_at_a_time(decks, args.deck_steps)
+        ncell = sum(d.x_cells * d.y_cells * d.z_cells for d in decks)
+        batched = dict(decks=args.decks, steps_per_deck=args.deck_steps, cells=ncell,
+                       workload="seeded ensemble: edges {16..96 step 8}^3, background rho U[0.1,0.5] e U[0.5,1.5], "
+                                "one cuboid rho U[0.5,2] e U[1.5,5] over 10-40% of each axis",
+                       value=args.decks / b64["wall_s"], unit="decks/s", wall_s=b64["wall_s"],
+                       cell_steps_per_s=ncell * args.deck_steps / b64["wall_s"],
+                       one_at_a_time=dict(value=args.decks / alone, unit="decks/s", wall_s=alone),
+                       speedup_vs_one_at_a_time=alone / b64["wall_s"],
+                       fp32_store=dict(value=args.decks / b32["wall_s"], unit="decks/s", wall_s=b32["wall_s"],
+                                       rel_err_vs_fp64=ensemble_errors(b32["sums"], b64["sums"]),
+                                       note="totals after the same step count; each deck's dt follows its "
+                                            "own state, so the FP32 and FP64 runs reach slightly different "
+                                            "simulated times"),
+                       small_dt_decks=b64["small"], setup_s=b64["setup_s"], device_gib=b64["device_gib"])
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -224,6 +315,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_store": fp32,
+        "batched": batched,
+        "tiled": tiled,
         "step_hbm_gbs_kbc": 1192 * value / 1e9,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
                                sorted(ktimes.items(), key=lambda kv: -kv[1][0])},

@@ -138,6 +138,42 @@ __global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(Lay g, Geo m, int mo
     cell_x_body<S>(hw_blk(), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out);
 }
 
+// The volume fluxes pre_vol reads for the y and z sweeps (modes 3-6), kept
+// raw so a face's loads can be issued a step ahead of their use.
+template <typename S>
+struct PreRaw {
+    S a, b, c, d, e, f;  // storage type: converted where used, so the loads stay in flight
+};
+
+template <typename S>
+__device__ inline PreRaw<S> load_pre_raw(int mode, const Lay &g, int i, const S *__restrict__ vfx,
+                                         const S *__restrict__ vfy, const S *__restrict__ vfz)
+{
+    PreRaw<S> r{0, 0, 0, 0, 0, 0};
+    switch (mode) {
+    case 3: r.a = vfy[i + g.ld]; r.b = vfy[i]; r.c = vfz[i + g.plane]; r.d = vfz[i]; break;
+    case 4: r.a = vfy[i + g.ld]; r.b = vfy[i]; r.c = vfx[i + 1]; r.d = vfx[i]; break;
+    case 5:
+        r.a = vfx[i + 1]; r.b = vfx[i]; r.c = vfy[i + g.ld]; r.d = vfy[i]; r.e = vfz[i + g.plane]; r.f = vfz[i];
+        break;
+    default: r.a = vfz[i + g.plane]; r.b = vfz[i]; break;
+    }
+    return r;
+}
+
+// cell_pre_vol's expressions for modes 3-6 on the raw values
+template <typename S>
+__device__ inline double pre_of_raw(int mode, double V0, const PreRaw<S> &r)
+{
+    const double a = r.a, b = r.b, c = r.c, d = r.d, e = r.e, f = r.f;
+    switch (mode) {
+    case 3:
+    case 4: return V0 + a - b + c - d;
+    case 5: return V0 + (a - b + c - d + e - f);
+    default: return V0 + a - b;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // y and z sweeps: lanes along x (interior j), threadIdx.y along the cross
 // axis, march along the sweep axis over faces [smin, smax+2] in chunks.
@@ -178,15 +214,26 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
         else
             cell_face_flux(vf_lo, pv_m1, vd, r_0, r_m1, r_m2, e_0, e_m1, e_m2, mf_lo, ef_lo);  // f0 <= smax+1
     }
+    // face f0's loads; each step issues the next face's before its arithmetic
+    S vf_n = vfs[OFF(f0)];
+    PreRaw<S> pr_n = load_pre_raw(mode, g, OFF(f0), vfx, vfy, vfz);
+    S r_n = (S)0, e_n = (S)0;  // upwind for vf <= 0 is min(f+1, smax+2)
+    if (f0 + 1 <= smax + 2) {
+        r_n = density1[OFF(f0 + 1)];
+        e_n = energy1[OFF(f0 + 1)];
+    }
     for (int f = f0; f <= f1; f++) {
         const int i = OFF(f);
-        const double vf = vfs[i];
-        const double pv_0 = pre(f);
-        // upwind for vf <= 0 is min(f+1, smax+2)
-        double r_p1 = r_0, e_p1 = e_0;
-        if (f + 1 <= smax + 2) {
-            r_p1 = density1[OFF(f + 1)];
-            e_p1 = energy1[OFF(f + 1)];
+        const double vf = (double)vf_n;
+        const double pv_0 = pre_of_raw(mode, m.vol, pr_n);
+        const double r_p1 = f + 1 <= smax + 2 ? (double)r_n : r_0, e_p1 = f + 1 <= smax + 2 ? (double)e_n : e_0;
+        if (f < f1) {
+            vf_n = vfs[OFF(f + 1)];
+            pr_n = load_pre_raw(mode, g, OFF(f + 1), vfx, vfy, vfz);
+            if (f + 2 <= smax + 2) {
+                r_n = density1[OFF(f + 2)];
+                e_n = energy1[OFF(f + 2)];
+            }
         }
         double mf, ef;
         if (vf > 0.0)

@@ -21,6 +21,7 @@
 #include "kernels.h"
 #include "runtime.h"
 #include "driver.h"
+#include "setup.h"
 
 namespace clover {
 
@@ -217,7 +218,7 @@ void decompose(const clover_deck &d, std::vector<Tile> &tiles)
 // initialise_chunk.f90:37-44 + initialise_chunk_kernel.f90:63-178 and
 // generate_chunk_kernel.f90:82-207, into dense Fortran arrays, then uploaded.
 
-void host_init(const clover_deck &d, const Tile &t, HostInit &h)
+void host_init(const clover_deck &d, const Tile &t, HostInit &h, bool fields)
 {
     const int nx = t.nx, ny = t.ny, nz = t.nz;
     const double dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
@@ -240,6 +241,7 @@ void host_init(const clover_deck &d, const Tile &t, HostInit &h)
     for (int k = -1; k <= ny + 2; k++) { h.celly[k + 1] = 0.5 * (h.vertexy[k + 1] + h.vertexy[k + 2]); h.celldy[k + 1] = dy; }
     for (int l = -1; l <= nz + 2; l++) { h.cellz[l + 1] = 0.5 * (h.vertexz[l + 1] + h.vertexz[l + 2]); h.celldz[l + 1] = dz; }
 
+    if (!fields) return;
     const size_t cx = nx + 4, cy = ny + 4, cz = nz + 4, vx = nx + 5, vy = ny + 5, vz = nz + 5;
     h.volume.assign(cx * cy * cz, 0.0);
     h.xarea.assign(vx * cy * cz, 0.0);
@@ -315,17 +317,6 @@ static double *dev_alloc_bytes(clover_chunk *c, size_t bytes)
 
 static double *dev_alloc(clover_chunk *c, size_t n) { return dev_alloc_bytes(c, n * sizeof(double)); }
 
-// a 3-D array of the chunk's element type, filled from a dense FP64 host array
-static void upload_field(clover_chunk *c, const Lay &g, int cls, const std::vector<double> &h, double *dev)
-{
-    if (c->store == ST_F64) {
-        upload_dense(g, cls, h.data(), dev, c->st);
-        return;
-    }
-    std::vector<float> f(h.begin(), h.end());  // round to nearest
-    upload_dense_any(g, cls, f.data(), dev, sizeof(float), c->st);
-    hip_check(hipStreamSynchronize(c->st), "sync");  // f dies here
-}
 
 static double *dev_upload_1d(clover_chunk *c, const std::vector<double> &v)
 {
@@ -744,17 +735,13 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
             const int64_t n = lay_elems(t.g);
             if (n >= (int64_t(1) << 31)) throw Error(E_ARG, "tile too large for 32-bit element offsets");
             for (int a = 0; a < NUM_ARRAYS; a++) t.f[a] = dev_alloc_bytes(c, (size_t)n * store_bytes(c->store));
+            // 1-D mesh on the host, the 3-D fields painted on the device
             HostInit h;
-            host_init(d, t, h);
-            upload_field(c, t.g, AC_CELL, h.volume, t.f[X_VOLUME]);
-            upload_field(c, t.g, AC_XF, h.xarea, t.f[X_XAREA]);
-            upload_field(c, t.g, AC_YF, h.yarea, t.f[X_YAREA]);
-            upload_field(c, t.g, AC_ZF, h.zarea, t.f[X_ZAREA]);
-            upload_field(c, t.g, AC_CELL, h.density0, t.f[F_DENSITY0]);
-            upload_field(c, t.g, AC_CELL, h.energy0, t.f[F_ENERGY0]);
-            upload_field(c, t.g, AC_VERT, h.xvel0, t.f[F_XVEL0]);
-            upload_field(c, t.g, AC_VERT, h.yvel0, t.f[F_YVEL0]);
-            upload_field(c, t.g, AC_VERT, h.zvel0, t.f[F_ZVEL0]);
+            host_init(d, t, h, false);
+            hip_check(launch_generate(c->store, t.g, gen_state(d, t.left, t.bottom, t.back), t.f[F_DENSITY0],
+                                      t.f[F_ENERGY0], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], t.f[X_VOLUME],
+                                      t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], c->st),
+                      "generate");
             t.cellx = dev_upload_1d(c, h.cellx);
             t.celly = dev_upload_1d(c, h.celly);
             t.cellz = dev_upload_1d(c, h.cellz);

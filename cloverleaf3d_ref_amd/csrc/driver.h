@@ -44,6 +44,8 @@ struct HostInit {
     std::vector<double> cellx, celly, cellz, celldx, celldy, celldz;
     std::vector<double> volume, xarea, yarea, zarea, density0, energy0, xvel0, yvel0, zvel0;
 };
-void host_init(const clover_deck &d, const Tile &t, HostInit &h);
+// fields=false fills only the 1-D arrays (the 3-D fields are painted on the
+// device, setup.h)
+void host_init(const clover_deck &d, const Tile &t, HostInit &h, bool fields = true);
 
 }  // namespace clover

@@ -19,6 +19,7 @@
 #include "batch.h"
 #include "kernels.h"
 #include "runtime.h"
+#include "setup.h"
 
 using namespace clover;
 
@@ -276,22 +277,11 @@ int clover_ensemble_create(const clover_deck *decks, int ndecks, int device, int
                 D.f[a] = p;
                 p += arr;
             }
-            // mesh and states on the host (generate_chunk), uploaded once
-            HostInit h;
-            host_init(decks[d], tiles[d], h);
-            struct Up { int id, cls; const std::vector<double> *v; } ups[] = {
-                {F_DENSITY0, AC_CELL, &h.density0}, {F_ENERGY0, AC_CELL, &h.energy0}, {F_XVEL0, AC_VERT, &h.xvel0},
-                {F_YVEL0, AC_VERT, &h.yvel0}, {F_ZVEL0, AC_VERT, &h.zvel0}};
-            for (auto &u : ups) {
-                if (e->store == ST_F64) {
-                    upload_dense(D.g, u.cls, u.v->data(), (double *)D.f[u.id], e->st);
-                    hip_check(hipStreamSynchronize(e->st), "sync");
-                } else {
-                    std::vector<float> fl(u.v->begin(), u.v->end());
-                    upload_dense_any(D.g, u.cls, fl.data(), D.f[u.id], sizeof(float), e->st);
-                    hip_check(hipStreamSynchronize(e->st), "sync");
-                }
-            }
+            // mesh and states painted on the device (generate_chunk)
+            hip_check(launch_generate(e->store, D.g, gen_state(decks[d], 1, 1, 1), D.f[F_DENSITY0], D.f[F_ENERGY0],
+                                      D.f[F_XVEL0], D.f[F_YVEL0], D.f[F_ZVEL0], nullptr, nullptr, nullptr, nullptr,
+                                      e->st),
+                      "generate");
         }
         hip_check(hipMalloc(&e->dev, sizeof(DeckDev) * ndecks), "hipMalloc decks");
         hip_check(hipMemcpy(e->dev, e->host.data(), sizeof(DeckDev) * ndecks, hipMemcpyHostToDevice), "H2D decks");

@@ -121,6 +121,72 @@ __device__ inline void exchange(double (*buf)[MW][64], int lane, int w, double a
     P.xw = buf[1][wd][lp];
 }
 
+// The loads of one plane, kept raw so that they can be issued a plane ahead
+// (after the barrier of the plane in hand) and consumed by the next plane:
+// the sweep's mass flux, density1 and the volume fluxes post_vol needs.
+template <typename S>
+struct Raw {
+    S a, rho, v0, v1, v2, v3;  // storage type: converted where used, so the loads stay in flight
+};
+
+template <typename S>
+__device__ inline Raw<S> load_raw(int mode, const Lay &g, int i, bool ok_a, bool ok_p, const S *__restrict__ mf,
+                                  const S *__restrict__ vfx, const S *__restrict__ vfy, const S *__restrict__ vfz,
+                                  const S *__restrict__ density1)
+{
+    Raw<S> r{0, 0, 0, 0, 0, 0};
+    if (ok_a) r.a = mf[i];
+    if (ok_p) {
+        r.rho = density1[i];
+        switch (mode) {
+        case 1: r.v0 = vfy[i + g.ld]; r.v1 = vfy[i]; r.v2 = vfz[i + g.plane]; r.v3 = vfz[i]; break;
+        case 2: r.v0 = vfx[i + 1]; r.v1 = vfx[i]; r.v2 = vfy[i + g.ld]; r.v3 = vfy[i]; break;
+        case 3: r.v0 = vfz[i + g.plane]; r.v1 = vfz[i]; break;
+        case 4: r.v0 = vfx[i + 1]; r.v1 = vfx[i]; break;
+        default: break;
+        }
+    }
+    return r;
+}
+
+// density1 * post_vol with mom_post_vol's expression order
+template <typename S>
+__device__ inline double raw_cell_mass(int mode, double V0, const Raw<S> &r, bool ok_p)
+{
+    if (!ok_p) return 0.0;
+    double pv;
+    switch (mode) {
+    case 1:
+    case 2: pv = V0 + (double)r.v0 - (double)r.v1 + (double)r.v2 - (double)r.v3; break;
+    case 3:
+    case 4: pv = V0 + (double)r.v0 - (double)r.v1; break;
+    default: pv = V0; break;
+    }
+    return (double)r.rho * pv;
+}
+
+// exchange() in two halves, so loads can be issued between them
+__device__ inline void publish(double (*buf)[MW][64], int lane, int w, double a, double p)
+{
+    buf[0][w][lane] = a;
+    buf[1][w][lane] = p;
+    __syncthreads();
+}
+
+template <int DXA, int DXP>
+__device__ inline void gather(double (*buf)[MW][64], int lane, int w, double a, double p, Quad &A, Quad &P)
+{
+    const int la = min(max(lane + DXA, 0), 63), lp = min(max(lane + DXP, 0), 63), wd = w > 0 ? w - 1 : 0;
+    A.c = a;
+    A.x = buf[0][w][la];
+    A.w = buf[0][wd][lane];
+    A.xw = buf[0][wd][la];
+    P.c = p;
+    P.x = buf[1][w][lp];
+    P.w = buf[1][wd][lane];
+    P.xw = buf[1][wd][lp];
+}
+
 // ---------------------------------------------------------------------------
 // y and z sweeps: lanes along x (lane 0 is the j-1 halo, 63 lanes store),
 // waves along the other axis, march along the sweep axis.
@@ -209,9 +275,26 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         }
     }
 
+    // the plane m+2 the first step needs, then one plane ahead of the step
+    Raw<S> rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1);
+    S vn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
     for (int m = m0; m <= m1; m++) {
+        const Raw<S> rc = rn;
+        double vp2[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) vp2[c] = (double)vn[c];
+        const double a = (double)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
+        publish(xbuf[b], lane, w, a, p);
+        if (m < m1) {
+            rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 3)] : (S)0;
+        }
         Quad Fc, Pc;
-        plane(m + 2, Fc, Pc, b);
+        gather<-1, -1>(xbuf[b], lane, w, a, p, Fc, Pc);
+        b ^= 1;
         const double nf1 = march_node_flux<DIR>(Fb, Fc);  // nf(m+1)
         const double mp2 = march_node_mass<DIR>(Pb, Pc);  // mp(m+2), garbage at mmax+3 (never used)
         const double mq1 = mp1 - nf0 + nf1;              // mq(m+1)
@@ -219,13 +302,12 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         const int i = OFF(m);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double vp2 = vel_at(c, m + 2);
-            const double fhi = mom_face_flux(nf0, sig, d, vm1[c], v0[c], vp1[c], vp2);  // F(m)
+            const double fhi = mom_face_flux(nf0, sig, d, vm1[c], v0[c], vp1[c], vp2[c]);  // F(m)
             if (store) v.out[c][i] = (v0[c] * mq0 + flo[c] - fhi) / mp0;
             flo[c] = fhi;
             vm1[c] = v0[c];
             v0[c] = vp1[c];
-            vp1[c] = vp2;
+            vp1[c] = vp2[c];
         }
         nf0 = nf1;
         mp0 = mp1;
@@ -286,9 +368,26 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     Quad Fp, Pp;  // plane l-1
     plane(l0 - 1, Fp, Pp, b);
     const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63, lp2 = lane < 62 ? lane + 2 : 63;
+    // plane l's loads are issued during plane l-1 (after its barrier)
+    Raw<S> rn = load_raw(mode, g, OFF(l0), ok_face, ok_cell, mf, vfx, vfy, vfz, density1);
+    S vn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l0)] : (S)0;
     for (int l = l0; l <= l1; l++) {
+        const Raw<S> rc = rn;
+        double vc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) vc[c] = (double)vn[c];
+        const double a = (double)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell);
+        publish(xbuf[b], lane, w, a, p);
+        if (l < l1) {
+            rn = load_raw(mode, g, OFF(l + 1), ok_face, ok_cell, mf, vfx, vfy, vfz, density1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l + 1)] : (S)0;
+        }
         Quad F, P;
-        plane(l, F, P, b);
+        gather<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
+        b ^= 1;
         // F.x = f(+1,0), F.w = f(0,-1), F.xw = f(+1,-1); P.x = cell(-1,0), ...
         const double nf = 0.125 * (F.w + F.c + F.xw + F.x + Fp.w + Fp.c + Fp.xw + Fp.x);
         const double mp = 0.125 * (P.w + P.c + P.xw + P.x + Pp.w + Pp.c + Pp.xw + Pp.x);
@@ -299,7 +398,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
         const int i = OFF(l);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double v0 = ok_vert ? (double)v.in[c][i] : 0.0;
+            const double v0 = vc[c];
             const double vm1 = shfl_d(v0, lm), vp1 = shfl_d(v0, lp), vp2 = shfl_d(v0, lp2);
             const double fhi = mom_face_flux(nf, sig, d, vm1, v0, vp1, vp2);  // face j | j+1
             const double flo = shfl_d(fhi, lm);                                // face j-1 | j
