@@ -40,6 +40,31 @@ __device__ inline Quad4 quad_down(const S *__restrict__ p, int i, int ld)
     return Quad4{{(double)p[i - 1 - ld], (double)p[i - ld], (double)p[i - 1], (double)p[i]}};
 }
 
+// the same corners in the storage type, for loads issued a plane ahead of
+// their use (converted by widen() where the arithmetic starts)
+template <typename S>
+struct QuadS {
+    S v[4];
+};
+
+template <typename S>
+__device__ inline QuadS<S> quad_up_raw(const S *__restrict__ p, int i, int ld)
+{
+    return QuadS<S>{{p[i], p[i + 1], p[i + ld], p[i + 1 + ld]}};
+}
+
+template <typename S>
+__device__ inline QuadS<S> quad_down_raw(const S *__restrict__ p, int i, int ld)
+{
+    return QuadS<S>{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
+}
+
+template <typename S>
+__device__ inline Quad4 widen(const QuadS<S> &q)
+{
+    return Quad4{{(double)q.v[0], (double)q.v[1], (double)q.v[2], (double)q.v[3]}};
+}
+
 // the column's planes [l0, l1] of the chunk this workgroup marches
 __device__ inline void chunk_planes(const Blk &b, int lstart, int lend, int &l0, int &l1)
 {
@@ -76,10 +101,20 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
         int i = idx(g, j, k, l0);
         Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
         double p_lo = (double)pressure[i - g.plane], p_c = (double)pressure[i];
+        // the upper plane's loads run one plane ahead of their use
+        QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
+                 wn = quad_up_raw(zvel0, i + g.plane, g.ld);
+        S pn = pressure[i + g.plane];
         for (int l = l0; l <= l1; l++, i += g.plane) {
-            const Quad4 uu = quad_up(xvel0, i + g.plane, g.ld), vu = quad_up(yvel0, i + g.plane, g.ld),
-                        wu = quad_up(zvel0, i + g.plane, g.ld);
-            const double p_hi = (double)pressure[i + g.plane];
+            const Quad4 uu = widen(un), vu = widen(vn), wu = widen(wn);
+            const double p_hi = (double)pn;
+            if (l < l1) {
+                const int i2 = i + 2 * g.plane;
+                un = quad_up_raw(xvel0, i2, g.ld);
+                vn = quad_up_raw(yvel0, i2, g.ld);
+                wn = quad_up_raw(zvel0, i2, g.ld);
+                pn = pressure[i2];
+            }
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
@@ -226,9 +261,23 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
     if (l0 > l1) return;
     int i = idx(g, j, k, l0);
     Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
+    // the next plane's loads run one plane ahead of their use
+    QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
+             wn = quad_up_raw(zvel0, i + g.plane, g.ld);
+    S dn = density0[i], en = energy0[i], pn = pressure[i], qn = viscosity[i];
     for (int l = l0; l <= l1; l++, i += g.plane) {
-        const Quad4 uu = quad_up(xvel0, i + g.plane, g.ld), vu = quad_up(yvel0, i + g.plane, g.ld),
-                    wu = quad_up(zvel0, i + g.plane, g.ld);
+        const Quad4 uu = widen(un), vu = widen(vn), wu = widen(wn);
+        const double d0 = dn, e0v = en, p0 = pn, q0 = qn;
+        if (l < l1) {
+            const int i1 = i + g.plane, i2 = i + 2 * g.plane;
+            un = quad_up_raw(xvel0, i2, g.ld);
+            vn = quad_up_raw(yvel0, i2, g.ld);
+            wn = quad_up_raw(zvel0, i2, g.ld);
+            dn = density0[i1];
+            en = energy0[i1];
+            pn = pressure[i1];
+            qn = viscosity[i1];
+        }
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
@@ -251,9 +300,8 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
         const double vol = m.vol;
         const double vol_change_s = vol / (vol + total_flux);
         const double recip_volume = 1.0 / vol;
-        const double d0 = density0[i];
-        const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
-        const double e1 = energy0[i] - energy_change;
+        const double energy_change = (p0 / d0 + q0 / d0) * total_flux * recip_volume;
+        const double e1 = e0v - energy_change;
         const double d1 = d0 * vol_change_s;
         pressure[i] = (1.4 - 1.0) * d1 * e1;
         ul = uu;
@@ -303,9 +351,22 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
           ql = quad_down(viscosity, i - g.plane, g.ld);
     const double vol = m.vol;
     const double xa = m.xa, ya = m.ya, za = m.za;
+    // the next plane's loads run one plane ahead of their use
+    QuadS<S> dn = quad_down_raw(density0, i, g.ld), pn = quad_down_raw(pressure, i, g.ld),
+             qn = quad_down_raw(viscosity, i, g.ld);
+    S un = xvel0[i], vn = yvel0[i], wn = zvel0[i];
     for (int l = l0; l <= l1; l++, i += g.plane) {
-        const Quad4 du = quad_down(density0, i, g.ld), pu = quad_down(pressure, i, g.ld),
-                    qu = quad_down(viscosity, i, g.ld);
+        const Quad4 du = widen(dn), pu = widen(pn), qu = widen(qn);
+        const double u0 = un, v0 = vn, w0 = wn;
+        if (l < l1) {
+            const int i1 = i + g.plane;
+            dn = quad_down_raw(density0, i1, g.ld);
+            pn = quad_down_raw(pressure, i1, g.ld);
+            qn = quad_down_raw(viscosity, i1, g.ld);
+            un = xvel0[i1];
+            vn = yvel0[i1];
+            wn = zvel0[i1];
+        }
         // (a, b, c) in {-1, 0}^3; c = 0 is plane l, c = -1 plane l-1
 #define RV(a, b, c) (((c) ? dl : du).v[((a) + 1) + 2 * ((b) + 1)] * vol)
 #define P(a, b, c) ((c) ? pl : pu).v[((a) + 1) + 2 * ((b) + 1)]
@@ -314,15 +375,15 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
                                    RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
                                   0.125;
         const double stepbymass_s = 0.25 * dt / nodal_mass;
-        const double xvel1_s = xvel0[i] - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
+        const double xvel1_s = u0 - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
                                                           xa * (P(0, -1, 0) - P(-1, -1, 0)) +
                                                           xa * (P(0, 0, -1) - P(-1, 0, -1)) +
                                                           xa * (P(0, -1, -1) - P(-1, -1, -1)));
-        const double yvel1_s = yvel0[i] - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
+        const double yvel1_s = v0 - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
                                                           ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
                                                           ya * (P(0, 0, -1) - P(0, -1, -1)) +
                                                           ya * (P(-1, 0, -1) - P(-1, -1, -1)));
-        const double zvel1_s = zvel0[i] - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
+        const double zvel1_s = w0 - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
                                                           za * (P(0, -1, 0) - P(0, -1, -1)) +
                                                           za * (P(-1, 0, 0) - P(-1, 0, -1)) +
                                                           za * (P(-1, -1, 0) - P(-1, -1, -1)));
@@ -390,19 +451,42 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
     Quad4 u0l = quad_up(xvel0, i, g.ld), u1l = quad_up(xvel1, i, g.ld), v0l = quad_up(yvel0, i, g.ld),
           v1l = quad_up(yvel1, i, g.ld), w0l = quad_up(zvel0, i, g.ld), w1l = quad_up(zvel1, i, g.ld);
     const double c0 = 0.125 * dt;
+    const bool cellcol = jin && kin;
+    // the plane above (needed only by cells and x/y faces, l <= z_max) and
+    // the cell's own values, loaded one plane ahead of their use
+    QuadS<S> u0n{}, u1n{}, v0n{}, v1n{}, w0n{}, w1n{};
+    S d0n = 0, e0n = 0, pn = 0, qn = 0;
+    auto load_next = [&](int ii, int ll) {
+        if (ll <= g.z_max) {
+            const int iu = ii + g.plane;
+            u0n = quad_up_raw(xvel0, iu, g.ld);
+            u1n = quad_up_raw(xvel1, iu, g.ld);
+            v0n = quad_up_raw(yvel0, iu, g.ld);
+            v1n = quad_up_raw(yvel1, iu, g.ld);
+            w0n = quad_up_raw(zvel0, iu, g.ld);
+            w1n = quad_up_raw(zvel1, iu, g.ld);
+            if (cellcol) {
+                d0n = density0[ii];
+                e0n = energy0[ii];
+                pn = pressure[ii];
+                qn = viscosity[ii];
+            }
+        }
+    };
+    load_next(i, l0);
     for (int l = l0; l <= l1; l++, i += g.plane) {
         const bool lin = l <= g.z_max;
-        // the plane above is needed only by cells and x/y faces (l <= z_max)
         Quad4 u0u = u0l, u1u = u1l, v0u = v0l, v1u = v1l, w0u = w0l, w1u = w1l;
         if (lin) {
-            const int iu = i + g.plane;
-            u0u = quad_up(xvel0, iu, g.ld);
-            u1u = quad_up(xvel1, iu, g.ld);
-            v0u = quad_up(yvel0, iu, g.ld);
-            v1u = quad_up(yvel1, iu, g.ld);
-            w0u = quad_up(zvel0, iu, g.ld);
-            w1u = quad_up(zvel1, iu, g.ld);
+            u0u = widen(u0n);
+            u1u = widen(u1n);
+            v0u = widen(v0n);
+            v1u = widen(v1n);
+            w0u = widen(w0n);
+            w1u = widen(w1n);
         }
+        const double d0 = d0n, e0v = e0n, p0 = pn, q0 = qn;
+        if (l < l1) load_next(i + g.plane, l + 1);
 #define U0(a, b, c) ((c) ? u0u : u0l).v[(a) + 2 * (b)]
 #define U1(a, b, c) ((c) ? u1u : u1l).v[(a) + 2 * (b)]
 #define V0(a, b, c) ((c) ? v0u : v0l).v[(a) + 2 * (b)]
@@ -437,9 +521,8 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
             const double vol = m.vol;
             const double vol_change_s = vol / (vol + total_flux);
             const double recip_volume = 1.0 / vol;
-            const double d0 = density0[i];
-            const double energy_change = (pressure[i] / d0 + viscosity[i] / d0) * total_flux * recip_volume;
-            energy1[i] = energy0[i] - energy_change;
+            const double energy_change = (p0 / d0 + q0 / d0) * total_flux * recip_volume;
+            energy1[i] = e0v - energy_change;
             density1[i] = d0 * vol_change_s;
         }
 #undef U0
