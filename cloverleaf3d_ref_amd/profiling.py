@@ -23,9 +23,17 @@ from collections import defaultdict
 from typing import Dict, List, Tuple
 
 
+# device symbols of template kernels -> the names the library's launch timer uses
+_TEMPLATE_NAMES = {"k_fast_mom_march<2": "k_fast_mom_y", "k_fast_mom_march<3": "k_fast_mom_z",
+                   "k_fast_cell_march<2": "k_fast_cell_y", "k_fast_cell_march<3": "k_fast_cell_z"}
+
+
 def _short(name: str) -> str:
-    name = name.split("(")[0].replace("void ", "").strip()
-    return name.replace("clover::", "")
+    name = name.split("(")[0].replace("void ", "").strip().replace("clover::", "")
+    for prefix, short in _TEMPLATE_NAMES.items():
+        if name.startswith(prefix):
+            return short
+    return name.split("<")[0]
 
 
 def _find(d: str, suffix: str) -> str:
