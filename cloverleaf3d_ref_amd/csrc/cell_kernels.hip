@@ -13,8 +13,10 @@
 // e_out) which then take the place of density1/energy1; only interior cells
 // of those are written (the halo update that follows every advec_cell
 // sweep in advection.f90 refreshes their halos before anything reads them).
-// Every value is formed with the expression and operand order of the
-// per-call kernels, so the results are bit-identical.
+// In the FP64 form every value is formed with the expression and operand
+// order of the per-call kernels, so the results are bit-identical.  The FP32
+// form computes the face fluxes in FP32 and keeps pre_vol and the cell's
+// mass and energy update in FP64.
 #include <hip/hip_runtime.h>
 
 #include "batch.h"
@@ -45,32 +47,34 @@ __device__ inline double cell_pre_vol(int mode, const Lay &g, int i, double V0, 
 #undef D
 }
 
-__device__ inline double cell_limiter(double sigma, double sigma3, double sigma4, double diffuw, double diffdw)
+template <typename T>
+__device__ inline T cell_limiter(T sigma, T sigma3, T sigma4, T diffuw, T diffdw)
 {
-    const double one_by_six = 1.0 / 6.0;
-    if (diffuw * diffdw > 0.0)
-        return (1.0 - sigma) * copysign(1.0, diffdw) *
-               dmin(dmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
-    return 0.0;
+    const T one_by_six = (T)(1.0 / 6.0);
+    if (diffuw * diffdw > (T)0)
+        return ((T)1 - sigma) * copysign((T)1, diffdw) *
+               tmin(tmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
+    return (T)0;
 }
 
 // Mass and energy flux through one face (advec_cell_kernel.f90:123-186 and
 // twins): ru/rd/rw, eu/ed/ew the density and energy of the upwind, donor and
-// downwind cells, pv the donor's pre_vol, vd the vertex spacing.
-__device__ inline void cell_face_flux(double vf, double pv, double vd, double ru, double rd, double rw, double eu,
-                                      double ed, double ew, double &mf, double &ef)
+// downwind cells, pv the donor's pre_vol, vd the vertex spacing; T is the
+// form's arithmetic type.
+template <typename T>
+__device__ inline void cell_face_flux(T vf, T pv, T vd, T ru, T rd, T rw, T eu, T ed, T ew, T &mf, T &ef)
 {
-    const double sigmat = fabs(vf) / pv;
-    const double sigma3 = (1.0 + sigmat) * (vd / vd);  // vertexd*(j)/vertexd*(dif) on the uniform mesh
-    const double sigma4 = 2.0 - sigmat;
-    double limiter = cell_limiter(sigmat, sigma3, sigma4, rd - ru, rw - rd);
+    const T sigmat = quo(fabs(vf), pv);
+    const T sigma3 = ((T)1 + sigmat) * (vd / vd);  // vertexd*(j)/vertexd*(dif) on the uniform mesh
+    const T sigma4 = (T)2 - sigmat;
+    T limiter = cell_limiter(sigmat, sigma3, sigma4, rd - ru, rw - rd);
     mf = vf * (rd + limiter);
-    const double sigmam = fabs(mf) / (rd * pv);
+    const T sigmam = quo(fabs(mf), rd * pv);
     limiter = cell_limiter(sigmam, sigma3, sigma4, ed - eu, ew - ed);
     ef = mf * (ed + limiter);
 }
 
-// cell update (advec_cell_kernel.f90:188-191 and twins)
+// cell update (advec_cell_kernel.f90:188-191 and twins), FP64 in both forms
 __device__ inline void cell_update(double rho, double e, double pv, double mf_lo, double mf_hi, double ef_lo,
                                    double ef_hi, double vf_lo, double vf_hi, double &rho_new, double &e_new)
 {
@@ -83,6 +87,7 @@ __device__ inline void cell_update(double rho, double e, double pv, double mf_lo
 }
 
 __device__ inline double shfl_c(double x, int src) { return __shfl(x, src, 64); }
+__device__ inline float shfl_c(float x, int src) { return __shfl(x, src, 64); }
 
 // ---------------------------------------------------------------------------
 // x sweep: faces j in [x_min, x_max+2], rows k, l interior.  Lane i holds
@@ -100,25 +105,27 @@ __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mod
     const int k = g.y_min + row % ny, l = g.z_min + row / ny;
     const int j = g.x_min - 2 + bk.x * 60 + lane;
     const int i = idx(g, j, k, l);
+    using T = S;  // arithmetic type of the fluxes
     const bool ok_cell = j <= g.x_max + 2;  // j >= x_min-2 always
-    const double rho = ok_cell ? density1[i] : 0.0;
-    const double e = ok_cell ? energy1[i] : 0.0;
+    const T rho = ok_cell ? density1[i] : (T)0;
+    const T e = ok_cell ? energy1[i] : (T)0;
     const double pv = ok_cell ? cell_pre_vol(mode, g, i, m.vol, vfx, vfy, vfz) : 0.0;
-    const double vf = j <= g.x_max + 3 ? vfx[i] : 0.0;
+    const T vf = j <= g.x_max + 3 ? vfx[i] : (T)0;
     const int lm2 = max(lane - 2, 0), lm1 = max(lane - 1, 0), lp1 = min(lane + 1, 63);
-    const double rm2 = shfl_c(rho, lm2), rm1 = shfl_c(rho, lm1), rp1 = shfl_c(rho, lp1);
-    const double em2 = shfl_c(e, lm2), em1 = shfl_c(e, lm1), ep1 = shfl_c(e, lp1);
+    const T rm2 = shfl_c(rho, lm2), rm1 = shfl_c(rho, lm1), rp1 = shfl_c(rho, lp1);
+    const T em2 = shfl_c(e, lm2), em1 = shfl_c(e, lm1), ep1 = shfl_c(e, lp1);
     const double pvm1 = shfl_c(pv, lm1);
+    const T vd = (T)m.dx;
     // face j: upwind/donor/downwind (advec_cell_kernel.f90:123-132); the
     // upwind index is clamped at x_max+2
-    double mf, ef;
-    if (vf > 0.0) {
-        cell_face_flux(vf, pvm1, m.dx, rm2, rm1, rho, em2, em1, e, mf, ef);
+    T mf, ef;
+    if (vf > (T)0) {
+        cell_face_flux(vf, (T)pvm1, vd, rm2, rm1, rho, em2, em1, e, mf, ef);
     } else {
         const bool clamp = j + 1 > g.x_max + 2;
-        cell_face_flux(vf, pv, m.dx, clamp ? rho : rp1, rho, rm1, clamp ? e : ep1, e, em1, mf, ef);
+        cell_face_flux(vf, (T)pv, vd, clamp ? rho : rp1, rho, rm1, clamp ? e : ep1, e, em1, mf, ef);
     }
-    const double mf_hi = shfl_c(mf, lp1), ef_hi = shfl_c(ef, lp1), vf_hi = shfl_c(vf, lp1);
+    const T mf_hi = shfl_c(mf, lp1), ef_hi = shfl_c(ef, lp1), vf_hi = shfl_c(vf, lp1);
     if (lane < 2 || lane > 61) return;
     if (j <= g.x_max + 2) mass_flux_x[i] = mf;
     if (j <= g.x_max) {
@@ -192,7 +199,8 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
     const bool cross_in = DIR == 3 || cr <= g.z_max;       // updates only on interior cells
     const int s = DIR == 3 ? g.plane : g.ld;
     const int smin = DIR == 3 ? g.z_min : g.y_min, smax = DIR == 3 ? g.z_max : g.y_max;
-    const double vd = DIR == 3 ? m.dz : m.dy;
+    using T = S;  // arithmetic type of the fluxes
+    const T vd = (T)(DIR == 3 ? m.dz : m.dy);
     const S *__restrict__ vfs = DIR == 3 ? vfz : vfy;
     const int f0 = smin + bk.z * CCHUNK;
     const int f1 = min(f0 + CCHUNK - 1, smax + 2);
@@ -201,18 +209,18 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
     auto pre = [&](int q) { return cell_pre_vol(mode, g, OFF(q), m.vol, vfx, vfy, vfz); };
 
     // window of old values at f-2, f-1, f, f+1 and pre_vol at f-1, f
-    double r_m2 = density1[OFF(f0 - 2)], r_m1 = density1[OFF(f0 - 1)], r_0 = density1[OFF(f0)];
-    double e_m2 = energy1[OFF(f0 - 2)], e_m1 = energy1[OFF(f0 - 1)], e_0 = energy1[OFF(f0)];
+    T r_m2 = density1[OFF(f0 - 2)], r_m1 = density1[OFF(f0 - 1)], r_0 = density1[OFF(f0)];
+    T e_m2 = energy1[OFF(f0 - 2)], e_m1 = energy1[OFF(f0 - 1)], e_0 = energy1[OFF(f0)];
     double pv_m1 = pre(f0 - 1);
     // flux of face f0-1, needed when cell f0-1 is updated here (f0 > smin)
-    double mf_lo = 0.0, ef_lo = 0.0, vf_lo = 0.0;
+    T mf_lo = 0, ef_lo = 0, vf_lo = 0;
     if (f0 > smin) {
         vf_lo = vfs[OFF(f0 - 1)];
-        const double r_m3 = density1[OFF(f0 - 3)], e_m3 = energy1[OFF(f0 - 3)];
-        if (vf_lo > 0.0)
-            cell_face_flux(vf_lo, pre(f0 - 2), vd, r_m3, r_m2, r_m1, e_m3, e_m2, e_m1, mf_lo, ef_lo);
+        const T r_m3 = density1[OFF(f0 - 3)], e_m3 = energy1[OFF(f0 - 3)];
+        if (vf_lo > (T)0)
+            cell_face_flux(vf_lo, (T)pre(f0 - 2), vd, r_m3, r_m2, r_m1, e_m3, e_m2, e_m1, mf_lo, ef_lo);
         else
-            cell_face_flux(vf_lo, pv_m1, vd, r_0, r_m1, r_m2, e_0, e_m1, e_m2, mf_lo, ef_lo);  // f0 <= smax+1
+            cell_face_flux(vf_lo, (T)pv_m1, vd, r_0, r_m1, r_m2, e_0, e_m1, e_m2, mf_lo, ef_lo);  // f0 <= smax+1
     }
     // face f0's loads; each step issues the next face's before its arithmetic
     S vf_n = vfs[OFF(f0)];
@@ -224,9 +232,9 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
     }
     for (int f = f0; f <= f1; f++) {
         const int i = OFF(f);
-        const double vf = (double)vf_n;
+        const T vf = vf_n;
         const double pv_0 = pre_of_raw(mode, m.vol, pr_n);
-        const double r_p1 = f + 1 <= smax + 2 ? (double)r_n : r_0, e_p1 = f + 1 <= smax + 2 ? (double)e_n : e_0;
+        const T r_p1 = f + 1 <= smax + 2 ? r_n : r_0, e_p1 = f + 1 <= smax + 2 ? e_n : e_0;
         if (f < f1) {
             vf_n = vfs[OFF(f + 1)];
             pr_n = load_pre_raw(mode, g, OFF(f + 1), vfx, vfy, vfz);
@@ -235,11 +243,11 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
                 e_n = energy1[OFF(f + 2)];
             }
         }
-        double mf, ef;
-        if (vf > 0.0)
-            cell_face_flux(vf, pv_m1, vd, r_m2, r_m1, r_0, e_m2, e_m1, e_0, mf, ef);
+        T mf, ef;
+        if (vf > (T)0)
+            cell_face_flux(vf, (T)pv_m1, vd, r_m2, r_m1, r_0, e_m2, e_m1, e_0, mf, ef);
         else
-            cell_face_flux(vf, pv_0, vd, r_p1, r_0, r_m1, e_p1, e_0, e_m1, mf, ef);
+            cell_face_flux(vf, (T)pv_0, vd, r_p1, r_0, r_m1, e_p1, e_0, e_m1, mf, ef);
         mass_flux[i] = mf;
         const int c = f - 1;  // cell between faces f-1 and f
         if (cross_in && c >= smin && c <= smax) {

@@ -1,10 +1,13 @@
-// Fused FP64 kernels of the resident driver's fast path (see fast.h).
+// Fused kernels of the resident driver's fast path (see fast.h), for both
+// store forms.
 //
-// Each kernel evaluates exactly the expressions of the reference loop nests
-// it replaces, in the same order and without FMA contraction, so its outputs
-// are bit-identical to the per-call kernels'; the savings are in HBM bytes:
-// scalar geometry for the uniform mesh, fused neighbours, dead stores
-// dropped, intermediates recomputed in registers.
+// In the FP64 form each kernel evaluates exactly the expressions of the
+// reference loop nests it replaces, in the same order and without FMA
+// contraction, so its outputs are bit-identical to the per-call kernels'; the
+// savings are in HBM bytes: scalar geometry for the uniform mesh, fused
+// neighbours, dead stores dropped, intermediates recomputed in registers.
+// The FP32 form runs the same expressions in FP32 (see by_const / quo); the
+// dt minimum and the summary sums stay FP64.
 //
 // The stencils here read two neighbouring planes (l and l+1 for cells and
 // faces, l-1 and l for nodes).  Each thread owns one (j, k) column and
@@ -21,27 +24,10 @@ namespace clover {
 
 constexpr int ZCH = 64;  // planes per marching workgroup
 
-// 2x2 corner values of an array around (j, k) in one plane:
-// v[a + 2b] = p(j+a, k+b) (a, b in {0, 1})
-struct Quad4 {
-    double v[4];
-};
-
-template <typename S>
-__device__ inline Quad4 quad_up(const S *__restrict__ p, int i, int ld)
-{
-    return Quad4{{(double)p[i], (double)p[i + 1], (double)p[i + ld], (double)p[i + 1 + ld]}};
-}
-
-// v[(a+1) + 2(b+1)] = p(j+a, k+b) (a, b in {-1, 0})
-template <typename S>
-__device__ inline Quad4 quad_down(const S *__restrict__ p, int i, int ld)
-{
-    return Quad4{{(double)p[i - 1 - ld], (double)p[i - ld], (double)p[i - 1], (double)p[i]}};
-}
-
-// the same corners in the storage type, for loads issued a plane ahead of
-// their use (converted by widen() where the arithmetic starts)
+// 2x2 corner values of an array around (j, k) in one plane, in the storage
+// type (which is also the arithmetic type of the form):
+//   quad_up:   v[a + 2b] = p(j+a, k+b)           (a, b in {0, 1})
+//   quad_down: v[(a+1) + 2(b+1)] = p(j+a, k+b)   (a, b in {-1, 0})
 template <typename S>
 struct QuadS {
     S v[4];
@@ -57,12 +43,6 @@ template <typename S>
 __device__ inline QuadS<S> quad_down_raw(const S *__restrict__ p, int i, int ld)
 {
     return QuadS<S>{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
-}
-
-template <typename S>
-__device__ inline Quad4 widen(const QuadS<S> &q)
-{
-    return Quad4{{(double)q.v[0], (double)q.v[1], (double)q.v[2], (double)q.v[3]}};
 }
 
 // the column's planes [l0, l1] of the chunk this workgroup marches
@@ -81,7 +61,7 @@ __host__ __device__ inline int chunks_for(int nplanes) { return (nplanes + ZCH -
 // viscosity only at the cell itself, so it runs on the value just computed
 // (the halo exchange of viscosity between the two in timestep.f90:80-84
 // does not touch interior cells).  Each workgroup leaves one partial of the
-// dt minimum.
+// dt minimum, which is FP64 in both forms.
 template <typename S>
 __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
@@ -89,25 +69,32 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
                const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
                double *__restrict__ partial)
 {
+    using T = S;  // arithmetic type of the form
     const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
     const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     int l0, l1;
     chunk_planes(bk, g.z_min, g.z_max, l0, l1);
     const bool in = j <= g.x_max && k <= g.y_max && l0 <= l1;  // every thread reaches the reduction
     const double mm = dmin(dmin(m.dx, m.dy), m.dz);
-    const double ds = 1.0 / (mm * mm);
+    const T ds = (T)(1.0 / (mm * mm));
+    const T dx = (T)m.dx, dy = (T)m.dy, dz = (T)m.dz;
+    const T dx2 = (T)(m.dx + m.dx), dy2 = (T)(m.dy + m.dy), dz2 = (T)(m.dz + m.dz);
+    const T rdx = (T)1 / dx, rdy = (T)1 / dy, rdz = (T)1 / dz;
+    const T rdx2 = (T)1 / dx2, rdy2 = (T)1 / dy2, rdz2 = (T)1 / dz2;
+    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol;
+    const T small = (T)g_small;
     double dtmin_thread = 1.0e300;
     if (in) {
         int i = idx(g, j, k, l0);
-        Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
-        double p_lo = (double)pressure[i - g.plane], p_c = (double)pressure[i];
+        QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
+        T p_lo = pressure[i - g.plane], p_c = pressure[i];
         // the upper plane's loads run one plane ahead of their use
         QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
                  wn = quad_up_raw(zvel0, i + g.plane, g.ld);
         S pn = pressure[i + g.plane];
         for (int l = l0; l <= l1; l++, i += g.plane) {
-            const Quad4 uu = widen(un), vu = widen(vn), wu = widen(wn);
-            const double p_hi = (double)pn;
+            const QuadS<S> uu = un, vu = vn, wu = wn;
+            const T p_hi = pn;
             if (l < l1) {
                 const int i2 = i + 2 * g.plane;
                 un = quad_up_raw(xvel0, i2, g.ld);
@@ -118,80 +105,79 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
-            const double ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
-            const double ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
-            const double ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
-            const double ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
-            const double ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
-            const double ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
-            const double vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
-            const double vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
-            const double vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
-            const double vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
-            const double vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
-            const double vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
-            const double wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
-            const double wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
-            const double wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
-            const double wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
-            const double wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
-            const double wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+            const T ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+            const T ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+            const T ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
+            const T ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
+            const T ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
+            const T ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
+            const T vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
+            const T vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
+            const T vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+            const T vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+            const T vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
+            const T vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
+            const T wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
+            const T wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
+            const T wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
+            const T wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
+            const T wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+            const T wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
             // calc_dt_kernel.f90:124-125 sums the z-face nodes in its own order
-            const double dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
-            const double dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
+            const T dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
+            const T dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
 #undef U
 #undef V
 #undef W
-            const double div = (m.xa * (ugradx2 - ugradx1) + m.ya * (vgrady2 - vgrady1)) + m.za * (wgradz2 - wgradz1);
-            const double xx = 0.25 * (ugradx2 - ugradx1) / (m.dx);
-            const double yy = 0.25 * (vgrady2 - vgrady1) / (m.dy);
-            const double zz = 0.25 * (wgradz2 - wgradz1) / (m.dz);
-            const double xy = 0.25 * (ugrady2 - ugrady1) / (m.dy) + 0.25 * (vgradx2 - vgradx1) / (m.dx);
-            const double xz = 0.25 * (ugradz2 - ugradz1) / (m.dz) + 0.25 * (wgradx2 - wgradx1) / (m.dx);
-            const double yz = 0.25 * (vgradz2 - vgradz1) / (m.dz) + 0.25 * (wgrady2 - wgrady1) / (m.dy);
-            double pgradx = ((double)pressure[i + 1] - (double)pressure[i - 1]) / (m.dx + m.dx);
-            double pgrady = ((double)pressure[i + g.ld] - (double)pressure[i - g.ld]) / (m.dy + m.dy);
-            double pgradz = (p_hi - p_lo) / (m.dz + m.dz);
-            const double pgradx2 = pgradx * pgradx;
-            const double pgrady2 = pgrady * pgrady;
-            const double pgradz2 = pgradz * pgradz;
-            const double limiter = (xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
-                                    xz * pgradx * pgradz + yz * pgrady * pgradz) /
-                                   dmax(pgradx2 + pgrady2 + pgradz2, 1.0e-16);
-            const double rho = density0[i];
-            double q;
-            if ((limiter > 0.0) || (div >= 0.0)) {
-                q = 0.0;
+            const T div = (xa * (ugradx2 - ugradx1) + ya * (vgrady2 - vgrady1)) + za * (wgradz2 - wgradz1);
+            const T xx = by_const((T)0.25 * (ugradx2 - ugradx1), dx, rdx);
+            const T yy = by_const((T)0.25 * (vgrady2 - vgrady1), dy, rdy);
+            const T zz = by_const((T)0.25 * (wgradz2 - wgradz1), dz, rdz);
+            const T xy = by_const((T)0.25 * (ugrady2 - ugrady1), dy, rdy) + by_const((T)0.25 * (vgradx2 - vgradx1), dx, rdx);
+            const T xz = by_const((T)0.25 * (ugradz2 - ugradz1), dz, rdz) + by_const((T)0.25 * (wgradx2 - wgradx1), dx, rdx);
+            const T yz = by_const((T)0.25 * (vgradz2 - vgradz1), dz, rdz) + by_const((T)0.25 * (wgrady2 - wgrady1), dy, rdy);
+            T pgradx = by_const((T)pressure[i + 1] - (T)pressure[i - 1], dx2, rdx2);
+            T pgrady = by_const((T)pressure[i + g.ld] - (T)pressure[i - g.ld], dy2, rdy2);
+            T pgradz = by_const(p_hi - p_lo, dz2, rdz2);
+            const T pgradx2 = pgradx * pgradx;
+            const T pgrady2 = pgrady * pgrady;
+            const T pgradz2 = pgradz * pgradz;
+            const T limiter = quo(xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
+                                      xz * pgradx * pgradz + yz * pgrady * pgradz,
+                                  tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16));
+            const T rho = density0[i];
+            T q;
+            if ((limiter > (T)0) || (div >= (T)0)) {
+                q = (T)0;
             } else {
-                pgradx = copysign(dmax(1.0e-16, fabs(pgradx)), pgradx);
-                pgrady = copysign(dmax(1.0e-16, fabs(pgrady)), pgrady);
-                pgradz = copysign(dmax(1.0e-16, fabs(pgradz)), pgradz);
-                const double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
-                const double xgrad = fabs(m.dx * pgrad / pgradx);
-                const double ygrad = fabs(m.dy * pgrad / pgrady);
-                const double zgrad = fabs(m.dz * pgrad / pgradz);
-                const double grad = dmin(dmin(xgrad, ygrad), zgrad);
-                const double grad2 = grad * grad;
-                q = 2.0 * rho * grad2 * limiter * limiter;
+                pgradx = copysign(tmax((T)1.0e-16, fabs(pgradx)), pgradx);
+                pgrady = copysign(tmax((T)1.0e-16, fabs(pgrady)), pgrady);
+                pgradz = copysign(tmax((T)1.0e-16, fabs(pgradz)), pgradz);
+                const T pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
+                const T xgrad = fabs(quo(dx * pgrad, pgradx));
+                const T ygrad = fabs(quo(dy * pgrad, pgrady));
+                const T zgrad = fabs(quo(dz * pgrad, pgradz));
+                const T grad = tmin(tmin(xgrad, ygrad), zgrad);
+                const T grad2 = grad * grad;
+                q = (T)2 * rho * grad2 * limiter * limiter;
             }
             viscosity[i] = q;
 
             // calc_dt
-            const double c = soundspeed[i];
-            double cc = c * c;
-            cc = cc + 2.0 * q / rho;
-            double dtct = ds * cc;
-            dtct = dtc_safe * 1.0 / dmax(sqrt(dtct), g_small);
-            const double vol = m.vol;
-            const double du1 = ugradx1 * m.xa, du2 = ugradx2 * m.xa;
-            const double dtut = dtu_safe * 4.0 * vol / dmax(dmax(fabs(du1), fabs(du2)), 1.0e-5 * vol);
-            const double dv1 = vgrady1 * m.ya, dv2 = vgrady2 * m.ya;
-            const double dtvt = dtv_safe * 4.0 * vol / dmax(dmax(fabs(dv1), fabs(dv2)), 1.0e-5 * vol);
-            const double dw1 = dw1s * m.za, dw2 = dw2s * m.za;
-            const double dtwt = dtw_safe * 4.0 * vol / dmax(dmax(fabs(dw1), fabs(dw2)), 1.0e-5 * vol);
-            const double dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
-            const double dtdivt = dtdiv_safe * 4.0 * (vol) / dmax(vol * 1.0e-05, fabs(dv));
-            dtmin_thread = dmin(dtmin_thread, dmin(dmin(dmin(dmin(dtct, dtut), dtvt), dtwt), dtdivt));
+            const T c = soundspeed[i];
+            T cc = c * c;
+            cc = cc + quo((T)2 * q, rho);
+            T dtct = ds * cc;
+            dtct = quo((T)dtc_safe * (T)1, tmax(sqrt(dtct), small));
+            const T du1 = ugradx1 * xa, du2 = ugradx2 * xa;
+            const T dtut = quo((T)dtu_safe * (T)4 * vol, tmax(tmax(fabs(du1), fabs(du2)), (T)1.0e-5 * vol));
+            const T dv1 = vgrady1 * ya, dv2 = vgrady2 * ya;
+            const T dtvt = quo((T)dtv_safe * (T)4 * vol, tmax(tmax(fabs(dv1), fabs(dv2)), (T)1.0e-5 * vol));
+            const T dw1 = dw1s * za, dw2 = dw2s * za;
+            const T dtwt = quo((T)dtw_safe * (T)4 * vol, tmax(tmax(fabs(dw1), fabs(dw2)), (T)1.0e-5 * vol));
+            const T dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
+            const T dtdivt = quo((T)dtdiv_safe * (T)4 * (vol), tmax(vol * (T)1.0e-05, fabs(dv)));
+            dtmin_thread = dmin(dtmin_thread, (double)tmin(tmin(tmin(tmin(dtct, dtut), dtvt), dtwt), dtdivt));
             ul = uu;
             vl = vu;
             wl = wu;
@@ -259,15 +245,17 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
     int l0, l1;
     chunk_planes(bk, g.z_min, g.z_max, l0, l1);
     if (l0 > l1) return;
+    using T = S;  // arithmetic type of the form
+    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol, tdt = (T)dt;
     int i = idx(g, j, k, l0);
-    Quad4 ul = quad_up(xvel0, i, g.ld), vl = quad_up(yvel0, i, g.ld), wl = quad_up(zvel0, i, g.ld);
+    QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
     // the next plane's loads run one plane ahead of their use
     QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
              wn = quad_up_raw(zvel0, i + g.plane, g.ld);
     S dn = density0[i], en = energy0[i], pn = pressure[i], qn = viscosity[i];
     for (int l = l0; l <= l1; l++, i += g.plane) {
-        const Quad4 uu = widen(un), vu = widen(vn), wu = widen(wn);
-        const double d0 = dn, e0v = en, p0 = pn, q0 = qn;
+        const QuadS<S> uu = un, vu = vn, wu = wn;
+        const T d0 = dn, e0v = en, p0 = pn, q0 = qn;
         if (l < l1) {
             const int i1 = i + g.plane, i2 = i + 2 * g.plane;
             un = quad_up_raw(xvel0, i2, g.ld);
@@ -281,29 +269,28 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
-        const double sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
-        const double sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
-        const double sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
-        const double stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
-        const double sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
-        const double sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+        const T sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+        const T sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+        const T sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+        const T stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+        const T sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+        const T sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
 #undef U
 #undef V
 #undef W
-        const double left_flux = (m.xa * sl) * 0.125 * dt * 0.5;
-        const double right_flux = (m.xa * sr) * 0.125 * dt * 0.5;
-        const double bottom_flux = (m.ya * sb) * 0.125 * dt * 0.5;
-        const double top_flux = (m.ya * stp) * 0.125 * dt * 0.5;
-        const double back_flux = (m.za * sk) * 0.125 * dt * 0.5;
-        const double front_flux = (m.za * sf) * 0.125 * dt * 0.5;
-        const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
-        const double vol = m.vol;
-        const double vol_change_s = vol / (vol + total_flux);
-        const double recip_volume = 1.0 / vol;
-        const double energy_change = (p0 / d0 + q0 / d0) * total_flux * recip_volume;
-        const double e1 = e0v - energy_change;
-        const double d1 = d0 * vol_change_s;
-        pressure[i] = (1.4 - 1.0) * d1 * e1;
+        const T left_flux = (xa * sl) * (T)0.125 * tdt * (T)0.5;
+        const T right_flux = (xa * sr) * (T)0.125 * tdt * (T)0.5;
+        const T bottom_flux = (ya * sb) * (T)0.125 * tdt * (T)0.5;
+        const T top_flux = (ya * stp) * (T)0.125 * tdt * (T)0.5;
+        const T back_flux = (za * sk) * (T)0.125 * tdt * (T)0.5;
+        const T front_flux = (za * sf) * (T)0.125 * tdt * (T)0.5;
+        const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+        const T vol_change_s = quo(vol, vol + total_flux);
+        const T recip_volume = (T)1 / vol;
+        const T energy_change = (quo(p0, d0) + quo(q0, d0)) * total_flux * recip_volume;
+        const T e1 = e0v - energy_change;
+        const T d1 = d0 * vol_change_s;
+        pressure[i] = (T)(1.4 - 1.0) * d1 * e1;
         ul = uu;
         vl = vu;
         wl = wu;
@@ -346,18 +333,19 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
     int l0, l1;
     chunk_planes(bk, g.z_min, g.z_max + 1, l0, l1);
     if (l0 > l1) return;
+    using T = S;  // arithmetic type of the form
     int i = idx(g, j, k, l0);
-    Quad4 dl = quad_down(density0, i - g.plane, g.ld), pl = quad_down(pressure, i - g.plane, g.ld),
-          ql = quad_down(viscosity, i - g.plane, g.ld);
-    const double vol = m.vol;
-    const double xa = m.xa, ya = m.ya, za = m.za;
+    QuadS<S> dl = quad_down_raw(density0, i - g.plane, g.ld), pl = quad_down_raw(pressure, i - g.plane, g.ld),
+             ql = quad_down_raw(viscosity, i - g.plane, g.ld);
+    const T vol = (T)m.vol;
+    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, tdt = (T)dt;
     // the next plane's loads run one plane ahead of their use
     QuadS<S> dn = quad_down_raw(density0, i, g.ld), pn = quad_down_raw(pressure, i, g.ld),
              qn = quad_down_raw(viscosity, i, g.ld);
     S un = xvel0[i], vn = yvel0[i], wn = zvel0[i];
     for (int l = l0; l <= l1; l++, i += g.plane) {
-        const Quad4 du = widen(dn), pu = widen(pn), qu = widen(qn);
-        const double u0 = un, v0 = vn, w0 = wn;
+        const QuadS<S> du = dn, pu = pn, qu = qn;
+        const T u0 = un, v0 = vn, w0 = wn;
         if (l < l1) {
             const int i1 = i + g.plane;
             dn = quad_down_raw(density0, i1, g.ld);
@@ -371,19 +359,19 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
 #define RV(a, b, c) (((c) ? dl : du).v[((a) + 1) + 2 * ((b) + 1)] * vol)
 #define P(a, b, c) ((c) ? pl : pu).v[((a) + 1) + 2 * ((b) + 1)]
 #define Q(a, b, c) ((c) ? ql : qu).v[((a) + 1) + 2 * ((b) + 1)]
-        const double nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
-                                   RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
-                                  0.125;
-        const double stepbymass_s = 0.25 * dt / nodal_mass;
-        const double xvel1_s = u0 - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
+        const T nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
+                              RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
+                             (T)0.125;
+        const T stepbymass_s = quo((T)0.25 * tdt, nodal_mass);
+        const T xvel1_s = u0 - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
                                                           xa * (P(0, -1, 0) - P(-1, -1, 0)) +
                                                           xa * (P(0, 0, -1) - P(-1, 0, -1)) +
                                                           xa * (P(0, -1, -1) - P(-1, -1, -1)));
-        const double yvel1_s = v0 - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
+        const T yvel1_s = v0 - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
                                                           ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
                                                           ya * (P(0, 0, -1) - P(0, -1, -1)) +
                                                           ya * (P(-1, 0, -1) - P(-1, -1, -1)));
-        const double zvel1_s = w0 - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
+        const T zvel1_s = w0 - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
                                                           za * (P(0, -1, 0) - P(0, -1, -1)) +
                                                           za * (P(-1, 0, 0) - P(-1, 0, -1)) +
                                                           za * (P(-1, -1, 0) - P(-1, -1, -1)));
@@ -447,10 +435,12 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
     chunk_planes(bk, g.z_min, g.z_max + 1, l0, l1);
     if (l0 > l1) return;
     const bool jin = j <= g.x_max, kin = k <= g.y_max;
+    using T = S;  // arithmetic type of the form
+    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol, tdt = (T)dt;
     int i = idx(g, j, k, l0);
-    Quad4 u0l = quad_up(xvel0, i, g.ld), u1l = quad_up(xvel1, i, g.ld), v0l = quad_up(yvel0, i, g.ld),
-          v1l = quad_up(yvel1, i, g.ld), w0l = quad_up(zvel0, i, g.ld), w1l = quad_up(zvel1, i, g.ld);
-    const double c0 = 0.125 * dt;
+    QuadS<S> u0l = quad_up_raw(xvel0, i, g.ld), u1l = quad_up_raw(xvel1, i, g.ld), v0l = quad_up_raw(yvel0, i, g.ld),
+             v1l = quad_up_raw(yvel1, i, g.ld), w0l = quad_up_raw(zvel0, i, g.ld), w1l = quad_up_raw(zvel1, i, g.ld);
+    const T c0 = (T)0.125 * tdt;
     const bool cellcol = jin && kin;
     // the plane above (needed only by cells and x/y faces, l <= z_max) and
     // the cell's own values, loaded one plane ahead of their use
@@ -476,16 +466,16 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
     load_next(i, l0);
     for (int l = l0; l <= l1; l++, i += g.plane) {
         const bool lin = l <= g.z_max;
-        Quad4 u0u = u0l, u1u = u1l, v0u = v0l, v1u = v1l, w0u = w0l, w1u = w1l;
+        QuadS<S> u0u = u0l, u1u = u1l, v0u = v0l, v1u = v1l, w0u = w0l, w1u = w1l;
         if (lin) {
-            u0u = widen(u0n);
-            u1u = widen(u1n);
-            v0u = widen(v0n);
-            v1u = widen(v1n);
-            w0u = widen(w0n);
-            w1u = widen(w1n);
+            u0u = u0n;
+            u1u = u1n;
+            v0u = v0n;
+            v1u = v1n;
+            w0u = w0n;
+            w1u = w1n;
         }
-        const double d0 = d0n, e0v = e0n, p0 = pn, q0 = qn;
+        const T d0 = d0n, e0v = e0n, p0 = pn, q0 = qn;
         if (l < l1) load_next(i + g.plane, l + 1);
 #define U0(a, b, c) ((c) ? u0u : u0l).v[(a) + 2 * (b)]
 #define U1(a, b, c) ((c) ? u1u : u1l).v[(a) + 2 * (b)]
@@ -493,35 +483,34 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
 #define V1(a, b, c) ((c) ? v1u : v1l).v[(a) + 2 * (b)]
 #define W0(a, b, c) ((c) ? w0u : w0l).v[(a) + 2 * (b)]
 #define W1(a, b, c) ((c) ? w1u : w1l).v[(a) + 2 * (b)]
-        double sl = 0.0, sb = 0.0;
+        T sl = 0, sb = 0;
         if (kin && lin) {
             sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
-            vol_flux_x[i] = c0 * m.xa * sl;
+            vol_flux_x[i] = c0 * xa * sl;
         }
         if (jin && lin) {
             sb = V0(0, 0, 0) + V0(1, 0, 0) + V0(0, 0, 1) + V0(1, 0, 1) + V1(0, 0, 0) + V1(1, 0, 0) + V1(0, 0, 1) + V1(1, 0, 1);
-            vol_flux_y[i] = c0 * m.ya * sb;
+            vol_flux_y[i] = c0 * ya * sb;
         }
         if (jin && kin)
-            vol_flux_z[i] = c0 * m.za *
+            vol_flux_z[i] = c0 * za *
                             (W0(0, 0, 0) + W0(1, 0, 0) + W0(1, 0, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) +
                              W1(1, 1, 0));
         if (jin && kin && lin) {
-            const double sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
-            const double stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
-            const double sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
-            const double sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
-            const double left_flux = (m.xa * sl) * 0.125 * dt;
-            const double right_flux = (m.xa * sr) * 0.125 * dt;
-            const double bottom_flux = (m.ya * sb) * 0.125 * dt;
-            const double top_flux = (m.ya * stp) * 0.125 * dt;
-            const double back_flux = (m.za * sk) * 0.125 * dt;
-            const double front_flux = (m.za * sf) * 0.125 * dt;
-            const double total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
-            const double vol = m.vol;
-            const double vol_change_s = vol / (vol + total_flux);
-            const double recip_volume = 1.0 / vol;
-            const double energy_change = (p0 / d0 + q0 / d0) * total_flux * recip_volume;
+            const T sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
+            const T stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
+            const T sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
+            const T sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
+            const T left_flux = (xa * sl) * (T)0.125 * tdt;
+            const T right_flux = (xa * sr) * (T)0.125 * tdt;
+            const T bottom_flux = (ya * sb) * (T)0.125 * tdt;
+            const T top_flux = (ya * stp) * (T)0.125 * tdt;
+            const T back_flux = (za * sk) * (T)0.125 * tdt;
+            const T front_flux = (za * sf) * (T)0.125 * tdt;
+            const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+            const T vol_change_s = quo(vol, vol + total_flux);
+            const T recip_volume = (T)1 / vol;
+            const T energy_change = (quo(p0, d0) + quo(q0, d0)) * total_flux * recip_volume;
             energy1[i] = e0v - energy_change;
             density1[i] = d0 * vol_change_s;
         }
@@ -660,14 +649,15 @@ __device__ __forceinline__ void ideal_gas_body(const Blk &bk, Lay g, Range r, co
     const int k = r.k0 + bk.y * BY + (int)threadIdx.y;
     const int l = r.l0 + bk.z;
     if (j > r.j1 || k > r.k1) return;
+    using T = S;  // arithmetic type of the form
     const int i = idx(g, j, k, l);
-    const double d = density[i];
-    const double v = 1.0 / d;
-    const double p = (1.4 - 1.0) * d * (double)energy[i];
+    const T d = density[i];
+    const T v = quo((T)1, d);
+    const T p = (T)(1.4 - 1.0) * d * (T)energy[i];
     pressure[i] = p;
-    const double pressurebyenergy = (1.4 - 1.0) * d;
-    const double pressurebyvolume = -d * p;
-    const double sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
+    const T pressurebyenergy = (T)(1.4 - 1.0) * d;
+    const T pressurebyvolume = -d * p;
+    const T sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
     soundspeed[i] = sqrt(sound_speed_squared);
 }
 

@@ -61,6 +61,31 @@ __device__ inline Blk hw_blk()
 __device__ inline double dmin(double a, double b) { return a < b ? a : b; }
 __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 
+// Arithmetic of the two store forms.  The FP64 form evaluates the reference's
+// expressions exactly (true divisions, same order); the FP32 form computes in
+// FP32 and divides by the uniform mesh's constants through reciprocals formed
+// once per thread, and by everything else through the hardware reciprocal.
+template <typename T>
+__device__ inline T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ inline T tmax(T a, T b) { return a > b ? a : b; }
+
+// a / d for a mesh constant d with rd = 1 / d
+template <typename T>
+__device__ inline T by_const(T a, T d, T rd)
+{
+    if constexpr (sizeof(T) == 8) return a / d;
+    else return a * rd;
+}
+
+// a / b
+template <typename T>
+__device__ inline T quo(T a, T b)
+{
+    if constexpr (sizeof(T) == 8) return a / b;
+    else return a * __builtin_amdgcn_rcpf(b);
+}
+
 constexpr int RED_THREADS = 256;
 
 __device__ inline double wave_min(double v)

@@ -10,8 +10,9 @@
 // and needs, for node n, nf at n-2s..n+s, mp at n-s..n+s and vel at n-2s..n+2s.
 // The per-call path stores nf, mp, mq and mom_flux in work arrays; here
 // every value is formed once, in registers, with the same expression and
-// operand order (so the results are bit-identical), and shared with the
-// neighbours that need it:
+// operand order (so the FP64 form's results are bit-identical; the FP32 form
+// runs the same expressions in FP32), and shared with the neighbours that
+// need it:
 //   * lanes of a wavefront lie along x (coalesced rows); neighbours along x
 //     come from lane shuffles;
 //   * the waves of a workgroup lie along the second axis; a neighbour row is
@@ -48,10 +49,10 @@ struct Vel3 {
 // 1 sweep 1 along x, 2 sweep 1 along z, 3 sweep 2 with advect_x,
 // 4 sweep 2 without, 5 and 6 sweep 3
 template <typename S>
-__device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, const S *__restrict__ vfx,
-                                      const S *__restrict__ vfy, const S *__restrict__ vfz)
+__device__ inline S mom_post_vol(int mode, const Lay &g, int i, S V0, const S *__restrict__ vfx,
+                                 const S *__restrict__ vfy, const S *__restrict__ vfz)
 {
-#define D(x) ((double)(x))
+#define D(x) (x)
     switch (mode) {
     case 1: return V0 + D(vfy[i + g.ld]) - D(vfy[i]) + D(vfz[i + g.plane]) - D(vfz[i]);
     case 2: return V0 + D(vfx[i + 1]) - D(vfx[i]) + D(vfy[i + g.ld]) - D(vfy[i]);
@@ -64,48 +65,55 @@ __device__ inline double mom_post_vol(int mode, const Lay &g, int i, double V0, 
 
 // momentum flux through the face between nodes n and n+s
 // (advec_mom_kernel.f90:213-249 and its y/z twins); vm1..vp2 are the
-// velocity at n-s, n, n+s, n+2s; sigma is |nf| / node_mass_pre(donor)
-__device__ inline double mom_face_flux(double nf, double sigma, double d, double vm1, double v0, double vp1,
-                                       double vp2)
+// velocity at n-s, n, n+s, n+2s; sigma is |nf| / node_mass_pre(donor); d the
+// node spacing along the sweep and rd = 1 / d
+template <typename T>
+__device__ inline T mom_face_flux(T nf, T sigma, T d, T rd, T vm1, T v0, T vp1, T vp2)
 {
-    double upwind, donor, downwind;
-    if (nf < 0.0) {
+    T upwind, donor, downwind;
+    if (nf < (T)0) {
         upwind = vp2; donor = vp1; downwind = v0;
     } else {
         upwind = vm1; donor = v0; downwind = vp1;
     }
-    const double width = d;
-    const double vdiffuw = donor - upwind;
-    const double vdiffdw = downwind - donor;
-    double limiter = 0.0;
-    if (vdiffuw * vdiffdw > 0.0) {
-        const double auw = fabs(vdiffuw);
-        const double adw = fabs(vdiffdw);
-        double wind = 1.0;
-        if (vdiffdw <= 0.0) wind = -1.0;
-        limiter = wind * dmin(dmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / d) / 6.0, auw), adw);
+    const T width = d;
+    const T vdiffuw = donor - upwind;
+    const T vdiffdw = downwind - donor;
+    T limiter = 0;
+    if (vdiffuw * vdiffdw > (T)0) {
+        const T auw = fabs(vdiffuw);
+        const T adw = fabs(vdiffdw);
+        T wind = 1;
+        if (vdiffdw <= (T)0) wind = -1;
+        limiter = wind * tmin(tmin(by_const(width * (by_const(((T)2 - sigma) * adw, width, rd) +
+                                                     by_const(((T)1 + sigma) * auw, d, rd)),
+                                            (T)6, (T)(1.0 / 6.0)),
+                                   auw),
+                              adw);
     }
-    const double advec_vel_s = donor + (1.0 - sigma) * limiter;
+    const T advec_vel_s = donor + ((T)1 - sigma) * limiter;
     return advec_vel_s * nf;
 }
 
-__device__ inline double face_sigma(double nf, double mq_lo, double mq_hi)
+template <typename T>
+__device__ inline T face_sigma(T nf, T mq_lo, T mq_hi)
 {
-    return fabs(nf) / (nf < 0.0 ? mq_hi : mq_lo);
+    return quo(fabs(nf), nf < (T)0 ? mq_hi : mq_lo);
 }
 
 // A value and its neighbours at -1 along the lane axis (or +1, see
 // exchange_x) and -1 along the wave axis.
+template <typename T>
 struct Quad {
-    double c, x, w, xw;
+    T c, x, w, xw;
 };
 
 // Publishes a and p of every thread of the workgroup and returns each
 // thread's neighbours: a at lane+dxa, p at lane+dxp, both also one wave
 // down.  Lanes and waves without such a neighbour get their own values
 // (used only by threads that store nothing).
-template <int DXA, int DXP>
-__device__ inline void exchange(double (*buf)[MW][64], int lane, int w, double a, double p, Quad &A, Quad &P)
+template <int DXA, int DXP, typename T>
+__device__ inline void exchange(T (*buf)[MW][64], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
 {
     buf[0][w][lane] = a;
     buf[1][w][lane] = p;
@@ -151,30 +159,31 @@ __device__ inline Raw<S> load_raw(int mode, const Lay &g, int i, bool ok_a, bool
 
 // density1 * post_vol with mom_post_vol's expression order
 template <typename S>
-__device__ inline double raw_cell_mass(int mode, double V0, const Raw<S> &r, bool ok_p)
+__device__ inline S raw_cell_mass(int mode, S V0, const Raw<S> &r, bool ok_p)
 {
-    if (!ok_p) return 0.0;
-    double pv;
+    if (!ok_p) return 0;
+    S pv;
     switch (mode) {
     case 1:
-    case 2: pv = V0 + (double)r.v0 - (double)r.v1 + (double)r.v2 - (double)r.v3; break;
+    case 2: pv = V0 + r.v0 - r.v1 + r.v2 - r.v3; break;
     case 3:
-    case 4: pv = V0 + (double)r.v0 - (double)r.v1; break;
+    case 4: pv = V0 + r.v0 - r.v1; break;
     default: pv = V0; break;
     }
-    return (double)r.rho * pv;
+    return r.rho * pv;
 }
 
 // exchange() in two halves, so loads can be issued between them
-__device__ inline void publish(double (*buf)[MW][64], int lane, int w, double a, double p)
+template <typename T>
+__device__ inline void publish(T (*buf)[MW][64], int lane, int w, T a, T p)
 {
     buf[0][w][lane] = a;
     buf[1][w][lane] = p;
     __syncthreads();
 }
 
-template <int DXA, int DXP>
-__device__ inline void gather(double (*buf)[MW][64], int lane, int w, double a, double p, Quad &A, Quad &P)
+template <int DXA, int DXP, typename T>
+__device__ inline void gather(T (*buf)[MW][64], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
 {
     const int la = min(max(lane + DXA, 0), 63), lp = min(max(lane + DXP, 0), 63), wd = w > 0 ? w - 1 : 0;
     A.c = a;
@@ -197,19 +206,19 @@ __device__ inline void gather(double (*buf)[MW][64], int lane, int w, double a, 
 // and node_mass_post (advec_mom_kernel.f90:267-275, 352-360) sums the cells
 // (dj,dk,dl) = (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1),
 // which is a different order in (x, m, w) for the two sweeps.
-template <int DIR>
-__device__ inline double march_node_flux(const Quad &f0, const Quad &f1)
+template <int DIR, typename T>
+__device__ inline T march_node_flux(const Quad<T> &f0, const Quad<T> &f1)
 {
-    return 0.125 * (f0.x + f0.c + f1.x + f1.c + f0.xw + f0.w + f1.xw + f1.w);
+    return (T)0.125 * (f0.x + f0.c + f1.x + f1.c + f0.xw + f0.w + f1.xw + f1.w);
 }
 
-template <int DIR>
-__device__ inline double march_node_mass(const Quad &pm, const Quad &p0)  // cell planes m-1, m
+template <int DIR, typename T>
+__device__ inline T march_node_mass(const Quad<T> &pm, const Quad<T> &p0)  // cell planes m-1, m
 {
     if (DIR == 2)  // k = m, l = w
-        return 0.125 * (pm.c + p0.c + pm.x + p0.x + pm.w + p0.w + pm.xw + p0.xw);
+        return (T)0.125 * (pm.c + p0.c + pm.x + p0.x + pm.w + p0.w + pm.xw + p0.xw);
     // DIR == 3: k = w, l = m
-    return 0.125 * (p0.w + p0.c + p0.xw + p0.x + pm.w + pm.c + pm.xw + pm.x);
+    return (T)0.125 * (p0.w + p0.c + p0.xw + p0.x + pm.w + pm.c + pm.xw + pm.x);
 }
 
 template <int DIR, typename S>
@@ -217,13 +226,14 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
-    __shared__ double xbuf[2][2][MW][64];
+    using T = S;  // arithmetic type of the form
+    __shared__ T xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int sM = DIR == 3 ? g.plane : g.ld;
     const int wmin = DIR == 3 ? g.y_min : g.z_min, wmax = DIR == 3 ? g.y_max : g.z_max;
     const int mmin = DIR == 3 ? g.z_min : g.y_min, mmax = DIR == 3 ? g.z_max : g.y_max;
-    const double d = DIR == 3 ? geo.dz : geo.dy;
-    const double V0 = geo.vol;
+    const T d = (T)(DIR == 3 ? geo.dz : geo.dy), rd = (T)1 / d;
+    const T V0 = (T)geo.vol;
 
     const int j = g.x_min - 1 + bk.x * 63 + lane;
     const int wp = wmin - 1 + bk.y * (MW - 1) + w;
@@ -236,42 +246,42 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     const int i0 = DIR == 3 ? idx(g, j, wp, mmin) : idx(g, j, mmin, wp);  // offset at march position mmin
 #define OFF(q) (i0 + ((q) - mmin) * sM)
 
-    auto plane = [&](int q, Quad &F, Quad &P, int &b) {
+    auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
-        const double a = ok_cell ? (double)mf[i] : 0.0;  // q <= mmax+3: inside the sweep's face extent
-        const double p = (ok_cell && q <= mmax + 2) ? (double)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
+        const T a = ok_cell ? (T)mf[i] : (T)0;  // q <= mmax+3: inside the sweep's face extent
+        const T p = (ok_cell && q <= mmax + 2) ? (T)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : (T)0;
         exchange<-1, -1>(xbuf[b], lane, w, a, p, F, P);
         b ^= 1;
     };
-    auto vel_at = [&](int c, int q) { return store ? (double)v.in[c][OFF(q)] : 0.0; };
+    auto vel_at = [&](int c, int q) { return store ? (T)v.in[c][OFF(q)] : (T)0; };
 
     int b = 0;
-    Quad Fa, Pa, Fb, Pb;
+    Quad<T> Fa, Pa, Fb, Pb;
     // prologue: nf(m0-2..m0), node masses at m0-1, m0, m0+1, velocities m0-2..m0+1
     plane(m0 - 2, Fa, Pa, b);
     plane(m0 - 1, Fb, Pb, b);
-    const double nf_m2 = march_node_flux<DIR>(Fa, Fb);
-    const double mp_m1 = march_node_mass<DIR>(Pa, Pb);
+    const T nf_m2 = march_node_flux<DIR, T>(Fa, Fb);
+    const T mp_m1 = march_node_mass<DIR, T>(Pa, Pb);
     plane(m0, Fa, Pa, b);
-    double nf_lo = march_node_flux<DIR>(Fb, Fa);  // nf(m-1)
-    double mp0 = march_node_mass<DIR>(Pb, Pa);    // mp(m)
-    const double mq_m1 = mp_m1 - nf_m2 + nf_lo;
+    T nf_lo = march_node_flux<DIR, T>(Fb, Fa);  // nf(m-1)
+    T mp0 = march_node_mass<DIR, T>(Pb, Pa);    // mp(m)
+    const T mq_m1 = mp_m1 - nf_m2 + nf_lo;
     plane(m0 + 1, Fb, Pb, b);
-    double nf0 = march_node_flux<DIR>(Fa, Fb);     // nf(m)
-    double mp1 = march_node_mass<DIR>(Pa, Pb);     // mp(m+1)
-    double mq0 = mp0 - nf_lo + nf0;                // mq(m)
+    T nf0 = march_node_flux<DIR, T>(Fa, Fb);     // nf(m)
+    T mp1 = march_node_mass<DIR, T>(Pa, Pb);     // mp(m+1)
+    T mq0 = mp0 - nf_lo + nf0;                // mq(m)
     // Fb, Pb hold plane m+1
 
-    double vm1[3], v0[3], vp1[3], flo[3];
+    T vm1[3], v0[3], vp1[3], flo[3];
     {
-        const double sig = face_sigma(nf_lo, mq_m1, mq0);
+        const T sig = face_sigma(nf_lo, mq_m1, mq0);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double vm2 = vel_at(c, m0 - 2);
+            const T vm2 = vel_at(c, m0 - 2);
             vm1[c] = vel_at(c, m0 - 1);
             v0[c] = vel_at(c, m0);
             vp1[c] = vel_at(c, m0 + 1);
-            flo[c] = mom_face_flux(nf_lo, sig, d, vm2, vm1[c], v0[c], vp1[c]);  // F(m0-1)
+            flo[c] = mom_face_flux(nf_lo, sig, d, rd, vm2, vm1[c], v0[c], vp1[c]);  // F(m0-1)
         }
     }
 
@@ -282,28 +292,28 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
     for (int m = m0; m <= m1; m++) {
         const Raw<S> rc = rn;
-        double vp2[3];
+        T vp2[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) vp2[c] = (double)vn[c];
-        const double a = (double)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
+        for (int c = 0; c < 3; c++) vp2[c] = (T)vn[c];
+        const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
         publish(xbuf[b], lane, w, a, p);
         if (m < m1) {
             rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 3)] : (S)0;
         }
-        Quad Fc, Pc;
+        Quad<T> Fc, Pc;
         gather<-1, -1>(xbuf[b], lane, w, a, p, Fc, Pc);
         b ^= 1;
-        const double nf1 = march_node_flux<DIR>(Fb, Fc);  // nf(m+1)
-        const double mp2 = march_node_mass<DIR>(Pb, Pc);  // mp(m+2), garbage at mmax+3 (never used)
-        const double mq1 = mp1 - nf0 + nf1;              // mq(m+1)
-        const double sig = face_sigma(nf0, mq0, mq1);
+        const T nf1 = march_node_flux<DIR, T>(Fb, Fc);  // nf(m+1)
+        const T mp2 = march_node_mass<DIR, T>(Pb, Pc);  // mp(m+2), garbage at mmax+3 (never used)
+        const T mq1 = mp1 - nf0 + nf1;              // mq(m+1)
+        const T sig = face_sigma(nf0, mq0, mq1);
         const int i = OFF(m);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double fhi = mom_face_flux(nf0, sig, d, vm1[c], v0[c], vp1[c], vp2[c]);  // F(m)
-            if (store) v.out[c][i] = (v0[c] * mq0 + flo[c] - fhi) / mp0;
+            const T fhi = mom_face_flux(nf0, sig, d, rd, vm1[c], v0[c], vp1[c], vp2[c]);  // F(m)
+            if (store) v.out[c][i] = quo(v0[c] * mq0 + flo[c] - fhi, mp0);
             flo[c] = fhi;
             vm1[c] = v0[c];
             v0[c] = vp1[c];
@@ -333,16 +343,18 @@ __global__ void __launch_bounds__(64 * MW) k_fast_mom_march(Lay g, Geo geo, int 
 //   nf(j)  = 0.125*(f(0,-1,0)+f(0,0,0)+f(1,-1,0)+f(1,0,0)+f(0,-1,-1)+f(0,0,-1)+f(1,-1,-1)+f(1,0,-1))
 //   mp(j)  = cells (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1)
 __device__ inline double shfl_d(double x, int src) { return __shfl(x, src, 64); }
+__device__ inline float shfl_d(float x, int src) { return __shfl(x, src, 64); }
 
 template <typename S>
 __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
              Vel3<S> v)
 {
-    __shared__ double xbuf[2][2][MW][64];
+    using T = S;  // arithmetic type of the form
+    __shared__ T xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
-    const double d = geo.dx;
-    const double V0 = geo.vol;
+    const T d = (T)geo.dx, rd = (T)1 / d;
+    const T V0 = (T)geo.vol;
     const int j = g.x_min - 2 + bk.x * 60 + lane;
     const int k = g.y_min - 1 + bk.y * (MW - 1) + w;
     const int l0 = g.z_min + bk.z * MCHUNK;
@@ -356,16 +368,16 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     const int i0 = idx(g, j, k, g.z_min);
 #define OFF(q) (i0 + ((q) - g.z_min) * g.plane)
 
-    auto plane = [&](int q, Quad &F, Quad &P, int &b) {
+    auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
-        const double a = ok_face ? (double)mf[i] : 0.0;
-        const double p = ok_cell ? (double)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : 0.0;
+        const T a = ok_face ? (T)mf[i] : (T)0;
+        const T p = ok_cell ? (T)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : (T)0;
         exchange<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
         b ^= 1;
     };
 
     int b = 0;
-    Quad Fp, Pp;  // plane l-1
+    Quad<T> Fp, Pp;  // plane l-1
     plane(l0 - 1, Fp, Pp, b);
     const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63, lp2 = lane < 62 ? lane + 2 : 63;
     // plane l's loads are issued during plane l-1 (after its barrier)
@@ -375,34 +387,34 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l0)] : (S)0;
     for (int l = l0; l <= l1; l++) {
         const Raw<S> rc = rn;
-        double vc[3];
+        T vc[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) vc[c] = (double)vn[c];
-        const double a = (double)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell);
+        for (int c = 0; c < 3; c++) vc[c] = (T)vn[c];
+        const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell);
         publish(xbuf[b], lane, w, a, p);
         if (l < l1) {
             rn = load_raw(mode, g, OFF(l + 1), ok_face, ok_cell, mf, vfx, vfy, vfz, density1);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l + 1)] : (S)0;
         }
-        Quad F, P;
+        Quad<T> F, P;
         gather<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
         b ^= 1;
         // F.x = f(+1,0), F.w = f(0,-1), F.xw = f(+1,-1); P.x = cell(-1,0), ...
-        const double nf = 0.125 * (F.w + F.c + F.xw + F.x + Fp.w + Fp.c + Fp.xw + Fp.x);
-        const double mp = 0.125 * (P.w + P.c + P.xw + P.x + Pp.w + Pp.c + Pp.xw + Pp.x);
-        const double nf_lo = shfl_d(nf, lm);
-        const double mq = mp - nf_lo + nf;
-        const double mq_hi = shfl_d(mq, lp);
-        const double sig = face_sigma(nf, mq, mq_hi);
+        const T nf = 0.125 * (F.w + F.c + F.xw + F.x + Fp.w + Fp.c + Fp.xw + Fp.x);
+        const T mp = 0.125 * (P.w + P.c + P.xw + P.x + Pp.w + Pp.c + Pp.xw + Pp.x);
+        const T nf_lo = shfl_d(nf, lm);
+        const T mq = mp - nf_lo + nf;
+        const T mq_hi = shfl_d(mq, lp);
+        const T sig = face_sigma(nf, mq, mq_hi);
         const int i = OFF(l);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double v0 = vc[c];
-            const double vm1 = shfl_d(v0, lm), vp1 = shfl_d(v0, lp), vp2 = shfl_d(v0, lp2);
-            const double fhi = mom_face_flux(nf, sig, d, vm1, v0, vp1, vp2);  // face j | j+1
-            const double flo = shfl_d(fhi, lm);                                // face j-1 | j
-            if (store) v.out[c][i] = (v0 * mq + flo - fhi) / mp;
+            const T v0 = vc[c];
+            const T vm1 = shfl_d(v0, lm), vp1 = shfl_d(v0, lp), vp2 = shfl_d(v0, lp2);
+            const T fhi = mom_face_flux(nf, sig, d, rd, vm1, v0, vp1, vp2);  // face j | j+1
+            const T flo = shfl_d(fhi, lm);                                // face j-1 | j
+            if (store) v.out[c][i] = quo(v0 * mq + flo - fhi, mp);
         }
         Fp = F;
         Pp = P;
