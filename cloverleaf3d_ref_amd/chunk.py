@@ -53,6 +53,7 @@ class SmallTimestep(RuntimeError):
 
 PER_CALL = 1  # CLOVER_CHUNK_PER_CALL
 FP32 = 2  # CLOVER_CHUNK_FP32
+HALO_CHECK = 4  # CLOVER_CHUNK_HALO_CHECK
 
 
 class Chunk:
@@ -62,16 +63,19 @@ class Chunk:
     per_call=False (default) runs the fused kernels; per_call=True runs one
     kernel per reference loop nest.  Both produce bit-identical fields.
     fp32=True holds the fields in FP32 (the FP32-store form, fused kernels
-    only); fetch() still returns FP64 arrays."""
+    only); fetch() still returns FP64 arrays.  halo_check=True is a test mode
+    of the fused kernels: the halos the fused step does not refresh (and the
+    reference does) get wrong values, so that reading one changes results."""
 
-    def __init__(self, deck: Deck, device: int = 0, per_call: bool = False, fp32: bool = False):
+    def __init__(self, deck: Deck, device: int = 0, per_call: bool = False, fp32: bool = False,
+                 halo_check: bool = False):
         self.deck = deck
         self.per_call = per_call
         self.fp32 = fp32
         self._L = _lib.lib()
         self._cdeck = to_c_deck(deck)
         h = ctypes.c_void_p()
-        flags = (PER_CALL if per_call else 0) | (FP32 if fp32 else 0)
+        flags = (PER_CALL if per_call else 0) | (FP32 if fp32 else 0) | (HALO_CHECK if halo_check else 0)
         _lib.check(self._L.clover_chunk_create_ex(ctypes.byref(self._cdeck), device, flags, ctypes.byref(h)),
                    "clover_chunk_create_ex")
         self._h = h

@@ -72,9 +72,10 @@ hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGr
 // advec_mom sweep `dir`; new velocities into X_W5..X_W7
 hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
                            int sweep_number, hipStream_t st);
-// reflective halo of the flagged fields along one axis, both faces
+// reflective halo of the flagged fields along one axis, both faces; axes
+// (null: all) as for launch_update_halo_store
 hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,
-                         const int *fields, hipStream_t st);
+                         const int *fields, hipStream_t st, const int *axes = nullptr);
 
 // Per-deck steps (one workgroup per deck).
 // timestep.f90:86-126 on the device: dt = min over the deck's partials,

@@ -125,6 +125,7 @@ struct clover_chunk {
     EventTimer *ktimer;  // non-null while per-kernel timing is on
     bool fast;           // fused kernels (fast.h) instead of the per-call ones
     Store store;         // element type of the field arrays (ST_F32 only with the fused kernels)
+    bool halo_check;     // CLOVER_CHUNK_HALO_CHECK
     Geo geo;             // the uniform mesh, for the fused kernels
 };
 
@@ -332,7 +333,8 @@ static void set_fields(int *fields, std::initializer_list<int> ids)
     for (int id : ids) fields[id] = 1;
 }
 
-static void update_halo(clover_chunk *c, const int *fields, int depth)
+// axes: null, or the per-field axis masks of a fused-step plan (driver.h)
+static void update_halo(clover_chunk *c, const int *fields, int depth, const int *axes = nullptr)
 {
     {
         Phase ph(c, P_TILE_HALO);
@@ -352,7 +354,8 @@ static void update_halo(clover_chunk *c, const int *fields, int depth)
                         fp[f] = tt.f[f];
                         np[f] = nt.f[f];
                     }
-                    hip_check(launch_tile_halo_store(c->store, face, tt.g, fp, nt.g, np, fields, depth, c->st),
+                    hip_check(launch_tile_halo_store(c->store, face, tt.g, fp, nt.g, np, fields, depth, c->st, axes,
+                                                     axes && c->halo_check),
                               "tile_halo");
                 }
     }
@@ -362,8 +365,16 @@ static void update_halo(clover_chunk *c, const int *fields, int depth)
         for (int f = 0; f < 6; f++) ext[f] = t.nbr[f] == -1;  // one chunk: every chunk face is external
         void *fp[NUM_FIELDS];
         for (int f = 0; f < NUM_FIELDS; f++) fp[f] = t.f[f];
-        hip_check(launch_update_halo_store(c->store, t.g, fp, fields, ext, depth, c->st), "update_halo");
+        hip_check(launch_update_halo_store(c->store, t.g, fp, fields, ext, depth, c->st, axes, axes && c->halo_check),
+                  "update_halo");
     }
+}
+
+// an exchange of the fused step, reduced to the halos its kernels read
+static void fused_halo(clover_chunk *c, int which)
+{
+    const HaloPlan p = fused_halo_plan(which, c->advect_x != 0);
+    update_halo(c, p.fields, p.depth, p.axes);
 }
 
 static void ideal_gas(clover_chunk *c, bool predict)
@@ -391,8 +402,12 @@ static int timestep(clover_chunk *c)
     const int T = (int)c->tiles.size();
     c->dt = c->g_big;
     ideal_gas(c, false);
-    set_fields(fields, {F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0});
-    update_halo(c, fields, 1);
+    if (c->fast) {
+        fused_halo(c, FH_TIMESTEP);
+    } else {
+        set_fields(fields, {F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0});
+        update_halo(c, fields, 1);
+    }
     int nparts[64] = {0};
     if (c->fast) {
         // viscosity and calc_dt in one pass: calc_dt reads viscosity only at
@@ -417,8 +432,12 @@ static int timestep(clover_chunk *c)
                                        t.f[F_YVEL0], t.f[F_ZVEL0], c->st),
                       "viscosity");
     }
-    set_fields(fields, {F_VISCOSITY});
-    update_halo(c, fields, 1);
+    if (c->fast) {
+        fused_halo(c, FH_VISCOSITY);
+    } else {
+        set_fields(fields, {F_VISCOSITY});
+        update_halo(c, fields, 1);
+    }
     {
         Phase ph(c, P_TIMESTEP);
         if (!c->fast) {
@@ -466,9 +485,7 @@ static void pdv(clover_chunk *c, bool predict)
                                                   t.f[F_ZVEL0], c->st),
                           "PdV predict");
         }
-        int fields[NUM_FIELDS];
-        set_fields(fields, {F_PRESSURE});
-        update_halo(c, fields, 1);
+        fused_halo(c, FH_PRESSURE);
         return;
     }
     if (c->fast) {
@@ -567,18 +584,23 @@ static void advection(clover_chunk *c)
     int fc[NUM_FIELDS], fm[NUM_FIELDS];
     set_fields(fc, {F_ENERGY1, F_DENSITY1, F_VOL_FLUX_X, F_VOL_FLUX_Y, F_VOL_FLUX_Z});
     set_fields(fm, {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1, F_MASS_FLUX_X, F_MASS_FLUX_Y, F_MASS_FLUX_Z});
+    // the fused step refreshes only what its kernels read (driver.h)
+    auto exchange = [&](const int *fields, int which) {
+        if (c->fast) fused_halo(c, which);
+        else update_halo(c, fields, 2);
+    };
     int dir = c->advect_x ? 1 : 3;
-    update_halo(c, fc, 2);
+    exchange(fc, FH_CELL1);
     advec_cell(c, 1, dir);
-    update_halo(c, fm, 2);
+    exchange(fm, FH_MOM1);
     advec_mom(c, 1, dir);
-    update_halo(c, fc, 2);
+    exchange(fc, FH_CELL2);
     advec_cell(c, 2, 2);
-    update_halo(c, fm, 2);
+    exchange(fm, FH_MOM2);
     advec_mom(c, 2, 2);
     dir = c->advect_x ? 3 : 1;
     advec_cell(c, 3, dir);
-    update_halo(c, fm, 2);
+    exchange(fm, FH_MOM3);
     advec_mom(c, 3, dir);
 }
 
@@ -702,14 +724,18 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         if (d.x_cells < 1 || d.y_cells < 1 || d.z_cells < 1) throw Error(E_ARG, "cells must be positive");
         if (d.number_of_states < 1 || d.number_of_states > CLOVER_MAX_STATES) throw Error(E_ARG, "bad state count");
         if (d.tiles_per_chunk < 1) throw Error(E_ARG, "tiles_per_chunk must be positive");
-        if (flags & ~(CLOVER_CHUNK_PER_CALL | CLOVER_CHUNK_FP32)) throw Error(E_ARG, "unknown flags");
+        if (flags & ~(CLOVER_CHUNK_PER_CALL | CLOVER_CHUNK_FP32 | CLOVER_CHUNK_HALO_CHECK))
+            throw Error(E_ARG, "unknown flags");
         if ((flags & CLOVER_CHUNK_PER_CALL) && (flags & CLOVER_CHUNK_FP32))
             throw Error(E_ARG, "the FP32-store form runs the fused kernels only");
+        if ((flags & CLOVER_CHUNK_PER_CALL) && (flags & CLOVER_CHUNK_HALO_CHECK))
+            throw Error(E_ARG, "the halo check applies to the fused kernels only");
         c = new clover_chunk();  // value-initialised: every scalar member starts at zero
         c->deck = d;
         c->device = device;
         c->fast = !(flags & CLOVER_CHUNK_PER_CALL);
         c->store = (flags & CLOVER_CHUNK_FP32) ? ST_F32 : ST_F64;
+        c->halo_check = (flags & CLOVER_CHUNK_HALO_CHECK) != 0;
         // the uniform mesh as initialise_chunk.f90:37-39 and
         // initialise_chunk_kernel.f90:151-178 compute it
         c->geo.dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;

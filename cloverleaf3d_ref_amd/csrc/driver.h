@@ -3,6 +3,7 @@
 // the tile decomposition and the host-side mesh/state fill.
 #pragma once
 
+#include <initializer_list>
 #include <vector>
 
 #include "../../include/clover_hip.h"
@@ -47,5 +48,105 @@ struct HostInit {
 // fields=false fills only the 1-D arrays (the 3-D fields are painted on the
 // device, setup.h)
 void host_init(const clover_deck &d, const Tile &t, HostInit &h, bool fields = true);
+
+// The halo exchanges of the fused step, with the fields the reference
+// refreshes and the axes of them that a fused kernel reads before the
+// field changes or is refreshed again; the rest are dead (the fused kernels
+// read the same cells as the reference's loop nests, but the reference
+// refreshes every flagged field on every face).
+//   FH_TIMESTEP   timestep.f90:69-71 (pressure, energy0, density0, xvel0,
+//                 yvel0): viscosity reads pressure at the face neighbours,
+//                 accelerate reads density0 at the 8 cells around each node
+//                 (corners included); energy0 and the velocities are read
+//                 at interior cells and nodes only.
+//   FH_VISCOSITY  timestep.f90:80-84: accelerate reads it around nodes.
+//   FH_PRESSURE   PdV.f90:99-101 after the predictor: accelerate.
+//   FH_CELL1      advection.f90 before the first cell sweep (energy1,
+//                 density1, vol_flux_x/y/z, depth 2): the sweep reads
+//                 density1/energy1 only along its axis (the rows across it
+//                 are interior).  The volume fluxes keep their values for
+//                 the whole advection and every momentum sweep reads them
+//                 around its nodes on all axes: refreshed here, once.
+//   FH_MOM1..3    before each momentum sweep (density1, energy1, x/y/zvel1,
+//                 mass_flux_x/y/z): the sweep reads density1 and its own
+//                 mass flux on all axes, the velocities along its axis only,
+//                 and not energy1.  FH_MOM2 also keeps energy1 along the
+//                 third sweep's axis: the third cell sweep follows the
+//                 second momentum sweep with no exchange of its own.  The
+//                 other two mass fluxes are left out altogether: a momentum
+//                 sweep reads face corners (e.g. mass_flux_y at y_max+3 and
+//                 z_min-1) that an exchange fills from halo cells written by
+//                 the exchange before it, and those hold the same values
+//                 whether or not the intermediate exchanges ran.
+//   FH_CELL2      before the second (y) cell sweep: density1/energy1 along
+//                 y, and density1 along z too: the y sweep also forms
+//                 mass_flux_y on the two planes above z_max (its face loop
+//                 runs to z_max+2, advec_cell_kernel.f90:215-217), and
+//                 FH_MOM2's y faces copy those values into corners the y
+//                 momentum sweep reads.  The volume fluxes still hold
+//                 FH_CELL1's halos, so they are left out of this exchange
+//                 altogether (neither refreshed nor, in the halo-check test
+//                 mode, overwritten).
+enum FusedHalo : int { FH_TIMESTEP, FH_VISCOSITY, FH_PRESSURE, FH_CELL1, FH_MOM1, FH_CELL2, FH_MOM2, FH_MOM3 };
+
+enum : int { AX_X = 1, AX_Y = 2, AX_Z = 4, AX_ALL = 7 };
+
+struct HaloPlan {
+    int depth;
+    int fields[NUM_FIELDS];  // 1: refreshed by the reference (and not still valid)
+    int axes[NUM_FIELDS];    // axis bits the fused kernels read
+};
+
+inline HaloPlan fused_halo_plan(int which, bool advect_x)
+{
+    HaloPlan p{};
+    auto ref = [&](std::initializer_list<int> ids) {
+        for (int f : ids) p.fields[f] = 1;
+    };
+    const int s1 = advect_x ? AX_X : AX_Z, s3 = advect_x ? AX_Z : AX_X;
+    auto mass_flux = [](int ax) { return ax == AX_X ? F_MASS_FLUX_X : ax == AX_Y ? F_MASS_FLUX_Y : F_MASS_FLUX_Z; };
+    const int fc_ids[] = {F_ENERGY1, F_DENSITY1, F_VOL_FLUX_X, F_VOL_FLUX_Y, F_VOL_FLUX_Z};
+    auto mom = [&](int s) {
+        ref({F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1, mass_flux(s)});
+        p.axes[F_DENSITY1] = AX_ALL;
+        p.axes[F_XVEL1] = p.axes[F_YVEL1] = p.axes[F_ZVEL1] = s;
+        p.axes[mass_flux(s)] = AX_ALL;
+    };
+    p.depth = 2;
+    switch (which) {
+    case FH_TIMESTEP:
+        ref({F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0});
+        p.axes[F_PRESSURE] = p.axes[F_DENSITY0] = AX_ALL;
+        p.depth = 1;
+        break;
+    case FH_VISCOSITY:
+        ref({F_VISCOSITY});
+        p.axes[F_VISCOSITY] = AX_ALL;
+        p.depth = 1;
+        break;
+    case FH_PRESSURE:
+        ref({F_PRESSURE});
+        p.axes[F_PRESSURE] = AX_ALL;
+        p.depth = 1;
+        break;
+    case FH_CELL1:
+        for (int f : fc_ids) p.fields[f] = 1;
+        p.axes[F_DENSITY1] = p.axes[F_ENERGY1] = s1;
+        p.axes[F_VOL_FLUX_X] = p.axes[F_VOL_FLUX_Y] = p.axes[F_VOL_FLUX_Z] = AX_ALL;
+        break;
+    case FH_CELL2:
+        ref({F_ENERGY1, F_DENSITY1});
+        p.axes[F_DENSITY1] = AX_Y | AX_Z;
+        p.axes[F_ENERGY1] = AX_Y;
+        break;
+    case FH_MOM1: mom(s1); break;
+    case FH_MOM2:
+        mom(AX_Y);
+        p.axes[F_ENERGY1] = s3;
+        break;
+    default: mom(s3); break;
+    }
+    return p;
+}
 
 }  // namespace clover

@@ -82,7 +82,16 @@ void set_fields(int *fields, std::initializer_list<int> ids)
 }
 
 // update_halo for single-tile decks: the reflective boundary on every face,
-// y faces, then x, then z (as launch_update_halo orders them)
+// y faces, then x, then z (as launch_update_halo orders them), reduced to
+// the halos the fused kernels read (fused_halo_plan, driver.h)
+int items_on_axis(const HaloPlan &p, int axis)
+{
+    int n = 0;
+    for (int f = 0; f < NUM_FIELDS; f++) n += p.fields[f] == 1 && ((p.axes[f] >> axis) & 1);
+    return 2 * n;  // both faces
+}
+
+// every flagged field on every axis (start.f90's exchange)
 void halo(clover_ensemble *e, const int *fields, int depth)
 {
     int n = 0;
@@ -92,6 +101,20 @@ void halo(clover_ensemble *e, const int *fields, int depth)
         const int axis = axis_order[ao];
         hip_check(batch_reflect(e->store, e->dev, e->nd, grid(e, 100 + axis, depth, 2 * n), axis, depth, fields,
                                 e->st),
+                  "halo");
+    }
+}
+
+void halo(clover_ensemble *e, int which)
+{
+    const HaloPlan p = fused_halo_plan(which, e->advect_x != 0);
+    static const int axis_order[3] = {1, 0, 2};
+    for (int ao = 0; ao < 3; ao++) {
+        const int axis = axis_order[ao];
+        const int n = items_on_axis(p, axis);
+        if (n == 0) continue;
+        hip_check(batch_reflect(e->store, e->dev, e->nd, grid(e, 100 + axis, p.depth, n), axis, p.depth, p.fields,
+                                e->st, p.axes),
                   "halo");
     }
 }
@@ -130,38 +153,31 @@ void advec_mom(clover_ensemble *e, int sweep, int dir)
 // hydro.f90:48-68 for every deck, device only
 void enqueue_step(clover_ensemble *e)
 {
-    int f[NUM_FIELDS];
     // timestep.f90:28-128
     ideal_gas(e);
-    set_fields(f, {F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0});
-    halo(e, f, 1);
+    halo(e, FH_TIMESTEP);
     const BatchGrid &vg = grid(e, BK_VISC_DT);
     hip_check(batch_visc_dt(e->store, e->dev, e->nd, vg, e->g_small, e->visc_partial, e->st), "visc_dt");
     hip_check(batch_deck_dt(e->dev, e->nd, vg.start, e->visc_partial, e->g_big, e->st), "deck dt");
-    set_fields(f, {F_VISCOSITY});
-    halo(e, f, 1);
+    halo(e, FH_VISCOSITY);
     // PdV predictor (+ ideal_gas on the predicted state), pressure halo
     hip_check(batch_pdv_predict(e->store, e->dev, e->nd, grid(e, BK_PDV_PREDICT), e->st), "PdV predict");
-    set_fields(f, {F_PRESSURE});
-    halo(e, f, 1);
+    halo(e, FH_PRESSURE);
     hip_check(batch_accelerate(e->store, e->dev, e->nd, grid(e, BK_ACCELERATE), e->st), "accelerate");
     hip_check(batch_pdv_correct_flux(e->store, e->dev, e->nd, grid(e, BK_PDV_CORRECT), e->st), "PdV correct");
     // advection.f90:26-181
-    int fc[NUM_FIELDS], fm[NUM_FIELDS];
-    set_fields(fc, {F_ENERGY1, F_DENSITY1, F_VOL_FLUX_X, F_VOL_FLUX_Y, F_VOL_FLUX_Z});
-    set_fields(fm, {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1, F_MASS_FLUX_X, F_MASS_FLUX_Y, F_MASS_FLUX_Z});
     int dir = e->advect_x ? 1 : 3;
-    halo(e, fc, 2);
+    halo(e, FH_CELL1);
     advec_cell(e, 1, dir);
-    halo(e, fm, 2);
+    halo(e, FH_MOM1);
     advec_mom(e, 1, dir);
-    halo(e, fc, 2);
+    halo(e, FH_CELL2);
     advec_cell(e, 2, 2);
-    halo(e, fm, 2);
+    halo(e, FH_MOM2);
     advec_mom(e, 2, 2);
     dir = e->advect_x ? 3 : 1;
     advec_cell(e, 3, dir);
-    halo(e, fm, 2);
+    halo(e, FH_MOM3);
     advec_mom(e, 3, dir);
     // reset_field as an exchange of levels, then time = time + dt
     swap(e, {{F_DENSITY0, F_DENSITY1}, {F_ENERGY0, F_ENERGY1}, {F_XVEL0, F_XVEL1}, {F_YVEL0, F_YVEL1},
@@ -174,10 +190,15 @@ void enqueue_step(clover_ensemble *e)
 void prepare_grids(clover_ensemble *e)
 {
     for (int k = 0; k < BK_COUNT; k++) grid(e, k);
-    // (depth, flagged fields) of the step's halo updates
-    static const int sets[][2] = {{1, 5}, {1, 1}, {2, 5}, {2, 8}, {2, 12}};
-    for (auto &s : sets)
-        for (int axis = 0; axis < 3; axis++) grid(e, 100 + axis, s[0], 2 * s[1]);
+    // the step's halo updates, for both sweep orders
+    for (int ax = 0; ax < 2; ax++)
+        for (int which = FH_TIMESTEP; which <= FH_MOM3; which++) {
+            const HaloPlan p = fused_halo_plan(which, ax != 0);
+            for (int axis = 0; axis < 3; axis++) {
+                const int n = items_on_axis(p, axis);
+                if (n) grid(e, 100 + axis, p.depth, n);
+            }
+        }
 }
 
 void destroy(clover_ensemble *e)

@@ -98,10 +98,15 @@ hipError_t launch_fast_field_summary(Store s, const Lay &g, const Geo &m, const 
 
 // Reflective halo and tile copies on arrays of either element type
 // (halo_kernels.hip; same tables and order as launch_update_halo and
-// launch_tile_halo of kernels.h).
+// launch_tile_halo of kernels.h).  fields[f] == 1 selects what the
+// reference refreshes; axes (null: all) narrows each selected field to a bit
+// mask of axes (1 x, 2 y, 4 z).  With poison, the selected (field, axis)
+// halos outside the mask get wrong values instead (a test of the mask).
 hipError_t launch_update_halo_store(Store s, const Lay &g, void *const *field_ptrs, const int *fields,
-                                    const int *ext, int depth, hipStream_t st);
+                                    const int *ext, int depth, hipStream_t st, const int *axes = nullptr,
+                                    bool poison = false);
 hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *field_ptrs, const Lay &gn,
-                                  void *const *nbr_ptrs, const int *fields, int depth, hipStream_t st);
+                                  void *const *nbr_ptrs, const int *fields, int depth, hipStream_t st,
+                                  const int *axes = nullptr, bool poison = false);
 
 }  // namespace clover

@@ -6,6 +6,11 @@
 // thick along it, else one face) for every flagged field: blockIdx.z picks
 // (field, face), threads cover the two tangential axes, each thread walks
 // the 1..depth halo layers.  Exact moves, no arithmetic.
+//
+// The resident driver refreshes only the (field, axis) halos its kernels
+// read (the axis masks below); in its test mode the halos it skips are
+// instead filled with wrong values (1.5 x + 0.25 of the value the reference
+// would put there), so any read of a skipped halo changes interior results.
 #include <hip/hip_runtime.h>
 
 #include "batch.h"
@@ -69,7 +74,15 @@ struct HaloItem {
     int kind;  // reflective: K_*; tile copy: staggered on the axis (0/1)
     int hi;    // 0 low face, 1 high face
     int ea, eb;
+    int poison;  // write the test-mode wrong value instead
 };
+
+__device__ inline double poisoned(double v) { return v * 1.5 + 0.25; }
+
+// does a launch for `axis` refresh field f?  fields[f] == 1 selects the
+// fields the reference refreshes; axes (null: every axis) narrows that to
+// a bit mask of axes (1 x, 2 y, 4 z) per field.
+static inline bool wanted(const int *axes, int f, int axis) { return !axes || ((axes[f] >> axis) & 1); }
 
 struct HaloBatch {
     HaloItem it[kMaxItems];
@@ -83,7 +96,7 @@ __device__ inline int tan_b(int axis) { return axis == 2 ? 1 : 2; }
 // reflective halo of one (field, face) item over the tangential block (bx, by)
 template <typename S>
 __device__ inline void reflect_body(const Lay &g, int axis, int depth, S *dst, double sgn, int kind, int hi, int ea,
-                                    int eb, int bx, int by)
+                                    int eb, int bx, int by, int poison)
 {
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
@@ -108,7 +121,8 @@ __device__ inline void reflect_body(const Lay &g, int axis, int depth, S *dst, d
         iw[axis] = w; ir[axis] = r;
         iw[ta] = ir[ta] = c;
         iw[tb] = ir[tb] = b;
-        dst[idx(g, iw[0], iw[1], iw[2])] = (S)(sgn * (double)dst[idx(g, ir[0], ir[1], ir[2])]);
+        const double v = sgn * (double)dst[idx(g, ir[0], ir[1], ir[2])];
+        dst[idx(g, iw[0], iw[1], iw[2])] = (S)(poison ? poisoned(v) : v);
     }
 }
 
@@ -117,7 +131,7 @@ __global__ void k_reflect(Lay g, int axis, int depth, HaloBatch batch)
 {
     const HaloItem &it = batch.it[blockIdx.z];
     reflect_body<S>(g, axis, depth, static_cast<S *>(it.dst), it.sgn, it.kind, it.hi, it.ea, it.eb, (int)blockIdx.x,
-                    (int)blockIdx.y);
+                    (int)blockIdx.y, it.poison);
 }
 
 static hipError_t run_batch_reflect(Store s, const Lay &g, int axis, int depth, const HaloBatch &b, int n,
@@ -137,7 +151,7 @@ static hipError_t run_batch_reflect(Store s, const Lay &g, int axis, int depth, 
 }
 
 hipError_t launch_update_halo_store(Store s, const Lay &g, void *const *fp, const int *fields, const int *ext,
-                                    int depth, hipStream_t st)
+                                    int depth, hipStream_t st, const int *axes, bool poison)
 {
     // The reference's order per field is bottom, top, left, right, back,
     // front; fields are independent arrays, so all fields' y faces go first,
@@ -156,8 +170,11 @@ hipError_t launch_update_halo_store(Store s, const Lay &g, void *const *fp, cons
             if (!ext[face]) continue;
             for (int f = 0; f < NUM_FIELDS; f++) {
                 if (fields[f] != 1 || !fp[f]) continue;
+                const bool want = wanted(axes, f, axis);
+                if (!want && !poison) continue;
                 const HaloDesc &h = kHalo[f];
                 HaloItem &it = b.it[n++];
+                it.poison = !want;
                 it.dst = fp[f];
                 it.src = fp[f];
                 it.sgn = h.neg[axis] ? -1.0 : 1.0;
@@ -211,12 +228,13 @@ __global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
         iw[axis] = w; ir[axis] = r;
         iw[ta] = ir[ta] = c;
         iw[tb] = ir[tb] = b;
-        dst[idx(g, iw[0], iw[1], iw[2])] = src[idx(gn, ir[0], ir[1], ir[2])];
+        const S v = src[idx(gn, ir[0], ir[1], ir[2])];
+        dst[idx(g, iw[0], iw[1], iw[2])] = it.poison ? (S)poisoned((double)v) : v;
     }
 }
 
 hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *fp, const Lay &gn, void *const *np,
-                                  const int *fields, int depth, hipStream_t st)
+                                  const int *fields, int depth, hipStream_t st, const int *axes, bool poison)
 {
     const int axis = (face - 1) / 2;
     const int hi = (face - 1) % 2;
@@ -233,7 +251,10 @@ hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *
         if (f == F_ZVEL0 && face == 2) src = F_YVEL0;
         if (fields[gate] != 1) continue;
         if (!fp[src] || !np[src]) continue;
+        const bool want = wanted(axes, src, axis);
+        if (!want && !poison) continue;
         HaloItem &it = b.it[n++];
+        it.poison = !want;
         it.dst = fp[src];
         it.src = np[src];
         it.sgn = 1.0;
@@ -296,18 +317,18 @@ __global__ void kb_reflect(const DeckDev *__restrict__ decks, const int *__restr
     const int item = lb / (ga * gb), r = lb % (ga * gb);
     const BItem &it = items.it[item];
     reflect_body<S>(D.g, axis, depth, static_cast<S *>(D.f[it.field]), it.sgn, it.kind, it.hi, it.ea, it.eb, r % ga,
-                    r / ga);
+                    r / ga, 0);
 }
 
 hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,
-                         const int *fields, hipStream_t st)
+                         const int *fields, hipStream_t st, const int *axes)
 {
     BItems b;
     b.n = 0;
     for (int side = 0; side < 2; side++) {
         const int face = 2 * axis + side;
         for (int f = 0; f < NUM_FIELDS; f++) {
-            if (fields[f] != 1) continue;
+            if (fields[f] != 1 || !wanted(axes, f, axis)) continue;
             const HaloDesc &h = kHalo[f];
             BItem &it = b.it[b.n++];
             it.field = f;

@@ -6,7 +6,9 @@
 // 41-147) for every class.  Sharing one stride set means a single offset per
 // thread addresses every array, and neighbours are +-1, +-ld, +-plane.  Rows
 // are padded so that the first interior element (j = x_min) of every row
-// starts on a 64-byte boundary.  Indices are the reference's tile-local
+// starts on a 32-element boundary (256 bytes of FP64, 128 of FP32): a
+// wavefront's 64 interior elements then fill whole 128-byte cache lines
+// instead of sharing partial lines with the neighbouring workgroups.  Indices are the reference's tile-local
 // Fortran indices (x_min = 1, halo down to x_min-2).
 #pragma once
 
@@ -14,8 +16,8 @@
 
 namespace clover {
 
-constexpr int kRowAlign = 8;    // elements (64 bytes of FP64)
-constexpr int kFrontPad = 6;    // (x_min-2) sits 6 elements into the slab so x_min is aligned
+constexpr int kRowAlign = 32;   // elements
+constexpr int kFrontPad = 30;   // (x_min-2) sits 30 elements into the slab so x_min is aligned
 
 struct Lay {
     int x_min, x_max, y_min, y_max, z_min, z_max;

@@ -184,12 +184,17 @@ int clover_chunk_destroy(clover_chunk *c);
  * 0/1 arrays); CLOVER_CHUNK_PER_CALL runs one kernel per reference loop nest
  * instead, every intermediate stored as the reference stores it. */
 #define CLOVER_CHUNK_PER_CALL 1
-/* The FP32-store form: every field array held in FP32 (half the HBM bytes),
- * values converted to FP64 on load, so the arithmetic, calc_dt's minimum,
- * the advection's mass and energy accumulation and the field_summary sums
- * (compensated) run in FP64.  Results differ from the FP64 form by the
- * rounding of the stored fields.  Fused kernels only. */
+/* The FP32-store form: every field array held in FP32 (half the HBM bytes)
+ * and the stencils computed in FP32, while calc_dt's minimum, the
+ * advection's mass and energy accumulation and the field_summary sums
+ * (compensated) run in FP64.  Fused kernels only. */
 #define CLOVER_CHUNK_FP32 2
+/* Test mode of the fused kernels: the fused step refreshes only the halos its
+ * kernels read; with this flag the halos the reference would refresh and the
+ * fused step skips are filled with wrong values instead, so that a read of
+ * one changes the results (which must then still equal the per-call
+ * kernels'). */
+#define CLOVER_CHUNK_HALO_CHECK 4
 int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clover_chunk **out);
 
 /* One step of hydro.f90:48-68: timestep, PdV predictor/corrector,

@@ -25,10 +25,10 @@ pytestmark = pytest.mark.gpu
 GOLDEN = json.loads((pathlib.Path(__file__).resolve().parent / "golden" / "config0_summaries.json").read_text())
 
 
-def _run(deck, per_call=False):
+def _run(deck, per_call=False, halo_check=False):
     print(f"[gpu] chunk {deck.x_cells}x{deck.y_cells}x{deck.z_cells} tiles={deck.tiles_per_chunk} "
-          f"{'per-call' if per_call else 'fused'}", flush=True)
-    with Chunk(deck, per_call=per_call) as c:
+          f"{'per-call' if per_call else 'fused'}{' halo-check' if halo_check else ''}", flush=True)
+    with Chunk(deck, per_call=per_call, halo_check=halo_check) as c:
         rows = [c.field_summary()]
         dts = []
         for _ in range(deck.end_step):
@@ -99,13 +99,13 @@ ODD_DECK = """*clover
 *endclover"""
 
 
-@pytest.mark.parametrize("per_call", [False, True])
+@pytest.mark.parametrize("per_call,halo_check", [(False, False), (True, False), (False, True)])
 @pytest.mark.parametrize("nx,ny,nz,tiles,td", [(23, 17, 13, 1, "3d"), (23, 17, 13, 6, "3d"), (19, 9, 8, 3, "1d"),
                                               (19, 9, 8, 4, "2d")])
-def test_odd_decks_against_oracle(nx, ny, nz, tiles, td, per_call):
+def test_odd_decks_against_oracle(nx, ny, nz, tiles, td, per_call, halo_check):
     deck = parse_deck(ODD_DECK.format(nx=nx, ny=ny, nz=nz, steps=30, tiles=tiles, td=td))
     want, want_dt = ol.run(deck)
-    rows, dts = _run(deck, per_call)
+    rows, dts = _run(deck, per_call, halo_check)
     assert np.array_equal(dts, want_dt)
     assert len(rows) == len(want)
     for g, w in zip(rows, want):
@@ -128,20 +128,25 @@ INTERIOR_FIELDS = ["density0", "energy0", "pressure", "viscosity", "xvel0", "yve
                                             (130, 9, 3, 2)])
 def test_fused_equals_per_call(nx, ny, nz, tiles):
     """The fused kernels write exactly the per-call kernels' values: equal
-    totals (same reduction tree) and bit-identical interiors after 12 steps."""
+    totals (same reduction tree) and bit-identical interiors after 12 steps
+    (six of each sweep order).  The fused step refreshes only the halos its
+    kernels read; in the halo-check run every halo it skips holds a wrong
+    value, so a kernel reading one would change the interiors."""
     deck = parse_deck(ODD_DECK.format(nx=nx, ny=ny, nz=nz, steps=12, tiles=tiles, td="3d"))
     out = {}
-    for per_call in (False, True):
-        rows, dts = _run(deck, per_call)
-        with Chunk(deck, per_call=per_call) as c:
+    for per_call, check in ((True, False), (False, False), (False, True)):
+        rows, dts = _run(deck, per_call, check)
+        with Chunk(deck, per_call=per_call, halo_check=check) as c:
             for _ in range(12):
                 c.step()
             fields = {(t, n): c.fetch(n, t) for t in range(c.ntiles) for n in INTERIOR_FIELDS}
-        out[per_call] = (rows, dts, fields)
-    (r0, d0, f0), (r1, d1, f1) = out[False], out[True]
-    assert np.array_equal(d0, d1)
-    assert r0 == r1
-    for key in f0:
-        a, b = f0[key], f1[key]
-        inner = tuple(slice(2, s - 2) for s in a.shape)  # interior and the +1 face/node planes
-        assert np.array_equal(a[inner].view(np.uint64), b[inner].view(np.uint64)), key
+        out[per_call, check] = (rows, dts, fields)
+    r1, d1, f1 = out[True, False]
+    for check in (False, True):
+        r0, d0, f0 = out[False, check]
+        assert np.array_equal(d0, d1), check
+        assert r0 == r1, check
+        for key in f0:
+            a, b = f0[key], f1[key]
+            inner = tuple(slice(2, s - 2) for s in a.shape)  # interior and the +1 face/node planes
+            assert np.array_equal(a[inner].view(np.uint64), b[inner].view(np.uint64)), (key, check)
