@@ -66,9 +66,10 @@ hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const B
 // five partials per workgroup at partial[5*workgroup]
 hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double *partial,
                                hipStream_t st);
-// advec_cell sweep `dir`; new density1/energy1 into the arrays X_W1/X_W2
+// advec_cell sweep `dir`; new density1/energy1 into the arrays X_W1/X_W2,
+// and with gas (third sweep only) ideal_gas of them into pressure/soundspeed
 hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
-                            int sweep_number, hipStream_t st);
+                            int sweep_number, bool gas, hipStream_t st);
 // advec_mom sweep `dir`; new velocities into X_W5..X_W7
 hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
                            int sweep_number, hipStream_t st);

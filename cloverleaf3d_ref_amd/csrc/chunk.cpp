@@ -126,6 +126,7 @@ struct clover_chunk {
     bool fast;           // fused kernels (fast.h) instead of the per-call ones
     Store store;         // element type of the field arrays (ST_F32 only with the fused kernels)
     bool halo_check;     // CLOVER_CHUNK_HALO_CHECK
+    bool gas_fresh;      // pressure/soundspeed interiors hold ideal_gas(density0, energy0)
     Geo geo;             // the uniform mesh, for the fused kernels
 };
 
@@ -379,6 +380,10 @@ static void fused_halo(clover_chunk *c, int which)
 
 static void ideal_gas(clover_chunk *c, bool predict)
 {
+    // the fused path's last cell sweep leaves the equation of state of the
+    // new level-0 fields behind; computing it again would write the same bits
+    if (c->fast && !predict && c->gas_fresh) return;
+    c->gas_fresh = !predict;
     Phase ph(c, P_IDEAL_GAS);
     for (auto &t : c->tiles) {
         if (c->fast && !predict)
@@ -484,6 +489,7 @@ static void pdv(clover_chunk *c, bool predict)
                                                   t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
                                                   t.f[F_ZVEL0], c->st),
                           "PdV predict");
+            c->gas_fresh = false;  // pressure now holds the predicted state's
         }
         fused_halo(c, FH_PRESSURE);
         return;
@@ -529,11 +535,15 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
             // new density1/energy1 land in spare arrays, which then take their
             // place (interior cells only; the halo update that follows every
             // advec_cell sweep refreshes the halos before anything reads them)
+            // the third sweep also writes the next step's ideal_gas results
+            const bool last = sweep == 3;
             hip_check(launch_fast_advec_cell(c->store, t.g, c->geo, c->advect_x, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
                                              t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y],
                                              t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W1],
-                                             t.f[X_W2], c->st),
+                                             t.f[X_W2], last ? t.f[F_PRESSURE] : nullptr,
+                                             last ? t.f[F_SOUNDSPEED] : nullptr, c->st),
                       "advec_cell");
+            if (last) c->gas_fresh = true;  // for the level 0 that reset_field makes of these
             std::swap(t.f[F_DENSITY1], t.f[X_W1]);
             std::swap(t.f[F_ENERGY1], t.f[X_W2]);
         } else

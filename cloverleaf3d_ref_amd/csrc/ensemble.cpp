@@ -2,10 +2,11 @@
 //
 // Each deck is one tile with its own mesh, states, dt and time.  A step is
 // the fast path of the single-deck driver (chunk.cpp) with every launch
-// covering all decks (batch.h): timestep (ideal_gas, halos, viscosity +
-// calc_dt, a per-deck dt on the device), PdV predictor, accelerate, PdV
-// corrector + flux_calc, the three advection sweeps and reset_field, the
-// array exchanges done in the device-side deck table.  No step reads
+// covering all decks (batch.h): timestep (halos, viscosity + calc_dt, a
+// per-deck dt on the device; its ideal_gas results come from the previous
+// step's third cell sweep), PdV predictor, accelerate, PdV corrector +
+// flux_calc, the three advection sweeps and reset_field, the array
+// exchanges done in the device-side deck table.  No step reads
 // anything back to the host, so the step for each advect_x parity is
 // captured once in a graph and replayed.
 #include <hip/hip_runtime.h>
@@ -139,7 +140,10 @@ void swap(clover_ensemble *e, std::initializer_list<std::pair<int, int>> pairs, 
 void advec_cell(clover_ensemble *e, int sweep, int dir)
 {
     const BatchKernel k = dir == 1 ? BK_CELL_X : dir == 2 ? BK_CELL_Y : BK_CELL_Z;
-    hip_check(batch_advec_cell(e->store, e->dev, e->nd, grid(e, k), e->advect_x, dir, sweep, e->st), "advec_cell");
+    // the third sweep also writes ideal_gas of the new state: the next step's
+    // (or a summary's) equation of state
+    hip_check(batch_advec_cell(e->store, e->dev, e->nd, grid(e, k), e->advect_x, dir, sweep, sweep == 3, e->st),
+              "advec_cell");
     swap(e, {{F_DENSITY1, X_W1}, {F_ENERGY1, X_W2}}, false);
 }
 
@@ -153,8 +157,8 @@ void advec_mom(clover_ensemble *e, int sweep, int dir)
 // hydro.f90:48-68 for every deck, device only
 void enqueue_step(clover_ensemble *e)
 {
-    // timestep.f90:28-128
-    ideal_gas(e);
+    // timestep.f90:28-128; its ideal_gas results were written by the previous
+    // step's third cell sweep (or by the start sequence)
     halo(e, FH_TIMESTEP);
     const BatchGrid &vg = grid(e, BK_VISC_DT);
     hip_check(batch_visc_dt(e->store, e->dev, e->nd, vg, e->g_small, e->visc_partial, e->st), "visc_dt");

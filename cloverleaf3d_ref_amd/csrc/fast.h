@@ -71,11 +71,13 @@ hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, dou
 // one advec_cell sweep, fluxes and cell update in one kernel
 // (cell_kernels.hip): mass_flux_<dir> is written over the reference's face
 // range, the new density1/energy1 of interior cells go to rho_out/e_out.
+// On the step's third sweep, pressure/soundspeed (both or neither) also get
+// ideal_gas of the new state: the next step's ideal_gas results.
 hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advect_x, int dir, int sweep_number,
                                   const void *density1, const void *energy1, void *mass_flux_x,
                                   const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
                                   void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
-                                  hipStream_t st);
+                                  void *pressure, void *soundspeed, hipStream_t st);
 
 // one advec_mom sweep for all three velocities (advec_mom_driver.f90 calls
 // the kernel for xvel1, yvel1, zvel1 with the node quantities of the first

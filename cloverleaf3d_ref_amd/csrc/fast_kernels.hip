@@ -649,16 +649,8 @@ __device__ __forceinline__ void ideal_gas_body(const Blk &bk, Lay g, Range r, co
     const int k = r.k0 + bk.y * BY + (int)threadIdx.y;
     const int l = r.l0 + bk.z;
     if (j > r.j1 || k > r.k1) return;
-    using T = S;  // arithmetic type of the form
     const int i = idx(g, j, k, l);
-    const T d = density[i];
-    const T v = quo((T)1, d);
-    const T p = (T)(1.4 - 1.0) * d * (T)energy[i];
-    pressure[i] = p;
-    const T pressurebyenergy = (T)(1.4 - 1.0) * d;
-    const T pressurebyvolume = -d * p;
-    const T sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
-    soundspeed[i] = sqrt(sound_speed_squared);
+    ideal_gas_cell<S>(density[i], energy[i], pressure[i], soundspeed[i]);
 }
 
 template <typename S>

@@ -42,6 +42,25 @@ __device__ inline Blk hw_blk()
     return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
 }
 
+// The hardware deals workgroups round-robin to the 8 XCDs, each with its own
+// L2.  hw_blk_xcd() hands XCD x the x-th contiguous eighth of the grid (x
+// fastest) instead, so tiles that share cache lines with their neighbours
+// along x and y run on the same XCD at about the same time.  It pays in the
+// y and z momentum sweeps and costs in the other kernels, which keep the
+// plain order.
+constexpr int kXcds = 8;
+
+__device__ inline Blk hw_blk_xcd()
+{
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y, gz = (int)gridDim.z;
+    const int n = gx * gy * gz;
+    const int hw = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
+    const int xcd = hw % kXcds, t = hw / kXcds;
+    const int q = n / kXcds, r = n % kXcds;  // XCDs below r take q+1 blocks
+    const int id = xcd < r ? xcd * (q + 1) + t : r * (q + 1) + (xcd - r) * q + t;
+    return Blk{id % gx, (id / gx) % gy, id / (gx * gy), gx, gy, gz};
+}
+
 #define TIMED_LAUNCH(name, st, ...)                           \
     do {                                                      \
         if (t_kernel_timer) t_kernel_timer->begin(name, st);  \
@@ -84,6 +103,23 @@ __device__ inline T quo(T a, T b)
 {
     if constexpr (sizeof(T) == 8) return a / b;
     else return a * __builtin_amdgcn_rcpf(b);
+}
+
+// ideal_gas_kernel.f90:50-57 for one cell, in the form's arithmetic (shared
+// by the ideal_gas kernels and the advection sweep that also writes the
+// next step's equation of state)
+template <typename S>
+__device__ inline void ideal_gas_cell(S density, S energy, S &pressure, S &soundspeed)
+{
+    using T = S;
+    const T d = density;
+    const T v = quo((T)1, d);
+    const T p = (T)(1.4 - 1.0) * d * (T)energy;
+    pressure = p;
+    const T pressurebyenergy = (T)(1.4 - 1.0) * d;
+    const T pressurebyvolume = -d * p;
+    const T sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
+    soundspeed = sqrt(sound_speed_squared);
 }
 
 constexpr int RED_THREADS = 256;

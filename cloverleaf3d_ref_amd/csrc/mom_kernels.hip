@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(64 * MW) k_fast_mom_march(Lay g, Geo geo, int 
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
-    mom_march_body<DIR, S>(hw_blk(), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+    mom_march_body<DIR, S>(hw_blk_xcd(), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 // ---------------------------------------------------------------------------

@@ -219,7 +219,12 @@ int clover_chunk_tile_info(const clover_chunk *c, int t, int *dims3, int *offs3,
 
 /* Copy one field of tile t into a dense Fortran array of the field's own
  * class.  field: 0..18 the FIELD_* ids minus one (data.f90:55-74); 19 volume,
- * 20 xarea, 21 yarea, 22 zarea. */
+ * 20 xarea, 21 yarea, 22 zarea.  Between steps the fused kernels' interiors
+ * equal the reference's, with three differences that nothing reads: the
+ * level-1 arrays hold the previous level 0 (reset_field is an exchange),
+ * halo cells no kernel reads are not refreshed, and pressure/soundspeed
+ * already hold ideal_gas of the new level 0 (the reference computes that at
+ * the next timestep or field_summary; after a field_summary both agree). */
 int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host);
 
 /* Per-phase device time (ms) accumulated with HIP events when profiling is
