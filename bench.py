@@ -54,9 +54,11 @@ ARRAYS_PER_CELL = {
     # 1/2/3), density1 and energy1 and write the sweep's mass flux, density1
     # and energy1; mom_x/y/z read the sweep's mass flux, density1, 2/1/0
     # volume fluxes and three velocities and write three velocities.  x and
-    # z run in sweeps 1 and 3, y always in sweep 2.
+    # z run in sweeps 1 and 3, y always in sweep 2; the third sweep also
+    # writes pressure and soundspeed (the next step's ideal_gas), one of every
+    # two x and z sweeps.
     "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accelerate": 9, "k_fast_pdv_correct_flux": 15,
-    "k_fast_cell_x": 7, "k_fast_cell_y": 7, "k_fast_cell_z": 7,
+    "k_fast_cell_x": 8, "k_fast_cell_y": 7, "k_fast_cell_z": 8,
     "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
     "k_fast_field_summary": 6, "k_fast_ideal_gas": 4,
 }
@@ -249,7 +251,8 @@ def main():
         value32 = n ** 3 * args.steps / wall32
         f64, f32 = info["final"], info32["final"]
         fp32 = dict(value=value32, unit="cell-steps/s", ms_per_step=wall32 / args.steps * 1e3,
-                    speedup_vs_fp64=value32 / value, dtype="f32 storage, f64 arithmetic",
+                    speedup_vs_fp64=value32 / value,
+                    dtype="f32 storage and stencils; f64 dt minimum, cell mass/energy update, summary sums",
                     rel_err_vs_fp64={k: abs(f32[k] / f64[k] - 1) for k in ("vol", "mass", "ie", "ke", "press")},
                     after_steps=f32["step"],
                     roofline=dominant_roofline(ktimes32, n, 4),
