@@ -75,7 +75,8 @@ def kernel_domain_cells(name: str, n: int) -> int:
     return n ** 3
 
 
-def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, fp32: bool = False, tiles: int = 1):
+def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, fp32: bool = False, tiles: int = 1,
+              tune: bool = True):
     from cloverleaf3d_ref_amd import _lib
     from cloverleaf3d_ref_amd.chunk import Chunk
     from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
@@ -84,6 +85,12 @@ def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, 
     t0 = time.perf_counter()
     c = Chunk(deck, fp32=fp32)
     setup_s = time.perf_counter() - t0
+    autotune = None
+    if tune:  # launch shapes picked on the chunk's own arrays, before the warmup
+        t0 = time.perf_counter()
+        before, after = c.autotune()
+        autotune = dict(seconds=time.perf_counter() - t0, pass_ms_before=before, pass_ms_after=after,
+                        shapes=c.shapes())
     L = _lib.lib()
     for _ in range(warmup):
         c.step()
@@ -99,7 +106,8 @@ def gpu_bench(cells: int, steps: int, warmup: int, summary_frequency: int = 25, 
         name, ms, launches = line.split()
         ktimes[name] = (float(ms), int(launches))
     final = c.field_summary()  # after the timed region
-    info = dict(setup_s=setup_s, device_bytes=c.device_bytes(), rows=rows, state=c.state(), final=final)
+    info = dict(setup_s=setup_s, device_bytes=c.device_bytes(), rows=rows, state=c.state(), final=final,
+                autotune=autotune)
     c.close()
     return wall, ktimes, info
 
@@ -120,7 +128,7 @@ def dominant_roofline(ktimes, n: int, bytes_per_value: int, traffic_file=None):
                 avg_launch_us=avg_s * 1e6, launches=launches, bytes_per_launch=per_launch_bytes)
 
 
-def ensemble_bench(ndecks: int, steps: int, fp32: bool = False, warmup: int = 2):
+def ensemble_bench(ndecks: int, steps: int, fp32: bool = False, warmup: int = 2, tune: bool = True):
     """The batched form: all decks advanced together (graph replays), then
     each deck's final summary.  Setup (mesh and states) and the warmup steps
     that capture the graph of each advect_x parity are outside the timed
@@ -132,13 +140,20 @@ def ensemble_bench(ndecks: int, steps: int, fp32: bool = False, warmup: int = 2)
     t0 = time.perf_counter()
     e = Ensemble(decks, fp32=fp32)
     setup_s = time.perf_counter() - t0
+    autotune = None
+    if tune:
+        t0 = time.perf_counter()
+        before, after = e.autotune()
+        autotune = dict(seconds=time.perf_counter() - t0, pass_ms_before=before, pass_ms_after=after,
+                        shapes=e.shapes())
     e.run(warmup)
     t0 = time.perf_counter()
     e.run(steps)
     sums = e.summaries()
     wall = time.perf_counter() - t0
     _, _, _, small = e.state()
-    out = dict(wall_s=wall, setup_s=setup_s, device_gib=e.device_bytes() / 2 ** 30, small=sum(small), sums=sums)
+    out = dict(wall_s=wall, setup_s=setup_s, device_gib=e.device_bytes() / 2 ** 30, small=sum(small), sums=sums,
+               autotune=autotune)
     e.close()
     return out, decks
 
@@ -229,6 +244,7 @@ def main():
     ap.add_argument("--tiled-steps", type=int, default=50)
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--deck-steps", type=int, default=200)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -240,14 +256,15 @@ def main():
 
     if args.calib_copy_mib:
         calibration_copy(args.calib_copy_mib)
-    wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup)
+    tune = not args.no_autotune
+    wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup, tune=tune)
     n = args.cells
     value = n ** 3 * args.steps / wall
     roofline = dominant_roofline(ktimes, n, 8, TRAFFIC_FILE if n == 512 else None)
 
     fp32 = None
     if not args.no_fp32:
-        wall32, ktimes32, info32 = gpu_bench(args.cells, args.steps, args.warmup, fp32=True)
+        wall32, ktimes32, info32 = gpu_bench(args.cells, args.steps, args.warmup, fp32=True, tune=tune)
         value32 = n ** 3 * args.steps / wall32
         f64, f32 = info["final"], info32["final"]
         fp32 = dict(value=value32, unit="cell-steps/s", ms_per_step=wall32 / args.steps * 1e3,
@@ -258,13 +275,13 @@ def main():
                     roofline=dominant_roofline(ktimes32, n, 4),
                     kernel_ms_per_step={k: round(v[0] / args.steps, 4) for k, v in
                                         sorted(ktimes32.items(), key=lambda kv: -kv[1][0])},
-                    device_gib=info32["device_bytes"] / 2 ** 30)
+                    device_gib=info32["device_bytes"] / 2 ** 30, autotune=info32["autotune"])
 
     tiled = None
     if args.tiled_cells:
         nt = args.tiled_cells
-        w64, k64, i64 = gpu_bench(nt, args.tiled_steps, 2, tiles=args.tiles)
-        w32, k32, i32 = gpu_bench(nt, args.tiled_steps, 2, tiles=args.tiles, fp32=True)
+        w64, k64, i64 = gpu_bench(nt, args.tiled_steps, 2, tiles=args.tiles, tune=tune)
+        w32, k32, i32 = gpu_bench(nt, args.tiled_steps, 2, tiles=args.tiles, fp32=True, tune=tune)
         f64, f32 = i64["final"], i32["final"]
         tiled = dict(workload=f"cuboid deck {nt}^3 cells as {args.tiles} tiles (3d decomposition), summary every 25",
                      steps=args.tiled_steps,
@@ -272,12 +289,13 @@ def main():
                      fp32_value=nt ** 3 * args.tiled_steps / w32, fp32_ms_per_step=w32 / args.tiled_steps * 1e3,
                      fp32_rel_err_vs_fp64={k: abs(f32[k] / f64[k] - 1) for k in ("mass", "ie", "ke", "press")},
                      tile_halo_ms_per_step=round(k64.get("k_tile_copy", (0.0, 0))[0] / args.tiled_steps, 4),
-                     device_gib=i64["device_bytes"] / 2 ** 30, setup_s=i64["setup_s"])
+                     device_gib=i64["device_bytes"] / 2 ** 30, setup_s=i64["setup_s"],
+                     autotune=i64["autotune"], fp32_autotune=i32["autotune"])
 
     batched = None
     if args.decks:
-        b64, decks = ensemble_bench(args.decks, args.deck_steps)
-        b32, _ = ensemble_bench(args.decks, args.deck_steps, fp32=True)
+        b64, decks = ensemble_bench(args.decks, args.deck_steps, tune=tune)
+        b32, _ = ensemble_bench(args.decks, args.deck_steps, fp32=True, tune=tune)
         alone = one_at_a_time(decks, args.deck_steps)
         ncell = sum(d.x_cells * d.y_cells * d.z_cells for d in decks)
         batched = dict(decks=args.decks, steps_per_deck=args.deck_steps, cells=ncell,
@@ -292,7 +310,8 @@ def main():
                                        note="totals after the same step count; each deck's dt follows its "
                                             "own state, so the FP32 and FP64 runs reach slightly different "
                                             "simulated times"),
-                       small_dt_decks=b64["small"], setup_s=b64["setup_s"], device_gib=b64["device_gib"])
+                       small_dt_decks=b64["small"], setup_s=b64["setup_s"], device_gib=b64["device_gib"],
+                       autotune=b64["autotune"], fp32_autotune=b32["autotune"])
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -325,6 +344,7 @@ def main():
                                sorted(ktimes.items(), key=lambda kv: -kv[1][0])},
         "device_time_fraction": total_ms / (wall * 1e3),
         "setup_s": info["setup_s"],
+        "autotune": info["autotune"],
         "device_gib": info["device_bytes"] / 2 ** 30,
     }
     print(json.dumps(out))

@@ -61,6 +61,11 @@ def header_functions() -> List[str]:
     return sorted(set(re.findall(r"\b(clover_[a-z0-9_A-Z]+)\s*\(", text)))
 
 
+# launch-shape kernel indices (CLOVER_SHAPE_* of clover_hip.h)
+SHAPE_KERNELS = ["visc_dt", "pdv_predict", "accelerate", "pdv_correct", "cell_x", "cell_y", "cell_z",
+                 "mom_x", "mom_y", "mom_z"]
+
+
 def lib() -> ctypes.CDLL:
     """The loaded library (raises CloverError if it is not built)."""
     global _lib
@@ -100,6 +105,12 @@ def lib() -> ctypes.CDLL:
     L.clover_ensemble_fetch_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.clover_ensemble_device_bytes.restype = ctypes.c_longlong
     L.clover_ensemble_device_bytes.argtypes = [vp]
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.clover_chunk_autotune.argtypes = [vp, dp, dp]
+    L.clover_chunk_get_shape.argtypes = [vp, ctypes.c_int, ip, ip]
+    L.clover_chunk_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.clover_ensemble_autotune.argtypes = [vp, dp, dp]
+    L.clover_ensemble_get_shape.argtypes = [vp, ctypes.c_int, ip, ip]
     L.clover_hip_fetch.argtypes = [vp]
     L.clover_hip_push.argtypes = [vp]
     L.clover_hip_forget.argtypes = [vp]

@@ -8,7 +8,7 @@ reference's own output (timestep.f90:115-118, field_summary.f90:111-113).
 from __future__ import annotations
 
 import ctypes
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -160,6 +160,29 @@ class Chunk:
 
     def set_profiling(self, on: bool) -> None:
         self._L.clover_chunk_set_profiling(self._h, 1 if on else 0)
+
+    def autotune(self) -> Tuple[float, float]:
+        """Pick the fastest launch shape of every fused marching kernel on
+        this chunk's own arrays (every field keeps its value).  Returns the
+        summed kernel milliseconds of one pass before and after."""
+        before, after = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._L.clover_chunk_autotune(self._h, ctypes.byref(before), ctypes.byref(after)),
+                   "clover_chunk_autotune")
+        return before.value, after.value
+
+    def shapes(self) -> Dict[str, Tuple[int, int]]:
+        """kernel -> (planes per workgroup, XCD-contiguous block order)"""
+        out = {}
+        for k, name in enumerate(_lib.SHAPE_KERNELS):
+            planes, xcd = ctypes.c_int(), ctypes.c_int()
+            _lib.check(self._L.clover_chunk_get_shape(self._h, k, ctypes.byref(planes), ctypes.byref(xcd)),
+                       "clover_chunk_get_shape")
+            out[name] = (planes.value, xcd.value)
+        return out
+
+    def set_shape(self, kernel: str, planes: int, xcd: int) -> None:
+        _lib.check(self._L.clover_chunk_set_shape(self._h, _lib.SHAPE_KERNELS.index(kernel), planes, xcd),
+                   "clover_chunk_set_shape")
 
     def profile(self) -> Dict[str, tuple]:
         ms = (ctypes.c_double * 32)()

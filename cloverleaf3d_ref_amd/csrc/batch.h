@@ -40,7 +40,7 @@ enum BatchKernel : int {
     BK_IDEAL_GAS, BK_VISC_DT, BK_PDV_PREDICT, BK_ACCELERATE, BK_PDV_CORRECT, BK_SUMMARY,
     BK_CELL_X, BK_CELL_Y, BK_CELL_Z, BK_MOM_X, BK_MOM_Y, BK_MOM_Z, BK_COUNT
 };
-int batch_blocks(BatchKernel k, const Lay &g);
+int batch_blocks(BatchKernel k, const Lay &g, int planes);
 // reflective halo along `axis` for `nitems` (field, face) items
 int batch_blocks_reflect(const Lay &g, int axis, int depth, int nitems);
 
@@ -58,21 +58,22 @@ __device__ inline int deck_of(const int *__restrict__ start, int nd, int b)
 // Launchers.  decks: device table; grid from the matching prefix table.
 hipError_t batch_ideal_gas(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
 // writes one dt partial per workgroup at partial[workgroup]
-hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double g_small,
+hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, double g_small,
                          double *partial, hipStream_t st);
-hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
-hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
-hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st);
+hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st);
+hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st);
+hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh,
+                                  hipStream_t st);
 // five partials per workgroup at partial[5*workgroup]
 hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double *partial,
                                hipStream_t st);
 // advec_cell sweep `dir`; new density1/energy1 into the arrays X_W1/X_W2,
 // and with gas (third sweep only) ideal_gas of them into pressure/soundspeed
 hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
-                            int sweep_number, bool gas, hipStream_t st);
+                            int sweep_number, bool gas, const Shape &sh, hipStream_t st);
 // advec_mom sweep `dir`; new velocities into X_W5..X_W7
 hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
-                           int sweep_number, hipStream_t st);
+                           int sweep_number, const Shape &sh, hipStream_t st);
 // reflective halo of the flagged fields along one axis, both faces; axes
 // (null: all) as for launch_update_halo_store
 hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,

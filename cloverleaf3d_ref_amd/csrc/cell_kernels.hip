@@ -30,7 +30,6 @@
 namespace clover {
 
 constexpr int CROWS = 4;   // rows (or columns) per workgroup
-constexpr int CCHUNK = 64; // faces per march chunk
 
 // pre_vol (advec_cell_kernel.f90:97-116, 193-210, 289-306) for `mode`:
 // 1 x sweep 1, 2 x sweep 3, 3 y sweep with advect_x, 4 y without,
@@ -143,12 +142,12 @@ __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mod
 }
 
 template <typename S>
-__global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+__global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(int xcd, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
               const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
               S *__restrict__ e_out, S *__restrict__ p_out, S *__restrict__ c_out)
 {
-    cell_x_body<S>(hw_blk(), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out, p_out, c_out);
+    cell_x_body<S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out, p_out, c_out);
 }
 
 // The volume fluxes pre_vol reads for the y and z sweeps (modes 3-6), kept
@@ -208,8 +207,9 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
     using T = S;  // arithmetic type of the fluxes
     const T vd = (T)(DIR == 3 ? m.dz : m.dy);
     const S *__restrict__ vfs = DIR == 3 ? vfz : vfy;
-    const int f0 = smin + bk.z * CCHUNK;
-    const int f1 = min(f0 + CCHUNK - 1, smax + 2);
+    int f0, f1;
+    chunk_range(bk, smin, smax + 2, f0, f1);
+    if (f0 > f1) return;
     const int ib = DIR == 3 ? idx(g, j, cr, smin) : idx(g, j, smin, cr);  // offset at sweep position smin
 #define OFF(q) (ib + ((q) - smin) * s)
     auto pre = [&](int q) { return cell_pre_vol(mode, g, OFF(q), m.vol, vfx, vfy, vfz); };
@@ -279,12 +279,12 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
 }
 
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * CROWS) k_fast_cell_march(Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
+__global__ void __launch_bounds__(64 * CROWS) k_fast_cell_march(int xcd, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
                   const S *__restrict__ vfz, const S *__restrict__ density1,
                   const S *__restrict__ energy1, S *__restrict__ mass_flux, S *__restrict__ rho_out,
                   S *__restrict__ e_out, S *__restrict__ p_out, S *__restrict__ c_out)
 {
-    cell_march_body<DIR, S>(hw_blk(), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux, rho_out, e_out, p_out,
+    cell_march_body<DIR, S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux, rho_out, e_out, p_out,
                             c_out);
 }
 
@@ -292,21 +292,21 @@ template <typename S>
 static hipError_t fast_advec_cell_t(const Lay &g, const Geo &m, int mode, int dir, const S *density1,
                                     const S *energy1, S *mass_flux_x, const S *vol_flux_x, S *mass_flux_y,
                                     const S *vol_flux_y, S *mass_flux_z, const S *vol_flux_z, S *rho_out, S *e_out,
-                                    S *p_out, S *c_out, hipStream_t st)
+                                    S *p_out, S *c_out, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const dim3 blk(64, CROWS, 1);
     if (dir == 1) {
         const dim3 grid((nx + 2 + 59) / 60, (ny * nz + CROWS - 1) / CROWS, 1);
-        TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x<S>, grid, blk, 0, st, g, m, mode, vol_flux_x, vol_flux_y,
+        TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x<S>, grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x, vol_flux_y,
                      vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out, p_out, c_out);
     } else if (dir == 2) {
-        const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, (ny + 2 + CCHUNK - 1) / CCHUNK);
-        TIMED_LAUNCH("k_fast_cell_y", st, (k_fast_cell_march<2, S>), grid, blk, 0, st, g, m, mode, vol_flux_x,
+        const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, nchunks(ny + 2, sh.planes));
+        TIMED_LAUNCH("k_fast_cell_y", st, (k_fast_cell_march<2, S>), grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x,
                      vol_flux_y, vol_flux_z, density1, energy1, mass_flux_y, rho_out, e_out, p_out, c_out);
     } else {
-        const dim3 grid((nx + 63) / 64, (ny + CROWS - 1) / CROWS, (nz + 2 + CCHUNK - 1) / CCHUNK);
-        TIMED_LAUNCH("k_fast_cell_z", st, (k_fast_cell_march<3, S>), grid, blk, 0, st, g, m, mode, vol_flux_x,
+        const dim3 grid((nx + 63) / 64, (ny + CROWS - 1) / CROWS, nchunks(nz + 2, sh.planes));
+        TIMED_LAUNCH("k_fast_cell_z", st, (k_fast_cell_march<3, S>), grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x,
                      vol_flux_y, vol_flux_z, density1, energy1, mass_flux_z, rho_out, e_out, p_out, c_out);
     }
     return hipGetLastError();
@@ -316,7 +316,7 @@ hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advec
                                   const void *density1, const void *energy1, void *mass_flux_x,
                                   const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
                                   void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
-                                  void *pressure, void *soundspeed, hipStream_t st)
+                                  void *pressure, void *soundspeed, const Shape &sh, hipStream_t st)
 {
     int mode = 0;
     if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
@@ -331,41 +331,43 @@ hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advec
                          static_cast<T *>(mass_flux_y), static_cast<const T *>(vol_flux_y),                      \
                          static_cast<T *>(mass_flux_z), static_cast<const T *>(vol_flux_z),                      \
                          static_cast<T *>(rho_out), static_cast<T *>(e_out), static_cast<T *>(pressure),         \
-                         static_cast<T *>(soundspeed), st)
+                         static_cast<T *>(soundspeed), sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
 
 // ---------------------------------------------------------------------------
 // Batched over many decks (batch.h): new density1/energy1 go to X_W1/X_W2.
-__host__ __device__ inline void dims_cell(int dir, const Lay &g, int &gx, int &gy, int &gz)
+__host__ __device__ inline void dims_cell(int dir, const Lay &g, int planes, int &gx, int &gy, int &gz)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     if (dir == 1) {
         gx = (nx + 2 + 59) / 60; gy = (ny * nz + CROWS - 1) / CROWS; gz = 1;
     } else if (dir == 2) {
-        gx = (nx + 63) / 64; gy = (nz + 2 + CROWS - 1) / CROWS; gz = (ny + 2 + CCHUNK - 1) / CCHUNK;
+        gx = (nx + 63) / 64; gy = (nz + 2 + CROWS - 1) / CROWS; gz = nchunks(ny + 2, planes);
     } else {
-        gx = (nx + 63) / 64; gy = (ny + CROWS - 1) / CROWS; gz = (nz + 2 + CCHUNK - 1) / CCHUNK;
+        gx = (nx + 63) / 64; gy = (ny + CROWS - 1) / CROWS; gz = nchunks(nz + 2, planes);
     }
 }
 
-int batch_blocks_cell(int dir, const Lay &g)
+int batch_blocks_cell(int dir, const Lay &g, int planes)
 {
     int gx, gy, gz;
-    dims_cell(dir, g, gx, gy, gz);
+    dims_cell(dir, g, planes, gx, gy, gz);
     return gx * gy * gz;
 }
 
 template <int DIR, typename S>
 __global__ void __launch_bounds__(64 * CROWS)
-kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int gas)
+kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int gas, int planes,
+        int xcd)
 {
-    const int d = deck_of(start, nd, (int)blockIdx.x);
+    const int hb = xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int d = deck_of(start, nd, hb);
     const DeckDev &D = decks[d];
     int gx, gy, gz;
-    dims_cell(DIR, D.g, gx, gy, gz);
-    const int lb = (int)blockIdx.x - start[d];
+    dims_cell(DIR, D.g, planes, gx, gy, gz);
+    const int lb = hb - start[d];
     const Blk bk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
 #define CF(id) static_cast<const S *>(D.f[id])
 #define MF(id) static_cast<S *>(D.f[id])
@@ -382,7 +384,7 @@ kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd
 }
 
 hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
-                            int sweep_number, bool gas, hipStream_t st)
+                            int sweep_number, bool gas, const Shape &sh, hipStream_t st)
 {
     int mode = 0;
     if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
@@ -392,7 +394,9 @@ hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGr
     if (bg.total == 0) return hipSuccess;
     const dim3 grid(bg.total), blk(64, CROWS);
     if (gas && sweep_number != 3) return hipErrorInvalidValue;
-#define L(D_, T_) TIMED_LAUNCH("kb_cell", st, (kb_cell<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode, gas ? 1 : 0)
+#define L(D_, T_)                                                                                          \
+    TIMED_LAUNCH("kb_cell", st, (kb_cell<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode, gas ? 1 : 0,   \
+                 sh.planes, sh.xcd)
     if (s == ST_F32) {
         if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float);
     } else {

@@ -127,6 +127,7 @@ struct clover_chunk {
     Store store;         // element type of the field arrays (ST_F32 only with the fused kernels)
     bool halo_check;     // CLOVER_CHUNK_HALO_CHECK
     bool gas_fresh;      // pressure/soundspeed interiors hold ideal_gas(density0, energy0)
+    Shapes shapes;       // launch shapes of the fused kernels (autotuner)
     Geo geo;             // the uniform mesh, for the fused kernels
 };
 
@@ -423,7 +424,8 @@ static int timestep(clover_chunk *c)
             hip_check(launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
                                           c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
                                           t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
-                                          c->partial + (size_t)ti * kMaxPartials * 5, &nparts[ti % 64], c->st),
+                                          c->partial + (size_t)ti * kMaxPartials * 5, &nparts[ti % 64],
+                                          c->shapes.k[SK_VISC_DT], c->st),
                       "visc_dt");
             hip_check(launch_min_final(c->partial + (size_t)ti * kMaxPartials * 5, nparts[ti % 64], c->g_big,
                                        c->scalar + 8 * ti, c->st),
@@ -487,7 +489,7 @@ static void pdv(clover_chunk *c, bool predict)
             for (auto &t : c->tiles)
                 hip_check(launch_fast_pdv_predict(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_ENERGY0],
                                                   t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
-                                                  t.f[F_ZVEL0], c->st),
+                                                  t.f[F_ZVEL0], c->shapes.k[SK_PDV_PREDICT], c->st),
                           "PdV predict");
             c->gas_fresh = false;  // pressure now holds the predicted state's
         }
@@ -502,7 +504,7 @@ static void pdv(clover_chunk *c, bool predict)
                                                    t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
                                                    t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1],
                                                    t.f[F_ZVEL0], t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y],
-                                                   t.f[F_VOL_FLUX_Z], c->st),
+                                                   t.f[F_VOL_FLUX_Z], c->shapes.k[SK_PDV_CORRECT], c->st),
                       "PdV correct + flux_calc");
         return;
     }
@@ -541,7 +543,7 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
                                              t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y],
                                              t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W1],
                                              t.f[X_W2], last ? t.f[F_PRESSURE] : nullptr,
-                                             last ? t.f[F_SOUNDSPEED] : nullptr, c->st),
+                                             last ? t.f[F_SOUNDSPEED] : nullptr, c->shapes.k[SK_CELL_X + dir - 1], c->st),
                       "advec_cell");
             if (last) c->gas_fresh = true;  // for the level 0 that reset_field makes of these
             std::swap(t.f[F_DENSITY1], t.f[X_W1]);
@@ -570,7 +572,7 @@ static void advec_mom(clover_chunk *c, int sweep, int dir)
             hip_check(launch_fast_advec_mom(c->store, t.g, c->geo, c->advect_x, sweep, dir, t.f[F_MASS_FLUX_X],
                                             t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y],
                                             t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[F_DENSITY1], vin, vout,
-                                            c->st),
+                                            c->shapes.k[SK_MOM_X + dir - 1], c->st),
                       "advec_mom");
             std::swap(t.f[F_XVEL1], t.f[X_W5]);
             std::swap(t.f[F_YVEL1], t.f[X_W6]);
@@ -642,6 +644,121 @@ static void field_summary(clover_chunk *c, double *out)
     for (int q = 0; q < 5; q++) out[q] = s[q];
 }
 
+// ---------------------------------------------------------------- autotuner
+// Launches shaped kernel k over every tile with its current shape, writing
+// only the spare work arrays: pdv_predict updates pressure in place, so it
+// runs on a copy of pressure in work array 3 (the copy is made by the caller,
+// outside the timing); every other output goes to W1..W7.  The inputs are
+// the chunk's own fields as a step leaves them.
+static void tune_launch(clover_chunk *c, int k)
+{
+    for (size_t ti = 0; ti < c->tiles.size(); ti++) {
+        Tile &t = c->tiles[ti];
+        const Shape &sh = c->shapes.k[k];
+        const clover_deck &d = c->deck;
+        hipError_t e = hipSuccess;
+        switch (k) {
+        case SK_VISC_DT: {
+            int np = 0;
+            e = launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe,
+                                    d.dtdiv_safe, c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
+                                    t.f[X_W3], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
+                                    c->partial + ti * kMaxPartials * 5, &np, sh, c->st);
+            break;
+        }
+        case SK_PDV_PREDICT:
+            e = launch_fast_pdv_predict(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[X_W3],
+                                        t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], sh, c->st);
+            break;
+        case SK_ACCELERATE:
+            e = launch_fast_accelerate(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[F_PRESSURE],
+                                       t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], t.f[X_W5],
+                                       t.f[X_W6], t.f[X_W7], sh, c->st);
+            break;
+        case SK_PDV_CORRECT:
+            e = launch_fast_pdv_correct_flux(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[X_W1],
+                                             t.f[F_ENERGY0], t.f[X_W2], t.f[F_PRESSURE], t.f[F_VISCOSITY],
+                                             t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1], t.f[F_ZVEL0],
+                                             t.f[F_ZVEL1], t.f[X_W3], t.f[X_W4], t.f[X_W5], sh, c->st);
+            break;
+        case SK_CELL_X:
+        case SK_CELL_Y:
+        case SK_CELL_Z: {
+            const int dir = k - SK_CELL_X + 1, sweep = dir == 2 ? 2 : 1;
+            e = launch_fast_advec_cell(c->store, t.g, c->geo, 1, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
+                                       t.f[X_W3], t.f[F_VOL_FLUX_X], t.f[X_W3], t.f[F_VOL_FLUX_Y], t.f[X_W3],
+                                       t.f[F_VOL_FLUX_Z], t.f[X_W1], t.f[X_W2], nullptr, nullptr, sh, c->st);
+            break;
+        }
+        default: {
+            const int dir = k - SK_MOM_X + 1, sweep = dir == 2 ? 2 : 1;
+            const void *vin[3] = {t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1]};
+            void *vout[3] = {t.f[X_W5], t.f[X_W6], t.f[X_W7]};
+            e = launch_fast_advec_mom(c->store, t.g, c->geo, 1, sweep, dir, t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X],
+                                      t.f[F_MASS_FLUX_Y], t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z],
+                                      t.f[F_DENSITY1], vin, vout, sh, c->st);
+            break;
+        }
+        }
+        hip_check(e, "autotune launch");
+    }
+}
+
+// average milliseconds of one tune_launch of kernel k (after one untimed run)
+static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b)
+{
+    const int reps = 3;
+    if (k == SK_PDV_PREDICT)
+        for (auto &t : c->tiles)
+            hip_check(hipMemcpyAsync(t.f[X_W3], t.f[F_PRESSURE], (size_t)lay_elems(t.g) * store_bytes(c->store),
+                                     hipMemcpyDeviceToDevice, c->st),
+                      "autotune copy");
+    tune_launch(c, k);
+    hip_check(hipEventRecord(a, c->st), "event");
+    for (int r = 0; r < reps; r++) tune_launch(c, k);
+    hip_check(hipEventRecord(b, c->st), "event");
+    hip_check(hipEventSynchronize(b), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, a, b), "event time");
+    return ms / reps;
+}
+
+// Every shaped kernel, every candidate shape: positions per workgroup along
+// its march from kTunePlanes (the x cell sweep has no march) times the two
+// block orders; the fastest is kept.  Shapes split the work differently but
+// never change a value, so later steps give the same bits whatever is chosen.
+static void autotune(clover_chunk *c, double *before, double *after)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&b), "event");
+    double t0 = 0.0, t1 = 0.0;
+    for (int k = 0; k < SK_COUNT; k++) {
+        const Shape start = c->shapes.k[k];
+        double best = tune_time(c, k, a, b);
+        Shape best_shape = start;
+        t0 += best;
+        for (int planes : kTunePlanes)
+            for (int xcd = 0; xcd < 2; xcd++) {
+                if (k == SK_CELL_X && planes != start.planes) continue;
+                const Shape cand{planes, xcd};
+                if (cand.planes == start.planes && cand.xcd == start.xcd) continue;
+                c->shapes.k[k] = cand;
+                const double ms = tune_time(c, k, a, b);
+                if (ms < best) {
+                    best = ms;
+                    best_shape = cand;
+                }
+            }
+        c->shapes.k[k] = best_shape;
+        t1 += best;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (before) *before = t0;
+    if (after) *after = t1;
+}
+
 static void do_step(clover_chunk *c)
 {
     pdv(c, true);
@@ -651,7 +768,7 @@ static void do_step(clover_chunk *c)
             if (c->fast)
                 hip_check(launch_fast_accelerate(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_PRESSURE],
                                                  t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
-                                                 t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->st),
+                                                 t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->shapes.k[SK_ACCELERATE], c->st),
                           "accelerate");
             else
                 hip_check(launch_accelerate(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[X_VOLUME],
@@ -746,6 +863,7 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         c->fast = !(flags & CLOVER_CHUNK_PER_CALL);
         c->store = (flags & CLOVER_CHUNK_FP32) ? ST_F32 : ST_F64;
         c->halo_check = (flags & CLOVER_CHUNK_HALO_CHECK) != 0;
+        c->shapes = default_shapes();
         // the uniform mesh as initialise_chunk.f90:37-39 and
         // initialise_chunk_kernel.f90:151-178 compute it
         c->geo.dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
@@ -982,6 +1100,35 @@ int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen)
     }
     if ((int)out.size() + 1 > buflen) throw Error(E_ARG, "buffer too small");
     memcpy(buf, out.c_str(), out.size() + 1);
+    CLOVER_API_END
+}
+
+int clover_chunk_autotune(clover_chunk *c, double *ms_before, double *ms_after)
+{
+    CLOVER_API_BEGIN
+    if (!c) throw Error(E_ARG, "null chunk");
+    if (!c->fast) throw Error(E_STATE, "the per-call kernels have no launch shapes");
+    autotune(c, ms_before, ms_after);
+    CLOVER_API_END
+}
+
+int clover_chunk_get_shape(const clover_chunk *c, int kernel, int *planes, int *xcd)
+{
+    CLOVER_API_BEGIN
+    if (!c || !planes || !xcd) throw Error(E_ARG, "null argument");
+    if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
+    *planes = c->shapes.k[kernel].planes;
+    *xcd = c->shapes.k[kernel].xcd;
+    CLOVER_API_END
+}
+
+int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd)
+{
+    CLOVER_API_BEGIN
+    if (!c) throw Error(E_ARG, "null chunk");
+    if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
+    if (planes < 1 || (xcd != 0 && xcd != 1)) throw Error(E_ARG, "planes must be positive, xcd 0 or 1");
+    c->shapes.k[kernel] = Shape{planes, xcd};
     CLOVER_API_END
 }
 

@@ -43,6 +43,7 @@ struct clover_ensemble {
     int step = 0;
     int advect_x = 1;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // index: advect_x
+    Shapes shapes = default_shapes();              // launch shapes (autotuner)
 };
 
 namespace {
@@ -60,8 +61,29 @@ BatchGrid make_grid(clover_ensemble *e, const std::vector<int> &counts)
     return BatchGrid{p, start[e->nd]};
 }
 
-// kind: 0..BK_COUNT-1 kernels (a, b unused); 100 + axis reflective halos
-const BatchGrid &grid(clover_ensemble *e, int kind, int a = 0, int b = 0)
+// the launch shape a batched kernel takes (none: -1)
+int shape_index(int bk)
+{
+    switch (bk) {
+    case BK_VISC_DT: return SK_VISC_DT;
+    case BK_PDV_PREDICT: return SK_PDV_PREDICT;
+    case BK_ACCELERATE: return SK_ACCELERATE;
+    case BK_PDV_CORRECT: return SK_PDV_CORRECT;
+    case BK_CELL_X: return SK_CELL_X;
+    case BK_CELL_Y: return SK_CELL_Y;
+    case BK_CELL_Z: return SK_CELL_Z;
+    case BK_MOM_X: return SK_MOM_X;
+    case BK_MOM_Y: return SK_MOM_Y;
+    case BK_MOM_Z: return SK_MOM_Z;
+    default: return -1;
+    }
+}
+
+const Shape &shape(clover_ensemble *e, int bk) { return e->shapes.k[shape_index(bk)]; }
+
+// kind: 0..BK_COUNT-1 kernels (a: planes of the launch shape, b unused);
+// 100 + axis reflective halos (a: depth, b: items)
+const BatchGrid &grid(clover_ensemble *e, int kind, int a, int b = 0)
 {
     auto key = std::make_tuple(kind, a, b);
     auto it = e->grids.find(key);
@@ -71,9 +93,16 @@ const BatchGrid &grid(clover_ensemble *e, int kind, int a = 0, int b = 0)
     if (cs != hipStreamCaptureStatusNone) throw Error(E_STATE, "ensemble: launch table missing during capture");
     std::vector<int> counts(e->nd);
     for (int d = 0; d < e->nd; d++)
-        counts[d] = kind < 100 ? batch_blocks((BatchKernel)kind, e->host[d].g)
+        counts[d] = kind < 100 ? batch_blocks((BatchKernel)kind, e->host[d].g, a)
                                : batch_blocks_reflect(e->host[d].g, kind - 100, a, b);
     return e->grids[key] = make_grid(e, counts);
+}
+
+// a kernel's grid for its current launch shape
+const BatchGrid &kgrid(clover_ensemble *e, int bk)
+{
+    const int si = shape_index(bk);
+    return grid(e, bk, si < 0 ? 0 : e->shapes.k[si].planes);
 }
 
 void set_fields(int *fields, std::initializer_list<int> ids)
@@ -122,7 +151,7 @@ void halo(clover_ensemble *e, int which)
 
 void ideal_gas(clover_ensemble *e)
 {
-    hip_check(batch_ideal_gas(e->store, e->dev, e->nd, grid(e, BK_IDEAL_GAS), e->st), "ideal_gas");
+    hip_check(batch_ideal_gas(e->store, e->dev, e->nd, kgrid(e, BK_IDEAL_GAS), e->st), "ideal_gas");
 }
 
 void swap(clover_ensemble *e, std::initializer_list<std::pair<int, int>> pairs, bool add_time)
@@ -142,7 +171,8 @@ void advec_cell(clover_ensemble *e, int sweep, int dir)
     const BatchKernel k = dir == 1 ? BK_CELL_X : dir == 2 ? BK_CELL_Y : BK_CELL_Z;
     // the third sweep also writes ideal_gas of the new state: the next step's
     // (or a summary's) equation of state
-    hip_check(batch_advec_cell(e->store, e->dev, e->nd, grid(e, k), e->advect_x, dir, sweep, sweep == 3, e->st),
+    hip_check(batch_advec_cell(e->store, e->dev, e->nd, kgrid(e, k), e->advect_x, dir, sweep, sweep == 3,
+                               shape(e, k), e->st),
               "advec_cell");
     swap(e, {{F_DENSITY1, X_W1}, {F_ENERGY1, X_W2}}, false);
 }
@@ -150,7 +180,8 @@ void advec_cell(clover_ensemble *e, int sweep, int dir)
 void advec_mom(clover_ensemble *e, int sweep, int dir)
 {
     const BatchKernel k = dir == 1 ? BK_MOM_X : dir == 2 ? BK_MOM_Y : BK_MOM_Z;
-    hip_check(batch_advec_mom(e->store, e->dev, e->nd, grid(e, k), e->advect_x, dir, sweep, e->st), "advec_mom");
+    hip_check(batch_advec_mom(e->store, e->dev, e->nd, kgrid(e, k), e->advect_x, dir, sweep, shape(e, k), e->st),
+              "advec_mom");
     swap(e, {{F_XVEL1, X_W5}, {F_YVEL1, X_W6}, {F_ZVEL1, X_W7}}, false);
 }
 
@@ -160,15 +191,20 @@ void enqueue_step(clover_ensemble *e)
     // timestep.f90:28-128; its ideal_gas results were written by the previous
     // step's third cell sweep (or by the start sequence)
     halo(e, FH_TIMESTEP);
-    const BatchGrid &vg = grid(e, BK_VISC_DT);
-    hip_check(batch_visc_dt(e->store, e->dev, e->nd, vg, e->g_small, e->visc_partial, e->st), "visc_dt");
+    const BatchGrid &vg = kgrid(e, BK_VISC_DT);
+    hip_check(batch_visc_dt(e->store, e->dev, e->nd, vg, shape(e, BK_VISC_DT), e->g_small, e->visc_partial, e->st),
+              "visc_dt");
     hip_check(batch_deck_dt(e->dev, e->nd, vg.start, e->visc_partial, e->g_big, e->st), "deck dt");
     halo(e, FH_VISCOSITY);
     // PdV predictor (+ ideal_gas on the predicted state), pressure halo
-    hip_check(batch_pdv_predict(e->store, e->dev, e->nd, grid(e, BK_PDV_PREDICT), e->st), "PdV predict");
+    hip_check(batch_pdv_predict(e->store, e->dev, e->nd, kgrid(e, BK_PDV_PREDICT), shape(e, BK_PDV_PREDICT), e->st),
+              "PdV predict");
     halo(e, FH_PRESSURE);
-    hip_check(batch_accelerate(e->store, e->dev, e->nd, grid(e, BK_ACCELERATE), e->st), "accelerate");
-    hip_check(batch_pdv_correct_flux(e->store, e->dev, e->nd, grid(e, BK_PDV_CORRECT), e->st), "PdV correct");
+    hip_check(batch_accelerate(e->store, e->dev, e->nd, kgrid(e, BK_ACCELERATE), shape(e, BK_ACCELERATE), e->st),
+              "accelerate");
+    hip_check(batch_pdv_correct_flux(e->store, e->dev, e->nd, kgrid(e, BK_PDV_CORRECT), shape(e, BK_PDV_CORRECT),
+                                     e->st),
+              "PdV correct");
     // advection.f90:26-181
     int dir = e->advect_x ? 1 : 3;
     halo(e, FH_CELL1);
@@ -193,7 +229,7 @@ void enqueue_step(clover_ensemble *e)
 // one allocates and copies, which a capture cannot contain)
 void prepare_grids(clover_ensemble *e)
 {
-    for (int k = 0; k < BK_COUNT; k++) grid(e, k);
+    for (int k = 0; k < BK_COUNT; k++) kgrid(e, k);
     // the step's halo updates, for both sweep orders
     for (int ax = 0; ax < 2; ax++)
         for (int which = FH_TIMESTEP; which <= FH_MOM3; which++) {
@@ -203,6 +239,119 @@ void prepare_grids(clover_ensemble *e)
                 if (n) grid(e, 100 + axis, p.depth, n);
             }
         }
+}
+
+// ---------------------------------------------------------------- autotuner
+// The batched twin of the single-deck tuner (chunk.cpp): each shaped kernel
+// runs over all decks with every candidate shape and the fastest is kept.
+// The kernels take their arrays from the deck table, so outputs are sent to
+// spare arrays by exchanging table slots around the timed launches (the
+// pressure pdv_predict updates in place becomes a spare array too; its
+// contents do not matter for timing).
+SlotSwap tune_slots(int bk)
+{
+    SlotSwap sw{};
+    auto pair = [&](int a, int b) {
+        sw.a[sw.n] = a;
+        sw.b[sw.n] = b;
+        sw.n++;
+    };
+    switch (bk) {
+    case BK_VISC_DT: pair(F_VISCOSITY, X_W1); break;
+    case BK_PDV_PREDICT: pair(F_PRESSURE, X_W1); break;
+    case BK_ACCELERATE: pair(F_XVEL1, X_W5); pair(F_YVEL1, X_W6); pair(F_ZVEL1, X_W7); break;
+    case BK_PDV_CORRECT:
+        pair(F_DENSITY1, X_W1); pair(F_ENERGY1, X_W2);
+        pair(F_VOL_FLUX_X, X_W5); pair(F_VOL_FLUX_Y, X_W6); pair(F_VOL_FLUX_Z, X_W7);
+        break;
+    case BK_CELL_X: pair(F_MASS_FLUX_X, X_W5); break;  // the new density/energy go to W1/W2 anyway
+    case BK_CELL_Y: pair(F_MASS_FLUX_Y, X_W5); break;
+    case BK_CELL_Z: pair(F_MASS_FLUX_Z, X_W5); break;
+    default: break;  // momentum sweeps write W5..W7 only
+    }
+    return sw;
+}
+
+void tune_launch(clover_ensemble *e, int bk, const Shape &sh)
+{
+    const BatchGrid &g = grid(e, bk, sh.planes);
+    hipError_t r = hipSuccess;
+    switch (bk) {
+    case BK_VISC_DT: r = batch_visc_dt(e->store, e->dev, e->nd, g, sh, e->g_small, e->visc_partial, e->st); break;
+    case BK_PDV_PREDICT: r = batch_pdv_predict(e->store, e->dev, e->nd, g, sh, e->st); break;
+    case BK_ACCELERATE: r = batch_accelerate(e->store, e->dev, e->nd, g, sh, e->st); break;
+    case BK_PDV_CORRECT: r = batch_pdv_correct_flux(e->store, e->dev, e->nd, g, sh, e->st); break;
+    case BK_CELL_X:
+    case BK_CELL_Y:
+    case BK_CELL_Z: {
+        const int dir = bk - BK_CELL_X + 1;
+        r = batch_advec_cell(e->store, e->dev, e->nd, g, 1, dir, dir == 2 ? 2 : 1, false, sh, e->st);
+        break;
+    }
+    default: {
+        const int dir = bk - BK_MOM_X + 1;
+        r = batch_advec_mom(e->store, e->dev, e->nd, g, 1, dir, dir == 2 ? 2 : 1, sh, e->st);
+        break;
+    }
+    }
+    hip_check(r, "autotune launch");
+}
+
+double tune_time(clover_ensemble *e, int bk, const Shape &sh, hipEvent_t a, hipEvent_t b)
+{
+    const int reps = 3;
+    grid(e, bk, sh.planes);  // any table allocation happens before the timing
+    const SlotSwap sw = tune_slots(bk);
+    if (sw.n) hip_check(batch_swap(e->dev, e->nd, sw, e->st), "swap");
+    tune_launch(e, bk, sh);
+    hip_check(hipEventRecord(a, e->st), "event");
+    for (int r = 0; r < reps; r++) tune_launch(e, bk, sh);
+    hip_check(hipEventRecord(b, e->st), "event");
+    if (sw.n) hip_check(batch_swap(e->dev, e->nd, sw, e->st), "swap back");
+    hip_check(hipEventSynchronize(b), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, a, b), "event time");
+    return ms / reps;
+}
+
+void autotune(clover_ensemble *e, double *before, double *after)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&b), "event");
+    double t0 = 0.0, t1 = 0.0;
+    for (int bk = 0; bk < BK_COUNT; bk++) {
+        const int si = shape_index(bk);
+        if (si < 0) continue;
+        const Shape start = e->shapes.k[si];
+        double best = tune_time(e, bk, start, a, b);
+        Shape best_shape = start;
+        t0 += best;
+        for (int planes : kTunePlanes)
+            for (int xcd = 0; xcd < 2; xcd++) {
+                if (si == SK_CELL_X && planes != start.planes) continue;
+                const Shape cand{planes, xcd};
+                if (cand.planes == start.planes && cand.xcd == start.xcd) continue;
+                const double ms = tune_time(e, bk, cand, a, b);
+                if (ms < best) {
+                    best = ms;
+                    best_shape = cand;
+                }
+            }
+        e->shapes.k[si] = best_shape;
+        t1 += best;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    // captured steps hold the old shapes
+    for (auto &g : e->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    prepare_grids(e);
+    if (before) *before = t0;
+    if (after) *after = t1;
 }
 
 void destroy(clover_ensemble *e)
@@ -311,8 +460,10 @@ int clover_ensemble_create(const clover_deck *decks, int ndecks, int device, int
         hip_check(hipMalloc(&e->dev, sizeof(DeckDev) * ndecks), "hipMalloc decks");
         hip_check(hipMemcpy(e->dev, e->host.data(), sizeof(DeckDev) * ndecks, hipMemcpyHostToDevice), "H2D decks");
         prepare_grids(e);
-        const BatchGrid &vg = grid(e, BK_VISC_DT);
-        const BatchGrid &sg = grid(e, BK_SUMMARY);
+        // calc_dt's partials: room for the visc/dt grid of every shape the
+        // autotuner tries (the most blocks at its fewest planes)
+        const BatchGrid &vg = grid(e, BK_VISC_DT, kTunePlanes[0]);
+        const BatchGrid &sg = kgrid(e, BK_SUMMARY);
         hip_check(hipMalloc(&e->visc_partial, sizeof(double) * (vg.total + 1)), "hipMalloc");
         hip_check(hipMalloc(&e->sum_partial, sizeof(double) * 5 * (sg.total + 1)), "hipMalloc");
         hip_check(hipMalloc(&e->sums, sizeof(double) * 5 * ndecks), "hipMalloc");
@@ -381,7 +532,7 @@ int clover_ensemble_summaries(clover_ensemble *e, double *out)
     if (!e || !out) throw Error(E_ARG, "bad argument");
     hip_check(hipSetDevice(e->device), "hipSetDevice");
     ideal_gas(e);  // field_summary.f90:49-60 recomputes pressure first
-    const BatchGrid &sg = grid(e, BK_SUMMARY);
+    const BatchGrid &sg = kgrid(e, BK_SUMMARY);
     hip_check(batch_field_summary(e->store, e->dev, e->nd, sg, e->sum_partial, e->st), "field_summary");
     hip_check(batch_deck_sum5(e->nd, sg.start, e->sum_partial, e->sums, e->st), "summary sums");
     hip_check(hipMemcpyAsync(out, e->sums, sizeof(double) * 5 * e->nd, hipMemcpyDeviceToHost, e->st), "D2H");
@@ -427,5 +578,24 @@ int clover_ensemble_fetch_field(clover_ensemble *e, int deck, int field, double 
 }
 
 long long clover_ensemble_device_bytes(const clover_ensemble *e) { return e ? e->bytes : 0; }
+
+int clover_ensemble_autotune(clover_ensemble *e, double *ms_before, double *ms_after)
+{
+    CLOVER_API_BEGIN
+    if (!e) throw Error(E_ARG, "null ensemble");
+    hip_check(hipSetDevice(e->device), "hipSetDevice");
+    autotune(e, ms_before, ms_after);
+    CLOVER_API_END
+}
+
+int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd)
+{
+    CLOVER_API_BEGIN
+    if (!e || !planes || !xcd) throw Error(E_ARG, "null argument");
+    if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
+    *planes = e->shapes.k[kernel].planes;
+    *xcd = e->shapes.k[kernel].xcd;
+    CLOVER_API_END
+}
 
 }  // extern "C"

@@ -1,8 +1,8 @@
 // Fused kernels of the resident driver's fast path.
 //
-// Every value they write is computed with the same FP64 expression, in the
-// same order, as the per-call kernels (kernels.h), so a fast run reproduces a
-// per-call run bit for bit; they only move fewer bytes:
+// In the FP64 form every value they write is computed with the same
+// expression, in the same order, as the per-call kernels (kernels.h), so a
+// fast run reproduces a per-call run bit for bit; they only move fewer bytes:
 //  * the mesh is uniform (initialise_chunk_kernel.f90:63-178 gives
 //    celldx = dx, xarea = celldy*celldz, volume = dx*dy*dz everywhere), so
 //    geometry comes in as scalars instead of arrays;
@@ -14,11 +14,14 @@
 //  * intermediates are recomputed where that is cheaper than storing them.
 //
 // Storage.  Each launcher takes the element type of the field arrays:
-// ST_F64 (the drop-in form) or ST_F32 (the FP32-store form: fields held in
-// FP32, every value converted to FP64 when loaded and rounded to FP32 when
-// stored, so all arithmetic, calc_dt's minimum and the advection's mass and
-// energy accumulation run in FP64).  Array arguments are untyped pointers
-// to arrays of that element type.
+// ST_F64 (the drop-in form) or ST_F32 (the FP32-store form: fields held and
+// stencils computed in FP32; calc_dt's minimum, advec_cell's mass and energy
+// update and the summary sums in FP64).  Array arguments are untyped
+// pointers to arrays of that element type.
+//
+// Shape.  The marching kernels take a launch shape (positions per workgroup
+// along the march, block order), which changes only how the work is split,
+// never a value; the autotuner (chunk.cpp, ensemble.cpp) picks it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +31,35 @@
 namespace clover {
 
 enum Store : int { ST_F64 = 0, ST_F32 = 1 };
+
+// The kernels with a launch shape, and the shape: `planes` positions per
+// workgroup along the kernel's march (planes for the z marches, faces or
+// nodes along the sweep for the y/z advection sweeps; the x cell sweep has
+// no march and ignores it) and `xcd`, the XCD-contiguous block order
+// (launch.h hw_blk_xcd) instead of the hardware's.
+// (the same indices as CLOVER_SHAPE_* of clover_hip.h)
+enum ShapeKernel : int {
+    SK_VISC_DT, SK_PDV_PREDICT, SK_ACCELERATE, SK_PDV_CORRECT, SK_CELL_X, SK_CELL_Y, SK_CELL_Z, SK_MOM_X,
+    SK_MOM_Y, SK_MOM_Z, SK_COUNT
+};
+struct Shape {
+    int planes;
+    int xcd;
+};
+struct Shapes {
+    Shape k[SK_COUNT];
+};
+// the autotuner's candidates for Shape::planes (each with both block orders)
+constexpr int kTunePlanes[] = {16, 32, 64, 128};
+constexpr int kTuneCount = sizeof(kTunePlanes) / sizeof(kTunePlanes[0]);
+
+// 64 positions per workgroup; the XCD order for the y/z momentum sweeps
+inline Shapes default_shapes()
+{
+    Shapes s;
+    for (int i = 0; i < SK_COUNT; i++) s.k[i] = Shape{64, i == SK_MOM_Y || i == SK_MOM_Z};
+    return s;
+}
 
 inline size_t store_bytes(Store s) { return s == ST_F32 ? sizeof(float) : sizeof(double); }
 
@@ -49,24 +81,24 @@ hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_s
                                double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
                                const void *density0, const void *pressure, const void *soundspeed, void *viscosity,
                                const void *xvel0, const void *yvel0, const void *zvel0, double *partial, int *nparts,
-                               hipStream_t st);
+                               const Shape &sh, hipStream_t st);
 
 // PdV predictor (PdV_kernel.f90:76-117) followed by ideal_gas on the
 // predicted density/energy (ideal_gas.f90:53-62): writes pressure only.
 hipError_t launch_fast_pdv_predict(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
                                    const void *energy0, void *pressure, const void *viscosity, const void *xvel0,
-                                   const void *yvel0, const void *zvel0, hipStream_t st);
+                                   const void *yvel0, const void *zvel0, const Shape &sh, hipStream_t st);
 
 hipError_t launch_fast_accelerate(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
                                   const void *pressure, const void *viscosity, const void *xvel0, const void *yvel0,
-                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, hipStream_t st);
+                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, const Shape &sh, hipStream_t st);
 
 // PdV corrector (PdV_kernel.f90:124-165) and flux_calc (flux_calc_kernel.f90)
 hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
                                         void *density1, const void *energy0, void *energy1, const void *pressure,
                                         const void *viscosity, const void *xvel0, const void *xvel1,
                                         const void *yvel0, const void *yvel1, const void *zvel0, const void *zvel1,
-                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, hipStream_t st);
+                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, const Shape &sh, hipStream_t st);
 
 // one advec_cell sweep, fluxes and cell update in one kernel
 // (cell_kernels.hip): mass_flux_<dir> is written over the reference's face
@@ -77,7 +109,7 @@ hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advec
                                   const void *density1, const void *energy1, void *mass_flux_x,
                                   const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
                                   void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
-                                  void *pressure, void *soundspeed, hipStream_t st);
+                                  void *pressure, void *soundspeed, const Shape &sh, hipStream_t st);
 
 // one advec_mom sweep for all three velocities (advec_mom_driver.f90 calls
 // the kernel for xvel1, yvel1, zvel1 with the node quantities of the first
@@ -89,7 +121,7 @@ hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect
                                  int direction, const void *mass_flux_x, const void *vol_flux_x,
                                  const void *mass_flux_y, const void *vol_flux_y, const void *mass_flux_z,
                                  const void *vol_flux_z, const void *density1, const void *const *vel_in,
-                                 void *const *vel_out, hipStream_t st);
+                                 void *const *vel_out, const Shape &sh, hipStream_t st);
 
 // field_summary_kernel.f90:51-78: five FP64 partials per block.  The FP64
 // form sums as the per-call kernel does (same tree); the FP32 form

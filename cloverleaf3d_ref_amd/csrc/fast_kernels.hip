@@ -22,8 +22,6 @@
 
 namespace clover {
 
-constexpr int ZCH = 64;  // planes per marching workgroup
-
 // 2x2 corner values of an array around (j, k) in one plane, in the storage
 // type (which is also the arithmetic type of the form):
 //   quad_up:   v[a + 2b] = p(j+a, k+b)           (a, b in {0, 1})
@@ -45,16 +43,6 @@ __device__ inline QuadS<S> quad_down_raw(const S *__restrict__ p, int i, int ld)
     return QuadS<S>{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
 }
 
-// the column's planes [l0, l1] of the chunk this workgroup marches
-__device__ inline void chunk_planes(const Blk &b, int lstart, int lend, int &l0, int &l1)
-{
-    const int n = lend - lstart + 1;
-    const int per = (n + b.gz - 1) / b.gz;
-    l0 = lstart + b.z * per;
-    l1 = min(l0 + per - 1, lend);
-}
-
-__host__ __device__ inline int chunks_for(int nplanes) { return (nplanes + ZCH - 1) / ZCH; }
 
 // ---------------------------------------------------------------------------
 // viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
@@ -73,7 +61,7 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
     const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
     const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     int l0, l1;
-    chunk_planes(bk, g.z_min, g.z_max, l0, l1);
+    chunk_range(bk, g.z_min, g.z_max, l0, l1);
     const bool in = j <= g.x_max && k <= g.y_max && l0 <= l1;  // every thread reaches the reduction
     const double mm = dmin(dmin(m.dx, m.dy), m.dz);
     const T ds = (T)(1.0 / (mm * mm));
@@ -198,13 +186,13 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_visc_dt(Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
+__global__ void __launch_bounds__(BX *BY) k_fast_visc_dt(int xcd, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
                const S *__restrict__ soundspeed, S *__restrict__ viscosity,
                const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
                double *__restrict__ partial)
 {
-    visc_dt_body<S>(hw_blk(), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0, yvel0, zvel0, partial);
+    visc_dt_body<S>(hw_blk(xcd), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0, yvel0, zvel0, partial);
 }
 
 template <typename S>
@@ -212,15 +200,15 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
                                double dtw_safe, double dtdiv_safe, double g_small, const S *density0,
                                const S *pressure, const S *soundspeed, S *viscosity,
                                const S *xvel0, const S *yvel0, const S *zvel0, double *partial,
-                               int *nparts, hipStream_t st)
+                               int *nparts, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const int bx = (nx + BX - 1) / BX, by = (ny + BY - 1) / BY;
     if (bx * by > kMaxPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
-    int gz = chunks_for(nz);
+    int gz = nchunks(nz, sh.planes);
     if (bx * by * gz > kMaxPartials) gz = kMaxPartials / (bx * by);
     *nparts = bx * by * gz;
-    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, g, m, dtc_safe,
+    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dtc_safe,
                  dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,
                  yvel0, zvel0, partial);
     return hipGetLastError();
@@ -243,7 +231,7 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
     const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     if (j > g.x_max || k > g.y_max) return;
     int l0, l1;
-    chunk_planes(bk, g.z_min, g.z_max, l0, l1);
+    chunk_range(bk, g.z_min, g.z_max, l0, l1);
     if (l0 > l1) return;
     using T = S;  // arithmetic type of the form
     const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol, tdt = (T)dt;
@@ -298,22 +286,22 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_pdv_predict(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
+__global__ void __launch_bounds__(BX *BY) k_fast_pdv_predict(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
                    S *__restrict__ pressure, const S *__restrict__ viscosity,
                    const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                    const S *__restrict__ zvel0)
 {
-    pdv_predict_body<S>(hw_blk(), g, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
+    pdv_predict_body<S>(hw_blk(xcd), g, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
 }
 
 template <typename S>
 static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, const S *density0,
                                    const S *energy0, S *pressure, const S *viscosity,
-                                   const S *xvel0, const S *yvel0, const S *zvel0, hipStream_t st)
+                                   const S *xvel0, const S *yvel0, const S *zvel0, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
-    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
-    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict<S>, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
+    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict<S>, grid, dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dt, density0,
                  energy0, pressure, viscosity, xvel0, yvel0, zvel0);
     return hipGetLastError();
 }
@@ -331,7 +319,7 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
     const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     if (j > g.x_max + 1 || k > g.y_max + 1) return;
     int l0, l1;
-    chunk_planes(bk, g.z_min, g.z_max + 1, l0, l1);
+    chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
     if (l0 > l1) return;
     using T = S;  // arithmetic type of the form
     int i = idx(g, j, k, l0);
@@ -391,23 +379,23 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_accelerate(Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
+__global__ void __launch_bounds__(BX *BY) k_fast_accelerate(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
                   const S *__restrict__ viscosity, const S *__restrict__ xvel0,
                   const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
                   S *__restrict__ yvel1, S *__restrict__ zvel1)
 {
-    accelerate_body<S>(hw_blk(), g, m, dt, density0, pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
+    accelerate_body<S>(hw_blk(xcd), g, m, dt, density0, pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
 }
 
 template <typename S>
 static hipError_t fast_accelerate_t(const Lay &g, const Geo &m, double dt, const S *density0,
                                   const S *pressure, const S *viscosity, const S *xvel0,
                                   const S *yvel0, const S *zvel0, S *xvel1, S *yvel1,
-                                  S *zvel1, hipStream_t st)
+                                  S *zvel1, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
-    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
-    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate<S>, grid, dim3(BX, BY, 1), 0, st, g, m, dt, density0,
+    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate<S>, grid, dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dt, density0,
                  pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
     return hipGetLastError();
 }
@@ -432,7 +420,7 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
     const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
     if (j > g.x_max + 1 || k > g.y_max + 1) return;
     int l0, l1;
-    chunk_planes(bk, g.z_min, g.z_max + 1, l0, l1);
+    chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
     if (l0 > l1) return;
     const bool jin = j <= g.x_max, kin = k <= g.y_max;
     using T = S;  // arithmetic type of the form
@@ -530,7 +518,7 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_pdv_correct_flux(Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
+__global__ void __launch_bounds__(BX *BY) k_fast_pdv_correct_flux(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
                         const S *__restrict__ energy0, S *__restrict__ energy1,
                         const S *__restrict__ pressure, const S *__restrict__ viscosity,
                         const S *__restrict__ xvel0, const S *__restrict__ xvel1,
@@ -539,7 +527,7 @@ __global__ void __launch_bounds__(BX *BY) k_fast_pdv_correct_flux(Lay g, Geo m, 
                         S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
                         S *__restrict__ vol_flux_z)
 {
-    pdv_correct_flux_body<S>(hw_blk(), g, m, dt, density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
+    pdv_correct_flux_body<S>(hw_blk(xcd), g, m, dt, density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
 }
 
 template <typename S>
@@ -548,11 +536,11 @@ static hipError_t fast_pdv_correct_flux_t(const Lay &g, const Geo &m, double dt,
                                         const S *pressure, const S *viscosity, const S *xvel0,
                                         const S *xvel1, const S *yvel0, const S *yvel1,
                                         const S *zvel0, const S *zvel1, S *vol_flux_x,
-                                        S *vol_flux_y, S *vol_flux_z, hipStream_t st)
+                                        S *vol_flux_y, S *vol_flux_z, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
-    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, chunks_for(nz));
-    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux<S>, grid, dim3(BX, BY, 1), 0, st, g, m, dt,
+    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux<S>, grid, dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dt,
                  density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1,
                  vol_flux_x, vol_flux_y, vol_flux_z);
     return hipGetLastError();
@@ -687,34 +675,34 @@ hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_s
                                double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
                                const void *density0, const void *pressure, const void *soundspeed, void *viscosity,
                                const void *xvel0, const void *yvel0, const void *zvel0, double *partial, int *nparts,
-                               hipStream_t st)
+                               const Shape &sh, hipStream_t st)
 {
 #define CALL(T)                                                                                                 \
     fast_visc_dt_t<T>(g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, CP(T, density0),      \
                       CP(T, pressure), CP(T, soundspeed), MP(T, viscosity), CP(T, xvel0), CP(T, yvel0),         \
-                      CP(T, zvel0), partial, nparts, st)
+                      CP(T, zvel0), partial, nparts, sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
 
 hipError_t launch_fast_pdv_predict(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
                                    const void *energy0, void *pressure, const void *viscosity, const void *xvel0,
-                                   const void *yvel0, const void *zvel0, hipStream_t st)
+                                   const void *yvel0, const void *zvel0, const Shape &sh, hipStream_t st)
 {
 #define CALL(T)                                                                                                 \
     fast_pdv_predict_t<T>(g, m, dt, CP(T, density0), CP(T, energy0), MP(T, pressure), CP(T, viscosity),         \
-                          CP(T, xvel0), CP(T, yvel0), CP(T, zvel0), st)
+                          CP(T, xvel0), CP(T, yvel0), CP(T, zvel0), sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
 
 hipError_t launch_fast_accelerate(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
                                   const void *pressure, const void *viscosity, const void *xvel0, const void *yvel0,
-                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, hipStream_t st)
+                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, const Shape &sh, hipStream_t st)
 {
 #define CALL(T)                                                                                                 \
     fast_accelerate_t<T>(g, m, dt, CP(T, density0), CP(T, pressure), CP(T, viscosity), CP(T, xvel0),            \
-                         CP(T, yvel0), CP(T, zvel0), MP(T, xvel1), MP(T, yvel1), MP(T, zvel1), st)
+                         CP(T, yvel0), CP(T, zvel0), MP(T, xvel1), MP(T, yvel1), MP(T, zvel1), sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
@@ -723,13 +711,13 @@ hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, dou
                                         void *density1, const void *energy0, void *energy1, const void *pressure,
                                         const void *viscosity, const void *xvel0, const void *xvel1,
                                         const void *yvel0, const void *yvel1, const void *zvel0, const void *zvel1,
-                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, hipStream_t st)
+                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, const Shape &sh, hipStream_t st)
 {
 #define CALL(T)                                                                                                 \
     fast_pdv_correct_flux_t<T>(g, m, dt, CP(T, density0), MP(T, density1), CP(T, energy0), MP(T, energy1),      \
                                CP(T, pressure), CP(T, viscosity), CP(T, xvel0), CP(T, xvel1), CP(T, yvel0),      \
                                CP(T, yvel1), CP(T, zvel0), CP(T, zvel1), MP(T, vol_flux_x), MP(T, vol_flux_y),   \
-                               MP(T, vol_flux_z), st)
+                               MP(T, vol_flux_z), sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
@@ -757,18 +745,18 @@ __host__ __device__ inline void dims_ideal(const Lay &g, int &gx, int &gy, int &
     gz = g.z_max - g.z_min + 1;
 }
 
-__host__ __device__ inline void dims_cells(const Lay &g, int &gx, int &gy, int &gz)
+__host__ __device__ inline void dims_cells(const Lay &g, int planes, int &gx, int &gy, int &gz)
 {
     gx = (g.x_max - g.x_min + BX) / BX;
     gy = (g.y_max - g.y_min + BY) / BY;
-    gz = chunks_for(g.z_max - g.z_min + 1);
+    gz = nchunks(g.z_max - g.z_min + 1, planes);
 }
 
-__host__ __device__ inline void dims_nodes(const Lay &g, int &gx, int &gy, int &gz)
+__host__ __device__ inline void dims_nodes(const Lay &g, int planes, int &gx, int &gy, int &gz)
 {
     gx = (g.x_max - g.x_min + 1 + BX) / BX;
     gy = (g.y_max - g.y_min + 1 + BY) / BY;
-    gz = chunks_for(g.z_max - g.z_min + 2);
+    gz = nchunks(g.z_max - g.z_min + 2, planes);
 }
 
 __host__ __device__ inline int summary_blocks(const Lay &g)
@@ -785,12 +773,15 @@ __device__ inline Blk local_blk(int lb, int gx, int gy, int gz)
     return Blk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
 }
 
-#define DECK_BLOCK(dimsfn)                                     \
-    const int d = deck_of(start, nd, (int)blockIdx.x);         \
-    const DeckDev &D = decks[d];                               \
-    int gx, gy, gz;                                            \
-    dimsfn(D.g, gx, gy, gz);                                   \
-    const Blk bk = local_blk((int)blockIdx.x - start[d], gx, gy, gz);
+// the deck and the block within it of this workgroup (xcd: in the
+// XCD-contiguous order over the whole launch)
+#define DECK_BLOCK(dims_call, xcd)                                                          \
+    const int hb = (xcd) ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;     \
+    const int d = deck_of(start, nd, hb);                                                   \
+    const DeckDev &D = decks[d];                                                            \
+    int gx, gy, gz;                                                                         \
+    dims_call;                                                                              \
+    const Blk bk = local_blk(hb - start[d], gx, gy, gz);
 #define CF(id) static_cast<const S *>(D.f[id])
 #define MF(id) static_cast<S *>(D.f[id])
 
@@ -798,7 +789,7 @@ template <typename S>
 __global__ void __launch_bounds__(BX *BY)
 kb_ideal_gas(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
 {
-    DECK_BLOCK(dims_ideal)
+    DECK_BLOCK(dims_ideal(D.g, gx, gy, gz), 0)
     const Lay g = D.g;
     const Range r{g.x_min, g.x_max, g.y_min, g.y_max, g.z_min, g.z_max};
     ideal_gas_body<S>(bk, g, r, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), MF(F_SOUNDSPEED));
@@ -806,10 +797,10 @@ kb_ideal_gas(const DeckDev *__restrict__ decks, const int *__restrict__ start, i
 
 template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, double g_small,
-           double *__restrict__ partial)
+kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd,
+           double g_small, double *__restrict__ partial)
 {
-    DECK_BLOCK(dims_cells)
+    DECK_BLOCK(dims_cells(D.g, planes, gx, gy, gz), xcd)
     visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
                     CF(F_DENSITY0), CF(F_PRESSURE), CF(F_SOUNDSPEED), MF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0),
                     CF(F_ZVEL0), partial + start[d]);
@@ -817,27 +808,27 @@ kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int
 
 template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
-    DECK_BLOCK(dims_cells)
+    DECK_BLOCK(dims_cells(D.g, planes, gx, gy, gz), xcd)
     pdv_predict_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), CF(F_VISCOSITY),
                         CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
 }
 
 template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-kb_accelerate(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+kb_accelerate(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
-    DECK_BLOCK(dims_nodes)
+    DECK_BLOCK(dims_nodes(D.g, planes, gx, gy, gz), xcd)
     accelerate_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0),
                        CF(F_YVEL0), CF(F_ZVEL0), MF(F_XVEL1), MF(F_YVEL1), MF(F_ZVEL1));
 }
 
 template <typename S>
 __global__ void __launch_bounds__(BX *BY)
-kb_pdv_correct_flux(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd)
+kb_pdv_correct_flux(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
-    DECK_BLOCK(dims_nodes)
+    DECK_BLOCK(dims_nodes(D.g, planes, gx, gy, gz), xcd)
     pdv_correct_flux_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), MF(F_DENSITY1), CF(F_ENERGY0), MF(F_ENERGY1),
                              CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0), CF(F_XVEL1), CF(F_YVEL0), CF(F_YVEL1),
                              CF(F_ZVEL0), CF(F_ZVEL1), MF(F_VOL_FLUX_X), MF(F_VOL_FLUX_Y), MF(F_VOL_FLUX_Z));
@@ -854,7 +845,7 @@ __global__ void __launch_bounds__(RED_THREADS)
 kb_field_summary(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd,
                  double *__restrict__ partial)
 {
-    DECK_BLOCK(dims_summary)
+    DECK_BLOCK(dims_summary(D.g, gx, gy, gz), 0)
     const Lay g = D.g;
     const int nj = g.x_max - g.x_min + 1, nk = g.y_max - g.y_min + 1, nl = g.z_max - g.z_min + 1;
     field_summary_body<S, COMP>(bk, g, D.m, nj, nk, nj * nk * nl, CF(F_DENSITY0), CF(F_ENERGY0), CF(F_PRESSURE),
@@ -864,25 +855,25 @@ kb_field_summary(const DeckDev *__restrict__ decks, const int *__restrict__ star
 #undef CF
 #undef MF
 
-int batch_blocks_mom(int dir, const Lay &g);
-int batch_blocks_cell(int dir, const Lay &g);
+int batch_blocks_mom(int dir, const Lay &g, int planes);
+int batch_blocks_cell(int dir, const Lay &g, int planes);
 
-int batch_blocks(BatchKernel k, const Lay &g)
+int batch_blocks(BatchKernel k, const Lay &g, int planes)
 {
     int gx = 1, gy = 1, gz = 1;
     switch (k) {
     case BK_IDEAL_GAS: dims_ideal(g, gx, gy, gz); break;
     case BK_VISC_DT:
-    case BK_PDV_PREDICT: dims_cells(g, gx, gy, gz); break;
+    case BK_PDV_PREDICT: dims_cells(g, planes, gx, gy, gz); break;
     case BK_ACCELERATE:
-    case BK_PDV_CORRECT: dims_nodes(g, gx, gy, gz); break;
+    case BK_PDV_CORRECT: dims_nodes(g, planes, gx, gy, gz); break;
     case BK_SUMMARY: dims_summary(g, gx, gy, gz); break;
-    case BK_CELL_X: return batch_blocks_cell(1, g);
-    case BK_CELL_Y: return batch_blocks_cell(2, g);
-    case BK_CELL_Z: return batch_blocks_cell(3, g);
-    case BK_MOM_X: return batch_blocks_mom(1, g);
-    case BK_MOM_Y: return batch_blocks_mom(2, g);
-    case BK_MOM_Z: return batch_blocks_mom(3, g);
+    case BK_CELL_X: return batch_blocks_cell(1, g, planes);
+    case BK_CELL_Y: return batch_blocks_cell(2, g, planes);
+    case BK_CELL_Z: return batch_blocks_cell(3, g, planes);
+    case BK_MOM_X: return batch_blocks_mom(1, g, planes);
+    case BK_MOM_Y: return batch_blocks_mom(2, g, planes);
+    case BK_MOM_Z: return batch_blocks_mom(3, g, planes);
     default: return 0;
     }
     return gx * gy * gz;
@@ -900,32 +891,34 @@ hipError_t batch_ideal_gas(Store s, const DeckDev *decks, int nd, const BatchGri
     return hipGetLastError();
 }
 
-hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double g_small,
+hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, double g_small,
                          double *partial, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_visc_dt", kb_visc_dt<float>, dim3(BX, BY), decks, bg.start, nd, g_small, partial);
-    else BLAUNCH("kb_visc_dt", kb_visc_dt<double>, dim3(BX, BY), decks, bg.start, nd, g_small, partial);
+    if (s == ST_F32)
+        BLAUNCH("kb_visc_dt", kb_visc_dt<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd, g_small, partial);
+    else
+        BLAUNCH("kb_visc_dt", kb_visc_dt<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd, g_small, partial);
     return hipGetLastError();
 }
 
-hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_pdv_predict", kb_pdv_predict<float>, dim3(BX, BY), decks, bg.start, nd);
-    else BLAUNCH("kb_pdv_predict", kb_pdv_predict<double>, dim3(BX, BY), decks, bg.start, nd);
+    if (s == ST_F32) BLAUNCH("kb_pdv_predict", kb_pdv_predict<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
+    else BLAUNCH("kb_pdv_predict", kb_pdv_predict<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
     return hipGetLastError();
 }
 
-hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_accelerate", kb_accelerate<float>, dim3(BX, BY), decks, bg.start, nd);
-    else BLAUNCH("kb_accelerate", kb_accelerate<double>, dim3(BX, BY), decks, bg.start, nd);
+    if (s == ST_F32) BLAUNCH("kb_accelerate", kb_accelerate<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
+    else BLAUNCH("kb_accelerate", kb_accelerate<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
     return hipGetLastError();
 }
 
-hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
+hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<float>, dim3(BX, BY), decks, bg.start, nd);
-    else BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<double>, dim3(BX, BY), decks, bg.start, nd);
+    if (s == ST_F32) BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
+    else BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
     return hipGetLastError();
 }
 

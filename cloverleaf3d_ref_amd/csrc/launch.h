@@ -50,15 +50,37 @@ __device__ inline Blk hw_blk()
 // plain order.
 constexpr int kXcds = 8;
 
+// the logical block that hardware block hw of n runs in the XCD-contiguous
+// order (a permutation of 0..n-1)
+__host__ __device__ inline int xcd_block(int hw, int n)
+{
+    const int xcd = hw % kXcds, t = hw / kXcds;
+    const int q = n / kXcds, r = n % kXcds;  // XCDs below r take q+1 blocks
+    return xcd < r ? xcd * (q + 1) + t : r * (q + 1) + (xcd - r) * q + t;
+}
+
 __device__ inline Blk hw_blk_xcd()
 {
     const int gx = (int)gridDim.x, gy = (int)gridDim.y, gz = (int)gridDim.z;
-    const int n = gx * gy * gz;
     const int hw = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
-    const int xcd = hw % kXcds, t = hw / kXcds;
-    const int q = n / kXcds, r = n % kXcds;  // XCDs below r take q+1 blocks
-    const int id = xcd < r ? xcd * (q + 1) + t : r * (q + 1) + (xcd - r) * q + t;
+    const int id = xcd_block(hw, gx * gy * gz);
     return Blk{id % gx, (id / gx) % gy, id / (gx * gy), gx, gy, gz};
+}
+
+__device__ inline Blk hw_blk(int xcd) { return xcd ? hw_blk_xcd() : hw_blk(); }
+
+// Marching kernels split the positions they march over into bk.gz chunks of
+// at most `planes` (the launch shape, fast.h Shape); chunk z of gz covers
+// [c0, c1] of [lo, hi] (empty when c0 > c1, which only a grid larger than
+// nchunks() gives).
+__host__ __device__ inline int nchunks(int npos, int planes) { return (npos + planes - 1) / planes; }
+
+__device__ inline void chunk_range(const Blk &b, int lo, int hi, int &c0, int &c1)
+{
+    const int n = hi - lo + 1;
+    const int per = (n + b.gz - 1) / b.gz;
+    c0 = lo + b.z * per;
+    c1 = min(c0 + per - 1, hi);
 }
 
 #define TIMED_LAUNCH(name, st, ...)                           \

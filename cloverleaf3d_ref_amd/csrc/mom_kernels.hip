@@ -37,7 +37,6 @@
 namespace clover {
 
 constexpr int MW = 8;       // waves per workgroup (wave 0 is the halo row)
-constexpr int MCHUNK = 64;  // march positions per workgroup
 
 template <typename S>
 struct Vel3 {
@@ -237,8 +236,9 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
 
     const int j = g.x_min - 1 + bk.x * 63 + lane;
     const int wp = wmin - 1 + bk.y * (MW - 1) + w;
-    const int m0 = mmin + bk.z * MCHUNK;
-    const int m1 = min(m0 + MCHUNK - 1, mmax + 1);
+    int m0, m1;
+    chunk_range(bk, mmin, mmax + 1, m0, m1);
+    if (m0 > m1) return;  // the whole workgroup: before any barrier
     // j >= x_min-1 and wp >= wmin-1 always; upper limits of the arrays read:
     // cells and the sweep's mass flux end at +2 across the sweep, vertices at +3
     const bool ok_cell = j <= g.x_max + 2 && wp <= wmax + 2;
@@ -330,11 +330,11 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
 }
 
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * MW) k_fast_mom_march(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+__global__ void __launch_bounds__(64 * MW) k_fast_mom_march(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
-    mom_march_body<DIR, S>(hw_blk_xcd(), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+    mom_march_body<DIR, S>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -357,8 +357,9 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     const T V0 = (T)geo.vol;
     const int j = g.x_min - 2 + bk.x * 60 + lane;
     const int k = g.y_min - 1 + bk.y * (MW - 1) + w;
-    const int l0 = g.z_min + bk.z * MCHUNK;
-    const int l1 = min(l0 + MCHUNK - 1, g.z_max + 1);
+    int l0, l1;
+    chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
+    if (l0 > l1) return;  // the whole workgroup: before any barrier
     // j >= x_min-2 and k >= y_min-1 always.  mass_flux_x ends at x_max+3 /
     // y_max+2, cells at x_max+2 / y_max+2, vertices at x_max+3 / y_max+3.
     const bool ok_face = j <= g.x_max + 3 && k <= g.y_max + 2;
@@ -423,18 +424,18 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
 }
 
 template <typename S>
-__global__ void __launch_bounds__(64 * MW) k_fast_mom_x(Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+__global__ void __launch_bounds__(64 * MW) k_fast_mom_x(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
              Vel3<S> v)
 {
-    mom_x_body<S>(hw_blk(), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+    mom_x_body<S>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 template <typename S>
 static hipError_t fast_advec_mom_t(const Lay &g, const Geo &m, int mode, int direction, const S *mass_flux_x,
                                    const S *vol_flux_x, const S *mass_flux_y, const S *vol_flux_y,
                                    const S *mass_flux_z, const S *vol_flux_z, const S *density1,
-                                   const void *const *vel_in, void *const *vel_out, hipStream_t st)
+                                   const void *const *vel_in, void *const *vel_out, const Shape &sh, hipStream_t st)
 {
     Vel3<S> v;
     for (int c = 0; c < 3; c++) {
@@ -444,16 +445,16 @@ static hipError_t fast_advec_mom_t(const Lay &g, const Geo &m, int mode, int dir
     const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
     const dim3 blk(64, MW, 1);
     if (direction == 1) {
-        const dim3 grid((nxn + 59) / 60, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
-        TIMED_LAUNCH("k_fast_mom_x", st, k_fast_mom_x<S>, grid, blk, 0, st, g, m, mode, mass_flux_x, vol_flux_x,
+        const dim3 grid((nxn + 59) / 60, (nyn + MW - 2) / (MW - 1), nchunks(nzn, sh.planes));
+        TIMED_LAUNCH("k_fast_mom_x", st, k_fast_mom_x<S>, grid, blk, 0, st, sh.xcd, g, m, mode, mass_flux_x, vol_flux_x,
                      vol_flux_y, vol_flux_z, density1, v);
     } else if (direction == 2) {
-        const dim3 grid((nxn + 62) / 63, (nzn + MW - 2) / (MW - 1), (nyn + MCHUNK - 1) / MCHUNK);
-        TIMED_LAUNCH("k_fast_mom_y", st, (k_fast_mom_march<2, S>), grid, blk, 0, st, g, m, mode, mass_flux_y,
+        const dim3 grid((nxn + 62) / 63, (nzn + MW - 2) / (MW - 1), nchunks(nyn, sh.planes));
+        TIMED_LAUNCH("k_fast_mom_y", st, (k_fast_mom_march<2, S>), grid, blk, 0, st, sh.xcd, g, m, mode, mass_flux_y,
                      vol_flux_x, vol_flux_y, vol_flux_z, density1, v);
     } else {
-        const dim3 grid((nxn + 62) / 63, (nyn + MW - 2) / (MW - 1), (nzn + MCHUNK - 1) / MCHUNK);
-        TIMED_LAUNCH("k_fast_mom_z", st, (k_fast_mom_march<3, S>), grid, blk, 0, st, g, m, mode, mass_flux_z,
+        const dim3 grid((nxn + 62) / 63, (nyn + MW - 2) / (MW - 1), nchunks(nzn, sh.planes));
+        TIMED_LAUNCH("k_fast_mom_z", st, (k_fast_mom_march<3, S>), grid, blk, 0, st, sh.xcd, g, m, mode, mass_flux_z,
                      vol_flux_x, vol_flux_y, vol_flux_z, density1, v);
     }
     return hipGetLastError();
@@ -463,7 +464,7 @@ hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect
                                  int direction, const void *mass_flux_x, const void *vol_flux_x,
                                  const void *mass_flux_y, const void *vol_flux_y, const void *mass_flux_z,
                                  const void *vol_flux_z, const void *density1, const void *const *vel_in,
-                                 void *const *vel_out, hipStream_t st)
+                                 void *const *vel_out, const Shape &sh, hipStream_t st)
 {
     int mode = 0;
     if (sweep_number == 1 && direction == 1) mode = 1;
@@ -477,41 +478,42 @@ hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect
     fast_advec_mom_t<T>(g, m, mode, direction, static_cast<const T *>(mass_flux_x),                             \
                         static_cast<const T *>(vol_flux_x), static_cast<const T *>(mass_flux_y),                \
                         static_cast<const T *>(vol_flux_y), static_cast<const T *>(mass_flux_z),                \
-                        static_cast<const T *>(vol_flux_z), static_cast<const T *>(density1), vel_in, vel_out, st)
+                        static_cast<const T *>(vol_flux_z), static_cast<const T *>(density1), vel_in, vel_out, sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
 
 // ---------------------------------------------------------------------------
 // Batched over many decks (batch.h): the new velocities go to X_W5..X_W7.
-__host__ __device__ inline void dims_mom(int dir, const Lay &g, int &gx, int &gy, int &gz)
+__host__ __device__ inline void dims_mom(int dir, const Lay &g, int planes, int &gx, int &gy, int &gz)
 {
     const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
     if (dir == 1) {
-        gx = (nxn + 59) / 60; gy = (nyn + MW - 2) / (MW - 1); gz = (nzn + MCHUNK - 1) / MCHUNK;
+        gx = (nxn + 59) / 60; gy = (nyn + MW - 2) / (MW - 1); gz = nchunks(nzn, planes);
     } else if (dir == 2) {
-        gx = (nxn + 62) / 63; gy = (nzn + MW - 2) / (MW - 1); gz = (nyn + MCHUNK - 1) / MCHUNK;
+        gx = (nxn + 62) / 63; gy = (nzn + MW - 2) / (MW - 1); gz = nchunks(nyn, planes);
     } else {
-        gx = (nxn + 62) / 63; gy = (nyn + MW - 2) / (MW - 1); gz = (nzn + MCHUNK - 1) / MCHUNK;
+        gx = (nxn + 62) / 63; gy = (nyn + MW - 2) / (MW - 1); gz = nchunks(nzn, planes);
     }
 }
 
-int batch_blocks_mom(int dir, const Lay &g)
+int batch_blocks_mom(int dir, const Lay &g, int planes)
 {
     int gx, gy, gz;
-    dims_mom(dir, g, gx, gy, gz);
+    dims_mom(dir, g, planes, gx, gy, gz);
     return gx * gy * gz;
 }
 
 template <int DIR, typename S>
 __global__ void __launch_bounds__(64 * MW)
-kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode)
+kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int planes, int xcd)
 {
-    const int d = deck_of(start, nd, (int)blockIdx.x);
+    const int hb = xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int d = deck_of(start, nd, hb);
     const DeckDev &D = decks[d];
     int gx, gy, gz;
-    dims_mom(DIR, D.g, gx, gy, gz);
-    const int lb = (int)blockIdx.x - start[d];
+    dims_mom(DIR, D.g, planes, gx, gy, gz);
+    const int lb = hb - start[d];
     const Blk bk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
     Vel3<S> v;
     v.in[0] = static_cast<const S *>(D.f[F_XVEL1]);
@@ -533,7 +535,7 @@ kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd,
 }
 
 hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
-                           int sweep_number, hipStream_t st)
+                           int sweep_number, const Shape &sh, hipStream_t st)
 {
     int mode = 0;
     if (sweep_number == 1 && dir == 1) mode = 1;
@@ -545,7 +547,8 @@ hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGri
     if (!mode || dir < 1 || dir > 3) return hipErrorInvalidValue;
     if (bg.total == 0) return hipSuccess;
     const dim3 grid(bg.total), blk(64, MW);
-#define L(D_, T_) TIMED_LAUNCH("kb_mom", st, (kb_mom<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode)
+#define L(D_, T_)                                                                                          \
+    TIMED_LAUNCH("kb_mom", st, (kb_mom<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode, sh.planes, sh.xcd)
     if (s == ST_F32) {
         if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float);
     } else {

@@ -9,7 +9,7 @@ each deck alone with the fused kernels.
 from __future__ import annotations
 
 import ctypes
-from typing import Dict, List, Sequence
+from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
 
@@ -72,3 +72,21 @@ class Ensemble:
 
     def device_bytes(self) -> int:
         return int(self._L.clover_ensemble_device_bytes(self._h))
+
+    def autotune(self) -> Tuple[float, float]:
+        """Pick the fastest launch shape of every batched marching kernel
+        (every field keeps its value).  Returns the summed kernel
+        milliseconds of one pass before and after."""
+        before, after = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._L.clover_ensemble_autotune(self._h, ctypes.byref(before), ctypes.byref(after)),
+                   "clover_ensemble_autotune")
+        return before.value, after.value
+
+    def shapes(self) -> Dict[str, Tuple[int, int]]:
+        out = {}
+        for k, name in enumerate(_lib.SHAPE_KERNELS):
+            planes, xcd = ctypes.c_int(), ctypes.c_int()
+            _lib.check(self._L.clover_ensemble_get_shape(self._h, k, ctypes.byref(planes), ctypes.byref(xcd)),
+                       "clover_ensemble_get_shape")
+            out[name] = (planes.value, xcd.value)
+        return out

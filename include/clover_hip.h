@@ -243,6 +243,33 @@ long long clover_chunk_device_bytes(const clover_chunk *c);
 int clover_chunk_set_kernel_timing(clover_chunk *c, int on);
 int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
 
+/* Launch shapes of the fused marching kernels (no counterpart in the
+ * reference, whose loops have no launch geometry).  A shape is `planes`,
+ * the positions a workgroup marches over (planes of the z marches, faces or
+ * nodes along the y/z advection sweeps; ignored by the x cell sweep), and
+ * `xcd`: 1 gives each of the 8 XCDs a contiguous eighth of the grid instead
+ * of the hardware's round-robin order.  A shape changes how the work is
+ * split, never a value.  Kernel indices: */
+#define CLOVER_SHAPE_VISC_DT 0
+#define CLOVER_SHAPE_PDV_PREDICT 1
+#define CLOVER_SHAPE_ACCELERATE 2
+#define CLOVER_SHAPE_PDV_CORRECT 3
+#define CLOVER_SHAPE_CELL_X 4
+#define CLOVER_SHAPE_CELL_Y 5
+#define CLOVER_SHAPE_CELL_Z 6
+#define CLOVER_SHAPE_MOM_X 7
+#define CLOVER_SHAPE_MOM_Y 8
+#define CLOVER_SHAPE_MOM_Z 9
+#define CLOVER_SHAPE_COUNT 10
+/* The autotuner: times every shaped kernel with every candidate shape (planes
+ * 16, 32, 64, 128 x both block orders) on this chunk's own arrays and keeps
+ * the fastest.  Candidates write only spare work arrays, so every field
+ * keeps its value.  ms_before/ms_after (may be null) get one pass of all
+ * shaped kernels with the old and the chosen shapes.  Fused kernels only. */
+int clover_chunk_autotune(clover_chunk *c, double *ms_before, double *ms_after);
+int clover_chunk_get_shape(const clover_chunk *c, int kernel, int *planes, int *xcd);
+int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd);
+
 /* ------------------------------------------------------------ ensembles */
 
 /* Many independent decks advanced together on one device: each deck is one
@@ -265,6 +292,12 @@ int clover_ensemble_state(clover_ensemble *e, int *step, double *time, double *d
 /* one field (0..18) of deck d as a dense Fortran array of its class */
 int clover_ensemble_fetch_field(clover_ensemble *e, int deck, int field, double *host);
 long long clover_ensemble_device_bytes(const clover_ensemble *e);
+/* The autotuner of the batched kernels: as clover_chunk_autotune, over all
+ * decks at once, with outputs sent to spare arrays by exchanging deck-table
+ * slots around the timed launches.  Drops the captured step graphs, which
+ * are captured again with the chosen shapes on the next run. */
+int clover_ensemble_autotune(clover_ensemble *e, double *ms_before, double *ms_after);
+int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd);
 
 #ifdef __cplusplus
 }
