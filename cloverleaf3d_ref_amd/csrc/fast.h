@@ -50,7 +50,7 @@ struct Shapes {
     Shape k[SK_COUNT];
 };
 // the autotuner's candidates for Shape::planes (each with both block orders)
-constexpr int kTunePlanes[] = {16, 32, 64, 128};
+constexpr int kTunePlanes[] = {8, 16, 32, 64, 128};
 constexpr int kTuneCount = sizeof(kTunePlanes) / sizeof(kTunePlanes[0]);
 
 // 64 positions per workgroup; the XCD order for the y/z momentum sweeps
@@ -76,7 +76,7 @@ hipError_t launch_fast_ideal_gas(Store s, const Lay &g, const void *density, con
 
 // viscosity_kernel + calc_dt_kernel on the same interior cells: writes
 // viscosity, and a block-partial minimum (FP64) of the cell dt into
-// partial[]; *nparts receives the number of partials (<= kMaxPartials).
+// partial[]; *nparts receives the number of partials (<= kMaxDtPartials).
 hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_safe, double dtu_safe,
                                double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
                                const void *density0, const void *pressure, const void *soundspeed, void *viscosity,

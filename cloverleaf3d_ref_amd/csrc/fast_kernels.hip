@@ -204,9 +204,9 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const int bx = (nx + BX - 1) / BX, by = (ny + BY - 1) / BY;
-    if (bx * by > kMaxPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
+    if (bx * by > kMaxDtPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
     int gz = nchunks(nz, sh.planes);
-    if (bx * by * gz > kMaxPartials) gz = kMaxPartials / (bx * by);
+    if (bx * by * gz > kMaxDtPartials) gz = kMaxDtPartials / (bx * by);
     *nparts = bx * by * gz;
     TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dtc_safe,
                  dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,

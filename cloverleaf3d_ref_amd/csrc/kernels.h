@@ -95,6 +95,9 @@ hipError_t launch_field_summary_partial(const Lay &g, const double *volume, cons
 hipError_t launch_sum5_final(const double *partial, int n, double *out, hipStream_t st);
 
 constexpr int kMaxPartials = 16384;
+// calc_dt's block minima (one double each) in the same buffers, which hold
+// 5 x kMaxPartials doubles for the summary's five sums
+constexpr int kMaxDtPartials = 5 * kMaxPartials;
 
 // Optional per-launch timing: while a timer is installed on the calling
 // thread, every launcher brackets each kernel with HIP events on its stream

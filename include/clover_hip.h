@@ -262,7 +262,7 @@ int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
 #define CLOVER_SHAPE_MOM_Z 9
 #define CLOVER_SHAPE_COUNT 10
 /* The autotuner: times every shaped kernel with every candidate shape (planes
- * 16, 32, 64, 128 x both block orders) on this chunk's own arrays and keeps
+ * 8, 16, 32, 64, 128 x both block orders) on this chunk's own arrays and keeps
  * the fastest.  Candidates write only spare work arrays, so every field
  * keeps its value.  ms_before/ms_after (may be null) get one pass of all
  * shaped kernels with the old and the chosen shapes.  Fused kernels only. */

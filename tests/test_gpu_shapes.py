@@ -71,7 +71,7 @@ def test_autotune_keeps_every_field(tiles):
     tuned = _run(deck, 10, tune_at=5)
     _same(base, tuned)
     for name, (planes, xcd) in tuned[3].items():
-        assert planes in (16, 32, 64, 128) and xcd in (0, 1), (name, planes, xcd)
+        assert planes in (8, 16, 32, 64, 128) and xcd in (0, 1), (name, planes, xcd)
 
 
 def test_shape_arguments_are_checked():
@@ -104,4 +104,4 @@ def test_ensemble_autotune_keeps_results(fp32):
     assert base == tuned
     assert base_state == tuned_state
     for name, (planes, xcd) in shapes.items():
-        assert planes in (16, 32, 64, 128) and xcd in (0, 1), (name, planes, xcd)
+        assert planes in (8, 16, 32, 64, 128) and xcd in (0, 1), (name, planes, xcd)
