@@ -107,6 +107,7 @@ def lib() -> ctypes.CDLL:
     L.clover_ensemble_device_bytes.argtypes = [vp]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     L.clover_chunk_autotune.argtypes = [vp, dp, dp]
+    L.clover_chunk_dt_control.argtypes = [vp, ip, ip, ip, dp, dp]
     L.clover_chunk_get_shape.argtypes = [vp, ctypes.c_int, ip, ip]
     L.clover_chunk_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.clover_ensemble_autotune.argtypes = [vp, dp, dp]

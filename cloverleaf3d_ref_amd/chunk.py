@@ -127,6 +127,15 @@ class Chunk:
             raise SmallTimestep("small timestep")
         return out
 
+    def dt_control(self):
+        """(control, jdt, kdt, x_pos, y_pos) of the last step's dt, as the
+        reference's step line reports them (timestep.f90:115-116)."""
+        ctl, j, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        x, y = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._L.clover_chunk_dt_control(self._h, ctypes.byref(ctl), ctypes.byref(j), ctypes.byref(k),
+                                                   ctypes.byref(x), ctypes.byref(y)), "clover_chunk_dt_control")
+        return ctl.value, j.value, k.value, x.value, y.value
+
     def state(self):
         step, time, dt = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         self._L.clover_chunk_state(self._h, ctypes.byref(step), ctypes.byref(time), ctypes.byref(dt))

@@ -128,6 +128,7 @@ struct clover_chunk {
     bool halo_check;     // CLOVER_CHUNK_HALO_CHECK
     bool gas_fresh;      // pressure/soundspeed interiors hold ideal_gas(density0, energy0)
     Shapes shapes;       // launch shapes of the fused kernels (autotuner)
+    int dt_tile;         // the tile whose dt the last timestep took (-1: none yet)
     Geo geo;             // the uniform mesh, for the fused kernels
 };
 
@@ -467,7 +468,10 @@ static int timestep(clover_chunk *c)
         // timestep.f90:93-100: the minimum over tiles, ties to the last tile
         for (int ti = 0; ti < T; ti++) {
             const double dtlp = c->h_scalar[8 * ti];
-            if (dtlp <= c->dt) c->dt = dtlp;
+            if (dtlp <= c->dt) {
+                c->dt = dtlp;
+                c->dt_tile = ti;
+            }
         }
     }
     double m = c->dt;
@@ -864,6 +868,7 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         c->store = (flags & CLOVER_CHUNK_FP32) ? ST_F32 : ST_F64;
         c->halo_check = (flags & CLOVER_CHUNK_HALO_CHECK) != 0;
         c->shapes = default_shapes();
+        c->dt_tile = -1;
         // the uniform mesh as initialise_chunk.f90:37-39 and
         // initialise_chunk_kernel.f90:151-178 compute it
         c->geo.dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
@@ -1129,6 +1134,28 @@ int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd)
     if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
     if (planes < 1 || (xcd != 0 && xcd != 1)) throw Error(E_ARG, "planes must be positive, xcd 0 or 1");
     c->shapes.k[kernel] = Shape{planes, xcd};
+    CLOVER_API_END
+}
+
+int clover_chunk_dt_control(const clover_chunk *c, int *control, int *jdt, int *kdt, double *x_pos, double *y_pos)
+{
+    CLOVER_API_BEGIN
+    if (!c || !control || !jdt || !kdt || !x_pos || !y_pos) throw Error(E_ARG, "null argument");
+    if (c->dt_tile < 0) throw Error(E_STATE, "no timestep taken yet");
+    const Tile &t = c->tiles[c->dt_tile];
+    // calc_dt_kernel.f90:97 and 154-160: jkl_control = 1.1 (a default-real
+    // literal), so the control is INT(10.01 * 0.1...) = 1 ('sound'),
+    // j = MOD(1, x_max) and k = INT(1 + 1.0/x_max), positions cellx(j) and
+    // celly(k) of the tile (tile-local x_max; arrays from x_min-2 = -1)
+    const double jkl = (double)1.1f;
+    *control = (int)((double)10.01f * (jkl - (double)(int)jkl));
+    const int xm = t.nx;
+    const int j = ((int)jkl) % xm;
+    const int k = (int)(1.0 + ((double)(int)jkl) / (double)xm);
+    *jdt = j;
+    *kdt = k;
+    *x_pos = t.h_cellx.at((size_t)(j + 1));
+    *y_pos = t.h_celly.at((size_t)(k + 1));
     CLOVER_API_END
 }
 

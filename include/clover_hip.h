@@ -213,6 +213,10 @@ int clover_chunk_run(clover_chunk *c, int max_steps, clover_summary *rows, int m
 
 /* step count, simulated time, last dt */
 int clover_chunk_state(const clover_chunk *c, int *step, double *time, double *dt);
+/* The rest of timestep.f90:115-116's step line for the last step: dt_control
+ * (1 'sound'), jdt, kdt, x_pos and y_pos as calc_dt_kernel.f90:154-160
+ * reports them, from the tile whose dt was taken (timestep.f90:93-100). */
+int clover_chunk_dt_control(const clover_chunk *c, int *control, int *jdt, int *kdt, double *x_pos, double *y_pos);
 int clover_chunk_tiles(const clover_chunk *c, int *ntiles);
 /* tile t: interior extents, global offsets, neighbours[6] */
 int clover_chunk_tile_info(const clover_chunk *c, int t, int *dims3, int *offs3, int *neighbours6);
