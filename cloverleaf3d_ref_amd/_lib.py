@@ -108,10 +108,10 @@ def lib() -> ctypes.CDLL:
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     L.clover_chunk_autotune.argtypes = [vp, dp, dp]
     L.clover_chunk_dt_control.argtypes = [vp, ip, ip, ip, dp, dp]
-    L.clover_chunk_get_shape.argtypes = [vp, ctypes.c_int, ip, ip]
-    L.clover_chunk_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.clover_chunk_get_shape.argtypes = [vp, ctypes.c_int, ip, ip, ip]
+    L.clover_chunk_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.clover_ensemble_autotune.argtypes = [vp, dp, dp]
-    L.clover_ensemble_get_shape.argtypes = [vp, ctypes.c_int, ip, ip]
+    L.clover_ensemble_get_shape.argtypes = [vp, ctypes.c_int, ip, ip, ip]
     L.clover_hip_fetch.argtypes = [vp]
     L.clover_hip_push.argtypes = [vp]
     L.clover_hip_forget.argtypes = [vp]

@@ -179,18 +179,19 @@ class Chunk:
                    "clover_chunk_autotune")
         return before.value, after.value
 
-    def shapes(self) -> Dict[str, Tuple[int, int]]:
-        """kernel -> (planes per workgroup, XCD-contiguous block order)"""
+    def shapes(self) -> Dict[str, Tuple[int, int, int]]:
+        """kernel -> (planes per workgroup, XCD-contiguous block order,
+        workgroup width)"""
         out = {}
         for k, name in enumerate(_lib.SHAPE_KERNELS):
-            planes, xcd = ctypes.c_int(), ctypes.c_int()
-            _lib.check(self._L.clover_chunk_get_shape(self._h, k, ctypes.byref(planes), ctypes.byref(xcd)),
-                       "clover_chunk_get_shape")
-            out[name] = (planes.value, xcd.value)
+            planes, xcd, bx = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _lib.check(self._L.clover_chunk_get_shape(self._h, k, ctypes.byref(planes), ctypes.byref(xcd),
+                                                      ctypes.byref(bx)), "clover_chunk_get_shape")
+            out[name] = (planes.value, xcd.value, bx.value)
         return out
 
-    def set_shape(self, kernel: str, planes: int, xcd: int) -> None:
-        _lib.check(self._L.clover_chunk_set_shape(self._h, _lib.SHAPE_KERNELS.index(kernel), planes, xcd),
+    def set_shape(self, kernel: str, planes: int, xcd: int, block_x: int = 64) -> None:
+        _lib.check(self._L.clover_chunk_set_shape(self._h, _lib.SHAPE_KERNELS.index(kernel), planes, xcd, block_x),
                    "clover_chunk_set_shape")
 
     def profile(self) -> Dict[str, tuple]:

@@ -729,8 +729,11 @@ static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b)
 
 // Every shaped kernel, every candidate shape: positions per workgroup along
 // its march from kTunePlanes (the x cell sweep has no march) times the two
-// block orders; the fastest is kept.  Shapes split the work differently but
-// never change a value, so later steps give the same bits whatever is chosen.
+// block orders, and for the z marches times the workgroup widths kTuneBx;
+// the fastest is kept.  Shapes split the work differently but never change a
+// value, so later steps give the same bits whatever is chosen.
+static bool z_march(int k) { return k == SK_VISC_DT || k == SK_PDV_PREDICT || k == SK_ACCELERATE || k == SK_PDV_CORRECT; }
+
 static void autotune(clover_chunk *c, double *before, double *after)
 {
     hipEvent_t a = nullptr, b = nullptr;
@@ -743,17 +746,19 @@ static void autotune(clover_chunk *c, double *before, double *after)
         Shape best_shape = start;
         t0 += best;
         for (int planes : kTunePlanes)
-            for (int xcd = 0; xcd < 2; xcd++) {
-                if (k == SK_CELL_X && planes != start.planes) continue;
-                const Shape cand{planes, xcd};
-                if (cand.planes == start.planes && cand.xcd == start.xcd) continue;
-                c->shapes.k[k] = cand;
-                const double ms = tune_time(c, k, a, b);
-                if (ms < best) {
-                    best = ms;
-                    best_shape = cand;
+            for (int xcd = 0; xcd < 2; xcd++)
+                for (int bx : kTuneBx) {
+                    if (k == SK_CELL_X && planes != start.planes) continue;
+                    if (!z_march(k) && bx != start.bx) continue;
+                    const Shape cand{planes, xcd, bx};
+                    if (cand.planes == start.planes && cand.xcd == start.xcd && cand.bx == start.bx) continue;
+                    c->shapes.k[k] = cand;
+                    const double ms = tune_time(c, k, a, b);
+                    if (ms < best) {
+                        best = ms;
+                        best_shape = cand;
+                    }
                 }
-            }
         c->shapes.k[k] = best_shape;
         t1 += best;
     }
@@ -1117,23 +1122,27 @@ int clover_chunk_autotune(clover_chunk *c, double *ms_before, double *ms_after)
     CLOVER_API_END
 }
 
-int clover_chunk_get_shape(const clover_chunk *c, int kernel, int *planes, int *xcd)
+int clover_chunk_get_shape(const clover_chunk *c, int kernel, int *planes, int *xcd, int *block_x)
 {
     CLOVER_API_BEGIN
-    if (!c || !planes || !xcd) throw Error(E_ARG, "null argument");
+    if (!c || !planes || !xcd || !block_x) throw Error(E_ARG, "null argument");
     if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
     *planes = c->shapes.k[kernel].planes;
     *xcd = c->shapes.k[kernel].xcd;
+    *block_x = c->shapes.k[kernel].bx;
     CLOVER_API_END
 }
 
-int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd)
+int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd, int block_x)
 {
     CLOVER_API_BEGIN
     if (!c) throw Error(E_ARG, "null chunk");
     if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
     if (planes < 1 || (xcd != 0 && xcd != 1)) throw Error(E_ARG, "planes must be positive, xcd 0 or 1");
-    c->shapes.k[kernel] = Shape{planes, xcd};
+    if (block_x != 64 && block_x != 128 && block_x != 256 && block_x != 512)
+        throw Error(E_ARG, "block_x must be 64, 128, 256 or 512");
+    if (block_x != 64 && !z_march(kernel)) throw Error(E_ARG, "only the z marches take a workgroup width");
+    c->shapes.k[kernel] = Shape{planes, xcd, block_x};
     CLOVER_API_END
 }
 

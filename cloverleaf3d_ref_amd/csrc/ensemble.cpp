@@ -330,7 +330,7 @@ void autotune(clover_ensemble *e, double *before, double *after)
         for (int planes : kTunePlanes)
             for (int xcd = 0; xcd < 2; xcd++) {
                 if (si == SK_CELL_X && planes != start.planes) continue;
-                const Shape cand{planes, xcd};
+                const Shape cand{planes, xcd, start.bx};  // the batched workgroups are fixed at 64 x 4
                 if (cand.planes == start.planes && cand.xcd == start.xcd) continue;
                 const double ms = tune_time(e, bk, cand, a, b);
                 if (ms < best) {
@@ -588,13 +588,14 @@ int clover_ensemble_autotune(clover_ensemble *e, double *ms_before, double *ms_a
     CLOVER_API_END
 }
 
-int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd)
+int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd, int *block_x)
 {
     CLOVER_API_BEGIN
-    if (!e || !planes || !xcd) throw Error(E_ARG, "null argument");
+    if (!e || !planes || !xcd || !block_x) throw Error(E_ARG, "null argument");
     if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
     *planes = e->shapes.k[kernel].planes;
     *xcd = e->shapes.k[kernel].xcd;
+    *block_x = e->shapes.k[kernel].bx;
     CLOVER_API_END
 }
 

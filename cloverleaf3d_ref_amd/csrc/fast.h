@@ -45,7 +45,15 @@ enum ShapeKernel : int {
 struct Shape {
     int planes;
     int xcd;
+    int bx;  // z marches: threads along x per workgroup (64, 128, 256 or 512; see block_of)
 };
+
+// Workgroup of a z march for shape sh: bx threads along x and 256/bx rows
+// (one row from 256 up), so wider workgroups keep more x neighbours inside
+// one workgroup.  The other kernels have fixed workgroups.
+constexpr int kMaxBlock = 512;
+inline dim3 block_of(const Shape &sh) { return dim3(sh.bx, sh.bx >= 256 ? 1 : 256 / sh.bx, 1); }
+constexpr int kTuneBx[] = {64, 128, 256, 512};
 struct Shapes {
     Shape k[SK_COUNT];
 };
@@ -53,11 +61,12 @@ struct Shapes {
 constexpr int kTunePlanes[] = {8, 16, 32, 64, 128};
 constexpr int kTuneCount = sizeof(kTunePlanes) / sizeof(kTunePlanes[0]);
 
-// 64 positions per workgroup; the XCD order for the y/z momentum sweeps
+// 64 positions per workgroup of 64 x 4; the XCD order for the y/z momentum
+// sweeps
 inline Shapes default_shapes()
 {
     Shapes s;
-    for (int i = 0; i < SK_COUNT; i++) s.k[i] = Shape{64, i == SK_MOM_Y || i == SK_MOM_Z};
+    for (int i = 0; i < SK_COUNT; i++) s.k[i] = Shape{64, i == SK_MOM_Y || i == SK_MOM_Z, 64};
     return s;
 }
 

@@ -58,8 +58,8 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
                double *__restrict__ partial)
 {
     using T = S;  // arithmetic type of the form
-    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
+    const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
+    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
     int l0, l1;
     chunk_range(bk, g.z_min, g.z_max, l0, l1);
     const bool in = j <= g.x_max && k <= g.y_max && l0 <= l1;  // every thread reaches the reduction
@@ -173,20 +173,21 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             p_c = p_hi;
         }
     }
-    __shared__ double red[BX * BY / 64];
-    const int tid = threadIdx.y * BX + threadIdx.x;
+    __shared__ double red[kMaxBlock / 64];
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const int nwaves = (int)(blockDim.x * blockDim.y) / 64;
     const double w = wave_min(dtmin_thread);
     if ((tid & 63) == 0) red[tid >> 6] = w;
     __syncthreads();
     if (tid == 0) {
         double b = red[0];
-        for (int q = 1; q < BX * BY / 64; q++) b = dmin(b, red[q]);
+        for (int q = 1; q < nwaves; q++) b = dmin(b, red[q]);
         partial[(bk.z * bk.gy + bk.y) * bk.gx + bk.x] = b;
     }
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_visc_dt(int xcd, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
+__global__ void __launch_bounds__(kMaxBlock) k_fast_visc_dt(int xcd, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
                const S *__restrict__ soundspeed, S *__restrict__ viscosity,
                const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
@@ -203,12 +204,13 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
                                int *nparts, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
-    const int bx = (nx + BX - 1) / BX, by = (ny + BY - 1) / BY;
+    const dim3 blk = block_of(sh);
+    const int bx = (nx + (int)blk.x - 1) / (int)blk.x, by = (ny + (int)blk.y - 1) / (int)blk.y;
     if (bx * by > kMaxDtPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
     int gz = nchunks(nz, sh.planes);
     if (bx * by * gz > kMaxDtPartials) gz = kMaxDtPartials / (bx * by);
     *nparts = bx * by * gz;
-    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dtc_safe,
+    TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), blk, 0, st, sh.xcd, g, m, dtc_safe,
                  dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,
                  yvel0, zvel0, partial);
     return hipGetLastError();
@@ -227,8 +229,8 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
                    const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                    const S *__restrict__ zvel0)
 {
-    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
+    const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
+    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
     if (j > g.x_max || k > g.y_max) return;
     int l0, l1;
     chunk_range(bk, g.z_min, g.z_max, l0, l1);
@@ -286,7 +288,7 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_pdv_predict(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
+__global__ void __launch_bounds__(kMaxBlock) k_fast_pdv_predict(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
                    S *__restrict__ pressure, const S *__restrict__ viscosity,
                    const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                    const S *__restrict__ zvel0)
@@ -300,8 +302,9 @@ static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, cons
                                    const S *xvel0, const S *yvel0, const S *zvel0, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
-    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nchunks(nz, sh.planes));
-    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict<S>, grid, dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dt, density0,
+    const dim3 blk = block_of(sh);
+    const dim3 grid((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict<S>, grid, blk, 0, st, sh.xcd, g, m, dt, density0,
                  energy0, pressure, viscosity, xvel0, yvel0, zvel0);
     return hipGetLastError();
 }
@@ -315,8 +318,8 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
                   const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
                   S *__restrict__ yvel1, S *__restrict__ zvel1)
 {
-    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
+    const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
+    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
     if (j > g.x_max + 1 || k > g.y_max + 1) return;
     int l0, l1;
     chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
@@ -379,7 +382,7 @@ __device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, dou
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_accelerate(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
+__global__ void __launch_bounds__(kMaxBlock) k_fast_accelerate(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
                   const S *__restrict__ viscosity, const S *__restrict__ xvel0,
                   const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
                   S *__restrict__ yvel1, S *__restrict__ zvel1)
@@ -394,8 +397,9 @@ static hipError_t fast_accelerate_t(const Lay &g, const Geo &m, double dt, const
                                   S *zvel1, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
-    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nchunks(nz, sh.planes));
-    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate<S>, grid, dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dt, density0,
+    const dim3 blk = block_of(sh);
+    const dim3 grid((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate<S>, grid, blk, 0, st, sh.xcd, g, m, dt, density0,
                  pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
     return hipGetLastError();
 }
@@ -416,8 +420,8 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
                         S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
                         S *__restrict__ vol_flux_z)
 {
-    const int j = g.x_min + (int)(bk.x * BX + threadIdx.x);
-    const int k = g.y_min + (int)(bk.y * BY + threadIdx.y);
+    const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
+    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
     if (j > g.x_max + 1 || k > g.y_max + 1) return;
     int l0, l1;
     chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
@@ -518,7 +522,7 @@ __device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo 
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY) k_fast_pdv_correct_flux(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
+__global__ void __launch_bounds__(kMaxBlock) k_fast_pdv_correct_flux(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
                         const S *__restrict__ energy0, S *__restrict__ energy1,
                         const S *__restrict__ pressure, const S *__restrict__ viscosity,
                         const S *__restrict__ xvel0, const S *__restrict__ xvel1,
@@ -539,8 +543,9 @@ static hipError_t fast_pdv_correct_flux_t(const Lay &g, const Geo &m, double dt,
                                         S *vol_flux_y, S *vol_flux_z, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
-    const dim3 grid((nx + BX - 1) / BX, (ny + BY - 1) / BY, nchunks(nz, sh.planes));
-    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux<S>, grid, dim3(BX, BY, 1), 0, st, sh.xcd, g, m, dt,
+    const dim3 blk = block_of(sh);
+    const dim3 grid((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux<S>, grid, blk, 0, st, sh.xcd, g, m, dt,
                  density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1,
                  vol_flux_x, vol_flux_y, vol_flux_z);
     return hipGetLastError();

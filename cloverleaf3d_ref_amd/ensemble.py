@@ -82,11 +82,11 @@ class Ensemble:
                    "clover_ensemble_autotune")
         return before.value, after.value
 
-    def shapes(self) -> Dict[str, Tuple[int, int]]:
+    def shapes(self) -> Dict[str, Tuple[int, int, int]]:
         out = {}
         for k, name in enumerate(_lib.SHAPE_KERNELS):
-            planes, xcd = ctypes.c_int(), ctypes.c_int()
-            _lib.check(self._L.clover_ensemble_get_shape(self._h, k, ctypes.byref(planes), ctypes.byref(xcd)),
-                       "clover_ensemble_get_shape")
-            out[name] = (planes.value, xcd.value)
+            planes, xcd, bx = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _lib.check(self._L.clover_ensemble_get_shape(self._h, k, ctypes.byref(planes), ctypes.byref(xcd),
+                                                         ctypes.byref(bx)), "clover_ensemble_get_shape")
+            out[name] = (planes.value, xcd.value, bx.value)
         return out

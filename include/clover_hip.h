@@ -250,10 +250,12 @@ int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
 /* Launch shapes of the fused marching kernels (no counterpart in the
  * reference, whose loops have no launch geometry).  A shape is `planes`,
  * the positions a workgroup marches over (planes of the z marches, faces or
- * nodes along the y/z advection sweeps; ignored by the x cell sweep), and
+ * nodes along the y/z advection sweeps; ignored by the x cell sweep),
  * `xcd`: 1 gives each of the 8 XCDs a contiguous eighth of the grid instead
- * of the hardware's round-robin order.  A shape changes how the work is
- * split, never a value.  Kernel indices: */
+ * of the hardware's round-robin order, and `block_x`: the workgroup width
+ * of the four z marches (64, 128, 256 or 512 threads along x, with 256/64,
+ * 256/128 or 1 rows; the other kernels' workgroups are fixed, block_x 64).
+ * A shape changes how the work is split, never a value.  Kernel indices: */
 #define CLOVER_SHAPE_VISC_DT 0
 #define CLOVER_SHAPE_PDV_PREDICT 1
 #define CLOVER_SHAPE_ACCELERATE 2
@@ -266,13 +268,14 @@ int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
 #define CLOVER_SHAPE_MOM_Z 9
 #define CLOVER_SHAPE_COUNT 10
 /* The autotuner: times every shaped kernel with every candidate shape (planes
- * 8, 16, 32, 64, 128 x both block orders) on this chunk's own arrays and keeps
+ * 8, 16, 32, 64, 128 x both block orders x, for the z marches, the four
+ * workgroup widths) on this chunk's own arrays and keeps
  * the fastest.  Candidates write only spare work arrays, so every field
  * keeps its value.  ms_before/ms_after (may be null) get one pass of all
  * shaped kernels with the old and the chosen shapes.  Fused kernels only. */
 int clover_chunk_autotune(clover_chunk *c, double *ms_before, double *ms_after);
-int clover_chunk_get_shape(const clover_chunk *c, int kernel, int *planes, int *xcd);
-int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd);
+int clover_chunk_get_shape(const clover_chunk *c, int kernel, int *planes, int *xcd, int *block_x);
+int clover_chunk_set_shape(clover_chunk *c, int kernel, int planes, int xcd, int block_x);
 
 /* ------------------------------------------------------------ ensembles */
 
@@ -301,7 +304,7 @@ long long clover_ensemble_device_bytes(const clover_ensemble *e);
  * slots around the timed launches.  Drops the captured step graphs, which
  * are captured again with the chosen shapes on the next run. */
 int clover_ensemble_autotune(clover_ensemble *e, double *ms_before, double *ms_after);
-int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd);
+int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd, int *block_x);
 
 #ifdef __cplusplus
 }

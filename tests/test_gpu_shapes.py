@@ -26,8 +26,8 @@ def _run(deck, steps, fp32=False, shapes=None, tune_at=None):
     print(f"[gpu] shapes {deck.x_cells}x{deck.y_cells}x{deck.z_cells} tiles={deck.tiles_per_chunk} "
           f"fp32={fp32} shapes={shapes} tune_at={tune_at}", flush=True)
     with Chunk(deck, fp32=fp32) as c:
-        for k, (planes, xcd) in (shapes or {}).items():
-            c.set_shape(k, planes, xcd)
+        for k, shape in (shapes or {}).items():
+            c.set_shape(k, *shape)
         dts = []
         for s in range(steps):
             if s == tune_at:
@@ -52,9 +52,9 @@ def _same(a, b):
         assert np.array_equal(f0[key][inner].view(np.uint64), f1[key][inner].view(np.uint64)), key
 
 
-ODD_SHAPES = {"visc_dt": (3, 1), "pdv_predict": (1, 0), "accelerate": (7, 1), "pdv_correct": (200, 0),
-              "cell_x": (64, 1), "cell_y": (5, 1), "cell_z": (2, 0), "mom_x": (1, 1), "mom_y": (9, 0),
-              "mom_z": (3, 1)}
+ODD_SHAPES = {"visc_dt": (3, 1, 512), "pdv_predict": (1, 0, 128), "accelerate": (7, 1, 256),
+              "pdv_correct": (200, 0, 512), "cell_x": (64, 1), "cell_y": (5, 1), "cell_z": (2, 0), "mom_x": (1, 1),
+              "mom_y": (9, 0), "mom_z": (3, 1)}
 
 
 @pytest.mark.parametrize("fp32", [False, True])
@@ -70,17 +70,20 @@ def test_autotune_keeps_every_field(tiles):
     base = _run(deck, 10)
     tuned = _run(deck, 10, tune_at=5)
     _same(base, tuned)
-    for name, (planes, xcd) in tuned[3].items():
-        assert planes in (8, 16, 32, 64, 128) and xcd in (0, 1), (name, planes, xcd)
+    for name, (planes, xcd, bx) in tuned[3].items():
+        assert planes in (8, 16, 32, 64, 128) and xcd in (0, 1) and bx in (64, 128, 256, 512), name
 
 
 def test_shape_arguments_are_checked():
     deck = parse_deck(ODD_DECK.format(nx=8, ny=8, nz=8, steps=1, tiles=1, td="3d"))
     with Chunk(deck) as c:
         L = _lib.lib()
-        assert L.clover_chunk_set_shape(c._h, len(_lib.SHAPE_KERNELS), 64, 0) != 0
-        assert L.clover_chunk_set_shape(c._h, 0, 0, 0) != 0
-        assert L.clover_chunk_set_shape(c._h, 0, 64, 2) != 0
+        assert L.clover_chunk_set_shape(c._h, len(_lib.SHAPE_KERNELS), 64, 0, 64) != 0
+        assert L.clover_chunk_set_shape(c._h, 0, 0, 0, 64) != 0
+        assert L.clover_chunk_set_shape(c._h, 0, 64, 2, 64) != 0
+        assert L.clover_chunk_set_shape(c._h, 0, 64, 0, 96) != 0
+        assert L.clover_chunk_set_shape(c._h, _lib.SHAPE_KERNELS.index("mom_y"), 64, 0, 256) != 0
+        assert L.clover_chunk_set_shape(c._h, 0, 64, 0, 512) == 0
     with Chunk(deck, per_call=True) as c:
         with pytest.raises(_lib.CloverError):
             c.autotune()
@@ -103,5 +106,5 @@ def test_ensemble_autotune_keeps_results(fp32):
         shapes = e.shapes()
     assert base == tuned
     assert base_state == tuned_state
-    for name, (planes, xcd) in shapes.items():
-        assert planes in (8, 16, 32, 64, 128) and xcd in (0, 1), (name, planes, xcd)
+    for name, (planes, xcd, bx) in shapes.items():
+        assert planes in (8, 16, 32, 64, 128) and xcd in (0, 1) and bx == 64, name
