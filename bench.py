@@ -57,7 +57,8 @@ ARRAYS_PER_CELL = {
     # z run in sweeps 1 and 3, y always in sweep 2; the third sweep also
     # writes pressure and soundspeed (the next step's ideal_gas), one of every
     # two x and z sweeps.
-    "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accelerate": 9, "k_fast_pdv_correct_flux": 15,
+    # accel_pdv reads rho0, e0, p, q, u0, v0, w0 and writes u1, v1, w1, rho1, e1 and the three volume fluxes
+    "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accel_pdv": 15,
     "k_fast_cell_x": 8, "k_fast_cell_y": 7, "k_fast_cell_z": 8,
     "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
     "k_fast_field_summary": 6, "k_fast_ideal_gas": 4,
@@ -69,7 +70,7 @@ def kernel_domain_cells(name: str, n: int) -> int:
     if name in ("k_cell_vols", "k_mom_vols"):
         return (n + 4) ** 3
     if name in ("k_accelerate", "k_reset_field", "k_flux_calc", "k_mom_flux", "k_mom_update", "k_node_flux",
-                "k_node_mass", "k_fast_accelerate", "k_fast_pdv_correct_flux", "k_fast_mom_x", "k_fast_mom_y",
+                "k_node_mass", "k_fast_accel_pdv", "k_fast_mom_x", "k_fast_mom_y",
                 "k_fast_mom_z"):
         return (n + 1) ** 3
     return n ** 3

@@ -62,7 +62,7 @@ def header_functions() -> List[str]:
 
 
 # launch-shape kernel indices (CLOVER_SHAPE_* of clover_hip.h)
-SHAPE_KERNELS = ["visc_dt", "pdv_predict", "accelerate", "pdv_correct", "cell_x", "cell_y", "cell_z",
+SHAPE_KERNELS = ["visc_dt", "pdv_predict", "accel_pdv", "cell_x", "cell_y", "cell_z",
                  "mom_x", "mom_y", "mom_z"]
 
 

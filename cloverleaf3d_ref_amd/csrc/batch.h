@@ -37,7 +37,7 @@ struct BatchGrid {
 // Workgroups one deck needs in each batched kernel (same shapes as the
 // single-deck launches).
 enum BatchKernel : int {
-    BK_IDEAL_GAS, BK_VISC_DT, BK_PDV_PREDICT, BK_ACCELERATE, BK_PDV_CORRECT, BK_SUMMARY,
+    BK_IDEAL_GAS, BK_VISC_DT, BK_PDV_PREDICT, BK_ACCEL_PDV, BK_SUMMARY,
     BK_CELL_X, BK_CELL_Y, BK_CELL_Z, BK_MOM_X, BK_MOM_Y, BK_MOM_Z, BK_COUNT
 };
 int batch_blocks(BatchKernel k, const Lay &g, int planes);
@@ -61,9 +61,8 @@ hipError_t batch_ideal_gas(Store s, const DeckDev *decks, int nd, const BatchGri
 hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, double g_small,
                          double *partial, hipStream_t st);
 hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st);
-hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st);
-hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh,
-                                  hipStream_t st);
+// accelerate + PdV corrector + flux_calc (launch_fast_accel_pdv)
+hipError_t batch_accel_pdv(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st);
 // five partials per workgroup at partial[5*workgroup]
 hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double *partial,
                                hipStream_t st);

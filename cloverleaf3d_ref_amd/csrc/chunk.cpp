@@ -500,18 +500,6 @@ static void pdv(clover_chunk *c, bool predict)
         fused_halo(c, FH_PRESSURE);
         return;
     }
-    if (c->fast) {
-        // PdV corrector fused with flux_calc, which follows it (hydro.f90:58-60)
-        Phase ph(c, P_PDV);
-        for (auto &t : c->tiles)
-            hip_check(launch_fast_pdv_correct_flux(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
-                                                   t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
-                                                   t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1],
-                                                   t.f[F_ZVEL0], t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y],
-                                                   t.f[F_VOL_FLUX_Z], c->shapes.k[SK_PDV_CORRECT], c->st),
-                      "PdV correct + flux_calc");
-        return;
-    }
     {
         Phase ph(c, P_PDV);
         for (auto &t : c->tiles)
@@ -674,16 +662,11 @@ static void tune_launch(clover_chunk *c, int k)
             e = launch_fast_pdv_predict(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[F_ENERGY0], t.f[X_W3],
                                         t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], sh, c->st);
             break;
-        case SK_ACCELERATE:
-            e = launch_fast_accelerate(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[F_PRESSURE],
-                                       t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0], t.f[X_W5],
-                                       t.f[X_W6], t.f[X_W7], sh, c->st);
-            break;
-        case SK_PDV_CORRECT:
-            e = launch_fast_pdv_correct_flux(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[X_W1],
-                                             t.f[F_ENERGY0], t.f[X_W2], t.f[F_PRESSURE], t.f[F_VISCOSITY],
-                                             t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1], t.f[F_ZVEL0],
-                                             t.f[F_ZVEL1], t.f[X_W3], t.f[X_W4], t.f[X_W5], sh, c->st);
+        case SK_ACCEL_PDV:
+            e = launch_fast_accel_pdv(c->store, t.g, c->geo, c->dtold, t.f[F_DENSITY0], t.f[X_W1], t.f[F_ENERGY0],
+                                      t.f[X_W2], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[X_W5],
+                                      t.f[F_YVEL0], t.f[X_W6], t.f[F_ZVEL0], t.f[X_W7], t.f[X_W3], t.f[X_W4],
+                                      t.f[X_W3], sh, c->st);
             break;
         case SK_CELL_X:
         case SK_CELL_Y:
@@ -732,7 +715,7 @@ static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b)
 // block orders, and for the z marches times the workgroup widths kTuneBx;
 // the fastest is kept.  Shapes split the work differently but never change a
 // value, so later steps give the same bits whatever is chosen.
-static bool z_march(int k) { return k == SK_VISC_DT || k == SK_PDV_PREDICT || k == SK_ACCELERATE || k == SK_PDV_CORRECT; }
+static bool z_march(int k) { return k == SK_VISC_DT || k == SK_PDV_PREDICT; }
 
 static void autotune(clover_chunk *c, double *before, double *after)
 {
@@ -771,23 +754,28 @@ static void autotune(clover_chunk *c, double *before, double *after)
 static void do_step(clover_chunk *c)
 {
     pdv(c, true);
-    {
+    if (c->fast) {
+        // accelerate, the PdV corrector and flux_calc in one march
         Phase ph(c, P_ACCEL);
-        for (auto &t : c->tiles) {
-            if (c->fast)
-                hip_check(launch_fast_accelerate(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_PRESSURE],
-                                                 t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
-                                                 t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1], c->shapes.k[SK_ACCELERATE], c->st),
-                          "accelerate");
-            else
+        for (auto &t : c->tiles)
+            hip_check(launch_fast_accel_pdv(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
+                                            t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
+                                            t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1], t.f[F_ZVEL0],
+                                            t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y], t.f[F_VOL_FLUX_Z],
+                                            c->shapes.k[SK_ACCEL_PDV], c->st),
+                      "accelerate + PdV correct + flux_calc");
+    } else {
+        {
+            Phase ph(c, P_ACCEL);
+            for (auto &t : c->tiles)
                 hip_check(launch_accelerate(t.g, c->dt, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.f[X_VOLUME],
                                             t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0],
                                             t.f[F_YVEL0], t.f[F_ZVEL0], t.f[F_XVEL1], t.f[F_YVEL1], t.f[F_ZVEL1],
                                             c->st),
                           "accelerate");
         }
+        pdv(c, false);
     }
-    pdv(c, false);
     if (!c->fast) {
         Phase ph(c, P_FLUX);
         for (auto &t : c->tiles)

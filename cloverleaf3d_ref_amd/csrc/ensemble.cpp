@@ -4,7 +4,7 @@
 // the fast path of the single-deck driver (chunk.cpp) with every launch
 // covering all decks (batch.h): timestep (halos, viscosity + calc_dt, a
 // per-deck dt on the device; its ideal_gas results come from the previous
-// step's third cell sweep), PdV predictor, accelerate, PdV corrector +
+// step's third cell sweep), PdV predictor, accelerate + PdV corrector +
 // flux_calc, the three advection sweeps and reset_field, the array
 // exchanges done in the device-side deck table.  No step reads
 // anything back to the host, so the step for each advect_x parity is
@@ -67,8 +67,7 @@ int shape_index(int bk)
     switch (bk) {
     case BK_VISC_DT: return SK_VISC_DT;
     case BK_PDV_PREDICT: return SK_PDV_PREDICT;
-    case BK_ACCELERATE: return SK_ACCELERATE;
-    case BK_PDV_CORRECT: return SK_PDV_CORRECT;
+    case BK_ACCEL_PDV: return SK_ACCEL_PDV;
     case BK_CELL_X: return SK_CELL_X;
     case BK_CELL_Y: return SK_CELL_Y;
     case BK_CELL_Z: return SK_CELL_Z;
@@ -200,11 +199,8 @@ void enqueue_step(clover_ensemble *e)
     hip_check(batch_pdv_predict(e->store, e->dev, e->nd, kgrid(e, BK_PDV_PREDICT), shape(e, BK_PDV_PREDICT), e->st),
               "PdV predict");
     halo(e, FH_PRESSURE);
-    hip_check(batch_accelerate(e->store, e->dev, e->nd, kgrid(e, BK_ACCELERATE), shape(e, BK_ACCELERATE), e->st),
-              "accelerate");
-    hip_check(batch_pdv_correct_flux(e->store, e->dev, e->nd, kgrid(e, BK_PDV_CORRECT), shape(e, BK_PDV_CORRECT),
-                                     e->st),
-              "PdV correct");
+    hip_check(batch_accel_pdv(e->store, e->dev, e->nd, kgrid(e, BK_ACCEL_PDV), shape(e, BK_ACCEL_PDV), e->st),
+              "accelerate + PdV correct");
     // advection.f90:26-181
     int dir = e->advect_x ? 1 : 3;
     halo(e, FH_CELL1);
@@ -259,10 +255,9 @@ SlotSwap tune_slots(int bk)
     switch (bk) {
     case BK_VISC_DT: pair(F_VISCOSITY, X_W1); break;
     case BK_PDV_PREDICT: pair(F_PRESSURE, X_W1); break;
-    case BK_ACCELERATE: pair(F_XVEL1, X_W5); pair(F_YVEL1, X_W6); pair(F_ZVEL1, X_W7); break;
-    case BK_PDV_CORRECT:
+    case BK_ACCEL_PDV:  // 8 outputs, 5 spare arrays: the volume fluxes keep their slots
         pair(F_DENSITY1, X_W1); pair(F_ENERGY1, X_W2);
-        pair(F_VOL_FLUX_X, X_W5); pair(F_VOL_FLUX_Y, X_W6); pair(F_VOL_FLUX_Z, X_W7);
+        pair(F_XVEL1, X_W5); pair(F_YVEL1, X_W6); pair(F_ZVEL1, X_W7);
         break;
     case BK_CELL_X: pair(F_MASS_FLUX_X, X_W5); break;  // the new density/energy go to W1/W2 anyway
     case BK_CELL_Y: pair(F_MASS_FLUX_Y, X_W5); break;
@@ -279,8 +274,7 @@ void tune_launch(clover_ensemble *e, int bk, const Shape &sh)
     switch (bk) {
     case BK_VISC_DT: r = batch_visc_dt(e->store, e->dev, e->nd, g, sh, e->g_small, e->visc_partial, e->st); break;
     case BK_PDV_PREDICT: r = batch_pdv_predict(e->store, e->dev, e->nd, g, sh, e->st); break;
-    case BK_ACCELERATE: r = batch_accelerate(e->store, e->dev, e->nd, g, sh, e->st); break;
-    case BK_PDV_CORRECT: r = batch_pdv_correct_flux(e->store, e->dev, e->nd, g, sh, e->st); break;
+    case BK_ACCEL_PDV: r = batch_accel_pdv(e->store, e->dev, e->nd, g, sh, e->st); break;
     case BK_CELL_X:
     case BK_CELL_Y:
     case BK_CELL_Z: {

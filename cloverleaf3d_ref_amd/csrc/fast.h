@@ -39,8 +39,8 @@ enum Store : int { ST_F64 = 0, ST_F32 = 1 };
 // (launch.h hw_blk_xcd) instead of the hardware's.
 // (the same indices as CLOVER_SHAPE_* of clover_hip.h)
 enum ShapeKernel : int {
-    SK_VISC_DT, SK_PDV_PREDICT, SK_ACCELERATE, SK_PDV_CORRECT, SK_CELL_X, SK_CELL_Y, SK_CELL_Z, SK_MOM_X,
-    SK_MOM_Y, SK_MOM_Z, SK_COUNT
+    SK_VISC_DT, SK_PDV_PREDICT, SK_ACCEL_PDV, SK_CELL_X, SK_CELL_Y, SK_CELL_Z, SK_MOM_X, SK_MOM_Y, SK_MOM_Z,
+    SK_COUNT
 };
 struct Shape {
     int planes;
@@ -98,16 +98,15 @@ hipError_t launch_fast_pdv_predict(Store s, const Lay &g, const Geo &m, double d
                                    const void *energy0, void *pressure, const void *viscosity, const void *xvel0,
                                    const void *yvel0, const void *zvel0, const Shape &sh, hipStream_t st);
 
-hipError_t launch_fast_accelerate(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
-                                  const void *pressure, const void *viscosity, const void *xvel0, const void *yvel0,
-                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, const Shape &sh, hipStream_t st);
-
-// PdV corrector (PdV_kernel.f90:124-165) and flux_calc (flux_calc_kernel.f90)
-hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
-                                        void *density1, const void *energy0, void *energy1, const void *pressure,
-                                        const void *viscosity, const void *xvel0, const void *xvel1,
-                                        const void *yvel0, const void *yvel1, const void *zvel0, const void *zvel1,
-                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, const Shape &sh, hipStream_t st);
+// accelerate (accelerate_kernel.f90:61-109), then the PdV corrector
+// (PdV_kernel.f90:124-165) and flux_calc (flux_calc_kernel.f90:56-90) on its
+// level-1 velocities, in one z-march: writes xvel1/yvel1/zvel1, density1,
+// energy1 and the three volume fluxes
+hipError_t launch_fast_accel_pdv(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                 void *density1, const void *energy0, void *energy1, const void *pressure,
+                                 const void *viscosity, const void *xvel0, void *xvel1, const void *yvel0,
+                                 void *yvel1, const void *zvel0, void *zvel1, void *vol_flux_x, void *vol_flux_y,
+                                 void *vol_flux_z, const Shape &sh, hipStream_t st);
 
 // one advec_cell sweep, fluxes and cell update in one kernel
 // (cell_kernels.hip): mass_flux_<dir> is written over the reference's face

@@ -309,246 +309,104 @@ static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, cons
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// accelerate_kernel.f90:61-109 with scalar areas and volume.  Node (j,k,l)
-// reads the cells (j-1..j, k-1..k, l-1..l); the march keeps plane l-1.
+// accelerate_kernel.f90:61-109 at one node: density0, pressure and viscosity
+// of the cells (j-1..j, k-1..k) in planes l (du, pu, qu) and l-1 (dl, pl, ql),
+// the node's level-0 velocity (u0, v0, w0) -> its level-1 velocity
 template <typename S>
-__device__ __forceinline__ void accelerate_body(const Blk &bk, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
-                  const S *__restrict__ viscosity, const S *__restrict__ xvel0,
-                  const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
-                  S *__restrict__ yvel1, S *__restrict__ zvel1)
+__device__ inline void accel_node(const QuadS<S> &du, const QuadS<S> &pu, const QuadS<S> &qu, const QuadS<S> &dl,
+                                  const QuadS<S> &pl, const QuadS<S> &ql, S u0, S v0, S w0, S vol, S xa, S ya, S za,
+                                  S tdt, S &u1, S &v1, S &w1)
 {
-    const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
-    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
-    if (j > g.x_max + 1 || k > g.y_max + 1) return;
-    int l0, l1;
-    chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
-    if (l0 > l1) return;
-    using T = S;  // arithmetic type of the form
-    int i = idx(g, j, k, l0);
-    QuadS<S> dl = quad_down_raw(density0, i - g.plane, g.ld), pl = quad_down_raw(pressure, i - g.plane, g.ld),
-             ql = quad_down_raw(viscosity, i - g.plane, g.ld);
-    const T vol = (T)m.vol;
-    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, tdt = (T)dt;
-    // the next plane's loads run one plane ahead of their use
-    QuadS<S> dn = quad_down_raw(density0, i, g.ld), pn = quad_down_raw(pressure, i, g.ld),
-             qn = quad_down_raw(viscosity, i, g.ld);
-    S un = xvel0[i], vn = yvel0[i], wn = zvel0[i];
-    for (int l = l0; l <= l1; l++, i += g.plane) {
-        const QuadS<S> du = dn, pu = pn, qu = qn;
-        const T u0 = un, v0 = vn, w0 = wn;
-        if (l < l1) {
-            const int i1 = i + g.plane;
-            dn = quad_down_raw(density0, i1, g.ld);
-            pn = quad_down_raw(pressure, i1, g.ld);
-            qn = quad_down_raw(viscosity, i1, g.ld);
-            un = xvel0[i1];
-            vn = yvel0[i1];
-            wn = zvel0[i1];
-        }
-        // (a, b, c) in {-1, 0}^3; c = 0 is plane l, c = -1 plane l-1
+    using T = S;
+    // (a, b, c) in {-1, 0}^3; c = 0 is plane l, c = -1 plane l-1
 #define RV(a, b, c) (((c) ? dl : du).v[((a) + 1) + 2 * ((b) + 1)] * vol)
 #define P(a, b, c) ((c) ? pl : pu).v[((a) + 1) + 2 * ((b) + 1)]
 #define Q(a, b, c) ((c) ? ql : qu).v[((a) + 1) + 2 * ((b) + 1)]
-        const T nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
-                              RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
-                             (T)0.125;
-        const T stepbymass_s = quo((T)0.25 * tdt, nodal_mass);
-        const T xvel1_s = u0 - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
-                                                          xa * (P(0, -1, 0) - P(-1, -1, 0)) +
-                                                          xa * (P(0, 0, -1) - P(-1, 0, -1)) +
-                                                          xa * (P(0, -1, -1) - P(-1, -1, -1)));
-        const T yvel1_s = v0 - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
-                                                          ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
-                                                          ya * (P(0, 0, -1) - P(0, -1, -1)) +
-                                                          ya * (P(-1, 0, -1) - P(-1, -1, -1)));
-        const T zvel1_s = w0 - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
-                                                          za * (P(0, -1, 0) - P(0, -1, -1)) +
-                                                          za * (P(-1, 0, 0) - P(-1, 0, -1)) +
-                                                          za * (P(-1, -1, 0) - P(-1, -1, -1)));
-        xvel1[i] = xvel1_s - stepbymass_s * (xa * (Q(0, 0, 0) - Q(-1, 0, 0)) + xa * (Q(0, -1, 0) - Q(-1, -1, 0)) +
-                                             xa * (Q(0, 0, -1) - Q(-1, 0, -1)) + xa * (Q(0, -1, -1) - Q(-1, -1, -1)));
-        yvel1[i] = yvel1_s - stepbymass_s * (ya * (Q(0, 0, 0) - Q(0, -1, 0)) + ya * (Q(-1, 0, 0) - Q(-1, -1, 0)) +
-                                             ya * (Q(0, 0, -1) - Q(0, -1, -1)) + ya * (Q(-1, 0, -1) - Q(-1, -1, -1)));
-        zvel1[i] = zvel1_s - stepbymass_s * (za * (Q(0, 0, 0) - Q(0, 0, -1)) + za * (Q(0, -1, 0) - Q(0, -1, -1)) +
-                                             za * (Q(-1, 0, 0) - Q(-1, 0, -1)) + za * (Q(-1, -1, 0) - Q(-1, -1, -1)));
+    const T nodal_mass = (RV(-1, -1, 0) + RV(0, -1, 0) + RV(0, 0, 0) + RV(-1, 0, 0) + RV(-1, -1, -1) +
+                          RV(0, -1, -1) + RV(0, 0, -1) + RV(-1, 0, -1)) *
+                         (T)0.125;
+    const T stepbymass_s = quo((T)0.25 * tdt, nodal_mass);
+    const T xvel1_s = u0 - stepbymass_s * (xa * (P(0, 0, 0) - P(-1, 0, 0)) +
+                                                      xa * (P(0, -1, 0) - P(-1, -1, 0)) +
+                                                      xa * (P(0, 0, -1) - P(-1, 0, -1)) +
+                                                      xa * (P(0, -1, -1) - P(-1, -1, -1)));
+    const T yvel1_s = v0 - stepbymass_s * (ya * (P(0, 0, 0) - P(0, -1, 0)) +
+                                                      ya * (P(-1, 0, 0) - P(-1, -1, 0)) +
+                                                      ya * (P(0, 0, -1) - P(0, -1, -1)) +
+                                                      ya * (P(-1, 0, -1) - P(-1, -1, -1)));
+    const T zvel1_s = w0 - stepbymass_s * (za * (P(0, 0, 0) - P(0, 0, -1)) +
+                                                      za * (P(0, -1, 0) - P(0, -1, -1)) +
+                                                      za * (P(-1, 0, 0) - P(-1, 0, -1)) +
+                                                      za * (P(-1, -1, 0) - P(-1, -1, -1)));
+    u1 = xvel1_s - stepbymass_s * (xa * (Q(0, 0, 0) - Q(-1, 0, 0)) + xa * (Q(0, -1, 0) - Q(-1, -1, 0)) +
+                                         xa * (Q(0, 0, -1) - Q(-1, 0, -1)) + xa * (Q(0, -1, -1) - Q(-1, -1, -1)));
+    v1 = yvel1_s - stepbymass_s * (ya * (Q(0, 0, 0) - Q(0, -1, 0)) + ya * (Q(-1, 0, 0) - Q(-1, -1, 0)) +
+                                         ya * (Q(0, 0, -1) - Q(0, -1, -1)) + ya * (Q(-1, 0, -1) - Q(-1, -1, -1)));
+    w1 = zvel1_s - stepbymass_s * (za * (Q(0, 0, 0) - Q(0, 0, -1)) + za * (Q(0, -1, 0) - Q(0, -1, -1)) +
+                                         za * (Q(-1, 0, 0) - Q(-1, 0, -1)) + za * (Q(-1, -1, 0) - Q(-1, -1, -1)));
 #undef RV
 #undef P
 #undef Q
-        dl = du;
-        pl = pu;
-        ql = qu;
-    }
 }
 
+// PdV corrector (PdV_kernel.f90:126-164) and flux_calc (flux_calc_kernel.f90:
+// 56-90) at plane l of column (j, k), offset i: node quads of the level-0
+// and level-1 velocities in planes l (..l) and l+1 (..u, read only when
+// lin: l <= z_max), the cell's level-0 state (d0, e0v, p0, q0)
 template <typename S>
-__global__ void __launch_bounds__(kMaxBlock) k_fast_accelerate(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ pressure,
-                  const S *__restrict__ viscosity, const S *__restrict__ xvel0,
-                  const S *__restrict__ yvel0, const S *__restrict__ zvel0, S *__restrict__ xvel1,
-                  S *__restrict__ yvel1, S *__restrict__ zvel1)
+__device__ inline void corrector_plane(int i, bool jin, bool kin, bool lin, const QuadS<S> &u0l, const QuadS<S> &u1l,
+                                       const QuadS<S> &v0l, const QuadS<S> &v1l, const QuadS<S> &w0l,
+                                       const QuadS<S> &w1l, const QuadS<S> &u0u, const QuadS<S> &u1u,
+                                       const QuadS<S> &v0u, const QuadS<S> &v1u, const QuadS<S> &w0u,
+                                       const QuadS<S> &w1u, S d0, S e0v, S p0, S q0, S xa, S ya, S za, S vol, S tdt,
+                                       S c0, S *__restrict__ density1, S *__restrict__ energy1,
+                                       S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
+                                       S *__restrict__ vol_flux_z)
 {
-    accelerate_body<S>(hw_blk(xcd), g, m, dt, density0, pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
-}
-
-template <typename S>
-static hipError_t fast_accelerate_t(const Lay &g, const Geo &m, double dt, const S *density0,
-                                  const S *pressure, const S *viscosity, const S *xvel0,
-                                  const S *yvel0, const S *zvel0, S *xvel1, S *yvel1,
-                                  S *zvel1, const Shape &sh, hipStream_t st)
-{
-    const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
-    const dim3 blk = block_of(sh);
-    const dim3 grid((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
-    TIMED_LAUNCH("k_fast_accelerate", st, k_fast_accelerate<S>, grid, blk, 0, st, sh.xcd, g, m, dt, density0,
-                 pressure, viscosity, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// PdV corrector (PdV_kernel.f90:126-164) on interior cells and flux_calc
-// (flux_calc_kernel.f90:56-90) on its face ranges, over the node range that
-// covers both.  The corrector's left/bottom face sums equal flux_calc's x/y
-// sums term for term; the z sums differ (flux_calc_kernel.f90:86 repeats
-// zvel0(j+1,k,l)) and are formed separately.
-template <typename S>
-__device__ __forceinline__ void pdv_correct_flux_body(const Blk &bk, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
-                        const S *__restrict__ energy0, S *__restrict__ energy1,
-                        const S *__restrict__ pressure, const S *__restrict__ viscosity,
-                        const S *__restrict__ xvel0, const S *__restrict__ xvel1,
-                        const S *__restrict__ yvel0, const S *__restrict__ yvel1,
-                        const S *__restrict__ zvel0, const S *__restrict__ zvel1,
-                        S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
-                        S *__restrict__ vol_flux_z)
-{
-    const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
-    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
-    if (j > g.x_max + 1 || k > g.y_max + 1) return;
-    int l0, l1;
-    chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
-    if (l0 > l1) return;
-    const bool jin = j <= g.x_max, kin = k <= g.y_max;
-    using T = S;  // arithmetic type of the form
-    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol, tdt = (T)dt;
-    int i = idx(g, j, k, l0);
-    QuadS<S> u0l = quad_up_raw(xvel0, i, g.ld), u1l = quad_up_raw(xvel1, i, g.ld), v0l = quad_up_raw(yvel0, i, g.ld),
-             v1l = quad_up_raw(yvel1, i, g.ld), w0l = quad_up_raw(zvel0, i, g.ld), w1l = quad_up_raw(zvel1, i, g.ld);
-    const T c0 = (T)0.125 * tdt;
-    const bool cellcol = jin && kin;
-    // the plane above (needed only by cells and x/y faces, l <= z_max) and
-    // the cell's own values, loaded one plane ahead of their use
-    QuadS<S> u0n{}, u1n{}, v0n{}, v1n{}, w0n{}, w1n{};
-    S d0n = 0, e0n = 0, pn = 0, qn = 0;
-    auto load_next = [&](int ii, int ll) {
-        if (ll <= g.z_max) {
-            const int iu = ii + g.plane;
-            u0n = quad_up_raw(xvel0, iu, g.ld);
-            u1n = quad_up_raw(xvel1, iu, g.ld);
-            v0n = quad_up_raw(yvel0, iu, g.ld);
-            v1n = quad_up_raw(yvel1, iu, g.ld);
-            w0n = quad_up_raw(zvel0, iu, g.ld);
-            w1n = quad_up_raw(zvel1, iu, g.ld);
-            if (cellcol) {
-                d0n = density0[ii];
-                e0n = energy0[ii];
-                pn = pressure[ii];
-                qn = viscosity[ii];
-            }
-        }
-    };
-    load_next(i, l0);
-    for (int l = l0; l <= l1; l++, i += g.plane) {
-        const bool lin = l <= g.z_max;
-        QuadS<S> u0u = u0l, u1u = u1l, v0u = v0l, v1u = v1l, w0u = w0l, w1u = w1l;
-        if (lin) {
-            u0u = u0n;
-            u1u = u1n;
-            v0u = v0n;
-            v1u = v1n;
-            w0u = w0n;
-            w1u = w1n;
-        }
-        const T d0 = d0n, e0v = e0n, p0 = pn, q0 = qn;
-        if (l < l1) load_next(i + g.plane, l + 1);
+    using T = S;
 #define U0(a, b, c) ((c) ? u0u : u0l).v[(a) + 2 * (b)]
 #define U1(a, b, c) ((c) ? u1u : u1l).v[(a) + 2 * (b)]
 #define V0(a, b, c) ((c) ? v0u : v0l).v[(a) + 2 * (b)]
 #define V1(a, b, c) ((c) ? v1u : v1l).v[(a) + 2 * (b)]
 #define W0(a, b, c) ((c) ? w0u : w0l).v[(a) + 2 * (b)]
 #define W1(a, b, c) ((c) ? w1u : w1l).v[(a) + 2 * (b)]
-        T sl = 0, sb = 0;
-        if (kin && lin) {
-            sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
-            vol_flux_x[i] = c0 * xa * sl;
-        }
-        if (jin && lin) {
-            sb = V0(0, 0, 0) + V0(1, 0, 0) + V0(0, 0, 1) + V0(1, 0, 1) + V1(0, 0, 0) + V1(1, 0, 0) + V1(0, 0, 1) + V1(1, 0, 1);
-            vol_flux_y[i] = c0 * ya * sb;
-        }
-        if (jin && kin)
-            vol_flux_z[i] = c0 * za *
-                            (W0(0, 0, 0) + W0(1, 0, 0) + W0(1, 0, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) +
-                             W1(1, 1, 0));
-        if (jin && kin && lin) {
-            const T sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
-            const T stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
-            const T sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
-            const T sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
-            const T left_flux = (xa * sl) * (T)0.125 * tdt;
-            const T right_flux = (xa * sr) * (T)0.125 * tdt;
-            const T bottom_flux = (ya * sb) * (T)0.125 * tdt;
-            const T top_flux = (ya * stp) * (T)0.125 * tdt;
-            const T back_flux = (za * sk) * (T)0.125 * tdt;
-            const T front_flux = (za * sf) * (T)0.125 * tdt;
-            const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
-            const T vol_change_s = quo(vol, vol + total_flux);
-            const T recip_volume = (T)1 / vol;
-            const T energy_change = (quo(p0, d0) + quo(q0, d0)) * total_flux * recip_volume;
-            energy1[i] = e0v - energy_change;
-            density1[i] = d0 * vol_change_s;
-        }
+    T sl = 0, sb = 0;
+    if (kin && lin) {
+        sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
+        vol_flux_x[i] = c0 * xa * sl;
+    }
+    if (jin && lin) {
+        sb = V0(0, 0, 0) + V0(1, 0, 0) + V0(0, 0, 1) + V0(1, 0, 1) + V1(0, 0, 0) + V1(1, 0, 0) + V1(0, 0, 1) + V1(1, 0, 1);
+        vol_flux_y[i] = c0 * ya * sb;
+    }
+    if (jin && kin)
+        vol_flux_z[i] = c0 * za *
+                        (W0(0, 0, 0) + W0(1, 0, 0) + W0(1, 0, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) +
+                         W1(1, 1, 0));
+    if (jin && kin && lin) {
+        const T sr = U0(1, 0, 0) + U0(1, 1, 0) + U0(1, 0, 1) + U0(1, 1, 1) + U1(1, 0, 0) + U1(1, 1, 0) + U1(1, 0, 1) + U1(1, 1, 1);
+        const T stp = V0(0, 1, 0) + V0(1, 1, 0) + V0(0, 1, 1) + V0(1, 1, 1) + V1(0, 1, 0) + V1(1, 1, 0) + V1(0, 1, 1) + V1(1, 1, 1);
+        const T sk = W0(0, 0, 0) + W0(1, 0, 0) + W0(0, 1, 0) + W0(1, 1, 0) + W1(0, 0, 0) + W1(1, 0, 0) + W1(0, 1, 0) + W1(1, 1, 0);
+        const T sf = W0(0, 0, 1) + W0(1, 0, 1) + W0(0, 1, 1) + W0(1, 1, 1) + W1(0, 0, 1) + W1(1, 0, 1) + W1(0, 1, 1) + W1(1, 1, 1);
+        const T left_flux = (xa * sl) * (T)0.125 * tdt;
+        const T right_flux = (xa * sr) * (T)0.125 * tdt;
+        const T bottom_flux = (ya * sb) * (T)0.125 * tdt;
+        const T top_flux = (ya * stp) * (T)0.125 * tdt;
+        const T back_flux = (za * sk) * (T)0.125 * tdt;
+        const T front_flux = (za * sf) * (T)0.125 * tdt;
+        const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+        const T vol_change_s = quo(vol, vol + total_flux);
+        const T recip_volume = (T)1 / vol;
+        const T energy_change = (quo(p0, d0) + quo(q0, d0)) * total_flux * recip_volume;
+        energy1[i] = e0v - energy_change;
+        density1[i] = d0 * vol_change_s;
+    }
 #undef U0
 #undef U1
 #undef V0
 #undef V1
 #undef W0
 #undef W1
-        u0l = u0u;
-        u1l = u1u;
-        v0l = v0u;
-        v1l = v1u;
-        w0l = w0u;
-        w1l = w1u;
-    }
-}
-
-template <typename S>
-__global__ void __launch_bounds__(kMaxBlock) k_fast_pdv_correct_flux(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
-                        const S *__restrict__ energy0, S *__restrict__ energy1,
-                        const S *__restrict__ pressure, const S *__restrict__ viscosity,
-                        const S *__restrict__ xvel0, const S *__restrict__ xvel1,
-                        const S *__restrict__ yvel0, const S *__restrict__ yvel1,
-                        const S *__restrict__ zvel0, const S *__restrict__ zvel1,
-                        S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
-                        S *__restrict__ vol_flux_z)
-{
-    pdv_correct_flux_body<S>(hw_blk(xcd), g, m, dt, density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
-}
-
-template <typename S>
-static hipError_t fast_pdv_correct_flux_t(const Lay &g, const Geo &m, double dt, const S *density0,
-                                        S *density1, const S *energy0, S *energy1,
-                                        const S *pressure, const S *viscosity, const S *xvel0,
-                                        const S *xvel1, const S *yvel0, const S *yvel1,
-                                        const S *zvel0, const S *zvel1, S *vol_flux_x,
-                                        S *vol_flux_y, S *vol_flux_z, const Shape &sh, hipStream_t st)
-{
-    const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
-    const dim3 blk = block_of(sh);
-    const dim3 grid((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
-    TIMED_LAUNCH("k_fast_pdv_correct_flux", st, k_fast_pdv_correct_flux<S>, grid, blk, 0, st, sh.xcd, g, m, dt,
-                 density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1,
-                 vol_flux_x, vol_flux_y, vol_flux_z);
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -664,6 +522,143 @@ static hipError_t fast_ideal_gas_t(const Lay &g, const S *density, const S *ener
 }
 
 // ---------------------------------------------------------------------------
+// accelerate (accelerate_kernel.f90:61-109), then the PdV corrector and
+// flux_calc on its results (PdV.f90 + flux_calc.f90 follow accelerate in
+// hydro.f90:56-60), in one z-march: the level-1 velocities are stored once
+// and handed to the corrector on chip instead of being read back.
+//
+// A workgroup is 64 lanes (x) by AW waves (y); lane 63 and wave AW-1 are a
+// halo column and row, so workgroups step by 63 and AW-1.  At node plane lp
+// every thread forms the level-1 velocity of its node (accel_node, as the
+// accelerate kernel does; owners store it), the plane's values go through
+// LDS (double-buffered, one barrier per plane), and the owners then run the
+// corrector and fluxes of plane lp-1 (corrector_plane) with node quads from
+// LDS (level 1) and memory (level 0).  Each chunk also forms the level-1
+// velocity of the plane above it, which the next chunk stores.
+constexpr int AW = 8;
+
+template <typename S>
+__device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, double dt,
+                                               const S *__restrict__ density0, S *__restrict__ density1,
+                                               const S *__restrict__ energy0, S *__restrict__ energy1,
+                                               const S *__restrict__ pressure, const S *__restrict__ viscosity,
+                                               const S *__restrict__ xvel0, S *__restrict__ xvel1,
+                                               const S *__restrict__ yvel0, S *__restrict__ yvel1,
+                                               const S *__restrict__ zvel0, S *__restrict__ zvel1,
+                                               S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
+                                               S *__restrict__ vol_flux_z)
+{
+    using T = S;  // arithmetic type of the form
+    __shared__ S vb[2][3][AW][64];
+    const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
+    const int j = g.x_min + bk.x * 63 + lane;
+    const int k = g.y_min + bk.y * (AW - 1) + w;
+    int l0, l1;
+    chunk_range(bk, g.z_min, g.z_max + 1, l0, l1);
+    if (l0 > l1) return;  // the whole workgroup, before any barrier
+    const bool node = j <= g.x_max + 1 && k <= g.y_max + 1;  // j >= x_min, k >= y_min always
+    const bool own = node && lane < 63 && w < AW - 1;
+    const bool jin = j <= g.x_max, kin = k <= g.y_max;
+    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol, tdt = (T)dt;
+    const T c0 = (T)0.125 * tdt;
+    const int ln = min(lane + 1, 63), wn = min(w + 1, AW - 1);
+    int i = idx(g, j, k, l0);
+    // accelerate's cells of the plane below the node plane
+    QuadS<S> dlo{}, plo{}, qlo{};
+    // the node plane's loads, issued one plane ahead of their use: cells of
+    // the plane (accelerate), level-0 node quads, and the cell's energy0
+    QuadS<S> dn{}, pn{}, qn{}, u0n{}, v0n{}, w0n{};
+    S e0n = 0;
+    auto load = [&](int ii) {
+        dn = quad_down_raw(density0, ii, g.ld);
+        pn = quad_down_raw(pressure, ii, g.ld);
+        qn = quad_down_raw(viscosity, ii, g.ld);
+        u0n = quad_up_raw(xvel0, ii, g.ld);
+        v0n = quad_up_raw(yvel0, ii, g.ld);
+        w0n = quad_up_raw(zvel0, ii, g.ld);
+    };
+    if (node) {
+        dlo = quad_down_raw(density0, i - g.plane, g.ld);
+        plo = quad_down_raw(pressure, i - g.plane, g.ld);
+        qlo = quad_down_raw(viscosity, i - g.plane, g.ld);
+        load(i);
+    }
+    QuadS<S> u0p{}, v0p{}, w0p{}, u1p{}, v1p{}, w1p{};  // node quads of the plane below
+    T e0p = 0;                                          // energy0 of the cell below
+    const int lend = min(l1 + 1, g.z_max + 1);
+    int b = 0;
+    for (int lp = l0; lp <= lend; lp++, i += g.plane) {
+        const QuadS<S> dhi = dn, phi = pn, qhi = qn, u0q = u0n, v0q = v0n, w0q = w0n;
+        const T e0q = (node && lp <= g.z_max && jin && kin) ? (T)energy0[i] : (T)0;
+        S u1 = 0, v1 = 0, w1 = 0;
+        if (node) {
+            accel_node<S>(dhi, phi, qhi, dlo, plo, qlo, u0q.v[0], v0q.v[0], w0q.v[0], vol, xa, ya, za, tdt, u1, v1,
+                          w1);
+            if (own && lp <= l1) {
+                xvel1[i] = u1;
+                yvel1[i] = v1;
+                zvel1[i] = w1;
+            }
+        }
+        vb[b][0][w][lane] = u1;
+        vb[b][1][w][lane] = v1;
+        vb[b][2][w][lane] = w1;
+        __syncthreads();
+        if (node && lp < lend) load(i + g.plane);
+        const QuadS<S> u1q{{u1, vb[b][0][w][ln], vb[b][0][wn][lane], vb[b][0][wn][ln]}};
+        const QuadS<S> v1q{{v1, vb[b][1][w][ln], vb[b][1][wn][lane], vb[b][1][wn][ln]}};
+        const QuadS<S> w1q{{w1, vb[b][2][w][ln], vb[b][2][wn][lane], vb[b][2][wn][ln]}};
+        b ^= 1;
+        if (own && lp > l0)  // plane lp-1, nodes of planes lp-1 and lp; its cell is the quads' (j, k) corner
+            corrector_plane<S>(i - g.plane, jin, kin, true, u0p, u1p, v0p, v1p, w0p, w1p, u0q, u1q, v0q, v1q, w0q,
+                               w1q, dlo.v[3], e0p, plo.v[3], qlo.v[3], xa, ya, za, vol, tdt, c0, density1, energy1,
+                               vol_flux_x, vol_flux_y, vol_flux_z);
+        dlo = dhi;
+        plo = phi;
+        qlo = qhi;
+        u0p = u0q;
+        v0p = v0q;
+        w0p = w0q;
+        u1p = u1q;
+        v1p = v1q;
+        w1p = w1q;
+        e0p = e0q;
+    }
+    // the top z faces (plane z_max+1) use only their own node plane
+    if (own && l1 == g.z_max + 1)
+        corrector_plane<S>(i - g.plane, jin, kin, false, u0p, u1p, v0p, v1p, w0p, w1p, u0p, u1p, v0p, v1p, w0p, w1p,
+                           (T)0, (T)0, (T)0, (T)0, xa, ya, za, vol, tdt, c0, density1, energy1, vol_flux_x,
+                           vol_flux_y, vol_flux_z);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(64 * AW)
+    k_fast_accel_pdv(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
+                     const S *__restrict__ energy0, S *__restrict__ energy1, const S *__restrict__ pressure,
+                     const S *__restrict__ viscosity, const S *__restrict__ xvel0, S *__restrict__ xvel1,
+                     const S *__restrict__ yvel0, S *__restrict__ yvel1, const S *__restrict__ zvel0,
+                     S *__restrict__ zvel1, S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
+                     S *__restrict__ vol_flux_z)
+{
+    accel_pdv_body<S>(hw_blk(xcd), g, m, dt, density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1,
+                      yvel0, yvel1, zvel0, zvel1, vol_flux_x, vol_flux_y, vol_flux_z);
+}
+
+template <typename S>
+static hipError_t fast_accel_pdv_t(const Lay &g, const Geo &m, double dt, const S *density0, S *density1,
+                                   const S *energy0, S *energy1, const S *pressure, const S *viscosity,
+                                   const S *xvel0, S *xvel1, const S *yvel0, S *yvel1, const S *zvel0, S *zvel1,
+                                   S *vol_flux_x, S *vol_flux_y, S *vol_flux_z, const Shape &sh, hipStream_t st)
+{
+    const int nx = g.x_max - g.x_min + 2, ny = g.y_max - g.y_min + 2, nz = g.z_max - g.z_min + 2;  // nodes
+    const dim3 grid((nx + 62) / 63, (ny + AW - 2) / (AW - 1), nchunks(nz, sh.planes));
+    TIMED_LAUNCH("k_fast_accel_pdv", st, k_fast_accel_pdv<S>, grid, dim3(64, AW, 1), 0, st, sh.xcd, g, m, dt,
+                 density0, density1, energy0, energy1, pressure, viscosity, xvel0, xvel1, yvel0, yvel1, zvel0, zvel1,
+                 vol_flux_x, vol_flux_y, vol_flux_z);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // storage dispatch
 #define CP(T, p) static_cast<const T *>(p)
 #define MP(T, p) static_cast<T *>(p)
@@ -701,28 +696,17 @@ hipError_t launch_fast_pdv_predict(Store s, const Lay &g, const Geo &m, double d
 #undef CALL
 }
 
-hipError_t launch_fast_accelerate(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
-                                  const void *pressure, const void *viscosity, const void *xvel0, const void *yvel0,
-                                  const void *zvel0, void *xvel1, void *yvel1, void *zvel1, const Shape &sh, hipStream_t st)
+hipError_t launch_fast_accel_pdv(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
+                                 void *density1, const void *energy0, void *energy1, const void *pressure,
+                                 const void *viscosity, const void *xvel0, void *xvel1, const void *yvel0,
+                                 void *yvel1, const void *zvel0, void *zvel1, void *vol_flux_x, void *vol_flux_y,
+                                 void *vol_flux_z, const Shape &sh, hipStream_t st)
 {
 #define CALL(T)                                                                                                 \
-    fast_accelerate_t<T>(g, m, dt, CP(T, density0), CP(T, pressure), CP(T, viscosity), CP(T, xvel0),            \
-                         CP(T, yvel0), CP(T, zvel0), MP(T, xvel1), MP(T, yvel1), MP(T, zvel1), sh, st)
-    return s == ST_F32 ? CALL(float) : CALL(double);
-#undef CALL
-}
-
-hipError_t launch_fast_pdv_correct_flux(Store s, const Lay &g, const Geo &m, double dt, const void *density0,
-                                        void *density1, const void *energy0, void *energy1, const void *pressure,
-                                        const void *viscosity, const void *xvel0, const void *xvel1,
-                                        const void *yvel0, const void *yvel1, const void *zvel0, const void *zvel1,
-                                        void *vol_flux_x, void *vol_flux_y, void *vol_flux_z, const Shape &sh, hipStream_t st)
-{
-#define CALL(T)                                                                                                 \
-    fast_pdv_correct_flux_t<T>(g, m, dt, CP(T, density0), MP(T, density1), CP(T, energy0), MP(T, energy1),      \
-                               CP(T, pressure), CP(T, viscosity), CP(T, xvel0), CP(T, xvel1), CP(T, yvel0),      \
-                               CP(T, yvel1), CP(T, zvel0), CP(T, zvel1), MP(T, vol_flux_x), MP(T, vol_flux_y),   \
-                               MP(T, vol_flux_z), sh, st)
+    fast_accel_pdv_t<T>(g, m, dt, CP(T, density0), MP(T, density1), CP(T, energy0), MP(T, energy1),             \
+                        CP(T, pressure), CP(T, viscosity), CP(T, xvel0), MP(T, xvel1), CP(T, yvel0),             \
+                        MP(T, yvel1), CP(T, zvel0), MP(T, zvel1), MP(T, vol_flux_x), MP(T, vol_flux_y),          \
+                        MP(T, vol_flux_z), sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
@@ -757,10 +741,10 @@ __host__ __device__ inline void dims_cells(const Lay &g, int planes, int &gx, in
     gz = nchunks(g.z_max - g.z_min + 1, planes);
 }
 
-__host__ __device__ inline void dims_nodes(const Lay &g, int planes, int &gx, int &gy, int &gz)
+__host__ __device__ inline void dims_accel_pdv(const Lay &g, int planes, int &gx, int &gy, int &gz)
 {
-    gx = (g.x_max - g.x_min + 1 + BX) / BX;
-    gy = (g.y_max - g.y_min + 1 + BY) / BY;
+    gx = (g.x_max - g.x_min + 2 + 62) / 63;
+    gy = (g.y_max - g.y_min + 2 + AW - 2) / (AW - 1);
     gz = nchunks(g.z_max - g.z_min + 2, planes);
 }
 
@@ -821,22 +805,13 @@ kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start,
 }
 
 template <typename S>
-__global__ void __launch_bounds__(BX *BY)
-kb_accelerate(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
+__global__ void __launch_bounds__(64 * AW)
+kb_accel_pdv(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
-    DECK_BLOCK(dims_nodes(D.g, planes, gx, gy, gz), xcd)
-    accelerate_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0),
-                       CF(F_YVEL0), CF(F_ZVEL0), MF(F_XVEL1), MF(F_YVEL1), MF(F_ZVEL1));
-}
-
-template <typename S>
-__global__ void __launch_bounds__(BX *BY)
-kb_pdv_correct_flux(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
-{
-    DECK_BLOCK(dims_nodes(D.g, planes, gx, gy, gz), xcd)
-    pdv_correct_flux_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), MF(F_DENSITY1), CF(F_ENERGY0), MF(F_ENERGY1),
-                             CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0), CF(F_XVEL1), CF(F_YVEL0), CF(F_YVEL1),
-                             CF(F_ZVEL0), CF(F_ZVEL1), MF(F_VOL_FLUX_X), MF(F_VOL_FLUX_Y), MF(F_VOL_FLUX_Z));
+    DECK_BLOCK(dims_accel_pdv(D.g, planes, gx, gy, gz), xcd)
+    accel_pdv_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), MF(F_DENSITY1), CF(F_ENERGY0), MF(F_ENERGY1),
+                      CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0), MF(F_XVEL1), CF(F_YVEL0), MF(F_YVEL1),
+                      CF(F_ZVEL0), MF(F_ZVEL1), MF(F_VOL_FLUX_X), MF(F_VOL_FLUX_Y), MF(F_VOL_FLUX_Z));
 }
 
 __host__ __device__ inline void dims_summary(const Lay &g, int &gx, int &gy, int &gz)
@@ -870,8 +845,7 @@ int batch_blocks(BatchKernel k, const Lay &g, int planes)
     case BK_IDEAL_GAS: dims_ideal(g, gx, gy, gz); break;
     case BK_VISC_DT:
     case BK_PDV_PREDICT: dims_cells(g, planes, gx, gy, gz); break;
-    case BK_ACCELERATE:
-    case BK_PDV_CORRECT: dims_nodes(g, planes, gx, gy, gz); break;
+    case BK_ACCEL_PDV: dims_accel_pdv(g, planes, gx, gy, gz); break;
     case BK_SUMMARY: dims_summary(g, gx, gy, gz); break;
     case BK_CELL_X: return batch_blocks_cell(1, g, planes);
     case BK_CELL_Y: return batch_blocks_cell(2, g, planes);
@@ -913,17 +887,10 @@ hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchG
     return hipGetLastError();
 }
 
-hipError_t batch_accelerate(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
+hipError_t batch_accel_pdv(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_accelerate", kb_accelerate<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
-    else BLAUNCH("kb_accelerate", kb_accelerate<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
-    return hipGetLastError();
-}
-
-hipError_t batch_pdv_correct_flux(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
-{
-    if (s == ST_F32) BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
-    else BLAUNCH("kb_pdv_correct_flux", kb_pdv_correct_flux<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
+    if (s == ST_F32) BLAUNCH("kb_accel_pdv", kb_accel_pdv<float>, dim3(64, AW), decks, bg.start, nd, sh.planes, sh.xcd);
+    else BLAUNCH("kb_accel_pdv", kb_accel_pdv<double>, dim3(64, AW), decks, bg.start, nd, sh.planes, sh.xcd);
     return hipGetLastError();
 }
 

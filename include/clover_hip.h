@@ -253,20 +253,20 @@ int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
  * nodes along the y/z advection sweeps; ignored by the x cell sweep),
  * `xcd`: 1 gives each of the 8 XCDs a contiguous eighth of the grid instead
  * of the hardware's round-robin order, and `block_x`: the workgroup width
- * of the four z marches (64, 128, 256 or 512 threads along x, with 256/64,
- * 256/128 or 1 rows; the other kernels' workgroups are fixed, block_x 64).
+ * of the visc/dt and PdV-predictor z marches (64, 128, 256 or 512 threads
+ * along x, with 256/64, 256/128 or 1 rows; the other kernels' workgroups are
+ * fixed, block_x 64).
  * A shape changes how the work is split, never a value.  Kernel indices: */
 #define CLOVER_SHAPE_VISC_DT 0
 #define CLOVER_SHAPE_PDV_PREDICT 1
-#define CLOVER_SHAPE_ACCELERATE 2
-#define CLOVER_SHAPE_PDV_CORRECT 3
-#define CLOVER_SHAPE_CELL_X 4
-#define CLOVER_SHAPE_CELL_Y 5
-#define CLOVER_SHAPE_CELL_Z 6
-#define CLOVER_SHAPE_MOM_X 7
-#define CLOVER_SHAPE_MOM_Y 8
-#define CLOVER_SHAPE_MOM_Z 9
-#define CLOVER_SHAPE_COUNT 10
+#define CLOVER_SHAPE_ACCEL_PDV 2 /* accelerate + PdV corrector + flux_calc */
+#define CLOVER_SHAPE_CELL_X 3
+#define CLOVER_SHAPE_CELL_Y 4
+#define CLOVER_SHAPE_CELL_Z 5
+#define CLOVER_SHAPE_MOM_X 6
+#define CLOVER_SHAPE_MOM_Y 7
+#define CLOVER_SHAPE_MOM_Z 8
+#define CLOVER_SHAPE_COUNT 9
 /* The autotuner: times every shaped kernel with every candidate shape (planes
  * 8, 16, 32, 64, 128 x both block orders x, for the z marches, the four
  * workgroup widths) on this chunk's own arrays and keeps
