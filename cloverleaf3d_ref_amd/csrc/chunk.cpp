@@ -864,13 +864,8 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         c->dt_tile = -1;
         // the uniform mesh as initialise_chunk.f90:37-39 and
         // initialise_chunk_kernel.f90:151-178 compute it
-        c->geo.dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
-        c->geo.dy = (d.ymax - d.ymin) / (double)(float)d.y_cells;
-        c->geo.dz = (d.zmax - d.zmin) / (double)(float)d.z_cells;
-        c->geo.vol = c->geo.dx * c->geo.dy * c->geo.dz;
-        c->geo.xa = c->geo.dy * c->geo.dz;
-        c->geo.ya = c->geo.dx * c->geo.dz;
-        c->geo.za = c->geo.dx * c->geo.dy;
+        c->geo = geo_of((d.xmax - d.xmin) / (double)(float)d.x_cells, (d.ymax - d.ymin) / (double)(float)d.y_cells,
+                        (d.zmax - d.zmin) / (double)(float)d.z_cells);
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&c->ev0), "event");

@@ -414,13 +414,9 @@ int clover_ensemble_create(const clover_deck *decks, int ndecks, int device, int
             memset(&D, 0, sizeof D);
             D.g = g;
             // the uniform mesh as initialise_chunk.f90:37-39 computes it
-            D.m.dx = (dk.xmax - dk.xmin) / (double)(float)dk.x_cells;
-            D.m.dy = (dk.ymax - dk.ymin) / (double)(float)dk.y_cells;
-            D.m.dz = (dk.zmax - dk.zmin) / (double)(float)dk.z_cells;
-            D.m.vol = D.m.dx * D.m.dy * D.m.dz;
-            D.m.xa = D.m.dy * D.m.dz;
-            D.m.ya = D.m.dx * D.m.dz;
-            D.m.za = D.m.dx * D.m.dy;
+            D.m = geo_of((dk.xmax - dk.xmin) / (double)(float)dk.x_cells,
+                         (dk.ymax - dk.ymin) / (double)(float)dk.y_cells,
+                         (dk.zmax - dk.zmin) / (double)(float)dk.z_cells);
             D.dt = dk.dtinit;
             D.dtold = dk.dtinit;
             D.time = 0.0;

@@ -26,6 +26,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "layout.h"
 
 namespace clover {
@@ -72,12 +74,57 @@ inline Shapes default_shapes()
 
 inline size_t store_bytes(Store s) { return s == ST_F32 ? sizeof(float) : sizeof(double); }
 
+// A positive divisor b that is constant over the mesh, prepared for division
+// without a divide instruction (launch.h by_const):
+//   hn = RN(1/b);  hd = the largest double <= 1/b (hn, or its predecessor);
+//   ld = RN(1/b - hd) >= 0, so hd + ld is 1/b to about 2^-105.
+// q = RN(a*hd + RN(a*ld)) is then within one ulp of a/b, e = q*b - a is exact
+// (FMA), and RN(q - e*hn) is the correctly rounded quotient RN(a/b): one
+// Newton-Raphson correction of a faithful quotient with the correctly rounded
+// reciprocal (P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990).  With ld >= 0 a
+// zero a keeps its sign, so the result equals a/b bit for bit for every
+// finite a whose products stay in the normal range.
+struct Recip {
+    double b, hn, hd, ld;
+};
+
+inline Recip recip_of(double b)
+{
+    Recip r;
+    r.b = b;
+    r.hn = 1.0 / b;
+    r.hd = std::fma(-r.hn, b, 1.0) >= 0.0 ? r.hn : std::nextafter(r.hn, 0.0);
+    r.ld = std::fma(-r.hd, b, 1.0) / b;  // 1 - b*hd is exact: hd is within one ulp of 1/b
+    return r;
+}
+
 // uniform mesh, as initialise_chunk_kernel.f90 computes it
 struct Geo {
     double dx, dy, dz;   // celld*, vertexd*
     double vol;          // dx*dy*dz
     double xa, ya, za;   // celldy*celldz, celldx*celldz, celldx*celldy
+    Recip rx, ry, rz;    // the spacings as divisors
+    Recip r6;            // 6, advec_mom_kernel.f90's limiter divisor
 };
+
+// initialise_chunk_kernel.f90:151-178 on a uniform mesh of spacing dx, dy, dz
+inline Geo geo_of(double dx, double dy, double dz)
+{
+    Geo m;
+    m.dx = dx;
+    m.dy = dy;
+    m.dz = dz;
+    m.vol = dx * dy * dz;
+    m.xa = dy * dz;
+    m.ya = dx * dz;
+    m.za = dx * dy;
+    m.rx = recip_of(dx);
+    m.ry = recip_of(dy);
+    m.rz = recip_of(dz);
+    m.r6 = recip_of(6.0);
+    return m;
+}
 
 // ideal_gas_kernel.f90:45-60 on interior cells
 hipError_t launch_fast_ideal_gas(Store s, const Lay &g, const void *density, const void *energy, void *pressure,

@@ -66,9 +66,6 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
     const double mm = dmin(dmin(m.dx, m.dy), m.dz);
     const T ds = (T)(1.0 / (mm * mm));
     const T dx = (T)m.dx, dy = (T)m.dy, dz = (T)m.dz;
-    const T dx2 = (T)(m.dx + m.dx), dy2 = (T)(m.dy + m.dy), dz2 = (T)(m.dz + m.dz);
-    const T rdx = (T)1 / dx, rdy = (T)1 / dy, rdz = (T)1 / dz;
-    const T rdx2 = (T)1 / dx2, rdy2 = (T)1 / dy2, rdz2 = (T)1 / dz2;
     const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol;
     const T small = (T)g_small;
     double dtmin_thread = 1.0e300;
@@ -118,15 +115,17 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 #undef V
 #undef W
             const T div = (xa * (ugradx2 - ugradx1) + ya * (vgrady2 - vgrady1)) + za * (wgradz2 - wgradz1);
-            const T xx = by_const((T)0.25 * (ugradx2 - ugradx1), dx, rdx);
-            const T yy = by_const((T)0.25 * (vgrady2 - vgrady1), dy, rdy);
-            const T zz = by_const((T)0.25 * (wgradz2 - wgradz1), dz, rdz);
-            const T xy = by_const((T)0.25 * (ugrady2 - ugrady1), dy, rdy) + by_const((T)0.25 * (vgradx2 - vgradx1), dx, rdx);
-            const T xz = by_const((T)0.25 * (ugradz2 - ugradz1), dz, rdz) + by_const((T)0.25 * (wgradx2 - wgradx1), dx, rdx);
-            const T yz = by_const((T)0.25 * (vgradz2 - vgradz1), dz, rdz) + by_const((T)0.25 * (wgrady2 - wgrady1), dy, rdy);
-            T pgradx = by_const((T)pressure[i + 1] - (T)pressure[i - 1], dx2, rdx2);
-            T pgrady = by_const((T)pressure[i + g.ld] - (T)pressure[i - g.ld], dy2, rdy2);
-            T pgradz = by_const(p_hi - p_lo, dz2, rdz2);
+            const T xx = by_const<T>((T)0.25 * (ugradx2 - ugradx1), m.rx);
+            const T yy = by_const<T>((T)0.25 * (vgrady2 - vgrady1), m.ry);
+            const T zz = by_const<T>((T)0.25 * (wgradz2 - wgradz1), m.rz);
+            const T xy = by_const<T>((T)0.25 * (ugrady2 - ugrady1), m.ry) + by_const<T>((T)0.25 * (vgradx2 - vgradx1), m.rx);
+            const T xz = by_const<T>((T)0.25 * (ugradz2 - ugradz1), m.rz) + by_const<T>((T)0.25 * (wgradx2 - wgradx1), m.rx);
+            const T yz = by_const<T>((T)0.25 * (vgradz2 - vgradz1), m.rz) + by_const<T>((T)0.25 * (wgrady2 - wgrady1), m.ry);
+            // p difference / (celld + celld): halving the difference instead
+            // of doubling the divisor gives the same quotient (both exact)
+            T pgradx = by_const<T>((T)0.5 * ((T)pressure[i + 1] - (T)pressure[i - 1]), m.rx);
+            T pgrady = by_const<T>((T)0.5 * ((T)pressure[i + g.ld] - (T)pressure[i - g.ld]), m.ry);
+            T pgradz = by_const<T>((T)0.5 * (p_hi - p_lo), m.rz);
             const T pgradx2 = pgradx * pgradx;
             const T pgrady2 = pgrady * pgrady;
             const T pgradz2 = pgradz * pgradz;

@@ -65,9 +65,9 @@ __device__ inline S mom_post_vol(int mode, const Lay &g, int i, S V0, const S *_
 // momentum flux through the face between nodes n and n+s
 // (advec_mom_kernel.f90:213-249 and its y/z twins); vm1..vp2 are the
 // velocity at n-s, n, n+s, n+2s; sigma is |nf| / node_mass_pre(donor); d the
-// node spacing along the sweep and rd = 1 / d
+// node spacing along the sweep (rd: the same as a divisor), r6: 6 as a divisor
 template <typename T>
-__device__ inline T mom_face_flux(T nf, T sigma, T d, T rd, T vm1, T v0, T vp1, T vp2)
+__device__ inline T mom_face_flux(T nf, T sigma, T d, const Recip &rd, const Recip &r6, T vm1, T v0, T vp1, T vp2)
 {
     T upwind, donor, downwind;
     if (nf < (T)0) {
@@ -84,9 +84,9 @@ __device__ inline T mom_face_flux(T nf, T sigma, T d, T rd, T vm1, T v0, T vp1, 
         const T adw = fabs(vdiffdw);
         T wind = 1;
         if (vdiffdw <= (T)0) wind = -1;
-        limiter = wind * tmin(tmin(by_const(width * (by_const(((T)2 - sigma) * adw, width, rd) +
-                                                     by_const(((T)1 + sigma) * auw, d, rd)),
-                                            (T)6, (T)(1.0 / 6.0)),
+        limiter = wind * tmin(tmin(by_const<T>(width * (by_const<T>(((T)2 - sigma) * adw, rd) +
+                                                        by_const<T>(((T)1 + sigma) * auw, rd)),
+                                               r6),
                                    auw),
                               adw);
     }
@@ -231,7 +231,8 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     const int sM = DIR == 3 ? g.plane : g.ld;
     const int wmin = DIR == 3 ? g.y_min : g.z_min, wmax = DIR == 3 ? g.y_max : g.z_max;
     const int mmin = DIR == 3 ? g.z_min : g.y_min, mmax = DIR == 3 ? g.z_max : g.y_max;
-    const T d = (T)(DIR == 3 ? geo.dz : geo.dy), rd = (T)1 / d;
+    const T d = (T)(DIR == 3 ? geo.dz : geo.dy);
+    const Recip rd = DIR == 3 ? geo.rz : geo.ry;
     const T V0 = (T)geo.vol;
 
     const int j = g.x_min - 1 + bk.x * 63 + lane;
@@ -281,7 +282,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
             vm1[c] = vel_at(c, m0 - 1);
             v0[c] = vel_at(c, m0);
             vp1[c] = vel_at(c, m0 + 1);
-            flo[c] = mom_face_flux(nf_lo, sig, d, rd, vm2, vm1[c], v0[c], vp1[c]);  // F(m0-1)
+            flo[c] = mom_face_flux(nf_lo, sig, d, rd, geo.r6, vm2, vm1[c], v0[c], vp1[c]);  // F(m0-1)
         }
     }
 
@@ -312,7 +313,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         const int i = OFF(m);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const T fhi = mom_face_flux(nf0, sig, d, rd, vm1[c], v0[c], vp1[c], vp2[c]);  // F(m)
+            const T fhi = mom_face_flux(nf0, sig, d, rd, geo.r6, vm1[c], v0[c], vp1[c], vp2[c]);  // F(m)
             if (store) v.out[c][i] = quo(v0[c] * mq0 + flo[c] - fhi, mp0);
             flo[c] = fhi;
             vm1[c] = v0[c];
@@ -353,7 +354,8 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     using T = S;  // arithmetic type of the form
     __shared__ T xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
-    const T d = (T)geo.dx, rd = (T)1 / d;
+    const T d = (T)geo.dx;
+    const Recip rd = geo.rx;
     const T V0 = (T)geo.vol;
     const int j = g.x_min - 2 + bk.x * 60 + lane;
     const int k = g.y_min - 1 + bk.y * (MW - 1) + w;
@@ -413,7 +415,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
         for (int c = 0; c < 3; c++) {
             const T v0 = vc[c];
             const T vm1 = shfl_d(v0, lm), vp1 = shfl_d(v0, lp), vp2 = shfl_d(v0, lp2);
-            const T fhi = mom_face_flux(nf, sig, d, rd, vm1, v0, vp1, vp2);  // face j | j+1
+            const T fhi = mom_face_flux(nf, sig, d, rd, geo.r6, vm1, v0, vp1, vp2);  // face j | j+1
             const T flo = shfl_d(fhi, lm);                                // face j-1 | j
             if (store) v.out[c][i] = quo(v0 * mq + flo - fhi, mp);
         }
