@@ -70,6 +70,7 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
     const T small = (T)g_small;
     double dtmin_thread = 1.0e300;
     if (in) {
+        T mc = 0, mu = 0, mv = 0, mw = 0, mdiv = 0;  // largest dt denominators so far (all are >= 0)
         int i = idx(g, j, k, l0);
         QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
         T p_lo = pressure[i - g.plane], p_c = pressure[i];
@@ -129,14 +130,17 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             const T pgradx2 = pgradx * pgradx;
             const T pgrady2 = pgrady * pgrady;
             const T pgradz2 = pgradz * pgradz;
-            const T limiter = quo(xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
-                                      xz * pgradx * pgradz + yz * pgrady * pgradz,
-                                  tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16));
+            // the limiter's denominator is positive, so its sign is the
+            // numerator's, and its value is needed only where q is formed
+            // (a positive quotient that underflows to zero gives q = 0 there)
+            const T lim_num = xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
+                              xz * pgradx * pgradz + yz * pgrady * pgradz;
             const T rho = density0[i];
             T q;
-            if ((limiter > (T)0) || (div >= (T)0)) {
+            if ((lim_num > (T)0) || (div >= (T)0)) {
                 q = (T)0;
             } else {
+                const T limiter = quo(lim_num, tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16));
                 pgradx = copysign(tmax((T)1.0e-16, fabs(pgradx)), pgradx);
                 pgrady = copysign(tmax((T)1.0e-16, fabs(pgrady)), pgrady);
                 pgradz = copysign(tmax((T)1.0e-16, fabs(pgradz)), pgradz);
@@ -150,27 +154,39 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             }
             viscosity[i] = q;
 
-            // calc_dt
+            // calc_dt: the denominators of the five candidates (their
+            // maxima over the thread's cells, see below)
             const T c = soundspeed[i];
             T cc = c * c;
-            cc = cc + quo((T)2 * q, rho);
-            T dtct = ds * cc;
-            dtct = quo((T)dtc_safe * (T)1, tmax(sqrt(dtct), small));
+            if (q != (T)0) cc = cc + quo((T)2 * q, rho);  // adding 2*0/rho would leave cc as it is
+            mc = tmax(mc, ds * cc);
             const T du1 = ugradx1 * xa, du2 = ugradx2 * xa;
-            const T dtut = quo((T)dtu_safe * (T)4 * vol, tmax(tmax(fabs(du1), fabs(du2)), (T)1.0e-5 * vol));
+            mu = tmax(mu, tmax(fabs(du1), fabs(du2)));
             const T dv1 = vgrady1 * ya, dv2 = vgrady2 * ya;
-            const T dtvt = quo((T)dtv_safe * (T)4 * vol, tmax(tmax(fabs(dv1), fabs(dv2)), (T)1.0e-5 * vol));
+            mv = tmax(mv, tmax(fabs(dv1), fabs(dv2)));
             const T dw1 = dw1s * za, dw2 = dw2s * za;
-            const T dtwt = quo((T)dtw_safe * (T)4 * vol, tmax(tmax(fabs(dw1), fabs(dw2)), (T)1.0e-5 * vol));
+            mw = tmax(mw, tmax(fabs(dw1), fabs(dw2)));
             const T dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
-            const T dtdivt = quo((T)dtdiv_safe * (T)4 * (vol), tmax(vol * (T)1.0e-05, fabs(dv)));
-            dtmin_thread = dmin(dtmin_thread, (double)tmin(tmin(tmin(tmin(dtct, dtut), dtvt), dtwt), dtdivt));
+            mdiv = tmax(mdiv, fabs(dv));
             ul = uu;
             vl = vu;
             wl = wu;
             p_lo = p_c;
             p_c = p_hi;
         }
+        // calc_dt_kernel.f90:127-149 per cell is dt = min(C_c / max(sqrt(ds*cc), g_small),
+        // C_u / max(|du1|, |du2|, 1e-5*volume), C_v / .., C_w / .., C_div / max(1e-5*volume, |div|))
+        // with constant numerators.  Rounded division by a positive
+        // denominator and rounded sqrt are monotone, so the minimum over
+        // cells of each candidate is its numerator over the largest
+        // denominator: the same value, with five divisions per thread
+        // instead of per cell.
+        const T dtct = quo((T)dtc_safe * (T)1, tmax(sqrt(mc), small));
+        const T dtut = quo((T)dtu_safe * (T)4 * vol, tmax(mu, (T)1.0e-5 * vol));
+        const T dtvt = quo((T)dtv_safe * (T)4 * vol, tmax(mv, (T)1.0e-5 * vol));
+        const T dtwt = quo((T)dtw_safe * (T)4 * vol, tmax(mw, (T)1.0e-5 * vol));
+        const T dtdivt = quo((T)dtdiv_safe * (T)4 * (vol), tmax(vol * (T)1.0e-05, mdiv));
+        dtmin_thread = (double)tmin(tmin(tmin(tmin(dtct, dtut), dtvt), dtwt), dtdivt);
     }
     __shared__ double red[kMaxBlock / 64];
     const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
@@ -276,7 +292,7 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
         const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
         const T vol_change_s = quo(vol, vol + total_flux);
         const T recip_volume = (T)1 / vol;
-        const T energy_change = (quo(p0, d0) + quo(q0, d0)) * total_flux * recip_volume;
+        const T energy_change = (quo(p0, d0) + quo_sparse(q0, d0)) * total_flux * recip_volume;
         const T e1 = e0v - energy_change;
         const T d1 = d0 * vol_change_s;
         pressure[i] = (T)(1.4 - 1.0) * d1 * e1;
@@ -396,7 +412,7 @@ __device__ inline void corrector_plane(int i, bool jin, bool kin, bool lin, cons
         const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
         const T vol_change_s = quo(vol, vol + total_flux);
         const T recip_volume = (T)1 / vol;
-        const T energy_change = (quo(p0, d0) + quo(q0, d0)) * total_flux * recip_volume;
+        const T energy_change = (quo(p0, d0) + quo_sparse(q0, d0)) * total_flux * recip_volume;
         energy1[i] = e0v - energy_change;
         density1[i] = d0 * vol_change_s;
     }
@@ -565,7 +581,7 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
     // accelerate's cells of the plane below the node plane
     QuadS<S> dlo{}, plo{}, qlo{};
     // the node plane's loads, issued one plane ahead of their use: cells of
-    // the plane (accelerate), level-0 node quads, and the cell's energy0
+    // the plane (accelerate) and level-0 node quads
     QuadS<S> dn{}, pn{}, qn{}, u0n{}, v0n{}, w0n{};
     S e0n = 0;
     auto load = [&](int ii) {
@@ -589,6 +605,9 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
     for (int lp = l0; lp <= lend; lp++, i += g.plane) {
         const QuadS<S> dhi = dn, phi = pn, qhi = qn, u0q = u0n, v0q = v0n, w0q = w0n;
         const T e0q = (node && lp <= g.z_max && jin && kin) ? (T)energy0[i] : (T)0;
+        // the next plane's loads go out before this plane's work, so they
+        // have the whole iteration to arrive
+        if (node && lp < lend) load(i + g.plane);
         S u1 = 0, v1 = 0, w1 = 0;
         if (node) {
             accel_node<S>(dhi, phi, qhi, dlo, plo, qlo, u0q.v[0], v0q.v[0], w0q.v[0], vol, xa, ya, za, tdt, u1, v1,
@@ -603,7 +622,6 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
         vb[b][1][w][lane] = v1;
         vb[b][2][w][lane] = w1;
         __syncthreads();
-        if (node && lp < lend) load(i + g.plane);
         const QuadS<S> u1q{{u1, vb[b][0][w][ln], vb[b][0][wn][lane], vb[b][0][wn][ln]}};
         const QuadS<S> v1q{{v1, vb[b][1][w][ln], vb[b][1][wn][lane], vb[b][1][wn][ln]}};
         const QuadS<S> w1q{{w1, vb[b][2][w][ln], vb[b][2][wn][lane], vb[b][2][wn][ln]}};

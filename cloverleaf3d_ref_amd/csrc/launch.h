@@ -135,6 +135,17 @@ __device__ inline T quo(T a, T b)
     else return a * __builtin_amdgcn_rcpf(b);
 }
 
+// a / b for b > 0 where a is mostly zero (the artificial viscosity outside
+// shocks): a zero a is its own quotient, and the divide is skipped when the
+// whole wavefront has a == 0
+template <typename T>
+__device__ inline T quo_sparse(T a, T b)
+{
+    T r = a;
+    if (a != (T)0) r = quo(a, b);
+    return r;
+}
+
 // ideal_gas_kernel.f90:50-57 for one cell, in the form's arithmetic (shared
 // by the ideal_gas kernels and the advection sweep that also writes the
 // next step's equation of state)
