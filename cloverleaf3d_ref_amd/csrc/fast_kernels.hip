@@ -136,11 +136,14 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             const T lim_num = xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
                               xz * pgradx * pgradz + yz * pgrady * pgradz;
             const T rho = density0[i];
+            const T lim_den = tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16);
+            T limiter = 0;
+            if constexpr (sizeof(T) == 4) limiter = quo(lim_num, lim_den);  // cheaper than a branch
             T q;
             if ((lim_num > (T)0) || (div >= (T)0)) {
                 q = (T)0;
             } else {
-                const T limiter = quo(lim_num, tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16));
+                if constexpr (sizeof(T) == 8) limiter = quo(lim_num, lim_den);
                 pgradx = copysign(tmax((T)1.0e-16, fabs(pgradx)), pgradx);
                 pgrady = copysign(tmax((T)1.0e-16, fabs(pgrady)), pgrady);
                 pgradz = copysign(tmax((T)1.0e-16, fabs(pgradz)), pgradz);
@@ -158,7 +161,9 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             // maxima over the thread's cells, see below)
             const T c = soundspeed[i];
             T cc = c * c;
-            if (q != (T)0) cc = cc + quo((T)2 * q, rho);  // adding 2*0/rho would leave cc as it is
+            // in FP64 the divide is skipped where q is zero (adding 2*0/rho
+            // leaves cc as it is); the FP32 reciprocal is cheaper than a branch
+            if (sizeof(T) == 4 || q != (T)0) cc = cc + quo((T)2 * q, rho);
             mc = tmax(mc, ds * cc);
             const T du1 = ugradx1 * xa, du2 = ugradx2 * xa;
             mu = tmax(mu, tmax(fabs(du1), fabs(du2)));

@@ -136,11 +136,13 @@ __device__ inline T quo(T a, T b)
 }
 
 // a / b for b > 0 where a is mostly zero (the artificial viscosity outside
-// shocks): a zero a is its own quotient, and the divide is skipped when the
-// whole wavefront has a == 0
+// shocks): in FP64 a zero a is its own quotient, and the divide sequence is
+// skipped when the whole wavefront has a == 0 (the FP32 reciprocal is
+// cheaper than the branch)
 template <typename T>
 __device__ inline T quo_sparse(T a, T b)
 {
+    if constexpr (sizeof(T) == 4) return quo(a, b);
     T r = a;
     if (a != (T)0) r = quo(a, b);
     return r;
