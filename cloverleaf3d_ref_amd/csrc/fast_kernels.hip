@@ -370,26 +370,23 @@ __device__ inline void accel_node(const QuadS<S> &du, const QuadS<S> &pu, const 
 }
 
 // PdV corrector (PdV_kernel.f90:126-164) and flux_calc (flux_calc_kernel.f90:
-// 56-90) at plane l of column (j, k), offset i: node quads of the level-0
-// and level-1 velocities in planes l (..l) and l+1 (..u, read only when
-// lin: l <= z_max), the cell's level-0 state (d0, e0v, p0, q0)
-template <typename S>
-__device__ inline void corrector_plane(int i, bool jin, bool kin, bool lin, const QuadS<S> &u0l, const QuadS<S> &u1l,
-                                       const QuadS<S> &v0l, const QuadS<S> &v1l, const QuadS<S> &w0l,
-                                       const QuadS<S> &w1l, const QuadS<S> &u0u, const QuadS<S> &u1u,
-                                       const QuadS<S> &v0u, const QuadS<S> &v1u, const QuadS<S> &w0u,
-                                       const QuadS<S> &w1u, S d0, S e0v, S p0, S q0, S xa, S ya, S za, S vol, S tdt,
-                                       S c0, S *__restrict__ density1, S *__restrict__ energy1,
-                                       S *__restrict__ vol_flux_x, S *__restrict__ vol_flux_y,
-                                       S *__restrict__ vol_flux_z)
+// 56-90) at plane l of column (j, k), offset i.  at(c, a, b, p) is the
+// velocity component c (0-2 level 0 x/y/z, 3-5 level 1 x/y/z) of node
+// (j+a, k+b) in node plane l+p; plane l+1 is read only when lin
+// (l <= z_max).  The cell's level-0 state is (d0, e0v, p0, q0).
+template <typename S, typename NodeAt>
+__device__ inline void corrector_plane(int i, bool jin, bool kin, bool lin, const NodeAt &at, S d0, S e0v, S p0,
+                                       S q0, S xa, S ya, S za, S vol, S tdt, S c0, S *__restrict__ density1,
+                                       S *__restrict__ energy1, S *__restrict__ vol_flux_x,
+                                       S *__restrict__ vol_flux_y, S *__restrict__ vol_flux_z)
 {
     using T = S;
-#define U0(a, b, c) ((c) ? u0u : u0l).v[(a) + 2 * (b)]
-#define U1(a, b, c) ((c) ? u1u : u1l).v[(a) + 2 * (b)]
-#define V0(a, b, c) ((c) ? v0u : v0l).v[(a) + 2 * (b)]
-#define V1(a, b, c) ((c) ? v1u : v1l).v[(a) + 2 * (b)]
-#define W0(a, b, c) ((c) ? w0u : w0l).v[(a) + 2 * (b)]
-#define W1(a, b, c) ((c) ? w1u : w1l).v[(a) + 2 * (b)]
+#define U0(a, b, c) at(0, a, b, c)
+#define V0(a, b, c) at(1, a, b, c)
+#define W0(a, b, c) at(2, a, b, c)
+#define U1(a, b, c) at(3, a, b, c)
+#define V1(a, b, c) at(4, a, b, c)
+#define W1(a, b, c) at(5, a, b, c)
     T sl = 0, sb = 0;
     if (kin && lin) {
         sl = U0(0, 0, 0) + U0(0, 1, 0) + U0(0, 0, 1) + U0(0, 1, 1) + U1(0, 0, 0) + U1(0, 1, 0) + U1(0, 0, 1) + U1(0, 1, 1);
@@ -422,10 +419,10 @@ __device__ inline void corrector_plane(int i, bool jin, bool kin, bool lin, cons
         density1[i] = d0 * vol_change_s;
     }
 #undef U0
-#undef U1
 #undef V0
-#undef V1
 #undef W0
+#undef U1
+#undef V1
 #undef W1
 }
 
@@ -550,11 +547,14 @@ static hipError_t fast_ideal_gas_t(const Lay &g, const S *density, const S *ener
 // A workgroup is 64 lanes (x) by AW waves (y); lane 63 and wave AW-1 are a
 // halo column and row, so workgroups step by 63 and AW-1.  At node plane lp
 // every thread forms the level-1 velocity of its node (accel_node, as the
-// accelerate kernel does; owners store it), the plane's values go through
-// LDS (double-buffered, one barrier per plane), and the owners then run the
-// corrector and fluxes of plane lp-1 (corrector_plane) with node quads from
-// LDS (level 1) and memory (level 0).  Each chunk also forms the level-1
-// velocity of the plane above it, which the next chunk stores.
+// accelerate kernel does; owners store it) from its cells' quads (read from
+// memory one plane ahead) and its own level-0 velocity.  Both velocity
+// levels of the plane go to LDS, into a ring of three planes with one
+// barrier per plane, and the owners then run the corrector and fluxes of
+// plane lp-1 (corrector_plane) reading the node quads of planes lp-1 and lp
+// from the ring; the plane below is read back rather than held in
+// registers.  Each chunk also forms the level-1 velocity of the plane above
+// it, which the next chunk stores.
 constexpr int AW = 8;
 
 template <typename S>
@@ -569,7 +569,12 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
                                                S *__restrict__ vol_flux_z)
 {
     using T = S;  // arithmetic type of the form
-    __shared__ S vb[2][3][AW][64];
+    // node velocities (level 0 x/y/z, level 1 x/y/z) of three node planes.
+    // Slot s is written at plane t (before the barrier of t) and read at
+    // planes t and t+1 (after their barriers); it is written again at t+3,
+    // past the barrier of t+2, which every thread reaches only after its
+    // reads of t+1.
+    __shared__ S nb[3][6][AW][64];
     const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
     const int j = g.x_min + bk.x * 63 + lane;
     const int k = g.y_min + bk.y * (AW - 1) + w;
@@ -585,72 +590,71 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
     int i = idx(g, j, k, l0);
     // accelerate's cells of the plane below the node plane
     QuadS<S> dlo{}, plo{}, qlo{};
-    // the node plane's loads, issued one plane ahead of their use: cells of
-    // the plane (accelerate) and level-0 node quads
-    QuadS<S> dn{}, pn{}, qn{}, u0n{}, v0n{}, w0n{};
-    S e0n = 0;
-    auto load = [&](int ii) {
+    // the next plane's loads, issued one plane ahead of their use: its cells
+    // (accelerate), the node's level-0 velocity and the cell's energy0
+    QuadS<S> dn{}, pn{}, qn{};
+    S u0n = 0, v0n = 0, w0n = 0, e0n = 0;
+    auto load = [&](int ii, int lp) {
         dn = quad_down_raw(density0, ii, g.ld);
         pn = quad_down_raw(pressure, ii, g.ld);
         qn = quad_down_raw(viscosity, ii, g.ld);
-        u0n = quad_up_raw(xvel0, ii, g.ld);
-        v0n = quad_up_raw(yvel0, ii, g.ld);
-        w0n = quad_up_raw(zvel0, ii, g.ld);
+        u0n = xvel0[ii];
+        v0n = yvel0[ii];
+        w0n = zvel0[ii];
+        e0n = (lp <= g.z_max && jin && kin) ? energy0[ii] : (S)0;
     };
     if (node) {
         dlo = quad_down_raw(density0, i - g.plane, g.ld);
         plo = quad_down_raw(pressure, i - g.plane, g.ld);
         qlo = quad_down_raw(viscosity, i - g.plane, g.ld);
-        load(i);
+        load(i, l0);
     }
-    QuadS<S> u0p{}, v0p{}, w0p{}, u1p{}, v1p{}, w1p{};  // node quads of the plane below
-    T e0p = 0;                                          // energy0 of the cell below
+    T e0p = 0;  // energy0 of the cell below
     const int lend = min(l1 + 1, g.z_max + 1);
-    int b = 0;
+    int s = 0, sp = 2;  // ring slots of the node plane in hand and the one below
     for (int lp = l0; lp <= lend; lp++, i += g.plane) {
-        const QuadS<S> dhi = dn, phi = pn, qhi = qn, u0q = u0n, v0q = v0n, w0q = w0n;
-        const T e0q = (node && lp <= g.z_max && jin && kin) ? (T)energy0[i] : (T)0;
+        const QuadS<S> dhi = dn, phi = pn, qhi = qn;
+        const S u0 = u0n, v0 = v0n, w0 = w0n;
+        const T e0q = (T)e0n;
         // the next plane's loads go out before this plane's work, so they
         // have the whole iteration to arrive
-        if (node && lp < lend) load(i + g.plane);
+        if (node && lp < lend) load(i + g.plane, lp + 1);
         S u1 = 0, v1 = 0, w1 = 0;
         if (node) {
-            accel_node<S>(dhi, phi, qhi, dlo, plo, qlo, u0q.v[0], v0q.v[0], w0q.v[0], vol, xa, ya, za, tdt, u1, v1,
-                          w1);
+            accel_node<S>(dhi, phi, qhi, dlo, plo, qlo, u0, v0, w0, vol, xa, ya, za, tdt, u1, v1, w1);
             if (own && lp <= l1) {
                 xvel1[i] = u1;
                 yvel1[i] = v1;
                 zvel1[i] = w1;
             }
         }
-        vb[b][0][w][lane] = u1;
-        vb[b][1][w][lane] = v1;
-        vb[b][2][w][lane] = w1;
+        nb[s][0][w][lane] = u0;
+        nb[s][1][w][lane] = v0;
+        nb[s][2][w][lane] = w0;
+        nb[s][3][w][lane] = u1;
+        nb[s][4][w][lane] = v1;
+        nb[s][5][w][lane] = w1;
         __syncthreads();
-        const QuadS<S> u1q{{u1, vb[b][0][w][ln], vb[b][0][wn][lane], vb[b][0][wn][ln]}};
-        const QuadS<S> v1q{{v1, vb[b][1][w][ln], vb[b][1][wn][lane], vb[b][1][wn][ln]}};
-        const QuadS<S> w1q{{w1, vb[b][2][w][ln], vb[b][2][wn][lane], vb[b][2][wn][ln]}};
-        b ^= 1;
-        if (own && lp > l0)  // plane lp-1, nodes of planes lp-1 and lp; its cell is the quads' (j, k) corner
-            corrector_plane<S>(i - g.plane, jin, kin, true, u0p, u1p, v0p, v1p, w0p, w1p, u0q, u1q, v0q, v1q, w0q,
-                               w1q, dlo.v[3], e0p, plo.v[3], qlo.v[3], xa, ya, za, vol, tdt, c0, density1, energy1,
-                               vol_flux_x, vol_flux_y, vol_flux_z);
+        const S(*lo)[AW][64] = nb[sp];
+        const S(*up)[AW][64] = nb[s];
+        if (own && lp > l0) {  // plane lp-1, nodes of planes lp-1 and lp; its cell is the (j, k) corner
+            const auto at = [&](int c, int a, int bb, int pl) { return (pl ? up : lo)[c][bb ? wn : w][a ? ln : lane]; };
+            corrector_plane<S>(i - g.plane, jin, kin, true, at, dlo.v[3], e0p, plo.v[3], qlo.v[3], xa, ya, za, vol,
+                               tdt, c0, density1, energy1, vol_flux_x, vol_flux_y, vol_flux_z);
+        }
+        // the top z faces (plane z_max+1) use only their own node plane
+        if (own && lp == lend && l1 == g.z_max + 1) {
+            const auto at = [&](int c, int a, int bb, int) { return up[c][bb ? wn : w][a ? ln : lane]; };
+            corrector_plane<S>(i, jin, kin, false, at, (T)0, (T)0, (T)0, (T)0, xa, ya, za, vol, tdt, c0, density1,
+                               energy1, vol_flux_x, vol_flux_y, vol_flux_z);
+        }
         dlo = dhi;
         plo = phi;
         qlo = qhi;
-        u0p = u0q;
-        v0p = v0q;
-        w0p = w0q;
-        u1p = u1q;
-        v1p = v1q;
-        w1p = w1q;
         e0p = e0q;
+        sp = s;
+        s = s == 2 ? 0 : s + 1;
     }
-    // the top z faces (plane z_max+1) use only their own node plane
-    if (own && l1 == g.z_max + 1)
-        corrector_plane<S>(i - g.plane, jin, kin, false, u0p, u1p, v0p, v1p, w0p, w1p, u0p, u1p, v0p, v1p, w0p, w1p,
-                           (T)0, (T)0, (T)0, (T)0, xa, ya, za, vol, tdt, c0, density1, energy1, vol_flux_x,
-                           vol_flux_y, vol_flux_z);
 }
 
 template <typename S>
