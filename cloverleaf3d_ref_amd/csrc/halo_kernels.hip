@@ -105,7 +105,12 @@ __device__ inline void reflect_body(const Lay &g, int axis, int depth, S *dst, d
     const int b = mn[tb] - depth + (by * HY + (int)threadIdx.y);
     if (c > mx[ta] + ea + depth || b > mx[tb] + eb + depth) return;
     const int amax = mx[axis];
-    for (int d = 1; d <= depth; d++) {
+    // every depth's source lies inside the tile and every target in the halo,
+    // so all loads are issued before the first store
+    int iw[2], ir[2];
+    S v[2];
+    for (int d = 1; d <= 2; d++) {
+        if (d > depth) break;
         int w, r;
         if (!hi) {
             w = 1 - d;
@@ -117,12 +122,18 @@ __device__ inline void reflect_body(const Lay &g, int axis, int depth, S *dst, d
         } else {
             w = amax + d; r = amax - d;
         }
-        int iw[3], ir[3];
-        iw[axis] = w; ir[axis] = r;
-        iw[ta] = ir[ta] = c;
-        iw[tb] = ir[tb] = b;
-        const double v = sgn * (double)dst[idx(g, ir[0], ir[1], ir[2])];
-        dst[idx(g, iw[0], iw[1], iw[2])] = (S)(poison ? poisoned(v) : v);
+        int jw[3], jr[3];
+        jw[axis] = w; jr[axis] = r;
+        jw[ta] = jr[ta] = c;
+        jw[tb] = jr[tb] = b;
+        iw[d - 1] = idx(g, jw[0], jw[1], jw[2]);
+        ir[d - 1] = idx(g, jr[0], jr[1], jr[2]);
+        v[d - 1] = dst[ir[d - 1]];
+    }
+    for (int d = 1; d <= 2; d++) {
+        if (d > depth) break;
+        const double x = sgn * (double)v[d - 1];
+        dst[iw[d - 1]] = (S)(poison ? poisoned(x) : x);
     }
 }
 
@@ -138,6 +149,7 @@ static hipError_t run_batch_reflect(Store s, const Lay &g, int axis, int depth, 
                                     hipStream_t st)
 {
     if (n == 0) return hipSuccess;
+    if (depth < 1 || depth > 2) return hipErrorInvalidValue;  // reflect_body holds two depths
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
     const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
@@ -323,6 +335,7 @@ __global__ void kb_reflect(const DeckDev *__restrict__ decks, const int *__restr
 hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,
                          const int *fields, hipStream_t st, const int *axes)
 {
+    if (depth < 1 || depth > 2) return hipErrorInvalidValue;  // reflect_body holds two depths
     BItems b;
     b.n = 0;
     for (int side = 0; side < 2; side++) {
