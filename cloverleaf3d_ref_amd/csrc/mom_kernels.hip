@@ -36,7 +36,10 @@
 
 namespace clover {
 
-constexpr int MW = 8;       // waves per workgroup (wave 0 is the halo row)
+// waves per workgroup (wave 0 is the halo row) of the x (DIR 1), y (2) and
+// z (3) sweeps: the y/z marches run faster in 4-wave workgroups despite the
+// larger share of halo rows, the x sweep in 8-wave ones
+constexpr int mw_of(int dir) { return dir == 1 ? 8 : 4; }
 
 template <typename S>
 struct Vel3 {
@@ -111,8 +114,8 @@ struct Quad {
 // thread's neighbours: a at lane+dxa, p at lane+dxp, both also one wave
 // down.  Lanes and waves without such a neighbour get their own values
 // (used only by threads that store nothing).
-template <int DXA, int DXP, typename T>
-__device__ inline void exchange(T (*buf)[MW][64], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
+template <int DXA, int DXP, typename T, int W>
+__device__ inline void exchange(T (*buf)[W][64], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
 {
     buf[0][w][lane] = a;
     buf[1][w][lane] = p;
@@ -173,16 +176,16 @@ __device__ inline S raw_cell_mass(int mode, S V0, const Raw<S> &r, bool ok_p)
 }
 
 // exchange() in two halves, so loads can be issued between them
-template <typename T>
-__device__ inline void publish(T (*buf)[MW][64], int lane, int w, T a, T p)
+template <typename T, int W>
+__device__ inline void publish(T (*buf)[W][64], int lane, int w, T a, T p)
 {
     buf[0][w][lane] = a;
     buf[1][w][lane] = p;
     __syncthreads();
 }
 
-template <int DXA, int DXP, typename T>
-__device__ inline void gather(T (*buf)[MW][64], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
+template <int DXA, int DXP, typename T, int W>
+__device__ inline void gather(T (*buf)[W][64], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
 {
     const int la = min(max(lane + DXA, 0), 63), lp = min(max(lane + DXP, 0), 63), wd = w > 0 ? w - 1 : 0;
     A.c = a;
@@ -226,6 +229,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
                  const S *__restrict__ density1, Vel3<S> v)
 {
     using T = S;  // arithmetic type of the form
+    constexpr int MW = mw_of(DIR);
     __shared__ T xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int sM = DIR == 3 ? g.plane : g.ld;
@@ -331,7 +335,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
 }
 
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * MW) k_fast_mom_march(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+__global__ void __launch_bounds__(64 * mw_of(DIR)) k_fast_mom_march(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
@@ -352,6 +356,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
              Vel3<S> v)
 {
     using T = S;  // arithmetic type of the form
+    constexpr int MW = mw_of(1);
     __shared__ T xbuf[2][2][MW][64];
     const int lane = threadIdx.x, w = threadIdx.y;
     const T d = (T)geo.dx;
@@ -426,7 +431,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
 }
 
 template <typename S>
-__global__ void __launch_bounds__(64 * MW) k_fast_mom_x(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
+__global__ void __launch_bounds__(64 * mw_of(1)) k_fast_mom_x(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
              Vel3<S> v)
 {
@@ -445,6 +450,7 @@ static hipError_t fast_advec_mom_t(const Lay &g, const Geo &m, int mode, int dir
         v.out[c] = static_cast<S *>(vel_out[c]);
     }
     const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
+    const int MW = mw_of(direction);
     const dim3 blk(64, MW, 1);
     if (direction == 1) {
         const dim3 grid((nxn + 59) / 60, (nyn + MW - 2) / (MW - 1), nchunks(nzn, sh.planes));
@@ -490,6 +496,7 @@ hipError_t launch_fast_advec_mom(Store s, const Lay &g, const Geo &m, int advect
 __host__ __device__ inline void dims_mom(int dir, const Lay &g, int planes, int &gx, int &gy, int &gz)
 {
     const int nxn = g.x_max - g.x_min + 2, nyn = g.y_max - g.y_min + 2, nzn = g.z_max - g.z_min + 2;  // nodes
+    const int MW = mw_of(dir);
     if (dir == 1) {
         gx = (nxn + 59) / 60; gy = (nyn + MW - 2) / (MW - 1); gz = nchunks(nzn, planes);
     } else if (dir == 2) {
@@ -507,7 +514,7 @@ int batch_blocks_mom(int dir, const Lay &g, int planes)
 }
 
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * MW)
+__global__ void __launch_bounds__(64 * mw_of(DIR))
 kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int planes, int xcd)
 {
     const int hb = xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
@@ -548,7 +555,7 @@ hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGri
     else if (sweep_number == 3 && dir == 3) mode = 6;
     if (!mode || dir < 1 || dir > 3) return hipErrorInvalidValue;
     if (bg.total == 0) return hipSuccess;
-    const dim3 grid(bg.total), blk(64, MW);
+    const dim3 grid(bg.total), blk(64, mw_of(dir));
 #define L(D_, T_)                                                                                          \
     TIMED_LAUNCH("kb_mom", st, (kb_mom<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode, sh.planes, sh.xcd)
     if (s == ST_F32) {
