@@ -146,7 +146,7 @@ def ensemble_bench(ndecks: int, steps: int, fp32: bool = False, warmup: int = 2,
         t0 = time.perf_counter()
         before, after = e.autotune()
         autotune = dict(seconds=time.perf_counter() - t0, pass_ms_before=before, pass_ms_after=after,
-                        shapes=e.shapes())
+                        shapes=e.shapes(), workgroups=e.workgroups())
     e.run(warmup)
     t0 = time.perf_counter()
     e.run(steps)

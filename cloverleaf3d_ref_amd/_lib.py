@@ -112,6 +112,8 @@ def lib() -> ctypes.CDLL:
     L.clover_chunk_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.clover_ensemble_autotune.argtypes = [vp, dp, dp]
     L.clover_ensemble_get_shape.argtypes = [vp, ctypes.c_int, ip, ip, ip]
+    L.clover_ensemble_get_workgroups.argtypes = [vp, ctypes.c_int, ip]
+    L.clover_ensemble_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.clover_hip_fetch.argtypes = [vp]
     L.clover_hip_push.argtypes = [vp]
     L.clover_hip_forget.argtypes = [vp]

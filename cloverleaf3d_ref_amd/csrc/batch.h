@@ -15,6 +15,7 @@
 
 #include "driver.h"
 #include "fast.h"
+#include "launch.h"
 
 namespace clover {
 
@@ -43,6 +44,38 @@ enum BatchKernel : int {
 int batch_blocks(BatchKernel k, const Lay &g, int planes);
 // reflective halo along `axis` for `nitems` (field, face) items
 int batch_blocks_reflect(const Lay &g, int axis, int depth, int nitems);
+
+// The tiles (workgroup-sized blocks of the decks' grids, numbered through
+// the prefix table) that this workgroup runs: t, t+step, ... below end.
+// With a workgroup per tile that is one tile, in the hardware's or the
+// XCD-contiguous order.  A persistent launch (fewer workgroups than tiles)
+// walks the list: with xcd, the workgroups on XCD x (b mod 8: the hardware
+// deals workgroups round-robin) share the x-th contiguous eighth of the
+// tiles, so neighbouring tiles keep meeting in one L2.
+__device__ inline void tile_walk(int total, int xcd, int &t, int &end, int &step)
+{
+    const int b = (int)blockIdx.x, G = (int)gridDim.x;
+    if (G >= total) {
+        t = xcd ? xcd_block(b, G) : b;
+        end = t + 1;
+        step = 1;
+    } else if (!xcd || G < kXcds) {
+        t = b;
+        end = total;
+        step = G;
+    } else {
+        const int x = b % kXcds;
+        t = (int)((long long)total * x / kXcds) + b / kXcds;
+        end = (int)((long long)total * (x + 1) / kXcds);
+        step = (G - x + kXcds - 1) / kXcds;  // workgroups on XCD x
+    }
+}
+
+// workgroups of a batched launch of bg's tiles with shape sh
+inline unsigned batch_wgs(const BatchGrid &bg, const Shape &sh)
+{
+    return (unsigned)(sh.wgs > 0 && sh.wgs < bg.total ? sh.wgs : bg.total);
+}
 
 __device__ inline int deck_of(const int *__restrict__ start, int nd, int b)
 {

@@ -362,25 +362,30 @@ __global__ void __launch_bounds__(64 * CROWS)
 kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int gas, int planes,
         int xcd)
 {
-    const int hb = xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-    const int d = deck_of(start, nd, hb);
-    const DeckDev &D = decks[d];
-    int gx, gy, gz;
-    dims_cell(DIR, D.g, planes, gx, gy, gz);
-    const int lb = hb - start[d];
-    const Blk bk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
+    int hb, end, step;  // the tiles of this workgroup (persistent when the grid is short)
+    tile_walk(start[nd], xcd, hb, end, step);
+    for (; hb < end; hb += step) {
+        const int d = deck_of(start, nd, hb);
+        const DeckDev &D = decks[d];
+        int gx, gy, gz;
+        dims_cell(DIR, D.g, planes, gx, gy, gz);
+        const int lb = hb - start[d];
+        const Blk bk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
 #define CF(id) static_cast<const S *>(D.f[id])
 #define MF(id) static_cast<S *>(D.f[id])
-    if constexpr (DIR == 1)
-        cell_x_body<S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z), CF(F_DENSITY1),
-                       CF(F_ENERGY1), MF(F_MASS_FLUX_X), MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr,
-                       gas ? MF(F_SOUNDSPEED) : nullptr);
-    else
-        cell_march_body<DIR, S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z),
-                                CF(F_DENSITY1), CF(F_ENERGY1), MF(DIR == 2 ? F_MASS_FLUX_Y : F_MASS_FLUX_Z),
-                                MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr, gas ? MF(F_SOUNDSPEED) : nullptr);
+        if constexpr (DIR == 1)
+            cell_x_body<S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z), CF(F_DENSITY1),
+                           CF(F_ENERGY1), MF(F_MASS_FLUX_X), MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr,
+                           gas ? MF(F_SOUNDSPEED) : nullptr);
+        else
+            cell_march_body<DIR, S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z),
+                                    CF(F_DENSITY1), CF(F_ENERGY1), MF(DIR == 2 ? F_MASS_FLUX_Y : F_MASS_FLUX_Z),
+                                    MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr,
+                                    gas ? MF(F_SOUNDSPEED) : nullptr);
 #undef CF
 #undef MF
+        __syncthreads();  // the next tile reuses the workgroup's LDS
+    }
 }
 
 hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
@@ -392,7 +397,7 @@ hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGr
     else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
     if (!mode) return hipErrorInvalidValue;
     if (bg.total == 0) return hipSuccess;
-    const dim3 grid(bg.total), blk(64, CROWS);
+    const dim3 grid(batch_wgs(bg, sh)), blk(64, CROWS);
     if (gas && sweep_number != 3) return hipErrorInvalidValue;
 #define L(D_, T_)                                                                                          \
     TIMED_LAUNCH("kb_cell", st, (kb_cell<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode, gas ? 1 : 0,   \

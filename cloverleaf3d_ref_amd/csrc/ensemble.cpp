@@ -308,8 +308,22 @@ double tune_time(clover_ensemble *e, int bk, const Shape &sh, hipEvent_t a, hipE
     return ms / reps;
 }
 
+// captured steps hold the old shapes: drop them (the next step captures
+// again) and build the launch tables of the new ones
+void shapes_changed(clover_ensemble *e)
+{
+    for (auto &g : e->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    prepare_grids(e);
+}
+
 void autotune(clover_ensemble *e, double *before, double *after)
 {
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device), "CU count");
     hipEvent_t a = nullptr, b = nullptr;
     hip_check(hipEventCreate(&a), "event");
     hip_check(hipEventCreate(&b), "event");
@@ -324,26 +338,34 @@ void autotune(clover_ensemble *e, double *before, double *after)
         for (int planes : kTunePlanes)
             for (int xcd = 0; xcd < 2; xcd++) {
                 if (si == SK_CELL_X && planes != start.planes) continue;
-                const Shape cand{planes, xcd, start.bx};  // the batched workgroups are fixed at 64 x 4
-                if (cand.planes == start.planes && cand.xcd == start.xcd) continue;
+                const Shape cand{planes, xcd, start.bx, 0};  // the batched workgroups are fixed at 64 x 4
+                if (cand.planes == start.planes && cand.xcd == start.xcd && start.wgs == 0) continue;
                 const double ms = tune_time(e, bk, cand, a, b);
                 if (ms < best) {
                     best = ms;
                     best_shape = cand;
                 }
             }
-        e->shapes.k[si] = best_shape;
+        // then a persistent grid of 1, 2, 4 or 8 workgroups per CU walking
+        // the tiles, where that is fewer workgroups than tiles
+        const int tiles = grid(e, bk, best_shape.planes).total;
+        Shape pick = best_shape;
+        for (int per_cu : {1, 2, 4, 8}) {
+            Shape cand = best_shape;
+            cand.wgs = per_cu * cus;
+            if (cand.wgs >= tiles) break;
+            const double ms = tune_time(e, bk, cand, a, b);
+            if (ms < best) {
+                best = ms;
+                pick = cand;
+            }
+        }
+        e->shapes.k[si] = pick;
         t1 += best;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    // captured steps hold the old shapes
-    for (auto &g : e->graph)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
-    prepare_grids(e);
+    shapes_changed(e);
     if (before) *before = t0;
     if (after) *after = t1;
 }
@@ -586,6 +608,34 @@ int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes,
     *planes = e->shapes.k[kernel].planes;
     *xcd = e->shapes.k[kernel].xcd;
     *block_x = e->shapes.k[kernel].bx;
+    CLOVER_API_END
+}
+
+int clover_ensemble_get_workgroups(const clover_ensemble *e, int kernel, int *workgroups)
+{
+    CLOVER_API_BEGIN
+    if (!e || !workgroups) throw Error(E_ARG, "null argument");
+    if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
+    *workgroups = e->shapes.k[kernel].wgs;
+    CLOVER_API_END
+}
+
+int clover_ensemble_set_shape(clover_ensemble *e, int kernel, int planes, int xcd, int workgroups)
+{
+    CLOVER_API_BEGIN
+    if (!e) throw Error(E_ARG, "null ensemble");
+    if (kernel < 0 || kernel >= SK_COUNT) throw Error(E_ARG, "kernel out of range");
+    // calc_dt's partial buffer holds the visc/dt tiles of the fewest planes the tuner tries
+    if (planes < kTunePlanes[0]) throw Error(E_ARG, "planes below the smallest tuned value");
+    if (xcd != 0 && xcd != 1) throw Error(E_ARG, "xcd must be 0 or 1");
+    if (workgroups < 0) throw Error(E_ARG, "workgroups must be >= 0");
+    hip_check(hipSetDevice(e->device), "hipSetDevice");
+    hip_check(hipStreamSynchronize(e->st), "sync");
+    Shape &sh = e->shapes.k[kernel];
+    sh.planes = planes;
+    sh.xcd = xcd;
+    sh.wgs = workgroups;
+    shapes_changed(e);
     CLOVER_API_END
 }
 

@@ -47,7 +47,9 @@ enum ShapeKernel : int {
 struct Shape {
     int planes;
     int xcd;
-    int bx;  // z marches: threads along x per workgroup (64, 128, 256 or 512; see block_of)
+    int bx;   // z marches: threads along x per workgroup (64, 128, 256 or 512; see block_of)
+    int wgs;  // batched launches: 0 for one workgroup per tile, else this many
+              // persistent workgroups walk the tile list (batch.h tile_walk)
 };
 
 // Workgroup of a z march for shape sh: bx threads along x and 256/bx rows

@@ -788,15 +788,23 @@ __device__ inline Blk local_blk(int lb, int gx, int gy, int gz)
     return Blk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz};
 }
 
-// the deck and the block within it of this workgroup (xcd: in the
-// XCD-contiguous order over the whole launch)
-#define DECK_BLOCK(dims_call, xcd)                                                          \
-    const int hb = (xcd) ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;     \
+// the deck and the block within it of tile hb of the launch
+#define DECK_TILE(hb, dims_call)                                                            \
     const int d = deck_of(start, nd, hb);                                                   \
     const DeckDev &D = decks[d];                                                            \
     int gx, gy, gz;                                                                         \
     dims_call;                                                                              \
-    const Blk bk = local_blk(hb - start[d], gx, gy, gz);
+    const Blk bk = local_blk((hb) - start[d], gx, gy, gz);
+// ... of this workgroup's only tile (a launch with a workgroup per tile)
+#define DECK_BLOCK(dims_call, xcd)                                                          \
+    const int hb = (xcd) ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;     \
+    DECK_TILE(hb, dims_call)
+// the tiles this workgroup walks (tile_walk); each tile's body ends with a
+// barrier, since the next one reuses the workgroup's LDS
+#define FOR_TILES(xcd)                                                                      \
+    int t_, end_, step_;                                                                    \
+    tile_walk(start[nd], xcd, t_, end_, step_);                                             \
+    for (; t_ < end_; t_ += step_)
 #define CF(id) static_cast<const S *>(D.f[id])
 #define MF(id) static_cast<S *>(D.f[id])
 
@@ -815,29 +823,41 @@ __global__ void __launch_bounds__(BX *BY)
 kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd,
            double g_small, double *__restrict__ partial)
 {
-    DECK_BLOCK(dims_cells(D.g, planes, gx, gy, gz), xcd)
-    visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
-                    CF(F_DENSITY0), CF(F_PRESSURE), CF(F_SOUNDSPEED), MF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0),
-                    CF(F_ZVEL0), partial + start[d]);
+    FOR_TILES(xcd)
+    {
+        DECK_TILE(t_, dims_cells(D.g, planes, gx, gy, gz))
+        visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
+                        CF(F_DENSITY0), CF(F_PRESSURE), CF(F_SOUNDSPEED), MF(F_VISCOSITY), CF(F_XVEL0),
+                        CF(F_YVEL0), CF(F_ZVEL0), partial + start[d]);
+        __syncthreads();
+    }
 }
 
 template <typename S>
 __global__ void __launch_bounds__(BX *BY)
 kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
-    DECK_BLOCK(dims_cells(D.g, planes, gx, gy, gz), xcd)
-    pdv_predict_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), CF(F_VISCOSITY),
-                        CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
+    FOR_TILES(xcd)
+    {
+        DECK_TILE(t_, dims_cells(D.g, planes, gx, gy, gz))
+        pdv_predict_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), CF(F_VISCOSITY),
+                            CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
+        __syncthreads();
+    }
 }
 
 template <typename S>
 __global__ void __launch_bounds__(64 * AW)
 kb_accel_pdv(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
-    DECK_BLOCK(dims_accel_pdv(D.g, planes, gx, gy, gz), xcd)
-    accel_pdv_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), MF(F_DENSITY1), CF(F_ENERGY0), MF(F_ENERGY1),
-                      CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0), MF(F_XVEL1), CF(F_YVEL0), MF(F_YVEL1),
-                      CF(F_ZVEL0), MF(F_ZVEL1), MF(F_VOL_FLUX_X), MF(F_VOL_FLUX_Y), MF(F_VOL_FLUX_Z));
+    FOR_TILES(xcd)
+    {
+        DECK_TILE(t_, dims_accel_pdv(D.g, planes, gx, gy, gz))
+        accel_pdv_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), MF(F_DENSITY1), CF(F_ENERGY0), MF(F_ENERGY1),
+                          CF(F_PRESSURE), CF(F_VISCOSITY), CF(F_XVEL0), MF(F_XVEL1), CF(F_YVEL0), MF(F_YVEL1),
+                          CF(F_ZVEL0), MF(F_ZVEL1), MF(F_VOL_FLUX_X), MF(F_VOL_FLUX_Y), MF(F_VOL_FLUX_Z));
+        __syncthreads();
+    }
 }
 
 __host__ __device__ inline void dims_summary(const Lay &g, int &gx, int &gy, int &gz)
@@ -858,6 +878,8 @@ kb_field_summary(const DeckDev *__restrict__ decks, const int *__restrict__ star
                                 CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0), partial + 5 * start[d]);
 }
 #undef DECK_BLOCK
+#undef DECK_TILE
+#undef FOR_TILES
 #undef CF
 #undef MF
 
@@ -888,6 +910,11 @@ int batch_blocks(BatchKernel k, const Lay &g, int planes)
     do {                                                                                                     \
         if (bg.total > 0) TIMED_LAUNCH(name, st, kern, dim3(bg.total), block, 0, st, __VA_ARGS__);           \
     } while (0)
+// shaped kernels: persistent when the shape asks for fewer workgroups than tiles
+#define BLAUNCH_SH(name, kern, block, ...)                                                                    \
+    do {                                                                                                     \
+        if (bg.total > 0) TIMED_LAUNCH(name, st, kern, dim3(batch_wgs(bg, sh)), block, 0, st, __VA_ARGS__);  \
+    } while (0)
 
 hipError_t batch_ideal_gas(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, hipStream_t st)
 {
@@ -900,23 +927,25 @@ hipError_t batch_visc_dt(Store s, const DeckDev *decks, int nd, const BatchGrid 
                          double *partial, hipStream_t st)
 {
     if (s == ST_F32)
-        BLAUNCH("kb_visc_dt", kb_visc_dt<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd, g_small, partial);
+        BLAUNCH_SH("kb_visc_dt", kb_visc_dt<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd, g_small,
+                   partial);
     else
-        BLAUNCH("kb_visc_dt", kb_visc_dt<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd, g_small, partial);
+        BLAUNCH_SH("kb_visc_dt", kb_visc_dt<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd, g_small,
+                   partial);
     return hipGetLastError();
 }
 
 hipError_t batch_pdv_predict(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_pdv_predict", kb_pdv_predict<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
-    else BLAUNCH("kb_pdv_predict", kb_pdv_predict<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
+    if (s == ST_F32) BLAUNCH_SH("kb_pdv_predict", kb_pdv_predict<float>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
+    else BLAUNCH_SH("kb_pdv_predict", kb_pdv_predict<double>, dim3(BX, BY), decks, bg.start, nd, sh.planes, sh.xcd);
     return hipGetLastError();
 }
 
 hipError_t batch_accel_pdv(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, const Shape &sh, hipStream_t st)
 {
-    if (s == ST_F32) BLAUNCH("kb_accel_pdv", kb_accel_pdv<float>, dim3(64, AW), decks, bg.start, nd, sh.planes, sh.xcd);
-    else BLAUNCH("kb_accel_pdv", kb_accel_pdv<double>, dim3(64, AW), decks, bg.start, nd, sh.planes, sh.xcd);
+    if (s == ST_F32) BLAUNCH_SH("kb_accel_pdv", kb_accel_pdv<float>, dim3(64, AW), decks, bg.start, nd, sh.planes, sh.xcd);
+    else BLAUNCH_SH("kb_accel_pdv", kb_accel_pdv<double>, dim3(64, AW), decks, bg.start, nd, sh.planes, sh.xcd);
     return hipGetLastError();
 }
 
@@ -931,5 +960,6 @@ hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const Batc
     return hipGetLastError();
 }
 #undef BLAUNCH
+#undef BLAUNCH_SH
 
 }  // namespace clover

@@ -514,10 +514,9 @@ int batch_blocks_mom(int dir, const Lay &g, int planes)
 }
 
 template <int DIR, typename S>
-__global__ void __launch_bounds__(64 * mw_of(DIR))
-kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int planes, int xcd)
+__device__ __forceinline__ void mom_tile(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd,
+                                         int mode, int planes, int hb)
 {
-    const int hb = xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     const int d = deck_of(start, nd, hb);
     const DeckDev &D = decks[d];
     int gx, gy, gz;
@@ -543,6 +542,18 @@ kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd,
                                vfz, rho, v);
 }
 
+template <int DIR, typename S>
+__global__ void __launch_bounds__(64 * mw_of(DIR))
+kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int planes, int xcd)
+{
+    int hb, end, step;  // the tiles of this workgroup (persistent when the grid is short)
+    tile_walk(start[nd], xcd, hb, end, step);
+    for (; hb < end; hb += step) {
+        mom_tile<DIR, S>(decks, start, nd, mode, planes, hb);
+        __syncthreads();  // the next tile reuses the workgroup's LDS
+    }
+}
+
 hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
                            int sweep_number, const Shape &sh, hipStream_t st)
 {
@@ -555,7 +566,7 @@ hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGri
     else if (sweep_number == 3 && dir == 3) mode = 6;
     if (!mode || dir < 1 || dir > 3) return hipErrorInvalidValue;
     if (bg.total == 0) return hipSuccess;
-    const dim3 grid(bg.total), blk(64, mw_of(dir));
+    const dim3 grid(batch_wgs(bg, sh)), blk(64, mw_of(dir));
 #define L(D_, T_)                                                                                          \
     TIMED_LAUNCH("kb_mom", st, (kb_mom<D_, T_>), grid, blk, 0, st, decks, bg.start, nd, mode, sh.planes, sh.xcd)
     if (s == ST_F32) {

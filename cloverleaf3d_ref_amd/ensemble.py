@@ -82,6 +82,22 @@ class Ensemble:
                    "clover_ensemble_autotune")
         return before.value, after.value
 
+    def workgroups(self) -> Dict[str, int]:
+        """Workgroups of each shaped batched kernel's launch: 0 for one per
+        tile, else a persistent grid walking the tile list."""
+        out = {}
+        for k, name in enumerate(_lib.SHAPE_KERNELS):
+            n = ctypes.c_int()
+            _lib.check(self._L.clover_ensemble_get_workgroups(self._h, k, ctypes.byref(n)),
+                       "clover_ensemble_get_workgroups")
+            out[name] = n.value
+        return out
+
+    def set_shape(self, kernel: str, planes: int, xcd: int, workgroups: int = 0) -> None:
+        """Launch shape of one batched kernel (planes >= 8); never changes a value."""
+        _lib.check(self._L.clover_ensemble_set_shape(self._h, _lib.SHAPE_KERNELS.index(kernel), planes, xcd,
+                                                     workgroups), "clover_ensemble_set_shape")
+
     def shapes(self) -> Dict[str, Tuple[int, int, int]]:
         out = {}
         for k, name in enumerate(_lib.SHAPE_KERNELS):

@@ -305,6 +305,15 @@ long long clover_ensemble_device_bytes(const clover_ensemble *e);
  * are captured again with the chosen shapes on the next run. */
 int clover_ensemble_autotune(clover_ensemble *e, double *ms_before, double *ms_after);
 int clover_ensemble_get_shape(const clover_ensemble *e, int kernel, int *planes, int *xcd, int *block_x);
+/* Workgroups of a batched kernel's launch: 0 for one per tile (a tile is
+ * one workgroup-sized block of one deck), else that many persistent
+ * workgroups walk the ensemble's tile list.  No counterpart in the
+ * reference (it has one deck); chosen by clover_ensemble_autotune. */
+int clover_ensemble_get_workgroups(const clover_ensemble *e, int kernel, int *workgroups);
+/* Set a batched kernel's launch shape by hand (CLOVER_SHAPE_* kernel, planes
+ * >= 8, xcd 0/1, workgroups as above).  Changes how the work is split,
+ * never a value; captured steps are dropped and captured again. */
+int clover_ensemble_set_shape(clover_ensemble *e, int kernel, int planes, int xcd, int workgroups);
 
 #ifdef __cplusplus
 }

@@ -94,3 +94,32 @@ def test_rejects_tiled_and_thin_decks():
     for bad in (tiled, thin):
         with pytest.raises(CloverError):
             Ensemble([bad])
+
+
+@pytest.mark.parametrize("wgs,xcd", [(3, 1), (8, 0), (13, 1), (200, 1)])
+def test_persistent_tile_walks_keep_every_bit(wgs, xcd):
+    """Short grids whose workgroups walk the tile list (fewer workgroups than
+    XCDs, a stride that does not divide the tiles, one that does) give the
+    fields of one workgroup per tile."""
+    from cloverleaf3d_ref_amd._lib import SHAPE_KERNELS
+    decks = _uneven()
+    print(f"[gpu] persistent walk wgs={wgs} xcd={xcd}", flush=True)
+    with Ensemble(decks) as a, Ensemble(decks) as b:
+        for name in SHAPE_KERNELS:
+            b.set_shape(name, 64 if name == "cell_x" else 16, xcd, wgs)
+        assert set(b.workgroups().values()) == {wgs}
+        a.run(9, graphs=True)
+        b.run(4, graphs=False)
+        b.run(5, graphs=True)
+        for d in range(len(decks)):
+            for name in ("density0", "energy0", "pressure", "xvel0", "yvel0", "zvel0", "viscosity"):
+                assert np.array_equal(a.fetch(d, name), b.fetch(d, name)), (d, name)
+        assert a.state()[2] == b.state()[2]
+
+
+def test_set_shape_rejects_bad_values():
+    from cloverleaf3d_ref_amd._lib import CloverError
+    with Ensemble(_uneven()[:2]) as e:
+        for args in (("visc_dt", 4, 0, 0), ("visc_dt", 16, 2, 0), ("mom_y", 16, 0, -1)):
+            with pytest.raises(CloverError):
+                e.set_shape(*args)
