@@ -203,6 +203,34 @@ def calibration_copy(mib: int) -> None:
     torch.cuda.empty_cache()
 
 
+def hbm_probe(mib: int = 1024, reps: int = 5) -> dict:
+    """What plain streaming kernels attain on this device (torch's
+    elementwise kernels on FP64 arrays of `mib` MiB): copy (1 read, 1
+    write), add (2 reads, 1 write) and fill (1 write), in GB/s.  The
+    context for the stencil kernels' roofline fractions, which are
+    against the published 8 TB/s."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
+    out = {}
+    for name, arrays, op in (("copy", 2, lambda: c.copy_(a)), ("add", 3, lambda: torch.add(a, b, out=c)),
+                             ("fill", 1, lambda: c.fill_(1.0))):
+        op()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op()
+        e1.record()
+        e1.synchronize()
+        out[f"{name}_gbs"] = arrays * n * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b, c
+    torch.cuda.empty_cache()
+    out["arrays_mib"] = mib
+    return out
+
+
 def cpu_baseline(cells: int, steps: int, threads: int):
     """The reference's compiled kernels through the hydro loop on the host
     (run in a child process so its OpenMP runtime picks up `threads`)."""
@@ -257,6 +285,7 @@ def main():
 
     if args.calib_copy_mib:
         calibration_copy(args.calib_copy_mib)
+    probe = hbm_probe()  # before the hydro library opens the device
     tune = not args.no_autotune
     wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup, tune=tune)
     n = args.cells
@@ -320,6 +349,8 @@ def main():
         cpu = cpu_baseline(args.cpu_cells, args.cpu_steps, threads)
 
     total_ms = sum(v[0] for v in ktimes.values())
+    if roofline.get("achieved"):
+        probe["dominant_frac_of_add"] = roofline["achieved"] / probe["add_gbs"]
     out = {
         "metric": "cell-steps/s at 512^3 FP64 vs FP32-store; HBM GB/s; batched decks/s",
         "value": value,
@@ -341,6 +372,7 @@ def main():
         "batched": batched,
         "tiled": tiled,
         "step_hbm_gbs_kbc": 1192 * value / 1e9,
+        "hbm_attainable": probe,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
                                sorted(ktimes.items(), key=lambda kv: -kv[1][0])},
         "device_time_fraction": total_ms / (wall * 1e3),
