@@ -3,17 +3,20 @@
 //
 // In the FP64 form each kernel evaluates exactly the expressions of the
 // reference loop nests it replaces, in the same order and without FMA
-// contraction, so its outputs are bit-identical to the per-call kernels'; the
-// savings are in HBM bytes: scalar geometry for the uniform mesh, fused
+// contraction, so its outputs are bit-identical to the per-call kernels'
+// (divisions by a mesh constant use launch.h by_const, whose FMA sequence
+// returns the correctly rounded quotient, the same bits as the division);
+// the savings are in HBM bytes: scalar geometry for the uniform mesh, fused
 // neighbours, dead stores dropped, intermediates recomputed in registers.
 // The FP32 form runs the same expressions in FP32 (see by_const / quo); the
 // dt minimum and the summary sums stay FP64.
 //
 // The stencils here read two neighbouring planes (l and l+1 for cells and
 // faces, l-1 and l for nodes).  Each thread owns one (j, k) column and
-// marches up a chunk of planes, keeping the lower plane's 2x2 corner values
-// in registers, so every plane is read from memory once per column instead
-// of once per plane that uses it.
+// marches up a chunk of planes, keeping the lower plane's values in
+// registers (or, in accel_pdv, in an LDS ring shared by the workgroup), so
+// every plane is read from memory once per column instead of once per
+// plane that uses it.
 #include <hip/hip_runtime.h>
 
 #include "batch.h"
