@@ -96,15 +96,16 @@ def test_rejects_tiled_and_thin_decks():
             Ensemble([bad])
 
 
-@pytest.mark.parametrize("wgs,xcd", [(3, 1), (8, 0), (13, 1), (200, 1)])
-def test_persistent_tile_walks_keep_every_bit(wgs, xcd):
+@pytest.mark.parametrize("wgs,xcd,fp32", [(3, 1, False), (8, 0, False), (13, 1, False), (200, 1, False),
+                                          (13, 1, True)])
+def test_persistent_tile_walks_keep_every_bit(wgs, xcd, fp32):
     """Short grids whose workgroups walk the tile list (fewer workgroups than
     XCDs, a stride that does not divide the tiles, one that does) give the
     fields of one workgroup per tile."""
     from cloverleaf3d_ref_amd._lib import SHAPE_KERNELS
     decks = _uneven()
-    print(f"[gpu] persistent walk wgs={wgs} xcd={xcd}", flush=True)
-    with Ensemble(decks) as a, Ensemble(decks) as b:
+    print(f"[gpu] persistent walk wgs={wgs} xcd={xcd} fp32={fp32}", flush=True)
+    with Ensemble(decks, fp32=fp32) as a, Ensemble(decks, fp32=fp32) as b:
         for name in SHAPE_KERNELS:
             b.set_shape(name, 64 if name == "cell_x" else 16, xcd, wgs)
         assert set(b.workgroups().values()) == {wgs}
