@@ -107,10 +107,20 @@ __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 // expressions exactly (true divisions, same order); the FP32 form computes in
 // FP32 and divides by the uniform mesh's constants through reciprocals formed
 // once per thread, and by everything else through the hardware reciprocal.
+// min/max: in FP64 the reference's comparison (the operand it keeps on a
+// tie, sign of zero included); in FP32 the hardware's min/max instruction
 template <typename T>
-__device__ inline T tmin(T a, T b) { return a < b ? a : b; }
+__device__ inline T tmin(T a, T b)
+{
+    if constexpr (sizeof(T) == 4) return __builtin_fminf(a, b);
+    else return a < b ? a : b;
+}
 template <typename T>
-__device__ inline T tmax(T a, T b) { return a > b ? a : b; }
+__device__ inline T tmax(T a, T b)
+{
+    if constexpr (sizeof(T) == 4) return __builtin_fmaxf(a, b);
+    else return a > b ? a : b;
+}
 
 // a / b for a mesh constant b (fast.h Recip): in FP64 the correctly rounded
 // quotient, equal to a / b, from three FMAs and a multiply instead of the
