@@ -119,17 +119,17 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 #undef V
 #undef W
             const T div = (xa * (ugradx2 - ugradx1) + ya * (vgrady2 - vgrady1)) + za * (wgradz2 - wgradz1);
-            const T xx = by_const<T>((T)0.25 * (ugradx2 - ugradx1), m.rx);
-            const T yy = by_const<T>((T)0.25 * (vgrady2 - vgrady1), m.ry);
-            const T zz = by_const<T>((T)0.25 * (wgradz2 - wgradz1), m.rz);
-            const T xy = by_const<T>((T)0.25 * (ugrady2 - ugrady1), m.ry) + by_const<T>((T)0.25 * (vgradx2 - vgradx1), m.rx);
-            const T xz = by_const<T>((T)0.25 * (ugradz2 - ugradz1), m.rz) + by_const<T>((T)0.25 * (wgradx2 - wgradx1), m.rx);
-            const T yz = by_const<T>((T)0.25 * (vgradz2 - vgradz1), m.rz) + by_const<T>((T)0.25 * (wgrady2 - wgrady1), m.ry);
+            const T xx = by_const_scaled<T>(ugradx2 - ugradx1, 0.25, m.rx);
+            const T yy = by_const_scaled<T>(vgrady2 - vgrady1, 0.25, m.ry);
+            const T zz = by_const_scaled<T>(wgradz2 - wgradz1, 0.25, m.rz);
+            const T xy = by_const_scaled<T>(ugrady2 - ugrady1, 0.25, m.ry) + by_const_scaled<T>(vgradx2 - vgradx1, 0.25, m.rx);
+            const T xz = by_const_scaled<T>(ugradz2 - ugradz1, 0.25, m.rz) + by_const_scaled<T>(wgradx2 - wgradx1, 0.25, m.rx);
+            const T yz = by_const_scaled<T>(vgradz2 - vgradz1, 0.25, m.rz) + by_const_scaled<T>(wgrady2 - wgrady1, 0.25, m.ry);
             // p difference / (celld + celld): halving the difference instead
             // of doubling the divisor gives the same quotient (both exact)
-            T pgradx = by_const<T>((T)0.5 * ((T)pressure[i + 1] - (T)pressure[i - 1]), m.rx);
-            T pgrady = by_const<T>((T)0.5 * ((T)pressure[i + g.ld] - (T)pressure[i - g.ld]), m.ry);
-            T pgradz = by_const<T>((T)0.5 * (p_hi - p_lo), m.rz);
+            T pgradx = by_const_scaled<T>((T)pressure[i + 1] - (T)pressure[i - 1], 0.5, m.rx);
+            T pgrady = by_const_scaled<T>((T)pressure[i + g.ld] - (T)pressure[i - g.ld], 0.5, m.ry);
+            T pgradz = by_const_scaled<T>(p_hi - p_lo, 0.5, m.rz);
             const T pgradx2 = pgradx * pgradx;
             const T pgrady2 = pgrady * pgrady;
             const T pgradz2 = pgradz * pgradz;

@@ -137,6 +137,15 @@ __device__ inline T by_const(T a, const Recip &r)
     }
 }
 
+// (s * a) / b for a constant scale s and a mesh constant b: FP64 forms s*a
+// and divides as by_const does; FP32 multiplies a by s/b (one product)
+template <typename T>
+__device__ inline T by_const_scaled(T a, double s, const Recip &r)
+{
+    if constexpr (sizeof(T) == 8) return by_const<T>((T)s * a, r);
+    else return a * (T)(s * r.hn);
+}
+
 // a / b
 template <typename T>
 __device__ inline T quo(T a, T b)
