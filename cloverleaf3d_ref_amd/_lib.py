@@ -84,6 +84,7 @@ def lib() -> ctypes.CDLL:
     L.clover_chunk_destroy.argtypes = [vp]
     L.clover_chunk_step.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.clover_chunk_field_summary.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.clover_chunk_visit_prepare.argtypes = [vp]
     L.clover_chunk_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(CloverSummary), ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int)]
     L.clover_chunk_state.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),

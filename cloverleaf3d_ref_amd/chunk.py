@@ -113,6 +113,12 @@ class Chunk:
         step, time, _ = self.state()
         return dict(step=step, time=time, vol=out[0], mass=out[1], ie=out[2], ke=out[3], press=out[4])
 
+    def visit_prepare(self) -> None:
+        """The device work visit.f90:65-87 does before its dump (ideal_gas,
+        pressure and velocity halos, viscosity); fetch() then returns what
+        the dump writes."""
+        _lib.check(self._L.clover_chunk_visit_prepare(self._h), "clover_chunk_visit_prepare")
+
     def run(self, max_steps: Optional[int] = None, max_rows: int = 4096) -> List[Dict[str, float]]:
         """The hydro loop to end_step/end_time; summary rows as the
         reference prints them (excluding the start row)."""

@@ -2,7 +2,7 @@
 summary lines, in its own formats, so a run here can be diffed against a
 reference run line by line.
 
-    python -m cloverleaf3d_ref_amd.clover_out clover.in [-o clover.out] [--fp32] [--per-call]
+    python -m cloverleaf3d_ref_amd.clover_out clover.in [-o clover.out] [--fp32] [--per-call] [--visit-dir D]
 
 Lines with explicit Fortran formats are reproduced character for character:
   * the step line, timestep.f90:115-116
@@ -55,14 +55,19 @@ def fmt_f(x: float, w: int, d: int) -> str:
 
 
 def fmt_e(x: float, w: int, d: int) -> str:
-    """Fortran Ew.d without a scale factor: 0.d1..dd E+xx."""
+    """Fortran Ew.d without a scale factor: 0.d1..dd E+xx, or 0.d1..dd +xxx
+    when the exponent needs three digits."""
     if x == 0.0:
         s = "0." + "0" * d + "E+00"
     else:
         m, e = f"{abs(x):.{d - 1}e}".split("e")  # d significant digits, correctly rounded
         digits = m.replace(".", "")
         exp = int(e) + 1
-        s = ("-" if x < 0 else "") + "0." + digits + ("E+" if exp >= 0 else "E-") + f"{abs(exp):02d}"
+        sign = "+" if exp >= 0 else "-"
+        if abs(exp) > 999:
+            return "*" * w
+        tail = ("E" + sign + f"{abs(exp):02d}") if abs(exp) <= 99 else (sign + f"{abs(exp):03d}")
+        s = ("-" if x < 0 else "") + "0." + digits + tail
     return s.rjust(w) if len(s) <= w else "*" * w
 
 
@@ -101,9 +106,14 @@ def test_problem_lines(deck: Deck, ke: float) -> List[str]:
     return [f" Test problem 1 is within {qa_diff!r} % of the expected solution", " " + verdict]
 
 
-def run(deck: Deck, out: TextIO, fp32: bool = False, per_call: bool = False, autotune: bool = True) -> int:
+def run(deck: Deck, out: TextIO, fp32: bool = False, per_call: bool = False, autotune: bool = True,
+        visit_dir: str = ".") -> int:
     """The hydro loop of hydro.f90:46-95 on the device, writing clover.out's
-    step and summary lines.  Returns the steps taken."""
+    step and summary lines, and, when the deck sets visit_frequency, the
+    VTK dumps of visit.f90 into visit_dir.  Returns the steps taken."""
+    from .visit import VisitWriter
+
+    visit = VisitWriter(deck, visit_dir) if deck.visit_frequency else None
     with Chunk(deck, per_call=per_call, fp32=fp32) as c:
         if autotune and not per_call:
             c.autotune()
@@ -112,6 +122,8 @@ def run(deck: Deck, out: TextIO, fp32: bool = False, per_call: bool = False, aut
         step, t, _ = c.state()
         for line in summary_lines(step, t, c.field_summary()):
             w(line + "\n")
+        if visit:  # start.f90:138
+            visit.dump(c)
         w(" Starting the calculation\n")
         t0 = _time.perf_counter()
         while True:
@@ -129,12 +141,16 @@ def run(deck: Deck, out: TextIO, fp32: bool = False, per_call: bool = False, aut
             if deck.summary_frequency and step % deck.summary_frequency == 0:
                 for line in summary_lines(step, t, c.field_summary()):
                     w(line + "\n")
+            if visit and step % deck.visit_frequency == 0:  # hydro.f90:73-75
+                visit.dump(c)
             if t + G_SMALL > deck.end_time or step >= deck.end_step:
                 s = c.field_summary()
                 for line in summary_lines(step, t, s):
                     w(line + "\n")
                 for line in test_problem_lines(deck, s["ke"]):
                     w(line + "\n")
+                if visit:  # hydro.f90:88
+                    visit.dump(c)
                 break
         wall = _time.perf_counter() - t0
         w("\n Calculation complete\n Clover is finishing\n")
@@ -149,11 +165,12 @@ def main(argv=None) -> int:
     ap.add_argument("--fp32", action="store_true", help="the FP32-store form")
     ap.add_argument("--per-call", action="store_true", help="one kernel per reference loop nest")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--visit-dir", default=".", help="where the VTK dumps go when the deck sets visit_frequency")
     a = ap.parse_args(argv)
     deck = parse_deck(open(a.deck).read())
     out = sys.stdout if a.out == "-" else open(a.out, "w")
     try:
-        run(deck, out, fp32=a.fp32, per_call=a.per_call, autotune=not a.no_autotune)
+        run(deck, out, fp32=a.fp32, per_call=a.per_call, autotune=not a.no_autotune, visit_dir=a.visit_dir)
     finally:
         if out is not sys.stdout:
             out.close()

@@ -636,6 +636,40 @@ static void field_summary(clover_chunk *c, double *out)
     for (int q = 0; q < 5; q++) out[q] = s[q];
 }
 
+// visit.f90:65-87: the device work before the dump: ideal_gas on every tile,
+// the pressure and level-0 velocity halos at depth 1, then viscosity.  The
+// next timestep recomputes the same values from the same inputs, so the run
+// continues unchanged.
+static void visit_prepare(clover_chunk *c)
+{
+    ideal_gas(c, false);
+    int fields[NUM_FIELDS];
+    set_fields(fields, {F_PRESSURE, F_XVEL0, F_YVEL0, F_ZVEL0});
+    update_halo(c, fields, 1);
+    const clover_deck &d = c->deck;
+    Phase ph(c, P_VISCOSITY);
+    for (size_t ti = 0; ti < c->tiles.size(); ti++) {
+        Tile &t = c->tiles[ti];
+        if (c->fast) {
+            // the fused kernel's viscosity is bit-identical to the per-call
+            // kernel's; its dt partials are overwritten by the next timestep
+            int np = 0;
+            hip_check(launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe,
+                                          d.dtdiv_safe, c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE],
+                                          t.f[F_SOUNDSPEED], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
+                                          t.f[F_ZVEL0], c->partial + ti * kMaxPartials * 5, &np,
+                                          c->shapes.k[SK_VISC_DT], c->st),
+                      "visc_dt");
+        } else {
+            hip_check(launch_viscosity(t.g, t.f[X_XAREA], t.f[X_YAREA], t.f[X_ZAREA], t.celldx, t.celldy, t.celldz,
+                                       t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0],
+                                       t.f[F_YVEL0], t.f[F_ZVEL0], c->st),
+                      "viscosity");
+        }
+    }
+    hip_check(hipStreamSynchronize(c->st), "sync");
+}
+
 // ---------------------------------------------------------------- autotuner
 // Launches shaped kernel k over every tile with its current shape, writing
 // only the spare work arrays: pdv_predict updates pressure in place, so it
@@ -956,6 +990,15 @@ int clover_chunk_field_summary(clover_chunk *c, double *out)
     if (!c || !out) throw Error(E_ARG, "null argument");
     KTIMER_SCOPE(c);
     field_summary(c, out);
+    CLOVER_API_END
+}
+
+int clover_chunk_visit_prepare(clover_chunk *c)
+{
+    CLOVER_API_BEGIN
+    if (!c) throw Error(E_ARG, "null chunk");
+    KTIMER_SCOPE(c);
+    visit_prepare(c);
     CLOVER_API_END
 }
 

@@ -206,6 +206,13 @@ int clover_chunk_step(clover_chunk *c, double *dt);
  * tile order).  out: vol, mass, ie, ke, press. */
 int clover_chunk_field_summary(clover_chunk *c, double *out);
 
+/* The device work of visit.f90:65-87 before a VTK dump: ideal_gas on every
+ * tile, the pressure and level-0 velocity halos at depth 1, viscosity.
+ * Afterwards clover_chunk_fetch_field returns the values visit.f90:127-198
+ * writes.  The run continues unchanged: the next timestep recomputes the
+ * same values. */
+int clover_chunk_visit_prepare(clover_chunk *c);
+
 /* The hydro loop until end_step / end_time (hydro.f90:46-87) or max_steps
  * more steps, with a summary row every summary_frequency steps and at the
  * end.  Row 0 (start) is not included; call clover_chunk_field_summary. */

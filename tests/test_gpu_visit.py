@@ -1,0 +1,104 @@
+"""visit.f90's VTK dumps from device runs.
+
+The dump's values are the fields after visit.f90's ideal_gas, halo update and
+viscosity (Chunk.visit_prepare).  They are checked against the per-call
+kernels' fields, which other tests pin bit for bit to the reference's
+compiled kernels, and the pressure against ideal_gas_kernel.f90:54
+evaluated here.  A run with dumps must also print the same clover.out as one
+without.
+"""
+from __future__ import annotations
+
+import io
+
+import numpy as np
+import pytest
+
+from cloverleaf3d_ref_amd.chunk import Chunk
+from cloverleaf3d_ref_amd.clover_out import run
+from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
+from cloverleaf3d_ref_amd.visit import file_name, vertex_coords, vtk_text
+
+pytestmark = pytest.mark.gpu
+
+
+def _deck(cells=12, end_step=7, visit_frequency=3, tiles=1):
+    d = parse_deck(cuboid_deck(cells, end_step, 5, tiles=tiles, ny=10, nz=8))
+    d.visit_frequency = visit_frequency
+    return d
+
+
+def _expected_dump(chunk: Chunk, deck, t: int = 0) -> str:
+    (nx, ny, nz), (ox, oy, oz), _ = chunk.tile_info(t)
+    cell = lambda n: chunk.fetch(n, t)[2:nx + 2, 2:ny + 2, 2:nz + 2]
+    vert = lambda n: chunk.fetch(n, t)[2:nx + 3, 2:ny + 3, 2:nz + 3]
+    return vtk_text(vertex_coords(deck, nx, ox, 0), vertex_coords(deck, ny, oy, 1), vertex_coords(deck, nz, oz, 2),
+                    cell("density0"), cell("energy0"), cell("pressure"), cell("viscosity"), vert("xvel0"),
+                    vert("yvel0"), vert("zvel0"))
+
+
+def _clover_out(deck, **kw) -> list:
+    buf = io.StringIO()
+    run(deck, buf, **kw)
+    return [l for l in buf.getvalue().splitlines() if "Wall clock" not in l]
+
+
+def test_dump_schedule_and_values(tmp_path):
+    deck = _deck()
+    print("[gpu] visit dumps, cuboid 12x10x8, 7 steps, every 3", flush=True)
+    with_dumps = _clover_out(deck, visit_dir=str(tmp_path))
+    index = (tmp_path / "clover.visit").read_text().split("\n")
+    assert index == ["!NBLOCKS     0"] + [file_name(1, s) for s in (0, 3, 6, 7)] + [""]
+    # the per-call kernels, stepped to 6 and prepared the same way
+    with Chunk(deck, per_call=True) as c:
+        for _ in range(6):
+            c.step()
+        c.visit_prepare()
+        want = _expected_dump(c, deck)
+        rho, e, p = c.fetch("density0"), c.fetch("energy0"), c.fetch("pressure")
+        q = c.fetch("viscosity")
+    assert (tmp_path / file_name(1, 6)).read_text() == want
+    # ideal_gas_kernel.f90:54, (1.4_8-1.0_8)*density*energy, on the interior
+    sl = (slice(2, 14), slice(2, 12), slice(2, 10))
+    assert np.array_equal(p[sl], ((1.4 - 1.0) * rho[sl]) * e[sl])
+    assert (q[sl] > 0).any()  # the shock has reached the viscosity by step 6
+    # dumps leave the run unchanged
+    deck0 = _deck(visit_frequency=0)
+    assert with_dumps == _clover_out(deck0)
+
+
+def test_fp32_dump_matches_its_own_fields(tmp_path):
+    deck = _deck(end_step=4, visit_frequency=4)
+    print("[gpu] visit dumps, FP32-store form", flush=True)
+    _clover_out(deck, visit_dir=str(tmp_path), fp32=True)
+    with Chunk(deck, fp32=True) as c:
+        for _ in range(4):
+            c.step()
+        c.visit_prepare()
+        want = _expected_dump(c, deck)
+    assert (tmp_path / file_name(1, 4)).read_text() == want
+
+
+def test_tiled_dump_files(tmp_path):
+    deck = _deck(end_step=2, visit_frequency=2, tiles=2)
+    print("[gpu] visit dumps, 2 tiles", flush=True)
+    _clover_out(deck, visit_dir=str(tmp_path))
+    index = (tmp_path / "clover.visit").read_text().split("\n")
+    names = [file_name(t, s, 0, 2) for s in (0, 2, 2) for t in (1, 2)]
+    assert index == ["!NBLOCKS     0"] + names + [""]
+    one = _deck(end_step=2, visit_frequency=2)
+    with Chunk(deck) as c, Chunk(one) as c1:
+        c1.visit_prepare()
+        rho1 = c1.fetch("density0")
+        for t in range(2):
+            text = (tmp_path / file_name(t + 1, 0, 0, 2)).read_text()
+            (nx, ny, nz), (ox, oy, oz), _ = c.tile_info(t)
+            lines = text.split("\n")
+            i = lines.index("X_COORDINATES " + f"{nx + 1:5d}" + " double")
+            xs = [float(v) for v in lines[i + 1:i + 2 + nx]]
+            assert xs == pytest.approx(vertex_coords(deck, nx, ox, 0), rel=5e-4)  # four significant digits
+            i = lines.index("density 1 " + f"{nx * ny * nz:20d}" + " double")
+            dens = np.array([float(v) for v in lines[i + 1:i + 1 + nx * ny * nz]]).reshape(nx, ny, nz, order="F")
+            # the start state of a tile is the same cells of the one-tile run
+            ref = rho1[ox + 1:ox + 1 + nx, oy + 1:oy + 1 + ny, oz + 1:oz + 1 + nz]
+            assert dens == pytest.approx(ref, rel=5e-4)
