@@ -690,6 +690,11 @@ static void tune_launch(clover_chunk *c, int k)
                                     d.dtdiv_safe, c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
                                     t.f[X_W3], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
                                     c->partial + ti * kMaxPartials * 5, &np, sh, c->st);
+            // the shape sets the number of partials, so their minimum is
+            // part of its cost (the value lands where the next timestep
+            // writes its own)
+            if (e == hipSuccess)
+                e = launch_min_final(c->partial + ti * kMaxPartials * 5, np, c->g_big, c->scalar + 8 * ti, c->st);
             break;
         }
         case SK_PDV_PREDICT:

@@ -77,19 +77,31 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
         int i = idx(g, j, k, l0);
         QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
         T p_lo = pressure[i - g.plane], p_c = pressure[i];
-        // the upper plane's loads run one plane ahead of their use
+        // loads run one plane ahead of their use: the upper node plane's
+        // velocities and pressure, and the next cell's density, soundspeed
+        // and x/y pressure neighbours
         QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
                  wn = quad_up_raw(zvel0, i + g.plane, g.ld);
         S pn = pressure[i + g.plane];
+        S rn = density0[i], cn = soundspeed[i];
+        S pxm = pressure[i - 1], pxp = pressure[i + 1], pym = pressure[i - g.ld], pyp = pressure[i + g.ld];
         for (int l = l0; l <= l1; l++, i += g.plane) {
             const QuadS<S> uu = un, vu = vn, wu = wn;
             const T p_hi = pn;
+            const T rho = rn, c = cn;
+            const T p_xm = pxm, p_xp = pxp, p_ym = pym, p_yp = pyp;
             if (l < l1) {
-                const int i2 = i + 2 * g.plane;
+                const int i1 = i + g.plane, i2 = i + 2 * g.plane;
                 un = quad_up_raw(xvel0, i2, g.ld);
                 vn = quad_up_raw(yvel0, i2, g.ld);
                 wn = quad_up_raw(zvel0, i2, g.ld);
                 pn = pressure[i2];
+                rn = density0[i1];
+                cn = soundspeed[i1];
+                pxm = pressure[i1 - 1];
+                pxp = pressure[i1 + 1];
+                pym = pressure[i1 - g.ld];
+                pyp = pressure[i1 + g.ld];
             }
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
@@ -127,8 +139,8 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             const T yz = by_const_scaled<T>(vgradz2 - vgradz1, 0.25, m.rz) + by_const_scaled<T>(wgrady2 - wgrady1, 0.25, m.ry);
             // p difference / (celld + celld): halving the difference instead
             // of doubling the divisor gives the same quotient (both exact)
-            T pgradx = by_const_scaled<T>((T)pressure[i + 1] - (T)pressure[i - 1], 0.5, m.rx);
-            T pgrady = by_const_scaled<T>((T)pressure[i + g.ld] - (T)pressure[i - g.ld], 0.5, m.ry);
+            T pgradx = by_const_scaled<T>(p_xp - p_xm, 0.5, m.rx);
+            T pgrady = by_const_scaled<T>(p_yp - p_ym, 0.5, m.ry);
             T pgradz = by_const_scaled<T>(p_hi - p_lo, 0.5, m.rz);
             const T pgradx2 = pgradx * pgradx;
             const T pgrady2 = pgrady * pgrady;
@@ -138,7 +150,6 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             // (a positive quotient that underflows to zero gives q = 0 there)
             const T lim_num = xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
                               xz * pgradx * pgradz + yz * pgrady * pgradz;
-            const T rho = density0[i];
             const T lim_den = tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16);
             T limiter = 0;
             if constexpr (sizeof(T) == 4) limiter = quo(lim_num, lim_den);  // cheaper than a branch
@@ -162,7 +173,6 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 
             // calc_dt: the denominators of the five candidates (their
             // maxima over the thread's cells, see below)
-            const T c = soundspeed[i];
             T cc = c * c;
             // in FP64 the divide is skipped where q is zero (adding 2*0/rho
             // leaves cc as it is); the FP32 reciprocal is cheaper than a branch
@@ -821,8 +831,9 @@ kb_ideal_gas(const DeckDev *__restrict__ decks, const int *__restrict__ start, i
     ideal_gas_body<S>(bk, g, r, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), MF(F_SOUNDSPEED));
 }
 
+// three waves per SIMD in FP64 (168 VGPRs), which the compiler would miss by one register
 template <typename S>
-__global__ void __launch_bounds__(BX *BY)
+__global__ void __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(3)))
 kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd,
            double g_small, double *__restrict__ partial)
 {

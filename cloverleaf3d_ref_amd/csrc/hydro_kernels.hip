@@ -220,25 +220,39 @@ hipError_t launch_calc_dt_cells(const Lay &g, double dtc_safe, double dtu_safe, 
     return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(RED_THREADS) k_min_final(const double *__restrict__ partial, int n, double seed,
+// One workgroup; each thread keeps four independent minima so that four
+// loads are in flight at a time.  The minimum does not depend on the order.
+constexpr int MIN_THREADS = 1024;
+
+__global__ void __launch_bounds__(MIN_THREADS) k_min_final(const double *__restrict__ partial, int n, double seed,
                                                            double *__restrict__ out)
 {
-    double m = seed;
-    for (int c = threadIdx.x; c < n; c += RED_THREADS) m = dmin(m, partial[c]);
-    __shared__ double red[RED_THREADS / 64];
+    double m0 = seed, m1 = seed, m2 = seed, m3 = seed;
+    int c = threadIdx.x;
+    for (; c + 3 * MIN_THREADS < n; c += 4 * MIN_THREADS) {
+        const double a = partial[c], b = partial[c + MIN_THREADS], d = partial[c + 2 * MIN_THREADS],
+                     e = partial[c + 3 * MIN_THREADS];
+        m0 = dmin(m0, a);
+        m1 = dmin(m1, b);
+        m2 = dmin(m2, d);
+        m3 = dmin(m3, e);
+    }
+    for (; c < n; c += MIN_THREADS) m0 = dmin(m0, partial[c]);
+    double m = dmin(dmin(m0, m1), dmin(m2, m3));
+    __shared__ double red[MIN_THREADS / 64];
     m = wave_min(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = red[0];
-        for (int w = 1; w < RED_THREADS / 64; w++) b = dmin(b, red[w]);
+        for (int w = 1; w < MIN_THREADS / 64; w++) b = dmin(b, red[w]);
         *out = b;
     }
 }
 
 hipError_t launch_min_final(const double *partial, int n, double seed, double *out, hipStream_t st)
 {
-    TIMED_LAUNCH("k_min_final", st, k_min_final, dim3(1), dim3(RED_THREADS), 0, st, partial, n, seed, out);
+    TIMED_LAUNCH("k_min_final", st, k_min_final, dim3(1), dim3(MIN_THREADS), 0, st, partial, n, seed, out);
     return hipGetLastError();
 }
 
