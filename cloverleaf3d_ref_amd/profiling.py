@@ -1,8 +1,13 @@
 """Summaries of rocprofv3 output for the hydro kernels.
 
-    python -m cloverleaf3d_ref_amd.profiling stats DIR --command CMD -o OUT.txt
+    python -m cloverleaf3d_ref_amd.profiling stats DIR --command CMD -o OUT.txt [--type double|float] [--last N]
         per-kernel calls / total / average from a --kernel-trace --stats run
-        (DIR holds *_kernel_stats.csv)
+        (DIR holds *_kernel_stats.csv and *_kernel_trace.csv), optionally only
+        the kernels instantiated for one element type.  With --last N, also
+        each marching kernel's average over its last N launches in the trace:
+        with N = the run's warmup + timed steps these are the steps' own
+        launches, which follow the autotuner's trial launches (the stats
+        table counts those trials too, for every candidate shape)
 
     python -m cloverleaf3d_ref_amd.profiling traffic FETCH_DIR WRITE_DIR --copy-mib M --command CMD -o OUT.json
         HBM bytes per dispatch of every kernel from two --pmc runs (FETCH_SIZE,
@@ -43,15 +48,49 @@ def _find(d: str, suffix: str) -> str:
     return hits[0]
 
 
-def kernel_stats(d: str) -> List[Tuple[str, int, float, float, float]]:
-    """(kernel, calls, total_us, avg_us, percent), largest total first."""
+def _elem_type(name: str) -> str:
+    """'float' or 'double' for a kernel template instantiated on one, else ''."""
+    head = name.split("(")[0]
+    if "<" not in head:
+        return ""
+    args = head.split("<", 1)[1]
+    return "float" if "float" in args else "double" if "double" in args else ""
+
+
+def _keep(name: str, elem: str) -> bool:
+    return not elem or _elem_type(name) in ("", elem)
+
+
+def kernel_stats(d: str, elem: str = "") -> List[Tuple[str, int, float, float, float]]:
+    """(kernel, calls, total_us, avg_us, percent of the kept rows), largest
+    total first; elem keeps only the kernels instantiated for that type
+    (and the untyped ones)."""
     rows = []
     with open(_find(d, "kernel_stats.csv")) as f:
         for r in csv.DictReader(f):
-            rows.append((_short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]) / 1e3,
-                         float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+            if _keep(r["Name"], elem):
+                rows.append((_short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]) / 1e3,
+                             float(r["AverageNs"]) / 1e3))
+    total = sum(r[2] for r in rows) or 1.0
+    rows = [(n, c, t, a, 100.0 * t / total) for n, c, t, a in rows]
     rows.sort(key=lambda r: -r[2])
     return rows
+
+
+def last_launches(d: str, n: int, elem: str = "") -> Dict[str, Tuple[int, float]]:
+    """kernel -> (launches, avg_us) over its last n launches in the kernel
+    trace (fewer if it ran fewer times)."""
+    per: Dict[str, List[Tuple[int, float]]] = defaultdict(list)
+    with open(_find(d, "kernel_trace.csv")) as f:
+        for r in csv.DictReader(f):
+            if _keep(r["Kernel_Name"], elem):
+                t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                per[_short(r["Kernel_Name"])].append((t0, (t1 - t0) / 1e3))
+    out = {}
+    for k, launches in per.items():
+        us = [t for _, t in sorted(launches)[-n:]]
+        out[k] = (len(us), sum(us) / len(us))
+    return out
 
 
 def counter_per_dispatch(d: str, counter: str) -> List[Tuple[str, float]]:
@@ -105,6 +144,9 @@ def main(argv=None) -> int:
     s.add_argument("dir")
     s.add_argument("--command", required=True)
     s.add_argument("-o", "--out", required=True)
+    s.add_argument("--type", choices=("double", "float"), default="",
+                   help="only the kernels instantiated for this element type")
+    s.add_argument("--last", type=int, default=0, help="also average each marching kernel's last N launches")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -113,12 +155,21 @@ def main(argv=None) -> int:
     t.add_argument("-o", "--out", required=True)
     a = ap.parse_args(argv)
     if a.what == "stats":
-        rows = kernel_stats(a.dir)
+        rows = kernel_stats(a.dir, a.type)
+        what = f", kernels instantiated for {a.type} (untyped ones count every launch)" if a.type else ""
         with open(a.out, "w") as f:
-            f.write(f"# rocprofv3 --kernel-trace --stats\n# command: {a.command}\n")
+            f.write(f"# rocprofv3 --kernel-trace --stats{what}\n# command: {a.command}\n")
+            f.write("# every launch, the autotuner's trials of each candidate shape included:\n")
             f.write(f"{'kernel':<40} {'calls':>7} {'total_us':>12} {'avg_us':>10} {'pct':>6}\n")
             for name, calls, tot, avg, pct in rows:
                 f.write(f"{name:<40} {calls:>7} {tot:>12.1f} {avg:>10.2f} {pct:>6.2f}\n")
+            if a.last:
+                tail = last_launches(a.dir, a.last, a.type)
+                f.write(f"# from the trace: each marching kernel's last {a.last} launches (the steps' own):\n")
+                f.write(f"{'kernel':<40} {'calls':>7} {'avg_us':>10}\n")
+                for name, calls, tot, avg, pct in rows:
+                    if name.startswith("k_fast_") and calls > 1:
+                        f.write(f"{name:<40} {tail[name][0]:>7} {tail[name][1]:>10.2f}\n")
     else:
         res = traffic(a.fetch_dir, a.write_dir, a.copy_mib)
         res["command"] = a.command
