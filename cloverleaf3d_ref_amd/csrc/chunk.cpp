@@ -163,11 +163,16 @@ void decompose(const clover_deck &d, std::vector<Tile> &tiles)
     const int T = d.tiles_per_chunk;
     const int xc = d.x_cells, yc = d.y_cells, zc = d.z_cells;
     int cur_x = 1, cur_y = 1, cur_z = T;
+    // clover.f90:282-285 and 323-326 form surface and volume in default REAL
+    // (single precision: `1.0*x_cells` is a REAL(4) product) and only widen
+    // the two results to REAL(8) before dividing.  Where two splits tie in
+    // exact arithmetic, the single-precision rounding decides which wins.
     auto metric = [&](int cx, int cy, int cz) {
-        double surface = (((1.0 * xc) / cx) * ((1.0 * yc) / cy) * 2) + (((1.0 * xc) / cx) * ((1.0 * zc) / cz) * 2) +
-                         (((1.0 * yc) / cy) * ((1.0 * zc) / cz) * 2);
-        double volume = ((1.0 * xc) / cx) * ((1.0 * yc) / cy) * ((1.0 * zc) / cz);
-        return surface / volume;
+        const float fx = (1.0f * (float)xc) / (float)cx, fy = (1.0f * (float)yc) / (float)cy,
+                    fz = (1.0f * (float)zc) / (float)cz;
+        const float s = ((fx * fy) * 2.0f + (fx * fz) * 2.0f) + (fy * fz) * 2.0f;
+        const float v = (fx * fy) * fz;
+        return (double)s / (double)v;
     };
     double best = metric(cur_x, cur_y, cur_z);
     int chunk_x = cur_x, chunk_y = cur_y, chunk_z = cur_z;
@@ -1071,6 +1076,29 @@ int clover_chunk_tile_info(const clover_chunk *c, int t, int *dims3, int *offs3,
     if (offs3) { offs3[0] = tt.left; offs3[1] = tt.bottom; offs3[2] = tt.back; }
     if (neighbours6) memcpy(neighbours6, tt.nbr, sizeof(tt.nbr));
     return 0;
+}
+
+int clover_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_chunk, int tile_dims, int max_tiles,
+                           int *dims3, int *offs3, int *neighbours6)
+{
+    if (x_cells < 1 || y_cells < 1 || z_cells < 1 || tiles_per_chunk < 1 || tile_dims < 1 || tile_dims > 3)
+        return E_ARG;
+    if (tiles_per_chunk > max_tiles) return E_ARG;
+    clover_deck d{};
+    d.x_cells = x_cells;
+    d.y_cells = y_cells;
+    d.z_cells = z_cells;
+    d.tiles_per_chunk = tiles_per_chunk;
+    d.tile_dims = tile_dims;
+    std::vector<Tile> tiles;
+    decompose(d, tiles);
+    for (size_t t = 0; t < tiles.size(); t++) {
+        const Tile &tt = tiles[t];
+        if (dims3) { dims3[3 * t] = tt.nx; dims3[3 * t + 1] = tt.ny; dims3[3 * t + 2] = tt.nz; }
+        if (offs3) { offs3[3 * t] = tt.left; offs3[3 * t + 1] = tt.bottom; offs3[3 * t + 2] = tt.back; }
+        if (neighbours6) memcpy(neighbours6 + 6 * t, tt.nbr, sizeof(tt.nbr));
+    }
+    return (int)tiles.size();
 }
 
 int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host)

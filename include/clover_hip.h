@@ -227,6 +227,13 @@ int clover_chunk_dt_control(const clover_chunk *c, int *control, int *jdt, int *
 int clover_chunk_tiles(const clover_chunk *c, int *ntiles);
 /* tile t: interior extents, global offsets, neighbours[6] */
 int clover_chunk_tile_info(const clover_chunk *c, int t, int *dims3, int *offs3, int *neighbours6);
+/* The tile split clover_chunk_create makes, on the host only (no device
+ * call): clover_decompose_tile, clover.f90:258-447, with its surface and
+ * volume in default REAL (clover.f90:282-285, 323-326).  Fills 3 extents,
+ * 3 global offsets (first cell, from 1) and 6 neighbours (tile ids from 1,
+ * -1 external) per tile; returns the tile count or E_ARG. */
+int clover_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_chunk, int tile_dims, int max_tiles,
+                           int *dims3, int *offs3, int *neighbours6);
 
 /* Copy one field of tile t into a dense Fortran array of the field's own
  * class.  field: 0..18 the FIELD_* ids minus one (data.f90:55-74); 19 volume,

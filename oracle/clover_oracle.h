@@ -210,6 +210,20 @@ int oracle_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_
                            int *t_nx, int *t_ny, int *t_nz, int *t_left, int *t_bottom, int *t_back,
                            int *neighbours);
 
+/* update_halo.f90:27-88 over caller-owned tiles (tile passes in
+ * update_tile_halo.f90's order, then each tile's reflective pass), with the
+ * kernels of lib (NULL: the C restatement; else symbols under the prefix
+ * set by oracle_set_kernel_prefix).  dims: 3 per tile; nbr: 6 per tile;
+ * arrays: 19 per tile in update_tile_halo argument order. */
+int oracle_update_halo_tiles(int ntiles, const int *dims, const int *nbr, double *const *arrays, const int *fields,
+                             int depth, const char *lib);
+
+/* Gather the final interior fields of the next oracle_run into whole-chunk
+ * arrays: 13 pointers (density0, energy0, pressure, viscosity, soundspeed,
+ * density1, energy1, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1), cells
+ * X*Y*Z and nodes (X+1)*(Y+1)*(Z+1), Fortran order; NULL entries skipped. */
+void oracle_set_dump(double *const *dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,14 @@ typedef void (*fn_tile_halo)(int, OR_DIMS, double *const *, int, int, int, int, 
 typedef void (*fn_summary)(OR_DIMS, const double *, const double *, const double *, const double *,
                            const double *, const double *, const double *, double *, double *, double *, double *,
                            double *);
+typedef void (*fn_init_chunk)(OR_DIMS, double, double, double, double, double, double, double *, double *,
+                              double *, double *, double *, double *, double *, double *, double *, double *,
+                              double *, double *, double *, double *, double *, double *);
+typedef void (*fn_gen_chunk)(OR_DIMS, const double *, const double *, const double *, const double *,
+                             const double *, const double *, double *, double *, double *, double *, double *, int,
+                             const double *, const double *, const double *, const double *, const double *,
+                             const double *, const double *, const double *, const double *, const double *,
+                             const double *, const double *, const int *);
 
 typedef struct {
     fn_ideal_gas ideal_gas;
@@ -72,6 +80,8 @@ typedef struct {
     fn_update_halo update_halo;
     fn_tile_halo tile_halo;
     fn_summary summary;
+    fn_init_chunk init_chunk;
+    fn_gen_chunk gen_chunk;
 } ktab;
 
 enum { F_D0, F_D1, F_E0, F_E1, F_P, F_Q, F_C, F_U0, F_U1, F_V0, F_V1, F_W0, F_W1, F_VFX, F_VFY, F_VFZ, F_MFX, F_MFY, F_MFZ };
@@ -130,17 +140,28 @@ static void free_field(otile *t)
     for (int i = 1; i <= 7; i++) free(t->w[i]);
 }
 
+/* clover.f90:282-285 and 323-326: surface and volume are default-REAL
+ * (single-precision) expressions, `1.0*x_cells` being a REAL(4) product;
+ * only their values are stored in the REAL(8) variables, and the metric is
+ * the REAL(8) quotient.  C's float arithmetic (SSE, no excess precision)
+ * evaluates the same operations in the same order. */
+static double tile_metric(int x_cells, int y_cells, int z_cells, int cx, int cy, int cz)
+{
+    float surface = (((1.0f * (float)x_cells) / (float)cx) * ((1.0f * (float)y_cells) / (float)cy) * 2.0f) +
+                    (((1.0f * (float)x_cells) / (float)cx) * ((1.0f * (float)z_cells) / (float)cz) * 2.0f) +
+                    (((1.0f * (float)y_cells) / (float)cy) * ((1.0f * (float)z_cells) / (float)cz) * 2.0f);
+    float volume = ((1.0f * (float)x_cells) / (float)cx) * ((1.0f * (float)y_cells) / (float)cy) *
+                   ((1.0f * (float)z_cells) / (float)cz);
+    return (double)surface / (double)volume;
+}
+
 /* clover.f90:258-447 */
 int oracle_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_chunk, int tile_dims,
                            int *t_nx, int *t_ny, int *t_nz, int *t_left, int *t_bottom, int *t_back,
                            int *neighbours)
 {
     int current_x = 1, current_y = 1, current_z = tiles_per_chunk;
-    double surface = (((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * 2) +
-                     (((1.0 * x_cells) / current_x) * ((1.0 * z_cells) / current_z) * 2) +
-                     (((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z) * 2);
-    double volume = ((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z);
-    double best_metric = surface / volume;
+    double best_metric = tile_metric(x_cells, y_cells, z_cells, current_x, current_y, current_z);
     int chunk_x = current_x, chunk_y = current_y, chunk_z = current_z;
     int x_bound = tiles_per_chunk, y_bound = tiles_per_chunk;
     if (tile_dims == 2) { x_bound = 1; y_bound = tiles_per_chunk; }
@@ -155,11 +176,7 @@ int oracle_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_
             current_y = j;
             if (tiles_per_chunk % (t * j) != 0) continue;
             current_z = tiles_per_chunk / (t * j);
-            surface = (((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * 2) +
-                      (((1.0 * x_cells) / current_x) * ((1.0 * z_cells) / current_z) * 2) +
-                      (((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z) * 2);
-            volume = ((1.0 * x_cells) / current_x) * ((1.0 * y_cells) / current_y) * ((1.0 * z_cells) / current_z);
-            double current_metric = surface / volume;
+            double current_metric = tile_metric(x_cells, y_cells, z_cells, current_x, current_y, current_z);
             if (current_metric < best_metric) {
                 chunk_x = current_x; chunk_y = current_y; chunk_z = current_z;
                 best_metric = current_metric;
@@ -459,6 +476,14 @@ static int load_lib(const char *path, const char *prefix, ktab *k)
     SYM(update_halo, "update_halo_kernel");
     SYM(summary, "field_summary_kernel");
 #undef SYM
+    /* The set-up kernels are optional: the reference's own are in _ref, a
+     * drop-in kernel library need not have them (the C ones run then). */
+    snprintf(name, sizeof name, "%sinitialise_chunk_kernel", prefix);
+    *(void **)(&k->init_chunk) = dlsym(h, name);
+    if (!k->init_chunk) k->init_chunk = oracle_initialise_chunk_kernel;
+    snprintf(name, sizeof name, "%sgenerate_chunk_kernel", prefix);
+    *(void **)(&k->gen_chunk) = dlsym(h, name);
+    if (!k->gen_chunk) k->gen_chunk = oracle_generate_chunk_kernel;
     static const char *tile_names[6] = {"update_tile_halo_left_kernel", "update_tile_halo_right_kernel",
                                         "update_tile_halo_bottom_kernel", "update_tile_halo_top_kernel",
                                         "update_tile_halo_back_kernel", "update_tile_halo_front_kernel"};
@@ -485,27 +510,101 @@ void oracle_set_kernel_prefix(const char *prefix)
     snprintf(kernel_prefix, sizeof kernel_prefix, "%s", prefix ? prefix : "ref_");
 }
 
+/* The kernel table: the C restatement (lib NULL) or a library's kernels. */
+static int get_ktab(const char *lib, ktab *k)
+{
+    if (lib) return load_lib(lib, kernel_prefix, k);
+    k->ideal_gas = oracle_ideal_gas_kernel;
+    k->viscosity = oracle_viscosity_kernel;
+    k->calc_dt = oracle_calc_dt_kernel;
+    k->pdv = oracle_PdV_kernel;
+    k->revert = oracle_revert_kernel;
+    k->accelerate = oracle_accelerate_kernel;
+    k->flux_calc = oracle_flux_calc_kernel;
+    k->advec_cell = oracle_advec_cell_kernel;
+    k->advec_mom = oracle_advec_mom_kernel;
+    k->reset = oracle_reset_field_kernel;
+    k->update_halo = oracle_update_halo_kernel;
+    k->tile_halo = oracle_update_tile_halo_face;
+    k->summary = oracle_field_summary_kernel;
+    k->init_chunk = oracle_initialise_chunk_kernel;
+    k->gen_chunk = oracle_generate_chunk_kernel;
+    return 0;
+}
+
+/* Test hook: update_halo.f90:27-88 (the tile passes of update_tile_halo.f90
+ * in their order, then each tile's reflective pass) over caller-owned
+ * tiles, through the same driver code oracle_run uses.  dims: 3 per tile;
+ * nbr: 6 per tile (tile ids from 1, -1 external); arrays: 19 per tile in
+ * update_tile_halo argument order.  Returns 0, or -1 if lib cannot load. */
+int oracle_update_halo_tiles(int ntiles, const int *dims, const int *nbr, double *const *arrays, const int *fields,
+                             int depth, const char *lib)
+{
+    ktab k;
+    if (get_ktab(lib, &k)) return -1;
+    ostate s;
+    memset(&s, 0, sizeof(s));
+    s.k = &k;
+    s.ntiles = ntiles;
+    s.tiles = calloc(ntiles, sizeof(otile));
+    for (int ti = 0; ti < ntiles; ti++) {
+        otile *t = &s.tiles[ti];
+        double *const *a = arrays + 19 * ti;
+        t->nx = dims[3 * ti]; t->ny = dims[3 * ti + 1]; t->nz = dims[3 * ti + 2];
+        memcpy(t->nbr, nbr + 6 * ti, 6 * sizeof(int));
+        t->d0 = a[0]; t->e0 = a[1]; t->p = a[2]; t->q = a[3]; t->c = a[4]; t->d1 = a[5]; t->e1 = a[6];
+        t->u0 = a[7]; t->v0 = a[8]; t->w0 = a[9]; t->u1 = a[10]; t->v1 = a[11]; t->w1 = a[12];
+        t->vfx = a[13]; t->vfy = a[14]; t->vfz = a[15]; t->mfx = a[16]; t->mfy = a[17]; t->mfz = a[18];
+    }
+    update_halo(&s, fields, depth);
+    free(s.tiles);
+    return 0;
+}
+
+/* Final-field dump of the next oracle_run: interior values gathered from
+ * every tile into whole-chunk arrays (Fortran order; cells X*Y*Z, nodes
+ * (X+1)*(Y+1)*(Z+1); a node on a tile boundary takes the last tile's
+ * value).  dst: 13 pointers, density0, energy0, pressure, viscosity,
+ * soundspeed, density1, energy1, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1;
+ * NULL entries are skipped.  The setting holds for one run. */
+static double *dump_dst[13];
+static int dump_on = 0;
+
+void oracle_set_dump(double *const *dst)
+{
+    dump_on = dst != NULL;
+    for (int i = 0; i < 13; i++) dump_dst[i] = dst ? dst[i] : NULL;
+}
+
+static void dump_fields(const ostate *s)
+{
+    const oracle_deck *d = s->deck;
+    const size_t X = d->x_cells, Y = d->y_cells;
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        const otile *t = &s->tiles[ti];
+        const double *src[13] = {t->d0, t->e0, t->p, t->q, t->c, t->d1, t->e1,
+                                 t->u0, t->v0, t->w0, t->u1, t->v1, t->w1};
+        for (int f = 0; f < 13; f++) {
+            if (!dump_dst[f]) continue;
+            const int node = f >= 7;
+            const size_t ex = t->nx + 4 + node, ey = t->ny + 4 + node; /* tile extents */
+            const size_t gx = X + node, gy = Y + node;                 /* dump extents */
+            for (int l = 1; l <= t->nz + node; l++)
+                for (int kk = 1; kk <= t->ny + node; kk++)
+                    for (int j = 1; j <= t->nx + node; j++) {
+                        size_t si = (size_t)(j + 1) + ex * ((size_t)(kk + 1) + ey * (size_t)(l + 1));
+                        size_t gj = t->left - 1 + j - 1, gk = t->bottom - 1 + kk - 1, gl = t->back - 1 + l - 1;
+                        dump_dst[f][gj + gx * (gk + gy * gl)] = src[f][si];
+                    }
+        }
+    }
+}
+
 int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *rows, int max_rows,
                double *dt_hist, int max_steps, int *steps_done)
 {
     ktab k;
-    if (ref_lib) {
-        if (load_lib(ref_lib, kernel_prefix, &k)) return -1;
-    } else {
-        k.ideal_gas = oracle_ideal_gas_kernel;
-        k.viscosity = oracle_viscosity_kernel;
-        k.calc_dt = oracle_calc_dt_kernel;
-        k.pdv = oracle_PdV_kernel;
-        k.revert = oracle_revert_kernel;
-        k.accelerate = oracle_accelerate_kernel;
-        k.flux_calc = oracle_flux_calc_kernel;
-        k.advec_cell = oracle_advec_cell_kernel;
-        k.advec_mom = oracle_advec_mom_kernel;
-        k.reset = oracle_reset_field_kernel;
-        k.update_halo = oracle_update_halo_kernel;
-        k.tile_halo = oracle_update_tile_halo_face;
-        k.summary = oracle_field_summary_kernel;
-    }
+    if (get_ktab(ref_lib, &k)) return -1;
     ostate s;
     memset(&s, 0, sizeof(s));
     s.deck = deck;
@@ -539,14 +638,14 @@ int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *row
         double xmin = deck->xmin + dx * (double)(float)(t->left - 1);
         double ymin = deck->ymin + dy * (double)(float)(t->bottom - 1);
         double zmin = deck->zmin + dz * (double)(float)(t->back - 1);
-        oracle_initialise_chunk_kernel(TD(t), xmin, ymin, zmin, dx, dy, dz, t->vertexx, t->vertexdx, t->vertexy,
-                                       t->vertexdy, t->vertexz, t->vertexdz, t->cellx, t->celldx, t->celly, t->celldy,
-                                       t->cellz, t->celldz, t->vol, t->xarea, t->yarea, t->zarea);
-        oracle_generate_chunk_kernel(TD(t), t->vertexx, t->vertexy, t->vertexz, t->cellx, t->celly, t->cellz, t->d0,
-                                     t->e0, t->u0, t->v0, t->w0, deck->number_of_states, deck->state_density,
-                                     deck->state_energy, deck->state_xvel, deck->state_yvel, deck->state_zvel,
-                                     deck->state_xmin, deck->state_xmax, deck->state_ymin, deck->state_ymax,
-                                     deck->state_zmin, deck->state_zmax, deck->state_radius, deck->state_geometry);
+        k.init_chunk(TD(t), xmin, ymin, zmin, dx, dy, dz, t->vertexx, t->vertexdx, t->vertexy, t->vertexdy,
+                     t->vertexz, t->vertexdz, t->cellx, t->celldx, t->celly, t->celldy, t->cellz, t->celldz, t->vol,
+                     t->xarea, t->yarea, t->zarea);
+        k.gen_chunk(TD(t), t->vertexx, t->vertexy, t->vertexz, t->cellx, t->celly, t->cellz, t->d0, t->e0, t->u0,
+                    t->v0, t->w0, deck->number_of_states, deck->state_density, deck->state_energy, deck->state_xvel,
+                    deck->state_yvel, deck->state_zvel, deck->state_xmin, deck->state_xmax, deck->state_ymin,
+                    deck->state_ymax, deck->state_zmin, deck->state_zmax, deck->state_radius,
+                    deck->state_geometry);
     }
     free(nx); free(ny); free(nz); free(lf); free(bt); free(bk); free(nb);
     s.advect_x = 1;
@@ -599,6 +698,8 @@ int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *row
     clock_gettime(CLOCK_MONOTONIC, &ts1);
     last_hydro_seconds = (double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec);
     if (steps_done) *steps_done = s.step;
+    if (dump_on) dump_fields(&s);
+    oracle_set_dump(NULL);
     for (int ti = 0; ti < s.ntiles; ti++) free_field(&s.tiles[ti]);
     free(s.tiles);
     return rc ? rc : nrows;

@@ -5,9 +5,12 @@
                            with 2x2x2 tiles, produced by the oracle driver
                            running the reference's own compiled kernels
                            (oracle/_ref/libclover_ref.so).  The file also
-                           carries, as `reference_program`, the totals the
-                           unmodified reference program printed at full
-                           precision for the same deck (8 OpenMP threads).
+                           carries, as `reference_program`, the totals a
+                           probe build of the whole reference program
+                           printed at full precision for the same deck (8
+                           OpenMP threads).  That build linked a stand-in MPI
+                           library, so those rows are regression data, not a
+                           pin; the `ref_kernels*` rows are the pin.
   kernel_digests.json      sha256 of every output array of each reference
                            kernel on seeded random inputs (tests/kernel_specs.py
                            fills), so the oracle can be pinned bit for bit

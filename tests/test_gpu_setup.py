@@ -1,6 +1,8 @@
 """Mesh and initial states painted on the device against the oracle's
-initialise_chunk + generate_chunk (C restatements pinned against the
-reference kernels), tile by tile, bit for bit.
+initialise_chunk + generate_chunk, tile by tile, bit for bit.  Those C
+restatements are pinned bit for bit against the reference's own
+initialise_chunk_kernel and generate_chunk_kernel (cuboid, sphere and point
+states, odd shapes, tile offsets) in test_oracle_setup_tiles.py.
 
 A chunk runs start.f90's halo update (depth 2, every painted field) right
 after painting, as the reference does, so the painted halo cells of

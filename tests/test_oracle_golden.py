@@ -3,8 +3,13 @@
 * the reference's only built-in known answer: test_problem 1 on the default
   10x2x2 deck ends with kinetic energy 3.64560737191257 (field_summary.f90:
   119-136; the reference accepts 0.001 %, the oracle is held to 1e-12);
-* the totals the reference program printed for the 64^3 cuboid deck, untiled
-  and with 2x2x2 tiles (tests/golden/config0_summaries.json);
+* the 64^3 cuboid deck run by the oracle driver with the reference's own
+  compiled kernels (`ref_kernels*` in tests/golden/config0_summaries.json;
+  the driver's call order is pinned by test_problem 1 and, for tiles, by
+  test_oracle_setup_tiles.py).  The same file keeps, as regression data
+  only, the totals a probe build of the whole reference program printed
+  (`reference_program*`); that build linked a stand-in MPI library, so
+  those rows are not a pin;
 * sha256 digests of every output array of every reference kernel on seeded
   inputs (tests/golden/kernel_digests.json);
 * closed forms: the volume total is exactly the domain volume; a single
@@ -39,7 +44,22 @@ def _cols(r):
     return [r["step"], r["vol"], r["mass"], r["press"] / r["vol"], r["ie"], r["ke"]]
 
 
+def test_config0_matches_reference_kernels():
+    rows, dts = ol.run(parse_deck(CONFIG0["deck"]))
+    got = [_cols(r) for r in rows]
+    ref = CONFIG0["ref_kernels"]
+    assert [g[0] for g in got] == [r[0] for r in ref]
+    for g, r in zip(got, ref):
+        for a, b in zip(g[1:], r[1:]):
+            assert a == pytest.approx(b, rel=1e-13, abs=1e-300), (g[0], a, b)
+    rows_t, _ = ol.run(parse_deck(cuboid_deck(64, 100, 10, tiles=8)))
+    for g, r in zip([_cols(x) for x in rows_t], CONFIG0["ref_kernels_tiled"]):
+        for a, b in zip(g[1:], r[1:]):
+            assert a == pytest.approx(b, rel=1e-13, abs=1e-300), (g[0], a, b)
+
+
 def test_config0_matches_reference_program():
+    """Regression data (see the module docstring): the probe build's totals."""
     rows, dts = ol.run(parse_deck(CONFIG0["deck"]))
     got = [_cols(r) for r in rows]
     ref = CONFIG0["reference_program"]
