@@ -15,7 +15,10 @@ Prints one JSON line:
                HBM bytes per launch / average launch time, against 8 TB/s
   cpu_baseline the reference's own kernels (compiled from its sources with
                flang -O3 -fopenmp, oracle/_ref) driven through the same
-               hydro loop on this host's cores, on a bounded sample deck
+               hydro loop on this host's cores, on a bounded sample of the
+               512^3 workload; also the 64^3 deck on 1 thread and all cores
+  fp32_store   the FP32-store form timed the same way, and its accuracy:
+               every total against the FP64 form after 300 steps (untimed)
 """
 from __future__ import annotations
 
@@ -231,15 +234,18 @@ def hbm_probe(mib: int = 1024, reps: int = 5) -> dict:
     return out
 
 
-def cpu_baseline(cells: int, steps: int, threads: int):
-    """The reference's compiled kernels through the hydro loop on the host
-    (run in a child process so its OpenMP runtime picks up `threads`)."""
+def cpu_run(cells: int, steps: int, threads: int, summary_frequency: int = 0) -> dict:
+    """The reference's compiled kernels (oracle/_ref: its kernel sources
+    built with flang -O3 -fopenmp) through the hydro loop on the host, timed
+    like the reference's Wall clock (hydro loop only, hydro.f90:44, 90).  A
+    child process, so that its OpenMP runtime starts with `threads` threads,
+    pinned one per core (OMP_PLACES=cores, OMP_PROC_BIND=close)."""
     code = f"""
 import sys, json
 sys.path[:0] = [{str(ROOT)!r}, {str(ROOT / 'tests')!r}]
 import oracle_lib as ol
 from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
-deck = parse_deck(cuboid_deck({cells}, {steps}, 0))
+deck = parse_deck(cuboid_deck({cells}, {steps}, {summary_frequency}))
 use_ref = ol.ref_available()
 ol.run(deck, use_ref=use_ref)
 import ctypes
@@ -247,14 +253,90 @@ f = ol.lib().oracle_last_hydro_seconds
 f.restype = ctypes.c_double
 print(json.dumps(dict(seconds=f(), ref=use_ref)))
 """
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
     if out.returncode != 0:
         raise RuntimeError(out.stderr[-2000:])
     r = json.loads(out.stdout.strip().splitlines()[-1])
-    rate = cells ** 3 * steps / r["seconds"]
-    return dict(value=rate, unit="cell-steps/s", cores=threads, kind="reference" if r["ref"] else "port",
-                sample=f"cuboid deck {cells}^3, {steps} steps, hydro loop only ({r['seconds']:.2f} s)")
+    return dict(cells=cells, steps=steps, threads=threads, seconds=r["seconds"],
+                value=cells ** 3 * steps / r["seconds"], kind="reference" if r["ref"] else "port")
+
+
+def host_cpu() -> dict:
+    """CPU model and core counts from lscpu, and the cores this process may use."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        sockets, cores = int(kv.get("Socket(s)", "1")), int(kv.get("Core(s) per socket", "0"))
+        info["physical_cores"] = sockets * cores if cores else None
+        info["logical_cpus"] = int(kv.get("CPU(s)", "0")) or None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    info["usable_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def cpu_baseline(cells: int, steps: int, threads: int) -> dict:
+    """BASELINE.md's CPU plan on this host: the 64^3 deck (100 steps) on one
+    thread and on `threads` threads, then a bounded sample of the GPU
+    workload (cells^3 for `steps` steps) on `threads` threads, which is the
+    reported value."""
+    t1 = cpu_run(64, 100, 1)
+    tn = cpu_run(64, 100, threads)
+    big = cpu_run(cells, steps, threads)
+    host = host_cpu()
+    return dict(value=big["value"], unit="cell-steps/s", cores=threads, kind=big["kind"],
+                sample=f"cuboid deck {cells}^3, {steps} steps, hydro loop only ({big['seconds']:.2f} s); "
+                       f"the reference's kernels (flang -O3 -fopenmp) through the reference's hydro loop order",
+                cpu_model=host.get("model"), physical_cores=host.get("physical_cores"),
+                usable_cpus=host.get("usable_cpus"), omp="OMP_PLACES=cores OMP_PROC_BIND=close",
+                config0_64cubed_100_steps={"threads_1": dict(seconds=t1["seconds"], value=t1["value"]),
+                                           f"threads_{threads}": dict(seconds=tn["seconds"], value=tn["value"])})
+
+
+def accuracy_leg(cells: int = 512, steps: int = 300) -> dict:
+    """The FP32-store form against the FP64 form on the same deck after
+    `steps` steps (untimed, default launch shapes): relative difference of
+    every field_summary total."""
+    from cloverleaf3d_ref_amd.chunk import Chunk
+    from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
+
+    deck = parse_deck(cuboid_deck(cells, steps, 25))
+    final = {}
+    for fp32 in (False, True):
+        with Chunk(deck, fp32=fp32) as c:
+            rows = c.run(max_steps=steps)
+        final[fp32] = rows[-1]
+    f64, f32 = final[False], final[True]
+    return dict(workload=f"cuboid deck {cells}^3", after_steps=f32["step"], time_fp64=f64["time"],
+                time_fp32=f32["time"],
+                rel_err_vs_fp64={k: abs(f32[k] / f64[k] - 1) for k in ("vol", "mass", "ie", "ke", "press")},
+                bound=1e-5)
+
+
+def ensemble_equal_time(ndecks: int, steps: int, dt: float) -> dict:
+    """FP32-store ensemble against FP64 at equal simulated time: both forms
+    on dt = max_timestep every step (pinned below every deck's stability
+    limit), so each deck reaches the same time in both and the totals
+    differ by the FP32 storage and arithmetic alone."""
+    from cloverleaf3d_ref_amd.deck import ensemble_decks
+    from cloverleaf3d_ref_amd.ensemble import Ensemble
+
+    decks = ensemble_decks(ndecks, end_step=steps)
+    for d in decks:
+        d.dtmax = dt
+    out = {}
+    for fp32 in (False, True):
+        with Ensemble(decks, fp32=fp32) as e:
+            e.run(steps)
+            out[fp32] = (e.summaries(), e.state())
+    (s64, st64), (s32, st32) = out[False], out[True]
+    same_time = st64[1] == st32[1] and set(st64[2]) == {dt} and set(st32[2]) == {dt}
+    return dict(pinned_dt=dt, steps=steps, same_time_every_deck=same_time,
+                rel_err_vs_fp64=ensemble_errors(s32, s64))
 
 
 def main():
@@ -263,8 +345,10 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=512)
-    ap.add_argument("--cpu-cells", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=300)  # about 10 s on 16 host cores
+    ap.add_argument("--cpu-cells", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=4)  # about 10-20 s of the reference on 16 host cores
+    ap.add_argument("--accuracy-steps", type=int, default=300,
+                    help="steps of the untimed FP32-vs-FP64 accuracy leg (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-store run")
@@ -336,16 +420,20 @@ def main():
                        one_at_a_time=dict(value=args.decks / alone, unit="decks/s", wall_s=alone),
                        speedup_vs_one_at_a_time=alone / b64["wall_s"],
                        fp32_store=dict(value=args.decks / b32["wall_s"], unit="decks/s", wall_s=b32["wall_s"],
-                                       rel_err_vs_fp64=ensemble_errors(b32["sums"], b64["sums"]),
-                                       note="totals after the same step count; each deck's dt follows its "
-                                            "own state, so the FP32 and FP64 runs reach slightly different "
-                                            "simulated times"),
+                                       equal_time=ensemble_equal_time(args.decks, args.deck_steps, 0.004)),
                        small_dt_decks=b64["small"], setup_s=b64["setup_s"], device_gib=b64["device_gib"],
                        autotune=b64["autotune"], fp32_autotune=b32["autotune"])
 
+    accuracy = None
+    if not args.no_fp32 and args.accuracy_steps:
+        accuracy = accuracy_leg(args.cells, args.accuracy_steps)
+        fp32["accuracy"] = accuracy
+
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every core this process may use: the GPU's share of the host where
+        # the environment sets one (OMP_NUM_THREADS), else the affinity mask
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         cpu = cpu_baseline(args.cpu_cells, args.cpu_steps, threads)
 
     total_ms = sum(v[0] for v in ktimes.values())

@@ -48,7 +48,12 @@ def to_c_deck(d: Deck) -> _lib.CloverDeck:
 
 
 class SmallTimestep(RuntimeError):
-    """dt fell below dtmin (timestep.f90:111, 122-124)."""
+    """dt fell below dtmin (timestep.f90:111, 120-122); `dt` is the small
+    time step that was formed (None where no single one applies)."""
+
+    def __init__(self, msg: str, dt: Optional[float] = None):
+        super().__init__(msg)
+        self.dt = dt
 
 
 PER_CALL = 1  # CLOVER_CHUNK_PER_CALL
@@ -103,7 +108,7 @@ class Chunk:
         dt = ctypes.c_double()
         rc = _lib.check(self._L.clover_chunk_step(self._h, ctypes.byref(dt)), "clover_chunk_step")
         if rc == 1:
-            raise SmallTimestep(f"small timestep: dt={dt.value}")
+            raise SmallTimestep(f"small timestep: dt={dt.value}", dt.value)
         return dt.value
 
     def field_summary(self) -> Dict[str, float]:

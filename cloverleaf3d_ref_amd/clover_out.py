@@ -106,11 +106,20 @@ def test_problem_lines(deck: Deck, ke: float) -> List[str]:
     return [f" Test problem 1 is within {qa_diff!r} % of the expected solution", " " + verdict]
 
 
+def error_lines(location: str, error: str) -> List[str]:
+    """report.f90:29-56's lines in clover.out: list-directed records, so
+    each starts with a blank, and empty WRITEs give empty lines."""
+    return ["", f" Error from {location}:", f" {error}", "", " CLOVER is terminating.", ""]
+
+
 def run(deck: Deck, out: TextIO, fp32: bool = False, per_call: bool = False, autotune: bool = True,
         visit_dir: str = ".") -> int:
     """The hydro loop of hydro.f90:46-95 on the device, writing clover.out's
     step and summary lines, and, when the deck sets visit_frequency, the
-    VTK dumps of visit.f90 into visit_dir.  Returns the steps taken."""
+    VTK dumps of visit.f90 into visit_dir.  Returns the steps taken.  On a
+    small time step it writes that step's line and report_error's lines, as
+    timestep.f90:112-122 does, and raises SmallTimestep (the reference
+    aborts there: no completion lines)."""
     from .visit import VisitWriter
 
     visit = VisitWriter(deck, visit_dir) if deck.visit_frequency else None
@@ -130,9 +139,13 @@ def run(deck: Deck, out: TextIO, fp32: bool = False, per_call: bool = False, aut
             step, t_before, _ = c.state()
             try:
                 dt = c.step()
-            except SmallTimestep:
-                w(" small timestep\n")
-                break
+            except SmallTimestep as err:
+                step, _, _ = c.state()
+                control, jdt, kdt, x, y = c.dt_control()
+                w(step_line(step, t_before, control, err.dt, jdt, kdt, x, y) + "\n")
+                for line in error_lines("timestep", "small timestep"):
+                    w(line + "\n")
+                raise
             step, t, _ = c.state()
             control, jdt, kdt, x, y = c.dt_control()
             w(step_line(step, t_before, control, dt, jdt, kdt, x, y) + "\n")
@@ -171,6 +184,8 @@ def main(argv=None) -> int:
     out = sys.stdout if a.out == "-" else open(a.out, "w")
     try:
         run(deck, out, fp32=a.fp32, per_call=a.per_call, autotune=not a.no_autotune, visit_dir=a.visit_dir)
+    except SmallTimestep:
+        return 1  # the reference aborts (report.f90:53, clover_abort)
     finally:
         if out is not sys.stdout:
             out.close()

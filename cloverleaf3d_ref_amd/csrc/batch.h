@@ -26,7 +26,8 @@ struct DeckDev {
     double dt, dtold, time;
     double dtmax, dtrise, dtmin;
     double dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe;
-    int small;  // set when dt fell below dtmin (timestep.f90:122-124)
+    int steps;  // timesteps taken (dt formed)
+    int small;  // 0, or the step at which dt first fell below dtmin (timestep.f90:120-122)
 };
 
 // A batched launch: total workgroups and the device prefix table (nd+1).

@@ -534,7 +534,13 @@ int clover_ensemble_run(clover_ensemble *e, int steps, int use_graphs)
         e->advect_x = !e->advect_x;
         e->step++;
     }
+    // the reference aborts on a small time step (timestep.f90:120-122): say
+    // so as clover_chunk_step does, with return code 1
+    std::vector<DeckDev> h(e->nd);
+    hip_check(hipMemcpyAsync(h.data(), e->dev, sizeof(DeckDev) * e->nd, hipMemcpyDeviceToHost, e->st), "D2H");
     hip_check(hipStreamSynchronize(e->st), "sync");
+    for (int d = 0; d < e->nd; d++)
+        if (h[d].small) return 1;
     CLOVER_API_END
 }
 

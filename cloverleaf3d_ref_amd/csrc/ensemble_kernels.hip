@@ -10,7 +10,10 @@ namespace clover {
 
 // timestep.f90:86-126 for one tile per deck: the minimum over the deck's
 // calc_dt partials (seeded with g_big, as the single-deck final pass), then
-// dt = MIN(dt, dtold*dtrise, dtmax), the small-dt flag and dtold = dt.
+// dt = MIN(dt, dtold*dtrise, dtmax), the small-dt flag and dtold = dt.  The
+// reference stops the run where dt falls below dtmin (timestep.f90:120-122);
+// a deck of the batch records that step and keeps stepping with the others,
+// so its later state is not one the reference produces.
 __global__ void __launch_bounds__(RED_THREADS)
 kb_deck_dt(DeckDev *__restrict__ decks, const int *__restrict__ start, const double *__restrict__ partial,
            double g_big)
@@ -34,7 +37,8 @@ kb_deck_dt(DeckDev *__restrict__ decks, const int *__restrict__ start, const dou
         if (D.dtold * D.dtrise < mm) mm = D.dtold * D.dtrise;
         if (D.dtmax < mm) mm = D.dtmax;
         dt = mm;
-        if (dt < D.dtmin) D.small = 1;
+        D.steps = D.steps + 1;
+        if (dt < D.dtmin && D.small == 0) D.small = D.steps;
         D.dt = dt;
         D.dtold = dt;
     }

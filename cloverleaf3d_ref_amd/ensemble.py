@@ -14,7 +14,7 @@ from typing import Dict, List, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from .chunk import FIELD_NAMES, FP32, _CLASS, _extents, to_c_deck
+from .chunk import FIELD_NAMES, FP32, _CLASS, SmallTimestep, _extents, to_c_deck
 from .deck import Deck
 
 
@@ -47,7 +47,14 @@ class Ensemble:
             pass
 
     def run(self, steps: int, graphs: bool = True) -> None:
-        _lib.check(self._L.clover_ensemble_run(self._h, steps, 1 if graphs else 0), "clover_ensemble_run")
+        """Advance every deck.  Raises SmallTimestep (after the steps ran)
+        if some deck's dt has fallen below its dtmin, where the reference
+        stops; state()'s small list gives the step it happened at."""
+        rc = _lib.check(self._L.clover_ensemble_run(self._h, steps, 1 if graphs else 0), "clover_ensemble_run")
+        if rc == 1:
+            small = self.state()[3]
+            raise SmallTimestep("small timestep in decks " +
+                                ", ".join(f"{d} (step {s})" for d, s in enumerate(small) if s))
 
     def summaries(self) -> List[Dict[str, float]]:
         out = (ctypes.c_double * (5 * len(self.decks)))()

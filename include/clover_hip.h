@@ -304,11 +304,16 @@ int clover_ensemble_create(const clover_deck *decks, int ndecks, int device, int
 int clover_ensemble_destroy(clover_ensemble *e);
 /* Advance every deck `steps` steps (hydro.f90:48-68, no summaries); with
  * use_graphs the step of each advect_x parity is captured in a graph once
- * and replayed. */
+ * and replayed.  Returns 1 if a deck's dt has fallen below its dtmin (the
+ * small time step on which the reference stops, timestep.f90:120-122): that
+ * deck keeps stepping with the others, so from that step on its state is
+ * not one the reference produces; clover_ensemble_state says which deck and
+ * from which step. */
 int clover_ensemble_run(clover_ensemble *e, int steps, int use_graphs);
 /* field_summary of every deck: out[5*d + (vol, mass, ie, ke, press)] */
 int clover_ensemble_summaries(clover_ensemble *e, double *out);
-/* steps taken; per deck: simulated time, last dt, small-dt flag */
+/* steps taken; per deck: simulated time, last dt, and small: 0, or the step
+ * at which the deck's dt first fell below dtmin */
 int clover_ensemble_state(clover_ensemble *e, int *step, double *time, double *dt, int *small);
 /* one field (0..18) of deck d as a dense Fortran array of its class */
 int clover_ensemble_fetch_field(clover_ensemble *e, int deck, int field, double *host);

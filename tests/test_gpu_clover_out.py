@@ -45,3 +45,36 @@ def test_default_deck_clover_out():
             final["ke"], final["ie"] + final["ke"]]
     assert vals == pytest.approx(want, rel=1e-8)
     assert any("This test is considered PASSED" in l for l in text)
+
+
+def test_small_timestep_clover_out(monkeypatch, tmp_path):
+    """dt below dtmin at step 1: the step line is written, then
+    report_error's lines (timestep.f90:112-122, report.f90:29-56), and the
+    run ends there with no completion lines and a failing exit status.
+    (The reference's decks cannot set dtmin, read_input.f90:63; the test
+    raises it on the parsed deck.)"""
+    from cloverleaf3d_ref_amd import clover_out
+    from cloverleaf3d_ref_amd.chunk import SmallTimestep
+
+    deck = parse_deck(DEFAULT_DECK)
+    deck.dtmin = 1.0
+    print("[gpu] clover_out small timestep", flush=True)
+    buf = io.StringIO()
+    with pytest.raises(SmallTimestep):
+        run(deck, buf, autotune=False)
+    lines = buf.getvalue().splitlines()
+    steps = [l for l in lines if l.startswith(" Step ")]
+    assert len(steps) == 1 and STEP.match(steps[0]).group(1) == "1"
+    assert lines[lines.index(steps[0]) + 1:] == ["", " Error from timestep:", " small timestep", "",
+                                                  " CLOVER is terminating.", ""]
+    assert not any("Calculation complete" in l for l in lines)
+
+    def small_deck(text):
+        d = parse_deck(text)
+        d.dtmin = 1.0
+        return d
+
+    monkeypatch.setattr(clover_out, "parse_deck", small_deck)
+    deck_file = tmp_path / "clover.in"
+    deck_file.write_text(DEFAULT_DECK)
+    assert clover_out.main([str(deck_file), "-o", str(tmp_path / "clover.out"), "--no-autotune"]) == 1

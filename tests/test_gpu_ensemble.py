@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from cloverleaf3d_ref_amd.chunk import Chunk
+from cloverleaf3d_ref_amd.chunk import Chunk, SmallTimestep
 from cloverleaf3d_ref_amd.deck import ensemble_decks, parse_deck
 from cloverleaf3d_ref_amd.ensemble import Ensemble
 
@@ -124,3 +124,26 @@ def test_set_shape_rejects_bad_values():
         for args in (("visc_dt", 4, 0, 0), ("visc_dt", 16, 2, 0), ("mom_y", 16, 0, -1)):
             with pytest.raises(CloverError):
                 e.set_shape(*args)
+
+
+def test_small_timestep_deck_is_reported():
+    """A deck whose dt falls below its dtmin: the reference stops there
+    (timestep.f90:120-122) and Chunk.step raises; the ensemble keeps the
+    batch going, records the step and raises after the run, and the other
+    decks are untouched by it."""
+    decks = _uneven()[:3]
+    decks[1].dtmin = 1.0  # above every dt this deck can take: small at step 1
+    with Chunk(decks[1]) as c:
+        with pytest.raises(SmallTimestep):
+            c.step()
+    with Ensemble(decks) as e:
+        with pytest.raises(SmallTimestep):
+            e.run(3)
+        step, times, dts, small = e.state()
+        sums = e.summaries()
+    assert step == 3 and small == [0, 1, 0]
+    for d in (0, 2):
+        want, time, dt = _alone(decks[d], 3, False)
+        assert times[d] == time and dts[d] == dt
+        for k in ("vol", "mass", "ie", "ke", "press"):
+            assert sums[d][k] == want[k], (d, k)
