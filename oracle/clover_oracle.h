@@ -221,8 +221,10 @@ int oracle_update_halo_tiles(int ntiles, const int *dims, const int *nbr, double
 /* Gather the final interior fields of the next oracle_run into whole-chunk
  * arrays: 13 pointers (density0, energy0, pressure, viscosity, soundspeed,
  * density1, energy1, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1), cells
- * X*Y*Z and nodes (X+1)*(Y+1)*(Z+1), Fortran order; NULL entries skipped. */
-void oracle_set_dump(double *const *dst);
+ * X*Y*Z and nodes (X+1)*(Y+1)*(Z+1), Fortran order; NULL entries skipped.
+ * visit != 0 first runs visit.f90:65-81's ideal_gas, halo update and
+ * viscosity, so the arrays hold what a visit dump at the last step writes. */
+void oracle_set_dump(double *const *dst, int visit);
 
 #ifdef __cplusplus
 }

@@ -568,16 +568,34 @@ int oracle_update_halo_tiles(int ntiles, const int *dims, const int *nbr, double
  * soundspeed, density1, energy1, xvel0, yvel0, zvel0, xvel1, yvel1, zvel1;
  * NULL entries are skipped.  The setting holds for one run. */
 static double *dump_dst[13];
-static int dump_on = 0;
+static int dump_on = 0, dump_visit = 0;
 
-void oracle_set_dump(double *const *dst)
+void oracle_set_dump(double *const *dst, int visit)
 {
     dump_on = dst != NULL;
+    dump_visit = dst != NULL && visit;
     for (int i = 0; i < 13; i++) dump_dst[i] = dst ? dst[i] : NULL;
 }
 
-static void dump_fields(const ostate *s)
+/* visit.f90:65-81: ideal_gas on level 0, the pressure and velocity halos
+ * (depth 1), then viscosity -- the values a visit dump writes */
+static void visit_prepare(ostate *s)
 {
+    ideal_gas(s, 0);
+    int fields[19];
+    static const int fv[] = {F_P, F_U0, F_V0, F_W0};
+    set_fields(fields, fv, 4);
+    update_halo(s, fields, 1);
+    for (int ti = 0; ti < s->ntiles; ti++) {
+        otile *t = &s->tiles[ti];
+        s->k->viscosity(TD(t), t->xarea, t->yarea, t->zarea, t->celldx, t->celldy, t->celldz, t->d0, t->p, t->q,
+                        t->u0, t->v0, t->w0);
+    }
+}
+
+static void dump_fields(ostate *s)
+{
+    if (dump_visit) visit_prepare(s);
     const oracle_deck *d = s->deck;
     const size_t X = d->x_cells, Y = d->y_cells;
     for (int ti = 0; ti < s->ntiles; ti++) {
@@ -699,7 +717,7 @@ int oracle_run(const oracle_deck *deck, const char *ref_lib, oracle_summary *row
     last_hydro_seconds = (double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec);
     if (steps_done) *steps_done = s.step;
     if (dump_on) dump_fields(&s);
-    oracle_set_dump(NULL);
+    oracle_set_dump(NULL, 0);
     for (int ti = 0; ti < s.ntiles; ti++) free_field(&s.tiles[ti]);
     free(s.tiles);
     return rc ? rc : nrows;

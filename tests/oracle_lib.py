@@ -139,3 +139,25 @@ def run_through(deck: Deck, lib_path: str, prefix: str, max_rows: int = 4096,
     out = [dict(step=r.step, time=r.time, vol=r.vol, mass=r.mass, press=r.press, ie=r.ie, ke=r.ke)
            for r in rows[:n]]
     return out, dts[:min(done.value, max_steps)].copy()
+
+
+DUMP_NAMES = ["density0", "energy0", "pressure", "viscosity", "soundspeed", "density1", "energy1",
+              "xvel0", "yvel0", "zvel0", "xvel1", "yvel1", "zvel1"]
+
+
+def run_with_fields(deck: Deck, use_ref: bool = False, visit: bool = False):
+    """ol.run plus the final interior fields, gathered over tiles into
+    whole-chunk Fortran-ordered arrays (cells (X,Y,Z), nodes (X+1,Y+1,Z+1)).
+    visit=True first runs visit.f90:65-81's ideal_gas, halo update and
+    viscosity, so the fields are what a visit dump at the last step holds."""
+    X, Y, Z = deck.x_cells, deck.y_cells, deck.z_cells
+    out = [np.zeros(X * Y * Z) if i < 7 else np.zeros((X + 1) * (Y + 1) * (Z + 1)) for i in range(13)]
+    L = lib()
+    L.oracle_set_dump.restype = None
+    L.oracle_set_dump((ctypes.c_void_p * 13)(*[a.ctypes.data for a in out]), 1 if visit else 0)
+    rows, dts = run(deck, use_ref=use_ref)
+    fields = {}
+    for i, (name, a) in enumerate(zip(DUMP_NAMES, out)):
+        shape = (X, Y, Z) if i < 7 else (X + 1, Y + 1, Z + 1)
+        fields[name] = a.reshape(shape, order="F")
+    return rows, dts, fields

@@ -106,3 +106,22 @@ def test_fortran_shims_compile():
         pytest.skip("no flang in this image")
     subprocess.run(["make", "-C", str(_lib.PKG / "fortran")], check=True, capture_output=True)
     assert (_lib.PKG / "_build" / "fortran" / "clover_hip_shims.o").exists()
+
+
+def test_fortran_shim_library_for_the_boundary_tests():
+    """oracle/_fshim/libfshim.so (the shims behind C-callable wrappers, built
+    by __graft_entry__.build() when flang is present) exports a wrapper per
+    kernel and resolves the product library by its soname."""
+    import shutil
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    if not pathlib.Path(flang).exists():
+        pytest.skip("no flang in this image")
+    so = _lib.PKG.parent / "oracle" / "_fshim" / "libfshim.so"
+    assert so.exists(), "build it: make -C oracle fshim"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(so)], capture_output=True, text=True, check=True).stdout
+    wrappers = {line.split()[-1] for line in out.splitlines() if " T ref_" in line}
+    assert len(wrappers) == 18
+    dyn = subprocess.run(["readelf", "-d", str(so)], capture_output=True, text=True, check=True).stdout
+    assert "[libclover_hip.so]" in dyn
+    dyn = subprocess.run(["readelf", "-d", str(_lib.LIB_PATH)], capture_output=True, text=True, check=True).stdout
+    assert "Library soname: [libclover_hip.so]" in dyn

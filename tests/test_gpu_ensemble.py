@@ -1,15 +1,20 @@
-"""The ensemble driver against each deck run alone.
+"""The ensemble driver against each deck run alone and against the oracle.
 
 Every deck of an ensemble evolves exactly as a Chunk running it alone with
 the fused kernels: after the same number of steps its summary totals, time
 and dt are bit-identical, in FP64 and in the FP32-store form, whether the
-steps run eagerly or as replays of the captured step graphs.
+steps run eagerly or as replays of the captured step graphs.  In FP64 each
+deck's time and dt also equal the oracle's run of that deck bit for bit,
+and its totals agree within 1e-12 (the oracle sums in another order).
 """
 from __future__ import annotations
+
+import dataclasses
 
 import numpy as np
 import pytest
 
+import oracle_lib as ol
 from cloverleaf3d_ref_amd.chunk import Chunk, SmallTimestep
 from cloverleaf3d_ref_amd.deck import ensemble_decks, parse_deck
 from cloverleaf3d_ref_amd.ensemble import Ensemble
@@ -60,6 +65,11 @@ def _check(decks, steps, fp32, graphs, split=None):
         assert times[d] == time and dts[d] == dt, d
         for k in ("vol", "mass", "ie", "ke", "press"):
             assert sums[d][k] == want[k], (d, k, sums[d][k], want[k])
+        if not fp32:
+            rows, odts = ol.run(dataclasses.replace(deck, end_step=steps, summary_frequency=0))
+            assert len(odts) == steps and dts[d] == odts[-1] and times[d] == rows[-1]["time"], d
+            for k in ("vol", "mass", "ie", "ke", "press"):
+                assert sums[d][k] == pytest.approx(rows[-1][k], rel=1e-12, abs=1e-300), (d, k)
 
 
 @pytest.mark.parametrize("graphs", [False, True])

@@ -1,19 +1,22 @@
 """visit.f90's VTK dumps from device runs.
 
 The dump's values are the fields after visit.f90's ideal_gas, halo update and
-viscosity (Chunk.visit_prepare).  They are checked against the per-call
-kernels' fields, which other tests pin bit for bit to the reference's
-compiled kernels, and the pressure against ideal_gas_kernel.f90:54
+viscosity (Chunk.visit_prepare).  They are checked against the oracle
+driver's run, which performs the same device work before gathering its
+fields (bit for bit, and the dump file's text), against the per-call
+kernels' fields, and the pressure against ideal_gas_kernel.f90:54
 evaluated here.  A run with dumps must also print the same clover.out as one
 without.
 """
 from __future__ import annotations
 
+import dataclasses
 import io
 
 import numpy as np
 import pytest
 
+import oracle_lib as ol
 from cloverleaf3d_ref_amd.chunk import Chunk
 from cloverleaf3d_ref_amd.clover_out import run
 from cloverleaf3d_ref_amd.deck import cuboid_deck, parse_deck
@@ -102,3 +105,41 @@ def test_tiled_dump_files(tmp_path):
             # the start state of a tile is the same cells of the one-tile run
             ref = rho1[ox + 1:ox + 1 + nx, oy + 1:oy + 1 + ny, oz + 1:oz + 1 + nz]
             assert dens == pytest.approx(ref, rel=5e-4)
+
+
+@pytest.mark.parametrize("tiles", [1, 2])
+def test_dump_against_oracle(tmp_path, tiles):
+    """At step 6 the dump of a device run holds exactly the oracle's
+    values after visit.f90's ideal_gas, halo update and viscosity: the
+    fields bit for bit, and the dump file's text.  With tiles, a node on the
+    face two tiles share is computed by both; the oracle's gathered arrays
+    hold the later tile's value there, so each tile is compared on the
+    nodes of its upper faces only where that face is external."""
+    deck = _deck(tiles=tiles)
+    print(f"[gpu] visit dump against the oracle, tiles={tiles}", flush=True)
+    _clover_out(deck, visit_dir=str(tmp_path))
+    _, _, want = ol.run_with_fields(dataclasses.replace(deck, end_step=6, visit_frequency=0), visit=True)
+    with Chunk(deck) as c:
+        for _ in range(6):
+            c.step()
+        c.visit_prepare()
+        for t in range(c.ntiles):
+            (nx, ny, nz), (ox, oy, oz), nb = c.tile_info(t)
+            cs = (slice(ox - 1, ox - 1 + nx), slice(oy - 1, oy - 1 + ny), slice(oz - 1, oz - 1 + nz))
+            vs = (slice(ox - 1, ox + nx), slice(oy - 1, oy + ny), slice(oz - 1, oz + nz))
+            own = tuple(slice(0, n + 1 if nb[hi] == -1 else n) for n, hi in zip((nx, ny, nz), (1, 3, 5)))
+            got = {}
+            for name in ("density0", "energy0", "pressure", "viscosity"):
+                got[name] = c.fetch(name, t)[2:nx + 2, 2:ny + 2, 2:nz + 2]
+                assert np.array_equal(got[name].view(np.uint64), want[name][cs].view(np.uint64)), (t, name)
+            for name in ("xvel0", "yvel0", "zvel0"):
+                got[name] = c.fetch(name, t)[2:nx + 3, 2:ny + 3, 2:nz + 3]
+                assert np.array_equal(got[name][own].view(np.uint64), want[name][vs][own].view(np.uint64)), (t, name)
+            if c.ntiles > 1:
+                continue
+            text = vtk_text(vertex_coords(deck, nx, ox, 0), vertex_coords(deck, ny, oy, 1),
+                            vertex_coords(deck, nz, oz, 2), want["density0"][cs], want["energy0"][cs],
+                            want["pressure"][cs], want["viscosity"][cs], want["xvel0"][vs], want["yvel0"][vs],
+                            want["zvel0"][vs])
+            assert (tmp_path / file_name(t + 1, 6, 0, c.ntiles)).read_text() == text, t
+        assert (got["viscosity"] > 0).any()

@@ -477,18 +477,12 @@ def test_tiled_update_halo_equals_untiled(tiling, fset, depth, kernels):
 
 # ----------------------------------------------------------------- whole runs over tiles
 
-DUMP_NAMES = ["density0", "energy0", "pressure", "viscosity", "soundspeed", "density1", "energy1",
-              "xvel0", "yvel0", "zvel0", "xvel1", "yvel1", "zvel1"]
+DUMP_NAMES = ol.DUMP_NAMES
 
 
 def _run_dump(deck, use_ref):
-    X, Y, Z = deck.x_cells, deck.y_cells, deck.z_cells
-    out = [np.zeros(X * Y * Z) if i < 7 else np.zeros((X + 1) * (Y + 1) * (Z + 1)) for i in range(13)]
-    L = ol.lib()
-    L.oracle_set_dump.restype = None
-    L.oracle_set_dump((P * 13)(*[a.ctypes.data for a in out]))
-    rows, dts = ol.run(deck, use_ref=use_ref)
-    return rows, dts, dict(zip(DUMP_NAMES, out))
+    rows, dts, fields = ol.run_with_fields(deck, use_ref=use_ref)
+    return rows, dts, {k: np.ravel(v, order="F") for k, v in fields.items()}
 
 
 def _tiled(deck_text, tiles, dims):
