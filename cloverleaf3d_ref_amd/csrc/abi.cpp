@@ -107,6 +107,7 @@ void ensure_init()
     hip_check(hipMalloc(&c.partial, sizeof(double) * 5 * kMaxPartials), "hipMalloc partials");
     hip_check(hipMalloc(&c.scalar, sizeof(double) * 8), "hipMalloc scalar");
     hip_check(hipHostMalloc(&c.h_scalar, sizeof(double) * 8, hipHostMallocDefault), "hipHostMalloc");
+    noalloc_scratch();  // with the device current
     c.init = true;
 }
 

@@ -768,9 +768,12 @@ hipError_t launch_reset_field(const Lay &g, double *density0, const double *dens
 
 // ---------------------------------------------------------------------------
 // field_summary_kernel.f90:51-78: block partial sums of vol, mass, ie, ke,
-// press (two-pass, fixed order, so a run is reproducible bit for bit).
+// press (two-pass, fixed order, so a run is reproducible bit for bit).  The
+// cell arrays are addressed through g, the vertex velocities through gv (the
+// same strides in the device layout; the reference's own vertex extents for
+// dense Fortran arrays).
 __global__ void __launch_bounds__(RED_THREADS)
-k_field_summary(Lay g, int nj, int nk, int ncells, const double *__restrict__ volume,
+k_field_summary(Lay g, Lay gv, int nj, int nk, int ncells, const double *__restrict__ volume,
                 const double *__restrict__ density0, const double *__restrict__ energy0,
                 const double *__restrict__ pressure, const double *__restrict__ xvel0,
                 const double *__restrict__ yvel0, const double *__restrict__ zvel0, double *__restrict__ partial)
@@ -781,11 +784,13 @@ k_field_summary(Lay g, int nj, int nk, int ncells, const double *__restrict__ vo
         const int k = g.y_min + (c / nj) % nk;
         const int l = g.z_min + c / (nj * nk);
         const int i = idx(g, j, k, l);
+        const int iv = idx(gv, j, k, l);
         double vsqrd = 0.0;
         for (int lv = 0; lv <= 1; lv++)
             for (int kv = 0; kv <= 1; kv++)
                 for (int jv = 0; jv <= 1; jv++) {
-                    const double u = AT(xvel0, jv, kv, lv), v = AT(yvel0, jv, kv, lv), w = AT(zvel0, jv, kv, lv);
+                    const int n = iv + jv + kv * gv.ld + lv * gv.plane;
+                    const double u = xvel0[n], v = yvel0[n], w = zvel0[n];
                     vsqrd = vsqrd + 0.125 * (u * u + v * v + w * w);
                 }
         const double cell_vol = volume[i];
@@ -809,7 +814,7 @@ k_field_summary(Lay g, int nj, int nk, int ncells, const double *__restrict__ vo
     }
 }
 
-hipError_t launch_field_summary_partial(const Lay &g, const double *volume, const double *density0,
+hipError_t launch_field_summary_partial(const Lay &g, const Lay &gv, const double *volume, const double *density0,
                                         const double *energy0, const double *pressure, const double *xvel0,
                                         const double *yvel0, const double *zvel0, double *partial, int *nparts,
                                         hipStream_t st)
@@ -818,9 +823,18 @@ hipError_t launch_field_summary_partial(const Lay &g, const double *volume, cons
     const int ncells = nj * nk * nl;
     const int nb = reduction_blocks(ncells);
     *nparts = nb;
-    TIMED_LAUNCH("k_field_summary", st, k_field_summary, dim3(nb), dim3(RED_THREADS), 0, st, g, nj, nk, ncells, volume, density0,
-                       energy0, pressure, xvel0, yvel0, zvel0, partial);
+    TIMED_LAUNCH("k_field_summary", st, k_field_summary, dim3(nb), dim3(RED_THREADS), 0, st, g, gv, nj, nk, ncells,
+                 volume, density0, energy0, pressure, xvel0, yvel0, zvel0, partial);
     return hipGetLastError();
+}
+
+hipError_t launch_field_summary_partial(const Lay &g, const double *volume, const double *density0,
+                                        const double *energy0, const double *pressure, const double *xvel0,
+                                        const double *yvel0, const double *zvel0, double *partial, int *nparts,
+                                        hipStream_t st)
+{
+    return launch_field_summary_partial(g, g, volume, density0, energy0, pressure, xvel0, yvel0, zvel0, partial,
+                                        nparts, st);
 }
 
 __global__ void __launch_bounds__(RED_THREADS) k_sum5_final(const double *__restrict__ partial, int n,

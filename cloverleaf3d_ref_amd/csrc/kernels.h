@@ -92,6 +92,12 @@ hipError_t launch_field_summary_partial(const Lay &g, const double *volume, cons
                                         const double *energy0, const double *pressure, const double *xvel0,
                                         const double *yvel0, const double *zvel0, double *partial, int *nparts,
                                         hipStream_t st);
+// the same with the vertex velocities in their own layout gv (dense Fortran
+// arrays: cell and vertex arrays have different extents)
+hipError_t launch_field_summary_partial(const Lay &g, const Lay &gv, const double *volume, const double *density0,
+                                        const double *energy0, const double *pressure, const double *xvel0,
+                                        const double *yvel0, const double *zvel0, double *partial, int *nparts,
+                                        hipStream_t st);
 hipError_t launch_sum5_final(const double *partial, int n, double *out, hipStream_t st);
 
 constexpr int kMaxPartials = 16384;

@@ -58,4 +58,8 @@ void download_dense_any(const Lay &g, int cls, const void *dev, void *host, size
 // element count of the dense host array of class cls
 int64_t dense_elems(const Lay &g, int cls);
 
+// the current device's address of the allocation-free entry point's fixed
+// scratch array (looked up on first use per device)
+double *noalloc_scratch();
+
 }  // namespace clover

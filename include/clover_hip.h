@@ -138,6 +138,23 @@ int clover_hip_field_summary_kernel(CLOVER_DIMS, double *volume, double *density
                                     double *pressure, double *xvel0, double *yvel0, double *zvel0, double *vol,
                                     double *mass, double *ie, double *ke, double *press);
 
+/* No counterpart in the reference: field_summary_kernel.f90:27-81 for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The arrays are the
+ * caller's DEVICE arrays in the reference's own shapes (build_field.f90:
+ * 41-147: volume, density0, energy0, pressure (x_min-2:x_max+2, ...),
+ * xvel0, yvel0, zvel0 (x_min-2:x_max+3, ...)); sums receives vol, mass, ie,
+ * ke, press (5 doubles of device memory).  Enqueued on `stream` (a
+ * hipStream_t; NULL for the default stream) without waiting and without any
+ * allocation: the block partial sums go to a fixed 640 KiB device array of
+ * the library, one per GPU, held until the process ends and shared by every
+ * call on that GPU, so calls that overlap on one GPU race.  Tiles whose
+ * vertex arrays exceed 2^31-1 elements are rejected (E_ARG).  The sums equal
+ * clover_hip_field_summary_kernel's bit for bit. */
+int clover_hip_field_summary_kernel_noalloc(CLOVER_DIMS, const double *volume, const double *density0,
+                                            const double *energy0, const double *pressure, const double *xvel0,
+                                            const double *yvel0, const double *zvel0, double *sums, void *stream);
+
 /* ------------------------------------------------ device-resident driver */
 
 #define CLOVER_MAX_STATES 16

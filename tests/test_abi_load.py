@@ -125,3 +125,17 @@ def test_fortran_shim_library_for_the_boundary_tests():
     assert "[libclover_hip.so]" in dyn
     dyn = subprocess.run(["readelf", "-d", str(_lib.LIB_PATH)], capture_output=True, text=True, check=True).stdout
     assert "Library soname: [libclover_hip.so]" in dyn
+
+
+def test_noalloc_field_summary_rejects_oversized_tile():
+    # the size check comes before any device call, so this runs without a GPU
+    L = _lib.lib()
+    fn = L.clover_hip_field_summary_kernel_noalloc
+    fn.restype = ctypes.c_int
+    L.clover_hip_last_error.restype = ctypes.c_char_p
+    dummy = ctypes.c_void_p(256)
+    rc = fn(1, 1300, 1, 1300, 1, 1300, *([dummy] * 8), ctypes.c_void_p(None))
+    assert rc == -2
+    assert b"exceeds the maximum" in L.clover_hip_last_error()
+    rc = fn(1, 8, 1, 8, 1, 8, *([dummy] * 7), ctypes.c_void_p(None), ctypes.c_void_p(None))
+    assert rc == -2 and b"null" in L.clover_hip_last_error()
