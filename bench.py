@@ -36,8 +36,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch of each kernel at the bench workload, from two
-# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a plain copy of
-# known size (python -m cloverleaf3d_ref_amd.profiling traffic ...)
+# rocprofv3 --pmc runs (read requests and write transactions to memory by
+# size), checked against the set-up equation-of-state kernel, whose traffic
+# is known exactly (python -m cloverleaf3d_ref_amd.profiling traffic ...)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_cuboid512_fp64.json"
 
 # Algorithmic (kernel-boundary compulsory) traffic of each kernel per launch,
@@ -189,21 +190,6 @@ def one_at_a_time(decks, steps: int) -> float:
             c.field_summary()
             total += time.perf_counter() - t0
     return total
-
-
-def calibration_copy(mib: int) -> None:
-    """One plain device-to-device copy of `mib` MiB: it reads and writes
-    exactly that many bytes, so its counter readings calibrate the HBM
-    traffic counters for the hydro kernels profiled in the same run."""
-    import torch
-    n = mib * (1 << 20) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    del src, dst
-    torch.cuda.empty_cache()
 
 
 def hbm_probe(mib: int = 1024, reps: int = 5) -> dict:
@@ -358,17 +344,12 @@ def main():
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--deck-steps", type=int, default=200)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="first run a plain device copy of this many MiB (the reference kernel for "
-                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the hydro path here is single-GPU by design "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
 
-    if args.calib_copy_mib:
-        calibration_copy(args.calib_copy_mib)
     probe = hbm_probe()  # before the hydro library opens the device
     tune = not args.no_autotune
     wall, ktimes, info = gpu_bench(args.cells, args.steps, args.warmup, tune=tune)
@@ -402,7 +383,10 @@ def main():
                      value=nt ** 3 * args.tiled_steps / w64, unit="cell-steps/s", ms_per_step=w64 / args.tiled_steps * 1e3,
                      fp32_value=nt ** 3 * args.tiled_steps / w32, fp32_ms_per_step=w32 / args.tiled_steps * 1e3,
                      fp32_rel_err_vs_fp64={k: abs(f32[k] / f64[k] - 1) for k in ("mass", "ie", "ke", "press")},
-                     tile_halo_ms_per_step=round(k64.get("k_tile_copy", (0.0, 0))[0] / args.tiled_steps, 4),
+                     tile_halo_ms_per_step=round(sum(k64.get(n, (0.0, 0))[0] for n in ("k_tile_copy", "kb_tile_copy"))
+                                                 / args.tiled_steps, 4),
+                     reflect_ms_per_step=round(sum(k64.get(n, (0.0, 0))[0] for n in ("k_reflect", "kb_tile_reflect"))
+                                               / args.tiled_steps, 4),
                      device_gib=i64["device_bytes"] / 2 ** 30, setup_s=i64["setup_s"],
                      autotune=i64["autotune"], fp32_autotune=i32["autotune"])
 

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "driver.h"
 #include "fast.h"
 #include "launch.h"
@@ -111,6 +113,34 @@ hipError_t batch_advec_mom(Store s, const DeckDev *decks, int nd, const BatchGri
 // (null: all) as for launch_update_halo_store
 hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int axis, int depth,
                          const int *fields, hipStream_t st, const int *axes = nullptr);
+
+// The tiles of one chunk as a batch (chunk.cpp, more than one tile): one
+// launch per pass of update_halo over every tile.  An item is one (tile,
+// face, field): a copy from tile `nbr` (update_tile_halo_kernel.f90), or the
+// reflective halo of an external face (update_halo_kernel.f90, nbr = tile).
+struct TileHaloItem {
+    int tile, nbr, field, hi, kind, ea, eb, poison;
+    double sgn;
+};
+// blocks one item needs along the face's two tangential axes for tile g
+void tile_halo_blocks(const Lay &g, int axis, int depth, int &per_a, int &per_b);
+// copy: the tile copies of one pass; else reflective halos along one axis
+// (items: device array; per_a x per_b: the largest tile's blocks)
+hipError_t batch_tile_halo(Store s, bool copy, const DeckDev *tiles, const TileHaloItem *items, int nitems, int per_a,
+                           int per_b, int axis, int depth, hipStream_t st);
+// the item lists of one update_halo over the tiles: copy[pass] for the three
+// tile passes (y, x, z faces), refl[k][side] for the reflective faces of the
+// k-th axis in y, x, z order; nbr6: 6 neighbours per tile (from 1, -1
+// external)
+void tile_halo_items(const int *nbr6, int ntiles, const Lay *lays, const int *fields, int depth, const int *axes,
+                     bool poison, std::vector<TileHaloItem> (&copy)[3], std::vector<TileHaloItem> (&refl)[3][2]);
+
+// Per-tile minimum of a batched calc_dt's partials (tile t's partials at
+// start[t]..start[t+1]-1), seeded with g_big, into out[stride * t]
+hipError_t batch_tile_min(int ntiles, const int *start, const double *partial, double g_big, double *out, int stride,
+                          hipStream_t st);
+// dt of every table entry
+hipError_t batch_set_dt(DeckDev *decks, int nd, double dt, hipStream_t st);
 
 // Per-deck steps (one workgroup per deck).
 // timestep.f90:86-126 on the device: dt = min over the deck's partials,

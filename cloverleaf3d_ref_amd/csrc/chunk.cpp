@@ -22,6 +22,7 @@
 #include "runtime.h"
 #include "driver.h"
 #include "setup.h"
+#include "batch.h"
 
 namespace clover {
 
@@ -130,6 +131,22 @@ struct clover_chunk {
     Shapes shapes;       // launch shapes of the fused kernels (autotuner)
     int dt_tile;         // the tile whose dt the last timestep took (-1: none yet)
     Geo geo;             // the uniform mesh, for the fused kernels
+    // More than one tile with the fused kernels: each stage is one launch over
+    // every tile (batch.h, the ensemble's batched kernels with the tiles as
+    // decks) and each update_halo one launch per pass.  The tiles' arrays
+    // are read from a device table that every array exchange keeps in step
+    // with `tiles`.
+    bool batched;
+    DeckDev *tdev;
+    std::map<std::pair<int, int>, BatchGrid> grids;  // (batched kernel, planes) -> prefix table
+    std::vector<int *> tables;
+    struct HaloTabs {
+        TileHaloItem *dev;
+        int n_copy[3], off_copy[3], pa_copy[3], pb_copy[3];
+        int n_refl[3][2], off_refl[3][2], pa_refl[3], pb_refl[3];
+        bool split[3];  // the faces of that reflective axis in two launches (a tile under 3 cells thick)
+    };
+    std::map<std::string, HaloTabs> halo_tabs;  // per update_halo signature
 };
 
 // installs the chunk's kernel timer (if any) for the duration of an API call
@@ -334,6 +351,175 @@ static double *dev_upload_1d(clover_chunk *c, const std::vector<double> &v)
     return p;
 }
 
+// ------------------------------------------------------------ tiles as a batch
+// the tile table on the device, from the tiles' current arrays
+static void tiles_table_init(clover_chunk *c)
+{
+    const int T = (int)c->tiles.size();
+    std::vector<DeckDev> h(T);
+    for (int t = 0; t < T; t++) {
+        DeckDev &D = h[t];
+        std::memset(&D, 0, sizeof(D));
+        D.g = c->tiles[t].g;
+        D.m = c->geo;
+        for (int a = 0; a < NUM_ARRAYS; a++) D.f[a] = c->tiles[t].f[a];
+        D.dtc_safe = c->deck.dtc_safe;
+        D.dtu_safe = c->deck.dtu_safe;
+        D.dtv_safe = c->deck.dtv_safe;
+        D.dtw_safe = c->deck.dtw_safe;
+        D.dtdiv_safe = c->deck.dtdiv_safe;
+    }
+    c->tdev = reinterpret_cast<DeckDev *>(dev_alloc_bytes(c, sizeof(DeckDev) * T));
+    hip_check(hipMemcpyAsync(c->tdev, h.data(), sizeof(DeckDev) * T, hipMemcpyHostToDevice, c->st), "tile table");
+    hip_check(hipStreamSynchronize(c->st), "sync");  // h dies here
+}
+
+// prefix table of batched kernel k with `planes` over the tiles (built once)
+static const BatchGrid &tgrid(clover_chunk *c, int k, int planes)
+{
+    auto key = std::make_pair(k, planes);
+    auto it = c->grids.find(key);
+    if (it != c->grids.end()) return it->second;
+    const int T = (int)c->tiles.size();
+    std::vector<int> start(T + 1, 0);
+    for (int t = 0; t < T; t++) start[t + 1] = start[t] + batch_blocks((BatchKernel)k, c->tiles[t].g, planes);
+    int *p = nullptr;
+    hip_check(hipMalloc(&p, sizeof(int) * (T + 1)), "hipMalloc");
+    c->tables.push_back(p);
+    c->bytes += (long long)sizeof(int) * (T + 1);
+    hip_check(hipMemcpyAsync(p, start.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice, c->st), "prefix table");
+    hip_check(hipStreamSynchronize(c->st), "sync");  // `start` dies here
+    return c->grids[key] = BatchGrid{p, start[T]};
+}
+
+// the shape index of a batched kernel (none: -1)
+static int shape_of(int bk)
+{
+    switch (bk) {
+    case BK_VISC_DT: return SK_VISC_DT;
+    case BK_PDV_PREDICT: return SK_PDV_PREDICT;
+    case BK_ACCEL_PDV: return SK_ACCEL_PDV;
+    case BK_CELL_X: return SK_CELL_X;
+    case BK_CELL_Y: return SK_CELL_Y;
+    case BK_CELL_Z: return SK_CELL_Z;
+    case BK_MOM_X: return SK_MOM_X;
+    case BK_MOM_Y: return SK_MOM_Y;
+    case BK_MOM_Z: return SK_MOM_Z;
+    default: return -1;
+    }
+}
+
+static const BatchGrid &kgrid(clover_chunk *c, int bk)
+{
+    const int si = shape_of(bk);
+    return tgrid(c, bk, si < 0 ? 0 : c->shapes.k[si].planes);
+}
+
+// exchange array roles in every tile, on the host and in the device table
+static void tswap(clover_chunk *c, std::initializer_list<std::pair<int, int>> pairs)
+{
+    SlotSwap sw{};
+    for (auto &pr : pairs) {
+        for (auto &t : c->tiles) std::swap(t.f[pr.first], t.f[pr.second]);
+        sw.a[sw.n] = pr.first;
+        sw.b[sw.n] = pr.second;
+        sw.n++;
+    }
+    if (c->batched) hip_check(batch_swap(c->tdev, (int)c->tiles.size(), sw, c->st), "swap");
+}
+
+// update_halo over the tiles in one launch per pass: the three tile passes,
+// then the reflective faces axis by axis (y, x, z)
+static void batched_update_halo(clover_chunk *c, const int *fields, int depth, const int *axes)
+{
+    const bool poison = axes && c->halo_check;
+    std::string key(reinterpret_cast<const char *>(fields), sizeof(int) * NUM_FIELDS);
+    key += std::string(1, (char)depth) + std::string(1, (char)poison);
+    if (axes) key += std::string(reinterpret_cast<const char *>(axes), sizeof(int) * NUM_FIELDS);
+    auto it = c->halo_tabs.find(key);
+    if (it == c->halo_tabs.end()) {
+        const int T = (int)c->tiles.size();
+        std::vector<int> nbr(6 * T);
+        std::vector<Lay> lays(T);
+        for (int t = 0; t < T; t++) {
+            for (int f = 0; f < 6; f++) nbr[6 * t + f] = c->tiles[t].nbr[f];
+            lays[t] = c->tiles[t].g;
+        }
+        std::vector<TileHaloItem> copy[3], refl[3][2];
+        tile_halo_items(nbr.data(), T, lays.data(), fields, depth, axes, poison, copy, refl);
+        clover_chunk::HaloTabs tab{};
+        std::vector<TileHaloItem> all;
+        static const int axis_order[3] = {1, 0, 2};
+        for (int p = 0; p < 3; p++) {
+            const int axis = axis_order[p];  // the passes go y, x, z too
+            tab.off_copy[p] = (int)all.size();
+            tab.n_copy[p] = (int)copy[p].size();
+            all.insert(all.end(), copy[p].begin(), copy[p].end());
+            int pa = 1, pb = 1;
+            for (auto &x : copy[p]) {
+                int a, b;
+                tile_halo_blocks(lays[x.tile], axis, depth, a, b);
+                pa = std::max(pa, a);
+                pb = std::max(pb, b);
+            }
+            tab.pa_copy[p] = pa;
+            tab.pb_copy[p] = pb;
+            pa = pb = 1;
+            bool split = false;
+            for (int sd = 0; sd < 2; sd++) {
+                tab.off_refl[p][sd] = (int)all.size();
+                tab.n_refl[p][sd] = (int)refl[p][sd].size();
+                all.insert(all.end(), refl[p][sd].begin(), refl[p][sd].end());
+                for (auto &x : refl[p][sd]) {
+                    int a, b;
+                    tile_halo_blocks(lays[x.tile], axis, depth, a, b);
+                    pa = std::max(pa, a);
+                    pb = std::max(pb, b);
+                    const Lay &g = lays[x.tile];
+                    const int mn[3] = {g.x_min, g.y_min, g.z_min}, mx[3] = {g.x_max, g.y_max, g.z_max};
+                    if (mx[axis] - mn[axis] + 1 < 3) split = true;
+                }
+            }
+            tab.pa_refl[p] = pa;
+            tab.pb_refl[p] = pb;
+            tab.split[p] = split;
+        }
+        tab.dev = nullptr;
+        if (!all.empty()) {
+            tab.dev = reinterpret_cast<TileHaloItem *>(dev_alloc_bytes(c, sizeof(TileHaloItem) * all.size()));
+            hip_check(hipMemcpyAsync(tab.dev, all.data(), sizeof(TileHaloItem) * all.size(), hipMemcpyHostToDevice,
+                                     c->st),
+                      "halo items");
+            hip_check(hipStreamSynchronize(c->st), "sync");  // `all` dies here
+        }
+        it = c->halo_tabs.emplace(key, tab).first;
+    }
+    const clover_chunk::HaloTabs &tab = it->second;
+    static const int axis_order[3] = {1, 0, 2};
+    {
+        Phase ph(c, P_TILE_HALO);
+        for (int p = 0; p < 3; p++)
+            hip_check(batch_tile_halo(c->store, true, c->tdev, tab.dev + tab.off_copy[p], tab.n_copy[p],
+                                      tab.pa_copy[p], tab.pb_copy[p], axis_order[p], depth, c->st),
+                      "tile halo");
+    }
+    Phase ph(c, P_SELF_HALO);
+    for (int p = 0; p < 3; p++) {
+        if (tab.split[p]) {
+            for (int sd = 0; sd < 2; sd++)
+                hip_check(batch_tile_halo(c->store, false, c->tdev, tab.dev + tab.off_refl[p][sd], tab.n_refl[p][sd],
+                                          tab.pa_refl[p], tab.pb_refl[p], axis_order[p], depth, c->st),
+                          "update_halo");
+        } else {
+            // the two faces' items are adjacent in the table
+            hip_check(batch_tile_halo(c->store, false, c->tdev, tab.dev + tab.off_refl[p][0],
+                                      tab.n_refl[p][0] + tab.n_refl[p][1], tab.pa_refl[p], tab.pb_refl[p],
+                                      axis_order[p], depth, c->st),
+                      "update_halo");
+        }
+    }
+}
+
 // ------------------------------------------------------------ phases
 static void set_fields(int *fields, std::initializer_list<int> ids)
 {
@@ -344,6 +530,10 @@ static void set_fields(int *fields, std::initializer_list<int> ids)
 // axes: null, or the per-field axis masks of a fused-step plan (driver.h)
 static void update_halo(clover_chunk *c, const int *fields, int depth, const int *axes = nullptr)
 {
+    if (c->batched) {
+        batched_update_halo(c, fields, depth, axes);
+        return;
+    }
     {
         Phase ph(c, P_TILE_HALO);
         // update_tile_halo.f90:47-423: top/bottom for all tiles, then
@@ -421,7 +611,16 @@ static int timestep(clover_chunk *c)
         update_halo(c, fields, 1);
     }
     int nparts[64] = {0};
-    if (c->fast) {
+    // every tile in one launch (batched) when the tiles' calc_dt partials at
+    // the current shape fit the partial buffer; each tile's partials are
+    // reduced to its own minimum, as the per-tile final pass does
+    if (c->batched && kgrid(c, BK_VISC_DT).total <= T * kMaxPartials * 5) {
+        Phase ph(c, P_VISCOSITY);
+        const BatchGrid &vg = kgrid(c, BK_VISC_DT);
+        hip_check(batch_visc_dt(c->store, c->tdev, T, vg, c->shapes.k[SK_VISC_DT], c->g_small, c->partial, c->st),
+                  "visc_dt");
+        hip_check(batch_tile_min(T, vg.start, c->partial, c->g_big, c->scalar, 8, c->st), "calc_dt min");
+    } else if (c->fast) {
         // viscosity and calc_dt in one pass: calc_dt reads viscosity only at
         // the cell itself, which the viscosity halo exchange does not change
         Phase ph(c, P_VISCOSITY);
@@ -485,6 +684,7 @@ static int timestep(clover_chunk *c)
     c->dt = m;
     if (c->dt < d.dtmin) return 1;
     c->dtold = c->dt;
+    if (c->batched) hip_check(batch_set_dt(c->tdev, T, c->dt, c->st), "set dt");
     return 0;
 }
 
@@ -495,11 +695,18 @@ static void pdv(clover_chunk *c, bool predict)
         // here because the predicted density/energy are never stored
         {
             Phase ph(c, P_PDV);
-            for (auto &t : c->tiles)
-                hip_check(launch_fast_pdv_predict(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_ENERGY0],
-                                                  t.f[F_PRESSURE], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
-                                                  t.f[F_ZVEL0], c->shapes.k[SK_PDV_PREDICT], c->st),
+            if (c->batched) {
+                hip_check(batch_pdv_predict(c->store, c->tdev, (int)c->tiles.size(), kgrid(c, BK_PDV_PREDICT),
+                                            c->shapes.k[SK_PDV_PREDICT], c->st),
                           "PdV predict");
+            } else {
+                for (auto &t : c->tiles)
+                    hip_check(launch_fast_pdv_predict(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0],
+                                                      t.f[F_ENERGY0], t.f[F_PRESSURE], t.f[F_VISCOSITY],
+                                                      t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
+                                                      c->shapes.k[SK_PDV_PREDICT], c->st),
+                              "PdV predict");
+            }
             c->gas_fresh = false;  // pressure now holds the predicted state's
         }
         fused_halo(c, FH_PRESSURE);
@@ -529,6 +736,15 @@ static void pdv(clover_chunk *c, bool predict)
 static void advec_cell(clover_chunk *c, int sweep, int dir)
 {
     Phase ph(c, P_CELL_ADVEC);
+    if (c->batched) {
+        const int k = dir == 1 ? BK_CELL_X : dir == 2 ? BK_CELL_Y : BK_CELL_Z;
+        hip_check(batch_advec_cell(c->store, c->tdev, (int)c->tiles.size(), kgrid(c, k), c->advect_x, dir, sweep,
+                                   sweep == 3, c->shapes.k[shape_of(k)], c->st),
+                  "advec_cell");
+        if (sweep == 3) c->gas_fresh = true;
+        tswap(c, {{F_DENSITY1, X_W1}, {F_ENERGY1, X_W2}});
+        return;
+    }
     for (auto &t : c->tiles) {
         if (c->fast) {
             // new density1/energy1 land in spare arrays, which then take their
@@ -558,6 +774,14 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
 static void advec_mom(clover_chunk *c, int sweep, int dir)
 {
     Phase ph(c, P_MOM_ADVEC);
+    if (c->batched) {
+        const int k = dir == 1 ? BK_MOM_X : dir == 2 ? BK_MOM_Y : BK_MOM_Z;
+        hip_check(batch_advec_mom(c->store, c->tdev, (int)c->tiles.size(), kgrid(c, k), c->advect_x, dir, sweep,
+                                  c->shapes.k[shape_of(k)], c->st),
+                  "advec_mom");
+        tswap(c, {{F_XVEL1, X_W5}, {F_YVEL1, X_W6}, {F_ZVEL1, X_W7}});
+        return;
+    }
     for (auto &t : c->tiles) {
         if (c->fast) {
             // the three velocity calls of advec_mom_driver.f90 in one pass; the
@@ -801,13 +1025,19 @@ static void do_step(clover_chunk *c)
     if (c->fast) {
         // accelerate, the PdV corrector and flux_calc in one march
         Phase ph(c, P_ACCEL);
-        for (auto &t : c->tiles)
-            hip_check(launch_fast_accel_pdv(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
-                                            t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
-                                            t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1], t.f[F_ZVEL0],
-                                            t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y], t.f[F_VOL_FLUX_Z],
-                                            c->shapes.k[SK_ACCEL_PDV], c->st),
+        if (c->batched) {
+            hip_check(batch_accel_pdv(c->store, c->tdev, (int)c->tiles.size(), kgrid(c, BK_ACCEL_PDV),
+                                      c->shapes.k[SK_ACCEL_PDV], c->st),
                       "accelerate + PdV correct + flux_calc");
+        } else {
+            for (auto &t : c->tiles)
+                hip_check(launch_fast_accel_pdv(c->store, t.g, c->geo, c->dt, t.f[F_DENSITY0], t.f[F_DENSITY1],
+                                                t.f[F_ENERGY0], t.f[F_ENERGY1], t.f[F_PRESSURE], t.f[F_VISCOSITY],
+                                                t.f[F_XVEL0], t.f[F_XVEL1], t.f[F_YVEL0], t.f[F_YVEL1],
+                                                t.f[F_ZVEL0], t.f[F_ZVEL1], t.f[F_VOL_FLUX_X], t.f[F_VOL_FLUX_Y],
+                                                t.f[F_VOL_FLUX_Z], c->shapes.k[SK_ACCEL_PDV], c->st),
+                          "accelerate + PdV correct + flux_calc");
+        }
     } else {
         {
             Phase ph(c, P_ACCEL);
@@ -829,7 +1059,12 @@ static void do_step(clover_chunk *c)
                       "flux_calc");
     }
     advection(c);
-    {
+    if (c->batched) {
+        // reset_field: the level 0/1 arrays exchange roles in every tile and
+        // in the device table, as the per-tile fused path below does
+        tswap(c, {{F_DENSITY0, F_DENSITY1}, {F_ENERGY0, F_ENERGY1}, {F_XVEL0, F_XVEL1}, {F_YVEL0, F_YVEL1},
+                  {F_ZVEL0, F_ZVEL1}});
+    } else {
         Phase ph(c, P_RESET);
         for (auto &t : c->tiles) {
             if (c->fast) {
@@ -864,6 +1099,10 @@ static void destroy(clover_chunk *c)
         for (double *p : oned)
             if (p) (void)hipFree(p);
     }
+    for (int *p : c->tables) (void)hipFree(p);
+    for (auto &kv : c->halo_tabs)
+        if (kv.second.dev) (void)hipFree(kv.second.dev);
+    if (c->tdev) (void)hipFree(c->tdev);
     if (c->partial) (void)hipFree(c->partial);
     if (c->scalar) (void)hipFree(c->scalar);
     if (c->h_scalar) (void)hipHostFree(c->h_scalar);
@@ -946,6 +1185,17 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
             t.h_celly = h.celly;
             t.h_cellz = h.cellz;
             hip_check(hipStreamSynchronize(c->st), "sync");  // host buffers die here
+        }
+        // more than one tile: the fused step as batched launches, if every
+        // tile's calc_dt partials fit the partial buffer at the shortest
+        // march chunk the tuner may pick
+        if (c->fast && T > 1) {
+            long long need = 0;
+            for (auto &t : c->tiles) need += batch_blocks(BK_VISC_DT, t.g, kTunePlanes[0]);
+            if (need <= (long long)T * kMaxPartials * 5) {
+                c->batched = true;
+                tiles_table_init(c);
+            }
         }
         // start.f90:58-129
         c->time = 0.0;

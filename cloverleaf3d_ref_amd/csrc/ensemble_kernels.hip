@@ -81,6 +81,47 @@ hipError_t batch_deck_sum5(int nd, const int *sum_start, const double *partial, 
     return hipGetLastError();
 }
 
+// a tile's partials seeded with g_big: the single-tile final pass's minimum
+__global__ void __launch_bounds__(RED_THREADS)
+kb_tile_min(const int *__restrict__ start, const double *__restrict__ partial, double g_big, double *__restrict__ out,
+            int stride)
+{
+    const int t = blockIdx.x;
+    const int n = start[t + 1] - start[t];
+    const double *p = partial + start[t];
+    double m = g_big;
+    for (int c = threadIdx.x; c < n; c += RED_THREADS) m = dmin(m, p[c]);
+    __shared__ double red[RED_THREADS / 64];
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = red[0];
+        for (int w = 1; w < RED_THREADS / 64; w++) b = dmin(b, red[w]);
+        out[stride * t] = b;
+    }
+}
+
+hipError_t batch_tile_min(int ntiles, const int *start, const double *partial, double g_big, double *out, int stride,
+                          hipStream_t st)
+{
+    TIMED_LAUNCH("kb_tile_min", st, kb_tile_min, dim3(ntiles), dim3(RED_THREADS), 0, st, start, partial, g_big, out,
+                 stride);
+    return hipGetLastError();
+}
+
+__global__ void kb_set_dt(DeckDev *__restrict__ decks, int nd, double dt)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < nd) decks[d].dt = dt;
+}
+
+hipError_t batch_set_dt(DeckDev *decks, int nd, double dt, hipStream_t st)
+{
+    TIMED_LAUNCH("kb_set_dt", st, kb_set_dt, dim3((nd + 63) / 64), dim3(64), 0, st, decks, nd, dt);
+    return hipGetLastError();
+}
+
 __global__ void kb_swap(DeckDev *__restrict__ decks, int nd, SlotSwap sw)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;

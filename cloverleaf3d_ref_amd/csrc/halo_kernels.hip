@@ -214,24 +214,22 @@ hipError_t launch_update_halo_store(Store s, const Lay &g, void *const *fp, cons
 //              f(max+1+d) = N(Nmin+d)     staggered
 // tangential ranges min-depth..max+depth, +1 where the field is staggered.
 template <typename S>
-__global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
+__device__ inline void tile_copy_body(const Lay &g, const Lay &gn, int axis, int depth, S *dst, const S *src, int kind,
+                                      int hi, int ea, int eb, int poison, int bx, int by)
 {
-    const HaloItem &it = batch.it[blockIdx.z];
-    S *const dst = static_cast<S *>(it.dst);
-    const S *const src = static_cast<const S *>(it.src);
     const int mn[3] = {g.x_min, g.y_min, g.z_min};
     const int mx[3] = {g.x_max, g.y_max, g.z_max};
     const int nmn[3] = {gn.x_min, gn.y_min, gn.z_min};
     const int nmx[3] = {gn.x_max, gn.y_max, gn.z_max};
     const int ta = tan_a(axis), tb = tan_b(axis);
-    const int c = mn[ta] - depth + (int)(blockIdx.x * HX + threadIdx.x);
-    const int b = mn[tb] - depth + (int)(blockIdx.y * HY + threadIdx.y);
-    if (c > mx[ta] + it.ea + depth || b > mx[tb] + it.eb + depth) return;
+    const int c = mn[ta] - depth + (bx * HX + (int)threadIdx.x);
+    const int b = mn[tb] - depth + (by * HY + (int)threadIdx.y);
+    if (c > mx[ta] + ea + depth || b > mx[tb] + eb + depth) return;
     for (int d = 1; d <= depth; d++) {
         int w, r;
-        if (!it.hi) {
+        if (!hi) {
             w = mn[axis] - d; r = nmx[axis] + 1 - d;
-        } else if (!it.kind) {
+        } else if (!kind) {
             w = mx[axis] + d; r = nmn[axis] - 1 + d;
         } else {
             w = mx[axis] + 1 + d; r = nmn[axis] + d;
@@ -241,8 +239,16 @@ __global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
         iw[ta] = ir[ta] = c;
         iw[tb] = ir[tb] = b;
         const S v = src[idx(gn, ir[0], ir[1], ir[2])];
-        dst[idx(g, iw[0], iw[1], iw[2])] = it.poison ? (S)poisoned((double)v) : v;
+        dst[idx(g, iw[0], iw[1], iw[2])] = poison ? (S)poisoned((double)v) : v;
     }
+}
+
+template <typename S>
+__global__ void k_tile_copy(Lay g, Lay gn, int axis, int depth, HaloBatch batch)
+{
+    const HaloItem &it = batch.it[blockIdx.z];
+    tile_copy_body<S>(g, gn, axis, depth, static_cast<S *>(it.dst), static_cast<const S *>(it.src), it.kind, it.hi,
+                      it.ea, it.eb, it.poison, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 hipError_t launch_tile_halo_store(Store s, int face, const Lay &g, void *const *fp, const Lay &gn, void *const *np,
@@ -358,6 +364,143 @@ hipError_t batch_reflect(Store s, const DeckDev *decks, int nd, const BatchGrid 
     else hipLaunchKernelGGL(kb_reflect<double>, dim3(bg.total), dim3(HX, HY), 0, st, decks, bg.start, nd, axis, depth, b);
     if (t_kernel_timer) t_kernel_timer->end(st);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The tiles of one chunk in one launch (batch.h TileHaloItem): every item is
+// one (tile, face, field) copy or reflection and owns per_a x per_b blocks
+// of the launch, enough for the largest tile's face; blocks past an item's
+// own face do nothing.
+__host__ __device__ inline void item_block(int b, int per_a, int per_b, int &item, int &bx, int &by)
+{
+    item = b / (per_a * per_b);
+    const int r = b % (per_a * per_b);
+    bx = r % per_a;
+    by = r / per_a;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(HX *HY)
+kb_tile_copy(const DeckDev *__restrict__ tiles, const TileHaloItem *__restrict__ items, int per_a, int per_b,
+             int axis, int depth)
+{
+    int item, bx, by;
+    item_block((int)blockIdx.x, per_a, per_b, item, bx, by);
+    const TileHaloItem it = items[item];
+    const DeckDev &D = tiles[it.tile];
+    const DeckDev &N = tiles[it.nbr];
+    tile_copy_body<S>(D.g, N.g, axis, depth, static_cast<S *>(D.f[it.field]), static_cast<const S *>(N.f[it.field]),
+                      it.kind, it.hi, it.ea, it.eb, it.poison, bx, by);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(HX *HY)
+kb_tile_reflect(const DeckDev *__restrict__ tiles, const TileHaloItem *__restrict__ items, int per_a, int per_b,
+                int axis, int depth)
+{
+    int item, bx, by;
+    item_block((int)blockIdx.x, per_a, per_b, item, bx, by);
+    const TileHaloItem it = items[item];
+    const DeckDev &D = tiles[it.tile];
+    reflect_body<S>(D.g, axis, depth, static_cast<S *>(D.f[it.field]), it.sgn, it.kind, it.hi, it.ea, it.eb, bx, by,
+                    it.poison);
+}
+
+void tile_halo_blocks(const Lay &g, int axis, int depth, int &per_a, int &per_b)
+{
+    dims_reflect(g, axis, depth, per_a, per_b);
+}
+
+hipError_t batch_tile_halo(Store s, bool copy, const DeckDev *tiles, const TileHaloItem *items, int nitems, int per_a,
+                           int per_b, int axis, int depth, hipStream_t st)
+{
+    if (nitems == 0) return hipSuccess;
+    if (depth < 1 || depth > 2) return hipErrorInvalidValue;  // reflect_body holds two depths
+    const dim3 grid((unsigned)(nitems * per_a * per_b)), blk(HX, HY);
+    if (copy) {
+        if (t_kernel_timer) t_kernel_timer->begin("kb_tile_copy", st);
+        if (s == ST_F32) hipLaunchKernelGGL(kb_tile_copy<float>, grid, blk, 0, st, tiles, items, per_a, per_b, axis, depth);
+        else hipLaunchKernelGGL(kb_tile_copy<double>, grid, blk, 0, st, tiles, items, per_a, per_b, axis, depth);
+    } else {
+        if (t_kernel_timer) t_kernel_timer->begin("kb_tile_reflect", st);
+        if (s == ST_F32) hipLaunchKernelGGL(kb_tile_reflect<float>, grid, blk, 0, st, tiles, items, per_a, per_b, axis, depth);
+        else hipLaunchKernelGGL(kb_tile_reflect<double>, grid, blk, 0, st, tiles, items, per_a, per_b, axis, depth);
+    }
+    if (t_kernel_timer) t_kernel_timer->end(st);
+    return hipGetLastError();
+}
+
+// The items of the tile passes and reflective passes of one update_halo of
+// a tiled chunk, in the order launch_tile_halo_store and
+// launch_update_halo_store would touch them tile by tile.
+void tile_halo_items(const int *nbr6, int ntiles, const Lay *lays, const int *fields, int depth, const int *axes,
+                     bool poison, std::vector<TileHaloItem> (&copy)[3], std::vector<TileHaloItem> (&refl)[3][2])
+{
+    (void)depth;
+    // update_tile_halo.f90:47-423: top/bottom, then left/right, then front/back
+    static const int passes[3][2] = {{4, 3}, {1, 2}, {6, 5}};
+    for (int p = 0; p < 3; p++) {
+        copy[p].clear();
+        for (int t = 0; t < ntiles; t++)
+            for (int sd = 0; sd < 2; sd++) {
+                const int face = passes[p][sd];
+                const int n = nbr6[6 * t + face - 1];
+                if (n == -1) continue;
+                const int axis = (face - 1) / 2, hi = (face - 1) % 2;
+                const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
+                for (int f = 0; f < NUM_FIELDS; f++) {
+                    int gate = f, src = f;
+                    if (f == F_ENERGY1 && face <= 4) gate = F_DENSITY1;
+                    if (f == F_ZVEL0 && face == 2) src = F_YVEL0;
+                    if (fields[gate] != 1) continue;
+                    const bool want = wanted(axes, src, axis);
+                    if (!want && !poison) continue;
+                    TileHaloItem it{};
+                    it.tile = t;
+                    it.nbr = n - 1;
+                    it.field = src;
+                    it.hi = hi;
+                    it.kind = kStag[src][axis];
+                    it.ea = kStag[src][ta];
+                    it.eb = kStag[src][tb];
+                    it.poison = !want;
+                    it.sgn = 1.0;
+                    copy[p].push_back(it);
+                }
+            }
+    }
+    // update_halo_kernel.f90 per tile: y faces, then x, then z; the two
+    // faces of an axis go to separate lists (refl[axis][side]) so that a
+    // tile under 3 cells thick can take them one after the other
+    static const int axis_order[3] = {1, 0, 2};
+    for (int ao = 0; ao < 3; ao++) {
+        const int axis = axis_order[ao];
+        for (int sd = 0; sd < 2; sd++) {
+            refl[ao][sd].clear();
+            const int face = 2 * axis + sd;
+            for (int t = 0; t < ntiles; t++) {
+                if (nbr6[6 * t + face] != -1) continue;
+                (void)lays;
+                for (int f = 0; f < NUM_FIELDS; f++) {
+                    if (fields[f] != 1) continue;
+                    const bool want = wanted(axes, f, axis);
+                    if (!want && !poison) continue;
+                    const HaloDesc &h = kHalo[f];
+                    TileHaloItem it{};
+                    it.tile = t;
+                    it.nbr = t;
+                    it.field = f;
+                    it.hi = sd;
+                    it.kind = h.kind[axis];
+                    it.ea = h.ext[face][0];
+                    it.eb = h.ext[face][1];
+                    it.poison = !want;
+                    it.sgn = h.neg[axis] ? -1.0 : 1.0;
+                    refl[ao][sd].push_back(it);
+                }
+            }
+        }
+    }
 }
 
 // FP64 entry points used by the drop-in ABI and the per-call driver
