@@ -9,12 +9,19 @@
         launches, which follow the autotuner's trial launches (the stats
         table counts those trials too, for every candidate shape)
 
-    python -m cloverleaf3d_ref_amd.profiling traffic FETCH_DIR WRITE_DIR --copy-mib M --command CMD -o OUT.json
-        HBM bytes per dispatch of every kernel from two --pmc runs (FETCH_SIZE,
-        then WRITE_SIZE; DIRs hold *_counter_collection.csv).  Both counters
-        are scaled by what they read for the plain device copy of M MiB that
-        the profiled command ran first (bench.py --calib-copy-mib M), which
-        reads and writes exactly M MiB.
+    python -m cloverleaf3d_ref_amd.profiling traffic READ_DIR WRITE_DIR --last N --cells C --elem-bytes B --command CMD -o OUT.json
+        HBM bytes per dispatch of every kernel from two --pmc runs of the same
+        command: READ_DIR with TCC_EA0_RDREQ_128B_sum, TCC_EA0_RDREQ_64B_sum
+        and TCC_EA0_RDREQ_32B_sum (read requests to memory by size), WRITE_DIR
+        with TCC_EA0_WRREQ_sum and TCC_EA0_WRREQ_64B_sum (64- and 32-byte
+        write transactions); each kernel averaged over its last N dispatches
+        (the timed steps', after the autotuner's trials).  The check: the
+        equation-of-state kernel the chunk runs at set-up reads density and
+        energy and writes pressure and soundspeed, 2 + 2 arrays of C^3
+        elements of B bytes each, so its counters must read exactly that.
+        (rocprofv3's FETCH_SIZE takes its 128-byte reads from TCC_BUBBLE,
+        which on this GPU sees few of them: it reads about 0.57 of the bytes
+        a plain copy moves; the request-size counters are used instead.)
 """
 from __future__ import annotations
 
@@ -93,48 +100,53 @@ def last_launches(d: str, n: int, elem: str = "") -> Dict[str, Tuple[int, float]
     return out
 
 
-def counter_per_dispatch(d: str, counter: str) -> List[Tuple[str, float]]:
-    """(kernel, value) for every dispatch in a --pmc run, in dispatch order."""
-    out = []
+READ_COUNTERS = ("TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_32B_sum")
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+CHECK_KERNEL = "k_fast_ideal_gas"
+
+
+def dispatch_counters(d: str) -> List[Tuple[str, Dict[str, float]]]:
+    """(kernel, {counter: value}) for every dispatch of a --pmc run, in
+    dispatch order."""
+    per: Dict[int, Tuple[str, Dict[str, float]]] = {}
     with open(_find(d, "counter_collection.csv")) as f:
         for r in csv.DictReader(f):
-            if r["Counter_Name"] == counter:
-                out.append((r["Kernel_Name"], float(r["Counter_Value"])))
-    return out
+            i = int(r["Dispatch_Id"])
+            if i not in per:
+                per[i] = (r["Kernel_Name"], {})
+            per[i][1][r["Counter_Name"]] = per[i][1].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return [per[i] for i in sorted(per)]
 
 
-def traffic(fetch_dir: str, write_dir: str, copy_mib: int) -> Dict:
-    known = copy_mib * (1 << 20)
-    fetch = counter_per_dispatch(fetch_dir, "FETCH_SIZE")
-    write = counter_per_dispatch(write_dir, "WRITE_SIZE")
+def _last_avg(rows: List[Tuple[str, float]], last: int) -> Dict[str, Tuple[int, float]]:
+    per: Dict[str, List[float]] = defaultdict(list)
+    for n, v in rows:
+        per[_short(n)].append(v)
+    return {k: (len(v[-last:]), sum(v[-last:]) / len(v[-last:])) for k, v in per.items()}
 
-    def calib(rows, what):
-        copies = [v for n, v in rows if "copy" in n.lower()]
-        if not copies:
-            raise RuntimeError(f"no copy dispatch in the {what} run")
-        v = max(copies) * 1024.0
-        if not 0.5 * known < v < 2.0 * known:
-            raise RuntimeError(f"{what}: largest copy dispatch read {v:.3e} B, expected about {known:.3e} B")
-        return known / v, v
 
-    ff, fraw = calib(fetch, "FETCH_SIZE")
-    wf, wraw = calib(write, "WRITE_SIZE")
-    acc: Dict[str, List[float]] = defaultdict(lambda: [0.0, 0.0, 0])
-    for n, v in fetch:
-        a = acc[_short(n)]
-        a[0] += v * 1024.0 * ff
-        a[2] += 1
-    wsum: Dict[str, float] = defaultdict(float)
-    for n, v in write:
-        wsum[_short(n)] += v * 1024.0 * wf
+def traffic(read_dir: str, write_dir: str, last: int, cells: int, elem_bytes: int) -> Dict:
+    reads, writes = [], []
+    for n, c in dispatch_counters(read_dir):
+        reads.append((n, 128.0 * c.get(READ_COUNTERS[0], 0.0) + 64.0 * c.get(READ_COUNTERS[1], 0.0) +
+                      32.0 * c.get(READ_COUNTERS[2], 0.0)))
+    for n, c in dispatch_counters(write_dir):
+        w, w64 = c.get(WRITE_COUNTERS[0], 0.0), c.get(WRITE_COUNTERS[1], 0.0)
+        writes.append((n, 64.0 * w64 + 32.0 * (w - w64)))
+    r, w = _last_avg(reads, last), _last_avg(writes, last)
     kernels = {}
-    for n, (fb, _, calls) in acc.items():
-        if "copy" in n.lower() or "fill" in n.lower():
+    for k in r:
+        if "copy" in k.lower() or "fill" in k.lower():
             continue
-        kernels[n] = dict(dispatches=calls, fetch_bytes=fb / calls, write_bytes=wsum[n] / calls,
-                          hbm_bytes=(fb + wsum[n]) / calls)
-    return dict(copy_mib=copy_mib, fetch_scale=ff, write_scale=wf, copy_fetch_raw_bytes=fraw,
-                copy_write_raw_bytes=wraw, kernels=kernels)
+        rb, wb = r[k][1], w.get(k, (0, 0.0))[1]
+        kernels[k] = dict(dispatches=r[k][0], read_bytes=rb, write_bytes=wb, hbm_bytes=rb + wb)
+    out = dict(read_counters=list(READ_COUNTERS), write_counters=list(WRITE_COUNTERS), last=last, kernels=kernels)
+    if CHECK_KERNEL in kernels:
+        want = 2.0 * cells ** 3 * elem_bytes  # read density, energy; write pressure, soundspeed
+        got = kernels[CHECK_KERNEL]
+        out["check"] = dict(kernel=CHECK_KERNEL, expected_read_bytes=want, expected_write_bytes=want,
+                            read_ratio=got["read_bytes"] / want, write_ratio=got["write_bytes"] / want)
+    return out
 
 
 def main(argv=None) -> int:
@@ -148,9 +160,11 @@ def main(argv=None) -> int:
                    help="only the kernels instantiated for this element type")
     s.add_argument("--last", type=int, default=0, help="also average each marching kernel's last N launches")
     t = sub.add_parser("traffic")
-    t.add_argument("fetch_dir")
+    t.add_argument("read_dir")
     t.add_argument("write_dir")
-    t.add_argument("--copy-mib", type=int, required=True)
+    t.add_argument("--last", type=int, required=True, help="average each kernel's last N dispatches")
+    t.add_argument("--cells", type=int, required=True, help="edge of the profiled cube (the check kernel's size)")
+    t.add_argument("--elem-bytes", type=int, default=8)
     t.add_argument("--command", required=True)
     t.add_argument("-o", "--out", required=True)
     a = ap.parse_args(argv)
@@ -171,7 +185,7 @@ def main(argv=None) -> int:
                     if name.startswith("k_fast_") and calls > 1:
                         f.write(f"{name:<40} {tail[name][0]:>7} {tail[name][1]:>10.2f}\n")
     else:
-        res = traffic(a.fetch_dir, a.write_dir, a.copy_mib)
+        res = traffic(a.read_dir, a.write_dir, a.last, a.cells, a.elem_bytes)
         res["command"] = a.command
         with open(a.out, "w") as f:
             json.dump(res, f, indent=1, sort_keys=True)

@@ -50,20 +50,29 @@ def test_last_launches(tmp_path):
     assert P.last_launches(str(tmp_path), 5, "float")["k_fast_accel_pdv"] == (2, 1.5)
 
 
-def test_traffic_scaled_by_the_copy(tmp_path):
-    fetch, write = tmp_path / "f", tmp_path / "w"
-    fetch.mkdir()
-    write.mkdir()
-    hdr = ["Kernel_Name", "Counter_Name", "Counter_Value"]
-    # the counters read a 1 MiB copy as 640 KiB fetched and 1600 KiB written
-    _write(fetch / "run_counter_collection.csv", hdr,
-           [["__amd_rocclr_copyBuffer", "FETCH_SIZE", 640], [D, "FETCH_SIZE", 100], [D, "FETCH_SIZE", 300]])
-    _write(write / "run_counter_collection.csv", hdr,
-           [["__amd_rocclr_copyBuffer", "WRITE_SIZE", 1600], [D, "WRITE_SIZE", 400], [D, "WRITE_SIZE", 400]])
-    t = P.traffic(str(fetch), str(write), 1)
-    assert t["fetch_scale"] == 1.6 and t["write_scale"] == 0.64
+def test_traffic_from_request_counters(tmp_path):
+    rd, wr = tmp_path / "r", tmp_path / "w"
+    rd.mkdir()
+    wr.mkdir()
+    hdr = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    G = "void clover::k_fast_ideal_gas<double>(clover::Lay)"
+    rows = []
+    # dispatch 1: the check kernel at 4^3 FP64 (2 arrays = 1024 B read, as 8 128-byte requests)
+    rows += [[1, G, "TCC_EA0_RDREQ_128B_sum", 8], [1, G, "TCC_EA0_RDREQ_64B_sum", 0], [1, G, "TCC_EA0_RDREQ_32B_sum", 0]]
+    # dispatches 2, 3: a trial and a timed launch of accel_pdv; only the last counts with --last 1
+    rows += [[2, D, "TCC_EA0_RDREQ_128B_sum", 100], [2, D, "TCC_EA0_RDREQ_64B_sum", 0], [2, D, "TCC_EA0_RDREQ_32B_sum", 0]]
+    rows += [[3, D, "TCC_EA0_RDREQ_128B_sum", 10], [3, D, "TCC_EA0_RDREQ_64B_sum", 3], [3, D, "TCC_EA0_RDREQ_32B_sum", 2]]
+    _write(rd / "run_counter_collection.csv", hdr, rows)
+    rows = [[1, G, "TCC_EA0_WRREQ_sum", 16], [1, G, "TCC_EA0_WRREQ_64B_sum", 16],
+            [2, D, "TCC_EA0_WRREQ_sum", 50], [2, D, "TCC_EA0_WRREQ_64B_sum", 50],
+            [3, D, "TCC_EA0_WRREQ_sum", 7], [3, D, "TCC_EA0_WRREQ_64B_sum", 5]]
+    _write(wr / "run_counter_collection.csv", hdr, rows)
+    t = P.traffic(str(rd), str(wr), 1, 4, 8)
     k = t["kernels"]["k_fast_accel_pdv"]
-    assert k["dispatches"] == 2
-    assert k["fetch_bytes"] == pytest.approx(200 * 1024 * 1.6)
-    assert k["write_bytes"] == pytest.approx(400 * 1024 * 0.64)
-    assert k["hbm_bytes"] == k["fetch_bytes"] + k["write_bytes"]
+    assert k["dispatches"] == 1
+    assert k["read_bytes"] == 10 * 128 + 3 * 64 + 2 * 32
+    assert k["write_bytes"] == 5 * 64 + 2 * 32
+    assert k["hbm_bytes"] == k["read_bytes"] + k["write_bytes"]
+    assert t["check"]["read_ratio"] == 1.0 and t["check"]["write_ratio"] == 1.0
+    t = P.traffic(str(rd), str(wr), 2, 4, 8)
+    assert t["kernels"]["k_fast_accel_pdv"]["read_bytes"] == (100 * 128 + 10 * 128 + 3 * 64 + 2 * 32) / 2
