@@ -353,22 +353,29 @@ static double *dev_upload_1d(clover_chunk *c, const std::vector<double> &v)
 
 // ------------------------------------------------------------ tiles as a batch
 // the tile table on the device, from the tiles' current arrays
+// tile t's table entry from the host's view of its arrays (which every
+// exchange keeps equal to the device table's)
+static DeckDev tile_entry(const clover_chunk *c, int t, double dt)
+{
+    DeckDev D;
+    std::memset(&D, 0, sizeof(D));
+    D.g = c->tiles[t].g;
+    D.m = c->geo;
+    for (int a = 0; a < NUM_ARRAYS; a++) D.f[a] = c->tiles[t].f[a];
+    D.dt = dt;
+    D.dtc_safe = c->deck.dtc_safe;
+    D.dtu_safe = c->deck.dtu_safe;
+    D.dtv_safe = c->deck.dtv_safe;
+    D.dtw_safe = c->deck.dtw_safe;
+    D.dtdiv_safe = c->deck.dtdiv_safe;
+    return D;
+}
+
 static void tiles_table_init(clover_chunk *c)
 {
     const int T = (int)c->tiles.size();
     std::vector<DeckDev> h(T);
-    for (int t = 0; t < T; t++) {
-        DeckDev &D = h[t];
-        std::memset(&D, 0, sizeof(D));
-        D.g = c->tiles[t].g;
-        D.m = c->geo;
-        for (int a = 0; a < NUM_ARRAYS; a++) D.f[a] = c->tiles[t].f[a];
-        D.dtc_safe = c->deck.dtc_safe;
-        D.dtu_safe = c->deck.dtu_safe;
-        D.dtv_safe = c->deck.dtv_safe;
-        D.dtw_safe = c->deck.dtw_safe;
-        D.dtdiv_safe = c->deck.dtdiv_safe;
-    }
+    for (int t = 0; t < T; t++) h[t] = tile_entry(c, t, 0.0);
     c->tdev = reinterpret_cast<DeckDev *>(dev_alloc_bytes(c, sizeof(DeckDev) * T));
     hip_check(hipMemcpyAsync(c->tdev, h.data(), sizeof(DeckDev) * T, hipMemcpyHostToDevice, c->st), "tile table");
     hip_check(hipStreamSynchronize(c->st), "sync");  // h dies here
@@ -959,8 +966,79 @@ static void tune_launch(clover_chunk *c, int k)
     }
 }
 
+// The batched twin of tune_launch for a chunk whose stages run as batched
+// launches: the kernel runs over every tile with a copy of the tile table
+// whose output slots point at spare arrays (the same ones tune_launch
+// writes), so the timed launch is the one a step makes and every field keeps
+// its value.  `ttab` is that table (device, one entry per tile).
+static void tune_launch_batched(clover_chunk *c, int k, DeckDev *ttab)
+{
+    const int T = (int)c->tiles.size();
+    const Shape &sh = c->shapes.k[k];
+    const int bk = k == SK_VISC_DT ? BK_VISC_DT : k == SK_PDV_PREDICT ? BK_PDV_PREDICT
+                 : k == SK_ACCEL_PDV ? BK_ACCEL_PDV : k <= SK_CELL_Z ? BK_CELL_X + (k - SK_CELL_X)
+                 : BK_MOM_X + (k - SK_MOM_X);
+    const BatchGrid &g = tgrid(c, bk, sh.planes);
+    hipError_t e = hipSuccess;
+    switch (k) {
+    case SK_VISC_DT:
+        if (g.total > T * kMaxPartials * 5) {  // the step takes the per-tile launches then
+            tune_launch(c, k);
+            return;
+        }
+        e = batch_visc_dt(c->store, ttab, T, g, sh, c->g_small, c->partial, c->st);
+        if (e == hipSuccess) e = batch_tile_min(T, g.start, c->partial, c->g_big, c->scalar, 8, c->st);
+        break;
+    case SK_PDV_PREDICT: e = batch_pdv_predict(c->store, ttab, T, g, sh, c->st); break;
+    case SK_ACCEL_PDV: e = batch_accel_pdv(c->store, ttab, T, g, sh, c->st); break;
+    case SK_CELL_X:
+    case SK_CELL_Y:
+    case SK_CELL_Z: {
+        const int dir = k - SK_CELL_X + 1;
+        e = batch_advec_cell(c->store, ttab, T, g, 1, dir, dir == 2 ? 2 : 1, false, sh, c->st);
+        break;
+    }
+    default: {
+        const int dir = k - SK_MOM_X + 1;
+        e = batch_advec_mom(c->store, ttab, T, g, 1, dir, dir == 2 ? 2 : 1, sh, c->st);
+        break;
+    }
+    }
+    hip_check(e, "autotune launch");
+}
+
+// the tile table with kernel k's outputs sent to the spare arrays
+// tune_launch uses (the batched cell and momentum sweeps write their new
+// fields to W1/W2 and W5..W7 anyway)
+static std::vector<DeckDev> tune_table(clover_chunk *c, int k)
+{
+    std::vector<DeckDev> h;
+    for (int t = 0; t < (int)c->tiles.size(); t++) h.push_back(tile_entry(c, t, c->dtold));
+    for (auto &D : h) {
+        switch (k) {
+        case SK_VISC_DT: D.f[F_VISCOSITY] = D.f[X_W3]; break;
+        case SK_PDV_PREDICT: D.f[F_PRESSURE] = D.f[X_W3]; break;  // a copy of pressure (tune_time)
+        case SK_ACCEL_PDV:
+            D.f[F_DENSITY1] = D.f[X_W1];
+            D.f[F_ENERGY1] = D.f[X_W2];
+            D.f[F_XVEL1] = D.f[X_W5];
+            D.f[F_YVEL1] = D.f[X_W6];
+            D.f[F_ZVEL1] = D.f[X_W7];
+            D.f[F_VOL_FLUX_X] = D.f[X_W3];
+            D.f[F_VOL_FLUX_Y] = D.f[X_W4];
+            D.f[F_VOL_FLUX_Z] = D.f[X_W3];
+            break;
+        case SK_CELL_X: D.f[F_MASS_FLUX_X] = D.f[X_W3]; break;
+        case SK_CELL_Y: D.f[F_MASS_FLUX_Y] = D.f[X_W3]; break;
+        case SK_CELL_Z: D.f[F_MASS_FLUX_Z] = D.f[X_W3]; break;
+        default: break;
+        }
+    }
+    return h;
+}
+
 // average milliseconds of one tune_launch of kernel k (after one untimed run)
-static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b)
+static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b, DeckDev *ttab = nullptr)
 {
     const int reps = 3;
     if (k == SK_PDV_PREDICT)
@@ -968,9 +1046,13 @@ static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b)
             hip_check(hipMemcpyAsync(t.f[X_W3], t.f[F_PRESSURE], (size_t)lay_elems(t.g) * store_bytes(c->store),
                                      hipMemcpyDeviceToDevice, c->st),
                       "autotune copy");
-    tune_launch(c, k);
+    auto launch = [&]() {
+        if (ttab) tune_launch_batched(c, k, ttab);
+        else tune_launch(c, k);
+    };
+    launch();
     hip_check(hipEventRecord(a, c->st), "event");
-    for (int r = 0; r < reps; r++) tune_launch(c, k);
+    for (int r = 0; r < reps; r++) launch();
     hip_check(hipEventRecord(b, c->st), "event");
     hip_check(hipEventSynchronize(b), "event sync");
     float ms = 0;
@@ -991,20 +1073,31 @@ static void autotune(clover_chunk *c, double *before, double *after)
     hip_check(hipEventCreate(&a), "event");
     hip_check(hipEventCreate(&b), "event");
     double t0 = 0.0, t1 = 0.0;
+    // batched chunks time the batched launches, through a tile table whose
+    // output slots point at spare arrays (their z marches have one
+    // workgroup width)
+    DeckDev *ttab = nullptr;
+    if (c->batched) hip_check(hipMalloc(&ttab, sizeof(DeckDev) * c->tiles.size()), "hipMalloc");
     for (int k = 0; k < SK_COUNT; k++) {
+        if (ttab) {
+            const std::vector<DeckDev> h = tune_table(c, k);
+            hip_check(hipMemcpyAsync(ttab, h.data(), sizeof(DeckDev) * h.size(), hipMemcpyHostToDevice, c->st),
+                      "tune table");
+            hip_check(hipStreamSynchronize(c->st), "sync");
+        }
         const Shape start = c->shapes.k[k];
-        double best = tune_time(c, k, a, b);
+        double best = tune_time(c, k, a, b, ttab);
         Shape best_shape = start;
         t0 += best;
         for (int planes : kTunePlanes)
             for (int xcd = 0; xcd < 2; xcd++)
                 for (int bx : kTuneBx) {
                     if (k == SK_CELL_X && planes != start.planes) continue;
-                    if (!z_march(k) && bx != start.bx) continue;
+                    if ((ttab || !z_march(k)) && bx != start.bx) continue;
                     const Shape cand{planes, xcd, bx};
                     if (cand.planes == start.planes && cand.xcd == start.xcd && cand.bx == start.bx) continue;
                     c->shapes.k[k] = cand;
-                    const double ms = tune_time(c, k, a, b);
+                    const double ms = tune_time(c, k, a, b, ttab);
                     if (ms < best) {
                         best = ms;
                         best_shape = cand;
@@ -1013,6 +1106,7 @@ static void autotune(clover_chunk *c, double *before, double *after)
         c->shapes.k[k] = best_shape;
         t1 += best;
     }
+    if (ttab) (void)hipFree(ttab);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (before) *before = t0;
