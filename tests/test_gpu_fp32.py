@@ -1,10 +1,13 @@
-"""The FP32-store form: fields held in FP32, arithmetic in FP64.
+"""The FP32-store form: fields held and stencils computed in FP32, with
+calc_dt's minimum, advec_cell's mass and energy update and the summary sums
+(compensated) in FP64.
 
-Its results differ from the FP64 form only through the rounding of the
-stored fields, so every summary total stays within 1e-5 relative of the
-FP64 oracle (the reference program's own totals for the 64^3 deck, the
-oracle driver for odd tiled decks), the fields are exactly representable in
-FP32, and the start state (volume, mass) is the FP64 one rounded once.
+Its results differ from the FP64 form through FP32 rounding of the stored
+fields and of the stencil arithmetic; every summary total stays within 1e-5
+relative of the FP64 oracle (the reference program's own totals for the
+64^3 deck, the oracle driver for odd tiled decks), the fields are exactly
+representable in FP32, and the start state (volume, mass) is the FP64 one
+rounded once.
 """
 from __future__ import annotations
 
