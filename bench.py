@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=512)
     ap.add_argument("--cpu-cells", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=4)  # about 10-20 s of the reference on 16 host cores
+    ap.add_argument("--cpu-steps", type=int, default=2)  # about 25 s of the reference on 16 host cores at 512^3
     ap.add_argument("--accuracy-steps", type=int, default=300,
                     help="steps of the untimed FP32-vs-FP64 accuracy leg (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
