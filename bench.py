@@ -59,11 +59,12 @@ ARRAYS_PER_CELL = {
     # and energy1; mom_x/y/z read the sweep's mass flux, density1, 2/1/0
     # volume fluxes and three velocities and write three velocities.  x and
     # z run in sweeps 1 and 3, y always in sweep 2; the third sweep also
-    # writes pressure and soundspeed (the next step's ideal_gas), one of every
-    # two x and z sweeps.
+    # writes pressure (the next step's ideal_gas), one of every two x and z
+    # sweeps.  visc_dt reads rho0, p, u0, v0, w0 (it forms the soundspeed
+    # from rho0 and p) and writes q.
     # accel_pdv reads rho0, e0, p, q, u0, v0, w0 and writes u1, v1, w1, rho1, e1 and the three volume fluxes
-    "k_fast_visc_dt": 7, "k_fast_pdv_predict": 8, "k_fast_accel_pdv": 15,
-    "k_fast_cell_x": 8, "k_fast_cell_y": 7, "k_fast_cell_z": 8,
+    "k_fast_visc_dt": 6, "k_fast_pdv_predict": 8, "k_fast_accel_pdv": 15,
+    "k_fast_cell_x": 7.5, "k_fast_cell_y": 7, "k_fast_cell_z": 7.5,
     "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
     "k_fast_field_summary": 6, "k_fast_ideal_gas": 4,
 }

@@ -103,7 +103,7 @@ hipError_t batch_accel_pdv(Store s, const DeckDev *decks, int nd, const BatchGri
 hipError_t batch_field_summary(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, double *partial,
                                hipStream_t st);
 // advec_cell sweep `dir`; new density1/energy1 into the arrays X_W1/X_W2,
-// and with gas (third sweep only) ideal_gas of them into pressure/soundspeed
+// and with gas (third sweep only) ideal_gas's pressure of them into pressure
 hipError_t batch_advec_cell(Store s, const DeckDev *decks, int nd, const BatchGrid &bg, int advect_x, int dir,
                             int sweep_number, bool gas, const Shape &sh, hipStream_t st);
 // advec_mom sweep `dir`; new velocities into X_W5..X_W7

@@ -13,8 +13,8 @@
 // e_out) which then take the place of density1/energy1; only interior cells
 // of those are written (the halo update that follows every advec_cell
 // sweep in advection.f90 refreshes their halos before anything reads them).
-// The step's last sweep can also write pressure and soundspeed of the new
-// state (p_out, c_out; null otherwise): after reset_field that state is the
+// The step's last sweep can also write the pressure of the new state (p_out;
+// null otherwise): after reset_field that state is the
 // next step's level 0, so these are the values the next ideal_gas call
 // (timestep.f90:48 or field_summary.f90:49-60) would compute.
 // In the FP64 form every value is formed with the expression and operand
@@ -99,7 +99,7 @@ template <typename S>
 __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
               const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
-              S *__restrict__ e_out, S *__restrict__ p_out, S *__restrict__ c_out)
+              S *__restrict__ e_out, S *__restrict__ p_out)
 {
     const int lane = threadIdx.x;
     const int ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
@@ -137,7 +137,7 @@ __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mod
         const S rs = (S)rn, es = (S)en;
         rho_out[i] = rs;
         e_out[i] = es;
-        if (p_out) ideal_gas_cell<S>(rs, es, p_out[i], c_out[i]);
+        if (p_out) p_out[i] = pressure_of<S>(rs, es);
     }
 }
 
@@ -145,9 +145,9 @@ template <typename S>
 __global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(int xcd, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
               const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
-              S *__restrict__ e_out, S *__restrict__ p_out, S *__restrict__ c_out)
+              S *__restrict__ e_out, S *__restrict__ p_out)
 {
-    cell_x_body<S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out, p_out, c_out);
+    cell_x_body<S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out, p_out);
 }
 
 // The volume fluxes pre_vol reads for the y and z sweeps (modes 3-6), kept
@@ -195,7 +195,7 @@ template <int DIR, typename S>
 __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
                   const S *__restrict__ vfz, const S *__restrict__ density1,
                   const S *__restrict__ energy1, S *__restrict__ mass_flux, S *__restrict__ rho_out,
-                  S *__restrict__ e_out, S *__restrict__ p_out, S *__restrict__ c_out)
+                  S *__restrict__ e_out, S *__restrict__ p_out)
 {
     const int j = g.x_min + (int)(bk.x * 64 + threadIdx.x);
     const int cr = (DIR == 3 ? g.y_min : g.z_min) + (int)(bk.y * CROWS + threadIdx.y);
@@ -262,7 +262,7 @@ __device__ __forceinline__ void cell_march_body(const Blk &bk, Lay g, Geo m, int
             const S rs = (S)rn, es = (S)en;
             rho_out[OFF(c)] = rs;
             e_out[OFF(c)] = es;
-            if (p_out) ideal_gas_cell<S>(rs, es, p_out[OFF(c)], c_out[OFF(c)]);
+            if (p_out) p_out[OFF(c)] = pressure_of<S>(rs, es);
         }
         mf_lo = mf;
         ef_lo = ef;
@@ -282,32 +282,32 @@ template <int DIR, typename S>
 __global__ void __launch_bounds__(64 * CROWS) k_fast_cell_march(int xcd, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
                   const S *__restrict__ vfz, const S *__restrict__ density1,
                   const S *__restrict__ energy1, S *__restrict__ mass_flux, S *__restrict__ rho_out,
-                  S *__restrict__ e_out, S *__restrict__ p_out, S *__restrict__ c_out)
+                  S *__restrict__ e_out, S *__restrict__ p_out)
 {
-    cell_march_body<DIR, S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux, rho_out, e_out, p_out,
-                            c_out);
+    cell_march_body<DIR, S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux, rho_out, e_out,
+                            p_out);
 }
 
 template <typename S>
 static hipError_t fast_advec_cell_t(const Lay &g, const Geo &m, int mode, int dir, const S *density1,
                                     const S *energy1, S *mass_flux_x, const S *vol_flux_x, S *mass_flux_y,
                                     const S *vol_flux_y, S *mass_flux_z, const S *vol_flux_z, S *rho_out, S *e_out,
-                                    S *p_out, S *c_out, const Shape &sh, hipStream_t st)
+                                    S *p_out, const Shape &sh, hipStream_t st)
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const dim3 blk(64, CROWS, 1);
     if (dir == 1) {
         const dim3 grid((nx + 2 + 59) / 60, (ny * nz + CROWS - 1) / CROWS, 1);
         TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x<S>, grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x, vol_flux_y,
-                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out, p_out, c_out);
+                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out, p_out);
     } else if (dir == 2) {
         const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, nchunks(ny + 2, sh.planes));
         TIMED_LAUNCH("k_fast_cell_y", st, (k_fast_cell_march<2, S>), grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_y, rho_out, e_out, p_out, c_out);
+                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_y, rho_out, e_out, p_out);
     } else {
         const dim3 grid((nx + 63) / 64, (ny + CROWS - 1) / CROWS, nchunks(nz + 2, sh.planes));
         TIMED_LAUNCH("k_fast_cell_z", st, (k_fast_cell_march<3, S>), grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x,
-                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_z, rho_out, e_out, p_out, c_out);
+                     vol_flux_y, vol_flux_z, density1, energy1, mass_flux_z, rho_out, e_out, p_out);
     }
     return hipGetLastError();
 }
@@ -316,22 +316,20 @@ hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advec
                                   const void *density1, const void *energy1, void *mass_flux_x,
                                   const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
                                   void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
-                                  void *pressure, void *soundspeed, const Shape &sh, hipStream_t st)
+                                  void *pressure, const Shape &sh, hipStream_t st)
 {
     int mode = 0;
     if (dir == 1) mode = sweep_number == 1 ? 1 : sweep_number == 3 ? 2 : 0;
     else if (dir == 2) mode = sweep_number == 2 ? (advect_x ? 3 : 4) : 0;
     else if (dir == 3) mode = sweep_number == 1 ? 5 : sweep_number == 3 ? 6 : 0;
     if (!mode) return hipErrorInvalidValue;  // the resident driver issues only the reference's sweeps
-    if ((pressure == nullptr) != (soundspeed == nullptr)) return hipErrorInvalidValue;
     if (pressure && sweep_number != 3) return hipErrorInvalidValue;  // only the last sweep's state is final
 #define CALL(T)                                                                                                 \
     fast_advec_cell_t<T>(g, m, mode, dir, static_cast<const T *>(density1), static_cast<const T *>(energy1),     \
                          static_cast<T *>(mass_flux_x), static_cast<const T *>(vol_flux_x),                      \
                          static_cast<T *>(mass_flux_y), static_cast<const T *>(vol_flux_y),                      \
                          static_cast<T *>(mass_flux_z), static_cast<const T *>(vol_flux_z),                      \
-                         static_cast<T *>(rho_out), static_cast<T *>(e_out), static_cast<T *>(pressure),         \
-                         static_cast<T *>(soundspeed), sh, st)
+                         static_cast<T *>(rho_out), static_cast<T *>(e_out), static_cast<T *>(pressure), sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
@@ -375,13 +373,11 @@ kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd
 #define MF(id) static_cast<S *>(D.f[id])
         if constexpr (DIR == 1)
             cell_x_body<S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z), CF(F_DENSITY1),
-                           CF(F_ENERGY1), MF(F_MASS_FLUX_X), MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr,
-                           gas ? MF(F_SOUNDSPEED) : nullptr);
+                           CF(F_ENERGY1), MF(F_MASS_FLUX_X), MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr);
         else
             cell_march_body<DIR, S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z),
                                     CF(F_DENSITY1), CF(F_ENERGY1), MF(DIR == 2 ? F_MASS_FLUX_Y : F_MASS_FLUX_Z),
-                                    MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr,
-                                    gas ? MF(F_SOUNDSPEED) : nullptr);
+                                    MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr);
 #undef CF
 #undef MF
         __syncthreads();  // the next tile reuses the workgroup's LDS

@@ -127,7 +127,8 @@ struct clover_chunk {
     bool fast;           // fused kernels (fast.h) instead of the per-call ones
     Store store;         // element type of the field arrays (ST_F32 only with the fused kernels)
     bool halo_check;     // CLOVER_CHUNK_HALO_CHECK
-    bool gas_fresh;      // pressure/soundspeed interiors hold ideal_gas(density0, energy0)
+    bool gas_fresh;      // pressure interiors hold ideal_gas(density0, energy0)
+    bool sound_fresh;    // and soundspeed interiors too (the fused step forms soundspeed where it reads it)
     Shapes shapes;       // launch shapes of the fused kernels (autotuner)
     int dt_tile;         // the tile whose dt the last timestep took (-1: none yet)
     Geo geo;             // the uniform mesh, for the fused kernels
@@ -582,12 +583,15 @@ static void fused_halo(clover_chunk *c, int which)
     update_halo(c, p.fields, p.depth, p.axes);
 }
 
-static void ideal_gas(clover_chunk *c, bool predict)
+// need_sound: the caller reads soundspeed too (a fetch); the fused kernels
+// form it from density and pressure where they need it
+static void ideal_gas(clover_chunk *c, bool predict, bool need_sound = false)
 {
-    // the fused path's last cell sweep leaves the equation of state of the
-    // new level-0 fields behind; computing it again would write the same bits
-    if (c->fast && !predict && c->gas_fresh) return;
+    // the fused path's last cell sweep leaves the new level-0 fields'
+    // pressure behind; computing it again would write the same bits
+    if (c->fast && !predict && c->gas_fresh && (c->sound_fresh || !need_sound)) return;
     c->gas_fresh = !predict;
+    c->sound_fresh = !predict;
     Phase ph(c, P_IDEAL_GAS);
     for (auto &t : c->tiles) {
         if (c->fast && !predict)
@@ -634,7 +638,7 @@ static int timestep(clover_chunk *c)
         for (int ti = 0; ti < T; ti++) {
             Tile &t = c->tiles[ti];
             hip_check(launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe, d.dtdiv_safe,
-                                          c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
+                                          c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE],
                                           t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
                                           c->partial + (size_t)ti * kMaxPartials * 5, &nparts[ti % 64],
                                           c->shapes.k[SK_VISC_DT], c->st),
@@ -748,7 +752,10 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
         hip_check(batch_advec_cell(c->store, c->tdev, (int)c->tiles.size(), kgrid(c, k), c->advect_x, dir, sweep,
                                    sweep == 3, c->shapes.k[shape_of(k)], c->st),
                   "advec_cell");
-        if (sweep == 3) c->gas_fresh = true;
+        if (sweep == 3) {
+            c->gas_fresh = true;
+            c->sound_fresh = false;
+        }
         tswap(c, {{F_DENSITY1, X_W1}, {F_ENERGY1, X_W2}});
         return;
     }
@@ -757,15 +764,18 @@ static void advec_cell(clover_chunk *c, int sweep, int dir)
             // new density1/energy1 land in spare arrays, which then take their
             // place (interior cells only; the halo update that follows every
             // advec_cell sweep refreshes the halos before anything reads them)
-            // the third sweep also writes the next step's ideal_gas results
+            // the third sweep also writes the next step's ideal_gas pressure
             const bool last = sweep == 3;
             hip_check(launch_fast_advec_cell(c->store, t.g, c->geo, c->advect_x, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
                                              t.f[F_MASS_FLUX_X], t.f[F_VOL_FLUX_X], t.f[F_MASS_FLUX_Y],
                                              t.f[F_VOL_FLUX_Y], t.f[F_MASS_FLUX_Z], t.f[F_VOL_FLUX_Z], t.f[X_W1],
                                              t.f[X_W2], last ? t.f[F_PRESSURE] : nullptr,
-                                             last ? t.f[F_SOUNDSPEED] : nullptr, c->shapes.k[SK_CELL_X + dir - 1], c->st),
+                                             c->shapes.k[SK_CELL_X + dir - 1], c->st),
                       "advec_cell");
-            if (last) c->gas_fresh = true;  // for the level 0 that reset_field makes of these
+            if (last) {  // for the level 0 that reset_field makes of these
+                c->gas_fresh = true;
+                c->sound_fresh = false;
+            }
             std::swap(t.f[F_DENSITY1], t.f[X_W1]);
             std::swap(t.f[F_ENERGY1], t.f[X_W2]);
         } else
@@ -892,7 +902,7 @@ static void visit_prepare(clover_chunk *c)
             int np = 0;
             hip_check(launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe,
                                           d.dtdiv_safe, c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE],
-                                          t.f[F_SOUNDSPEED], t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
+                                          t.f[F_VISCOSITY], t.f[F_XVEL0], t.f[F_YVEL0],
                                           t.f[F_ZVEL0], c->partial + ti * kMaxPartials * 5, &np,
                                           c->shapes.k[SK_VISC_DT], c->st),
                       "visc_dt");
@@ -923,7 +933,7 @@ static void tune_launch(clover_chunk *c, int k)
         case SK_VISC_DT: {
             int np = 0;
             e = launch_fast_visc_dt(c->store, t.g, c->geo, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtw_safe,
-                                    d.dtdiv_safe, c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE], t.f[F_SOUNDSPEED],
+                                    d.dtdiv_safe, c->g_small, t.f[F_DENSITY0], t.f[F_PRESSURE],
                                     t.f[X_W3], t.f[F_XVEL0], t.f[F_YVEL0], t.f[F_ZVEL0],
                                     c->partial + ti * kMaxPartials * 5, &np, sh, c->st);
             // the shape sets the number of partials, so their minimum is
@@ -949,7 +959,7 @@ static void tune_launch(clover_chunk *c, int k)
             const int dir = k - SK_CELL_X + 1, sweep = dir == 2 ? 2 : 1;
             e = launch_fast_advec_cell(c->store, t.g, c->geo, 1, dir, sweep, t.f[F_DENSITY1], t.f[F_ENERGY1],
                                        t.f[X_W3], t.f[F_VOL_FLUX_X], t.f[X_W3], t.f[F_VOL_FLUX_Y], t.f[X_W3],
-                                       t.f[F_VOL_FLUX_Z], t.f[X_W1], t.f[X_W2], nullptr, nullptr, sh, c->st);
+                                       t.f[F_VOL_FLUX_Z], t.f[X_W1], t.f[X_W2], nullptr, sh, c->st);
             break;
         }
         default: {
@@ -1450,6 +1460,9 @@ int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host)
     CLOVER_API_BEGIN
     if (!c || !host || t < 0 || t >= (int)c->tiles.size()) throw Error(E_ARG, "bad argument");
     if (field < 0 || field > X_ZAREA) throw Error(E_ARG, "bad field id");
+    KTIMER_SCOPE(c);
+    // the fused step forms soundspeed where it reads it instead of storing it
+    if (field == F_SOUNDSPEED && c->fast) ideal_gas(c, false, true);
     Tile &tt = c->tiles[t];
     if (c->store == ST_F64) {
         download_dense(tt.g, class_of(field), tt.f[field], host, c->st);

@@ -579,6 +579,10 @@ int clover_ensemble_fetch_field(clover_ensemble *e, int deck, int field, double 
     CLOVER_API_BEGIN
     if (!e || !host || deck < 0 || deck >= e->nd) throw Error(E_ARG, "bad argument");
     if (field < 0 || field >= NUM_FIELDS) throw Error(E_ARG, "bad field id");
+    // the step forms soundspeed where it reads it instead of storing it:
+    // ideal_gas of the current level 0 (its pressure comes out with the same
+    // bits the step left)
+    if (field == F_SOUNDSPEED) ideal_gas(e);
     DeckDev D;
     hip_check(hipMemcpyAsync(&D, e->dev + deck, sizeof D, hipMemcpyDeviceToHost, e->st), "D2H");
     hip_check(hipStreamSynchronize(e->st), "sync");

@@ -137,9 +137,9 @@ hipError_t launch_fast_ideal_gas(Store s, const Lay &g, const void *density, con
 // partial[]; *nparts receives the number of partials (<= kMaxDtPartials).
 hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_safe, double dtu_safe,
                                double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
-                               const void *density0, const void *pressure, const void *soundspeed, void *viscosity,
-                               const void *xvel0, const void *yvel0, const void *zvel0, double *partial, int *nparts,
-                               const Shape &sh, hipStream_t st);
+                               const void *density0, const void *pressure, void *viscosity, const void *xvel0,
+                               const void *yvel0, const void *zvel0, double *partial, int *nparts, const Shape &sh,
+                               hipStream_t st);
 
 // PdV predictor (PdV_kernel.f90:76-117) followed by ideal_gas on the
 // predicted density/energy (ideal_gas.f90:53-62): writes pressure only.
@@ -166,7 +166,7 @@ hipError_t launch_fast_advec_cell(Store s, const Lay &g, const Geo &m, int advec
                                   const void *density1, const void *energy1, void *mass_flux_x,
                                   const void *vol_flux_x, void *mass_flux_y, const void *vol_flux_y,
                                   void *mass_flux_z, const void *vol_flux_z, void *rho_out, void *e_out,
-                                  void *pressure, void *soundspeed, const Shape &sh, hipStream_t st);
+                                  void *pressure, const Shape &sh, hipStream_t st);
 
 // one advec_mom sweep for all three velocities (advec_mom_driver.f90 calls
 // the kernel for xvel1, yvel1, zvel1 with the node quantities of the first

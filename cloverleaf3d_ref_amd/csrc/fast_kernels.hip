@@ -56,9 +56,8 @@ __device__ inline QuadS<S> quad_down_raw(const S *__restrict__ p, int i, int ld)
 template <typename S>
 __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
-               const S *__restrict__ soundspeed, S *__restrict__ viscosity,
-               const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
-               double *__restrict__ partial)
+               S *__restrict__ viscosity, const S *__restrict__ xvel0, const S *__restrict__ yvel0,
+               const S *__restrict__ zvel0, double *__restrict__ partial)
 {
     using T = S;  // arithmetic type of the form
     const int j = g.x_min + bk.x * (int)blockDim.x + (int)threadIdx.x;
@@ -78,17 +77,20 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
         QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
         T p_lo = pressure[i - g.plane], p_c = pressure[i];
         // loads run one plane ahead of their use: the upper node plane's
-        // velocities and pressure, and the next cell's density, soundspeed
-        // and x/y pressure neighbours
+        // velocities and pressure, and the next cell's density and x/y
+        // pressure neighbours.  The cell's soundspeed is not read: it is
+        // ideal_gas's of the cell's density and pressure, formed here with
+        // the same operations (soundspeed_of), so it has the same bits
         QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
                  wn = quad_up_raw(zvel0, i + g.plane, g.ld);
         S pn = pressure[i + g.plane];
-        S rn = density0[i], cn = soundspeed[i];
+        S rn = density0[i];
         S pxm = pressure[i - 1], pxp = pressure[i + 1], pym = pressure[i - g.ld], pyp = pressure[i + g.ld];
         for (int l = l0; l <= l1; l++, i += g.plane) {
             const QuadS<S> uu = un, vu = vn, wu = wn;
             const T p_hi = pn;
-            const T rho = rn, c = cn;
+            const T rho = rn;
+            const T c = soundspeed_of<T>(rho, p_c);
             const T p_xm = pxm, p_xp = pxp, p_ym = pym, p_yp = pyp;
             if (l < l1) {
                 const int i1 = i + g.plane, i2 = i + 2 * g.plane;
@@ -97,7 +99,6 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
                 wn = quad_up_raw(zvel0, i2, g.ld);
                 pn = pressure[i2];
                 rn = density0[i1];
-                cn = soundspeed[i1];
                 pxm = pressure[i1 - 1];
                 pxp = pressure[i1 + 1];
                 pym = pressure[i1 - g.ld];
@@ -222,17 +223,17 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 template <typename S>
 __global__ void __launch_bounds__(kMaxBlock) k_fast_visc_dt(int xcd, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
-               const S *__restrict__ soundspeed, S *__restrict__ viscosity,
-               const S *__restrict__ xvel0, const S *__restrict__ yvel0, const S *__restrict__ zvel0,
-               double *__restrict__ partial)
+               S *__restrict__ viscosity, const S *__restrict__ xvel0, const S *__restrict__ yvel0,
+               const S *__restrict__ zvel0, double *__restrict__ partial)
 {
-    visc_dt_body<S>(hw_blk(xcd), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0, yvel0, zvel0, partial);
+    visc_dt_body<S>(hw_blk(xcd), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure,
+                    viscosity, xvel0, yvel0, zvel0, partial);
 }
 
 template <typename S>
 static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, double dtu_safe, double dtv_safe,
                                double dtw_safe, double dtdiv_safe, double g_small, const S *density0,
-                               const S *pressure, const S *soundspeed, S *viscosity,
+                               const S *pressure, S *viscosity,
                                const S *xvel0, const S *yvel0, const S *zvel0, double *partial,
                                int *nparts, const Shape &sh, hipStream_t st)
 {
@@ -244,8 +245,8 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
     if (bx * by * gz > kMaxDtPartials) gz = kMaxDtPartials / (bx * by);
     *nparts = bx * by * gz;
     TIMED_LAUNCH("k_fast_visc_dt", st, k_fast_visc_dt<S>, dim3(bx, by, gz), blk, 0, st, sh.xcd, g, m, dtc_safe,
-                 dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, soundspeed, viscosity, xvel0,
-                 yvel0, zvel0, partial);
+                 dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure, viscosity, xvel0, yvel0,
+                 zvel0, partial);
     return hipGetLastError();
 }
 
@@ -712,14 +713,14 @@ hipError_t launch_fast_ideal_gas(Store s, const Lay &g, const void *density, con
 
 hipError_t launch_fast_visc_dt(Store s, const Lay &g, const Geo &m, double dtc_safe, double dtu_safe,
                                double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
-                               const void *density0, const void *pressure, const void *soundspeed, void *viscosity,
-                               const void *xvel0, const void *yvel0, const void *zvel0, double *partial, int *nparts,
-                               const Shape &sh, hipStream_t st)
+                               const void *density0, const void *pressure, void *viscosity, const void *xvel0,
+                               const void *yvel0, const void *zvel0, double *partial, int *nparts, const Shape &sh,
+                               hipStream_t st)
 {
 #define CALL(T)                                                                                                 \
     fast_visc_dt_t<T>(g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, CP(T, density0),      \
-                      CP(T, pressure), CP(T, soundspeed), MP(T, viscosity), CP(T, xvel0), CP(T, yvel0),         \
-                      CP(T, zvel0), partial, nparts, sh, st)
+                      CP(T, pressure), MP(T, viscosity), CP(T, xvel0), CP(T, yvel0), CP(T, zvel0), partial,     \
+                      nparts, sh, st)
     return s == ST_F32 ? CALL(float) : CALL(double);
 #undef CALL
 }
@@ -841,7 +842,7 @@ kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int
     {
         DECK_TILE(t_, dims_cells(D.g, planes, gx, gy, gz))
         visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
-                        CF(F_DENSITY0), CF(F_PRESSURE), CF(F_SOUNDSPEED), MF(F_VISCOSITY), CF(F_XVEL0),
+                        CF(F_DENSITY0), CF(F_PRESSURE), MF(F_VISCOSITY), CF(F_XVEL0),
                         CF(F_YVEL0), CF(F_ZVEL0), partial + start[d]);
         __syncthreads();
     }

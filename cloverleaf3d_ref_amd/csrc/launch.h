@@ -170,18 +170,34 @@ __device__ inline T quo_sparse(T a, T b)
 // ideal_gas_kernel.f90:50-57 for one cell, in the form's arithmetic (shared
 // by the ideal_gas kernels and the advection sweep that also writes the
 // next step's equation of state)
+template <typename T>
+__device__ inline T pressure_of(T density, T energy)
+{
+    return (T)(1.4 - 1.0) * density * energy;
+}
+
+// ideal_gas_kernel.f90:52-57: the soundspeed of a cell from its density and
+// its pressure (the pressure ideal_gas forms from density and energy); the
+// same operations on the same values as ideal_gas_cell, so the same bits
+template <typename T>
+__device__ inline T soundspeed_of(T density, T pressure)
+{
+    const T d = density, p = pressure;
+    const T v = quo((T)1, d);
+    const T pressurebyenergy = (T)(1.4 - 1.0) * d;
+    const T pressurebyvolume = -d * p;
+    const T sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
+    return sqrt(sound_speed_squared);
+}
+
 template <typename S>
 __device__ inline void ideal_gas_cell(S density, S energy, S &pressure, S &soundspeed)
 {
     using T = S;
     const T d = density;
-    const T v = quo((T)1, d);
-    const T p = (T)(1.4 - 1.0) * d * (T)energy;
+    const T p = pressure_of<T>(d, (T)energy);
     pressure = p;
-    const T pressurebyenergy = (T)(1.4 - 1.0) * d;
-    const T pressurebyvolume = -d * p;
-    const T sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
-    soundspeed = sqrt(sound_speed_squared);
+    soundspeed = soundspeed_of<T>(d, p);
 }
 
 constexpr int RED_THREADS = 256;

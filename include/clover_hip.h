@@ -259,7 +259,9 @@ int clover_decompose_tiles(int x_cells, int y_cells, int z_cells, int tiles_per_
  * level-1 arrays hold the previous level 0 (reset_field is an exchange),
  * halo cells no kernel reads are not refreshed, and pressure/soundspeed
  * already hold ideal_gas of the new level 0 (the reference computes that at
- * the next timestep or field_summary; after a field_summary both agree). */
+ * the next timestep or field_summary; after a field_summary both agree).
+ * The fused kernels form soundspeed where they read it, so fetching it
+ * first runs that ideal_gas (the pressure it writes has the bits it had). */
 int clover_chunk_fetch_field(clover_chunk *c, int t, int field, double *host);
 
 /* Per-phase device time (ms) accumulated with HIP events when profiling is
