@@ -61,9 +61,10 @@ ARRAYS_PER_CELL = {
     # z run in sweeps 1 and 3, y always in sweep 2; the third sweep also
     # writes pressure (the next step's ideal_gas), one of every two x and z
     # sweeps.  visc_dt reads rho0, p, u0, v0, w0 (it forms the soundspeed
-    # from rho0 and p) and writes q.
+    # from rho0 and p) and writes q; pdv_predict reads rho0, e0, q, u0, v0, w0
+    # (it forms the level-0 pressure from rho0 and e0) and writes p.
     # accel_pdv reads rho0, e0, p, q, u0, v0, w0 and writes u1, v1, w1, rho1, e1 and the three volume fluxes
-    "k_fast_visc_dt": 6, "k_fast_pdv_predict": 8, "k_fast_accel_pdv": 15,
+    "k_fast_visc_dt": 6, "k_fast_pdv_predict": 7, "k_fast_accel_pdv": 15,
     "k_fast_cell_x": 7.5, "k_fast_cell_y": 7, "k_fast_cell_z": 7.5,
     "k_fast_mom_x": 9, "k_fast_mom_y": 9, "k_fast_mom_z": 9,
     "k_fast_field_summary": 6, "k_fast_ideal_gas": 4,

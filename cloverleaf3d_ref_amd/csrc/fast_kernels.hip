@@ -276,10 +276,14 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
     // the next plane's loads run one plane ahead of their use
     QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
              wn = quad_up_raw(zvel0, i + g.plane, g.ld);
-    S dn = density0[i], en = energy0[i], pn = pressure[i], qn = viscosity[i];
+    // The cell's pressure is not read: at this point it is ideal_gas's of
+    // the cell's level-0 density and energy, formed here with the same
+    // expression (pressure_of), so it has the same bits.
+    S dn = density0[i], en = energy0[i], qn = viscosity[i];
     for (int l = l0; l <= l1; l++, i += g.plane) {
         const QuadS<S> uu = un, vu = vn, wu = wn;
-        const T d0 = dn, e0v = en, p0 = pn, q0 = qn;
+        const T d0 = dn, e0v = en, q0 = qn;
+        const T p0 = pressure_of<T>(d0, e0v);
         if (l < l1) {
             const int i1 = i + g.plane, i2 = i + 2 * g.plane;
             un = quad_up_raw(xvel0, i2, g.ld);
@@ -287,7 +291,6 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
             wn = quad_up_raw(zvel0, i2, g.ld);
             dn = density0[i1];
             en = energy0[i1];
-            pn = pressure[i1];
             qn = viscosity[i1];
         }
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
