@@ -94,50 +94,83 @@ __device__ inline float shfl_c(float x, int src) { return __shfl(x, src, 64); }
 
 // ---------------------------------------------------------------------------
 // x sweep: faces j in [x_min, x_max+2], rows k, l interior.  Lane i holds
-// cell/face j = jt - 2 + i.
+// cell/face j = jt - 2 + i.  A wave walks `rows` consecutive rows (k
+// fastest) and issues the next row's loads before the current row's work;
+// one row per wave leaves too few bytes in flight in the FP32 form.
 template <typename S>
-__device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
-              const S *__restrict__ vfz, const S *__restrict__ density1,
+__device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mode, int rows, const S *__restrict__ vfx,
+              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
               S *__restrict__ e_out, S *__restrict__ p_out)
 {
     const int lane = threadIdx.x;
     const int ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
-    const int row = bk.y * CROWS + (int)threadIdx.y;
-    if (row >= ny * nz) return;  // whole wave: no shuffle partner is lost
-    const int k = g.y_min + row % ny, l = g.z_min + row / ny;
+    const int nrows = ny * nz;
+    const int r0 = (bk.y * CROWS + (int)threadIdx.y) * rows;
+    if (r0 >= nrows) return;  // whole wave: no shuffle partner is lost
+    const int r1 = min(r0 + rows, nrows) - 1;
     const int j = g.x_min - 2 + bk.x * 60 + lane;
-    const int i = idx(g, j, k, l);
     using T = S;  // arithmetic type of the fluxes
     const bool ok_cell = j <= g.x_max + 2;  // j >= x_min-2 always
-    const T rho = ok_cell ? density1[i] : (T)0;
-    const T e = ok_cell ? energy1[i] : (T)0;
-    const double pv = ok_cell ? cell_pre_vol(mode, g, i, m.vol, vfx, vfy, vfz) : 0.0;
-    const T vf = j <= g.x_max + 3 ? vfx[i] : (T)0;
-    const int lm2 = max(lane - 2, 0), lm1 = max(lane - 1, 0), lp1 = min(lane + 1, 63);
-    const T rm2 = shfl_c(rho, lm2), rm1 = shfl_c(rho, lm1), rp1 = shfl_c(rho, lp1);
-    const T em2 = shfl_c(e, lm2), em1 = shfl_c(e, lm1), ep1 = shfl_c(e, lp1);
-    const double pvm1 = shfl_c(pv, lm1);
-    const T vd = (T)m.dx;
-    // face j: upwind/donor/downwind (advec_cell_kernel.f90:123-132); the
-    // upwind index is clamped at x_max+2
-    T mf, ef;
-    if (vf > (T)0) {
-        cell_face_flux(vf, (T)pvm1, vd, rm2, rm1, rho, em2, em1, e, mf, ef);
-    } else {
-        const bool clamp = j + 1 > g.x_max + 2;
-        cell_face_flux(vf, (T)pv, vd, clamp ? rho : rp1, rho, rm1, clamp ? e : ep1, e, em1, mf, ef);
-    }
-    const T mf_hi = shfl_c(mf, lp1), ef_hi = shfl_c(ef, lp1), vf_hi = shfl_c(vf, lp1);
-    if (lane < 2 || lane > 61) return;
-    if (j <= g.x_max + 2) mass_flux_x[i] = mf;
-    if (j <= g.x_max) {
-        double rn, en;
-        cell_update(rho, e, pv, mf, mf_hi, ef, ef_hi, vf, vf_hi, rn, en);
-        const S rs = (S)rn, es = (S)en;
-        rho_out[i] = rs;
-        e_out[i] = es;
-        if (p_out) p_out[i] = pressure_of<S>(rs, es);
+    const bool ok_face = j <= g.x_max + 3;
+    // pre_vol's volume fluxes (advec_cell_kernel.f90:97-116): all three axes
+    // in sweep 1 (mode 1), x only in sweep 3 (mode 2)
+    const bool all3 = mode == 1;
+    // the next row's values, raw (converted where used, so the loads stay
+    // in flight): density, energy, vol_flux_x at j and j+1, vol_flux_y at k
+    // and k+1, vol_flux_z at l and l+1
+    S rn = 0, en = 0, fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0, fz0 = 0, fz1 = 0;
+    auto row_at = [&](int r) { return idx(g, j, g.y_min + r % ny, g.z_min + r / ny); };
+    auto load = [&](int ii) {
+        rn = ok_cell ? density1[ii] : (S)0;
+        en = ok_cell ? energy1[ii] : (S)0;
+        fx0 = ok_face ? vfx[ii] : (S)0;
+        fx1 = ok_cell ? vfx[ii + 1] : (S)0;
+        fy0 = (ok_cell && all3) ? vfy[ii] : (S)0;
+        fy1 = (ok_cell && all3) ? vfy[ii + g.ld] : (S)0;
+        fz0 = (ok_cell && all3) ? vfz[ii] : (S)0;
+        fz1 = (ok_cell && all3) ? vfz[ii + g.plane] : (S)0;
+    };
+    int in = row_at(r0);
+    load(in);
+    for (int r = r0; r <= r1; r++) {
+        const int i = in;
+        const T rho = rn, e = en;
+        const S x0 = fx0, x1 = fx1, y0 = fy0, y1 = fy1, z0 = fz0, z1 = fz1;
+        if (r < r1) {
+            in = row_at(r + 1);
+            load(in);
+        }
+        double pv = 0.0;
+        if (ok_cell)
+            pv = all3 ? m.vol + ((double)x1 - (double)x0 + (double)y1 - (double)y0 + (double)z1 - (double)z0)
+                      : m.vol + (double)x1 - (double)x0;
+        const T vf = x0;
+        const int lm2 = max(lane - 2, 0), lm1 = max(lane - 1, 0), lp1 = min(lane + 1, 63);
+        const T rm2 = shfl_c(rho, lm2), rm1 = shfl_c(rho, lm1), rp1 = shfl_c(rho, lp1);
+        const T em2 = shfl_c(e, lm2), em1 = shfl_c(e, lm1), ep1 = shfl_c(e, lp1);
+        const double pvm1 = shfl_c(pv, lm1);
+        const T vd = (T)m.dx;
+        // face j: upwind/donor/downwind (advec_cell_kernel.f90:123-132); the
+        // upwind index is clamped at x_max+2
+        T mf, ef;
+        if (vf > (T)0) {
+            cell_face_flux(vf, (T)pvm1, vd, rm2, rm1, rho, em2, em1, e, mf, ef);
+        } else {
+            const bool clamp = j + 1 > g.x_max + 2;
+            cell_face_flux(vf, (T)pv, vd, clamp ? rho : rp1, rho, rm1, clamp ? e : ep1, e, em1, mf, ef);
+        }
+        const T mf_hi = shfl_c(mf, lp1), ef_hi = shfl_c(ef, lp1), vf_hi = shfl_c(vf, lp1);
+        if (lane < 2 || lane > 61) continue;
+        if (j <= g.x_max + 2) mass_flux_x[i] = mf;
+        if (j <= g.x_max) {
+            double rnew, enew;
+            cell_update(rho, e, pv, mf, mf_hi, ef, ef_hi, vf, vf_hi, rnew, enew);
+            const S rs = (S)rnew, es = (S)enew;
+            rho_out[i] = rs;
+            e_out[i] = es;
+            if (p_out) p_out[i] = pressure_of<S>(rs, es);
+        }
     }
 }
 
@@ -145,9 +178,10 @@ template <typename S>
 __global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(int xcd, Lay g, Geo m, int mode, const S *__restrict__ vfx, const S *__restrict__ vfy,
               const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
-              S *__restrict__ e_out, S *__restrict__ p_out)
+              S *__restrict__ e_out, S *__restrict__ p_out, int rows)
 {
-    cell_x_body<S>(hw_blk(xcd), g, m, mode, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out, p_out);
+    cell_x_body<S>(hw_blk(xcd), g, m, mode, rows, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out,
+                   p_out);
 }
 
 // The volume fluxes pre_vol reads for the y and z sweeps (modes 3-6), kept
@@ -297,9 +331,10 @@ static hipError_t fast_advec_cell_t(const Lay &g, const Geo &m, int mode, int di
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const dim3 blk(64, CROWS, 1);
     if (dir == 1) {
-        const dim3 grid((nx + 2 + 59) / 60, (ny * nz + CROWS - 1) / CROWS, 1);
+        const int rows = cell_x_rows(sh.planes);
+        const dim3 grid((nx + 2 + 59) / 60, ((ny * nz + rows - 1) / rows + CROWS - 1) / CROWS, 1);
         TIMED_LAUNCH("k_fast_cell_x", st, k_fast_cell_x<S>, grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x, vol_flux_y,
-                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out, p_out);
+                     vol_flux_z, density1, energy1, mass_flux_x, rho_out, e_out, p_out, rows);
     } else if (dir == 2) {
         const dim3 grid((nx + 63) / 64, (nz + 2 + CROWS - 1) / CROWS, nchunks(ny + 2, sh.planes));
         TIMED_LAUNCH("k_fast_cell_y", st, (k_fast_cell_march<2, S>), grid, blk, 0, st, sh.xcd, g, m, mode, vol_flux_x,
@@ -340,7 +375,8 @@ __host__ __device__ inline void dims_cell(int dir, const Lay &g, int planes, int
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     if (dir == 1) {
-        gx = (nx + 2 + 59) / 60; gy = (ny * nz + CROWS - 1) / CROWS; gz = 1;
+        const int rows = cell_x_rows(planes);
+        gx = (nx + 2 + 59) / 60; gy = ((ny * nz + rows - 1) / rows + CROWS - 1) / CROWS; gz = 1;
     } else if (dir == 2) {
         gx = (nx + 63) / 64; gy = (nz + 2 + CROWS - 1) / CROWS; gz = nchunks(ny + 2, planes);
     } else {
@@ -372,7 +408,7 @@ kb_cell(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd
 #define CF(id) static_cast<const S *>(D.f[id])
 #define MF(id) static_cast<S *>(D.f[id])
         if constexpr (DIR == 1)
-            cell_x_body<S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z), CF(F_DENSITY1),
+            cell_x_body<S>(bk, D.g, D.m, mode, cell_x_rows(planes), CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z), CF(F_DENSITY1),
                            CF(F_ENERGY1), MF(F_MASS_FLUX_X), MF(X_W1), MF(X_W2), gas ? MF(F_PRESSURE) : nullptr);
         else
             cell_march_body<DIR, S>(bk, D.g, D.m, mode, CF(F_VOL_FLUX_X), CF(F_VOL_FLUX_Y), CF(F_VOL_FLUX_Z),

@@ -1071,7 +1071,7 @@ static double tune_time(clover_chunk *c, int k, hipEvent_t a, hipEvent_t b, Deck
 }
 
 // Every shaped kernel, every candidate shape: positions per workgroup along
-// its march from kTunePlanes (the x cell sweep has no march) times the two
+// its march from kTunePlanes (the x cell sweep: rows per wave, cell_x_rows) times the two
 // block orders, and for the z marches times the workgroup widths kTuneBx;
 // the fastest is kept.  Shapes split the work differently but never change a
 // value, so later steps give the same bits whatever is chosen.
@@ -1102,7 +1102,6 @@ static void autotune(clover_chunk *c, double *before, double *after)
         for (int planes : kTunePlanes)
             for (int xcd = 0; xcd < 2; xcd++)
                 for (int bx : kTuneBx) {
-                    if (k == SK_CELL_X && planes != start.planes) continue;
                     if ((ttab || !z_march(k)) && bx != start.bx) continue;
                     const Shape cand{planes, xcd, bx};
                     if (cand.planes == start.planes && cand.xcd == start.xcd && cand.bx == start.bx) continue;

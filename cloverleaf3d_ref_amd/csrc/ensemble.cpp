@@ -337,7 +337,6 @@ void autotune(clover_ensemble *e, double *before, double *after)
         t0 += best;
         for (int planes : kTunePlanes)
             for (int xcd = 0; xcd < 2; xcd++) {
-                if (si == SK_CELL_X && planes != start.planes) continue;
                 const Shape cand{planes, xcd, start.bx, 0};  // the batched workgroups are fixed at 64 x 4
                 if (cand.planes == start.planes && cand.xcd == start.xcd && start.wgs == 0) continue;
                 const double ms = tune_time(e, bk, cand, a, b);

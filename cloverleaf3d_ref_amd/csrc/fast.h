@@ -36,9 +36,9 @@ enum Store : int { ST_F64 = 0, ST_F32 = 1 };
 
 // The kernels with a launch shape, and the shape: `planes` positions per
 // workgroup along the kernel's march (planes for the z marches, faces or
-// nodes along the sweep for the y/z advection sweeps; the x cell sweep has
-// no march and ignores it) and `xcd`, the XCD-contiguous block order
-// (launch.h hw_blk_xcd) instead of the hardware's.
+// nodes along the sweep for the y/z advection sweeps; for the x cell sweep,
+// which has no march, cell_x_rows below) and `xcd`, the XCD-contiguous
+// block order (launch.h hw_blk_xcd) instead of the hardware's.
 // (the same indices as CLOVER_SHAPE_* of clover_hip.h)
 enum ShapeKernel : int {
     SK_VISC_DT, SK_PDV_PREDICT, SK_ACCEL_PDV, SK_CELL_X, SK_CELL_Y, SK_CELL_Z, SK_MOM_X, SK_MOM_Y, SK_MOM_Z,
@@ -65,12 +65,16 @@ struct Shapes {
 constexpr int kTunePlanes[] = {8, 16, 32, 64, 128};
 constexpr int kTuneCount = sizeof(kTunePlanes) / sizeof(kTunePlanes[0]);
 
-// 64 positions per workgroup of 64 x 4; the XCD order for the y/z momentum
-// sweeps
+// rows of the x cell sweep per wave for `planes`: planes/32, at least one
+// (the tuner's candidates give 1, 2 and 4)
+__host__ __device__ inline int cell_x_rows(int planes) { return planes >= 64 ? planes / 32 : 1; }
+
+// 64 positions per workgroup of 64 x 4 (one row per wave for the x cell
+// sweep); the XCD order for the y/z momentum sweeps
 inline Shapes default_shapes()
 {
     Shapes s;
-    for (int i = 0; i < SK_COUNT; i++) s.k[i] = Shape{64, i == SK_MOM_Y || i == SK_MOM_Z, 64};
+    for (int i = 0; i < SK_COUNT; i++) s.k[i] = Shape{i == SK_CELL_X ? 32 : 64, i == SK_MOM_Y || i == SK_MOM_Z, 64};
     return s;
 }
 

@@ -283,7 +283,8 @@ int clover_chunk_kernel_times(clover_chunk *c, char *buf, int buflen);
 /* Launch shapes of the fused marching kernels (no counterpart in the
  * reference, whose loops have no launch geometry).  A shape is `planes`,
  * the positions a workgroup marches over (planes of the z marches, faces or
- * nodes along the y/z advection sweeps; ignored by the x cell sweep),
+ * nodes along the y/z advection sweeps; the x cell sweep, which has no
+ * march, walks planes/32 rows per wave, at least one),
  * `xcd`: 1 gives each of the 8 XCDs a contiguous eighth of the grid instead
  * of the hardware's round-robin order, and `block_x`: the workgroup width
  * of the visc/dt and PdV-predictor z marches (64, 128, 256 or 512 threads

@@ -52,7 +52,7 @@ def _same(a, b):
         assert np.array_equal(f0[key][inner].view(np.uint64), f1[key][inner].view(np.uint64)), key
 
 
-ODD_SHAPES = {"visc_dt": (3, 1, 512), "pdv_predict": (1, 0, 128), "accel_pdv": (7, 1), "cell_x": (64, 1),
+ODD_SHAPES = {"visc_dt": (3, 1, 512), "pdv_predict": (1, 0, 128), "accel_pdv": (7, 1), "cell_x": (96, 1),
               "cell_y": (5, 1), "cell_z": (2, 0), "mom_x": (1, 1), "mom_y": (9, 0), "mom_z": (3, 1)}
 
 
