@@ -76,34 +76,21 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
         int i = idx(g, j, k, l0);
         QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
         T p_lo = pressure[i - g.plane], p_c = pressure[i];
-        // loads run one plane ahead of their use: the upper node plane's
-        // velocities and pressure, and the next cell's density and x/y
-        // pressure neighbours.  The cell's soundspeed is not read: it is
-        // ideal_gas's of the cell's density and pressure, formed here with
-        // the same operations (soundspeed_of), so it has the same bits
-        QuadS<S> un = quad_up_raw(xvel0, i + g.plane, g.ld), vn = quad_up_raw(yvel0, i + g.plane, g.ld),
-                 wn = quad_up_raw(zvel0, i + g.plane, g.ld);
-        S pn = pressure[i + g.plane];
-        S rn = density0[i];
-        S pxm = pressure[i - 1], pxp = pressure[i + 1], pym = pressure[i - g.ld], pyp = pressure[i + g.ld];
+        // The cell's soundspeed is not read: it is ideal_gas's of the cell's
+        // density and pressure, formed here with the same operations
+        // (soundspeed_of), so it has the same bits.
         for (int l = l0; l <= l1; l++, i += g.plane) {
-            const QuadS<S> uu = un, vu = vn, wu = wn;
-            const T p_hi = pn;
-            const T rho = rn;
+            // the upper node plane's velocities and pressure, the cell's
+            // density and its x/y pressure neighbours, loaded at the top of
+            // the plane: holding the next plane's loads in registers as well
+            // took 150 VGPRs in FP64 (three waves per SIMD) against 126 (four)
+            // and measured 1.55 against 1.49 ms at 512^3
+            const QuadS<S> uu = quad_up_raw(xvel0, i + g.plane, g.ld), vu = quad_up_raw(yvel0, i + g.plane, g.ld),
+                           wu = quad_up_raw(zvel0, i + g.plane, g.ld);
+            const T p_hi = pressure[i + g.plane];
+            const T rho = density0[i];
             const T c = soundspeed_of<T>(rho, p_c);
-            const T p_xm = pxm, p_xp = pxp, p_ym = pym, p_yp = pyp;
-            if (l < l1) {
-                const int i1 = i + g.plane, i2 = i + 2 * g.plane;
-                un = quad_up_raw(xvel0, i2, g.ld);
-                vn = quad_up_raw(yvel0, i2, g.ld);
-                wn = quad_up_raw(zvel0, i2, g.ld);
-                pn = pressure[i2];
-                rn = density0[i1];
-                pxm = pressure[i1 - 1];
-                pxp = pressure[i1 + 1];
-                pym = pressure[i1 - g.ld];
-                pyp = pressure[i1 + g.ld];
-            }
+            const T p_xm = pressure[i - 1], p_xp = pressure[i + 1], p_ym = pressure[i - g.ld], p_yp = pressure[i + g.ld];
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
