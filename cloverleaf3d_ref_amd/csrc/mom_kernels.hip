@@ -290,19 +290,32 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         }
     }
 
-    // the plane m+2 the first step needs, then one plane ahead of the step
-    Raw<S> rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1);
-    S vn[3];
+    // FP64: the plane m+2 the first step needs, then each step issues the
+    // next plane's loads right after its barrier.  FP32: each step loads its
+    // own plane m+2 first (no registers held across the step), which
+    // measured faster at 512^3 (mom_y 1.42 -> 1.35 ms, mom_z 1.36 -> 1.31 ms;
+    // FP64 runs 0.01 ms slower that way).
+    constexpr bool ahead = sizeof(S) == 8;
+    Raw<S> rn{0, 0, 0, 0, 0, 0};
+    S vn[3] = {0, 0, 0};
+    if constexpr (ahead) {
+        rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1);
 #pragma unroll
-    for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
+        for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
+    }
     for (int m = m0; m <= m1; m++) {
+        if constexpr (!ahead) {
+            rn = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 2)] : (S)0;
+        }
         const Raw<S> rc = rn;
         T vp2[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) vp2[c] = (T)vn[c];
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
         publish(xbuf[b], lane, w, a, p);
-        if (m < m1) {
+        if (ahead && m < m1) {
             rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 3)] : (S)0;
