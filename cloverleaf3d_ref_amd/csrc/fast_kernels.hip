@@ -559,7 +559,11 @@ static hipError_t fast_ideal_gas_t(const Lay &g, const S *density, const S *ener
 // from the ring; the plane below is read back rather than held in
 // registers.  Each chunk also forms the level-1 velocity of the plane above
 // it, which the next chunk stores.
-constexpr int AW = 8;
+// 12 waves: in FP64 the kernel takes 144 VGPRs, three waves per SIMD, so
+// one 12-wave workgroup per CU keeps 12 waves in flight where 8-wave ones
+// kept 8 (4.14 -> 4.05 ms FP64, 2.26 -> 2.23 ms FP32 at 512^3); 10 waves
+// were slower in FP32.
+constexpr int AW = 12;
 
 template <typename S>
 __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, double dt,
@@ -577,9 +581,9 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
     // Slot s is written at plane t (before the first barrier of t) and read
     // at planes t and t+1 (between their two barriers); it is written again
     // at t+2, past the second barrier of t+1, which every thread reaches
-    // only after its reads of t+1.  Two planes rather than three keep the
-    // workgroup's LDS at 48 KiB in FP64, so three workgroups fit a CU
-    // (measured at 512^3: 4.29 -> 4.16 ms FP64, 2.29 -> 2.24 ms FP32).
+    // only after its reads of t+1.  Two planes rather than three measured
+    // faster (8-wave workgroups at 512^3: 4.29 -> 4.16 ms FP64, 2.29 ->
+    // 2.24 ms FP32).
     __shared__ S nb[2][6][AW][64];
     const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
     const int j = g.x_min + bk.x * 63 + lane;
