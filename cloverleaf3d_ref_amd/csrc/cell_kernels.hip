@@ -121,9 +121,10 @@ __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mod
     // and k+1, vol_flux_z at l and l+1
     S rn = 0, en = 0, fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0, fz0 = 0, fz1 = 0;
     auto row_at = [&](int r) { return idx(g, j, g.y_min + r % ny, g.z_min + r / ny); };
+    const int dxc = cell_xm(g, j) - j;  // a mirrored x face: the interior cell its halo copies
     auto load = [&](int ii) {
-        rn = ok_cell ? density1[ii] : (S)0;
-        en = ok_cell ? energy1[ii] : (S)0;
+        rn = ok_cell ? density1[ii + dxc] : (S)0;
+        en = ok_cell ? energy1[ii + dxc] : (S)0;
         fx0 = ok_face ? vfx[ii] : (S)0;
         fx1 = ok_cell ? vfx[ii + 1] : (S)0;
         fy0 = (ok_cell && all3) ? vfy[ii] : (S)0;

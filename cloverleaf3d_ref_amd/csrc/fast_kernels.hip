@@ -46,6 +46,25 @@ __device__ inline QuadS<S> quad_down_raw(const S *__restrict__ p, int i, int ld)
     return QuadS<S>{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
 }
 
+// A quad_down_raw quad at a mirrored x face (Lay::xmirror): at the x_min
+// face (lo) the halo column j-1 takes the values of column j, which its
+// reflection copies, at the x_max face (hi) column j those of j-1.  The
+// halo values were loaded (in bounds) and are replaced where the quad is
+// used, so the loads keep their constant offsets and stay in flight.
+template <typename S>
+__device__ inline QuadS<S> quad_xm(QuadS<S> q, bool lo, bool hi)
+{
+    if (lo) {
+        q.v[0] = q.v[1];
+        q.v[2] = q.v[3];
+    }
+    if (hi) {
+        q.v[1] = q.v[0];
+        q.v[3] = q.v[2];
+    }
+    return q;
+}
+
 
 // ---------------------------------------------------------------------------
 // viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
@@ -76,6 +95,9 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
         int i = idx(g, j, k, l0);
         QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
         T p_lo = pressure[i - g.plane], p_c = pressure[i];
+        // a mirrored x face: the x neighbour there is a halo cell, read as
+        // the cell itself, which its reflection copies
+        const bool mxm = cell_xm(g, j - 1) != j - 1, mxp = cell_xm(g, j + 1) != j + 1;
         // The cell's soundspeed is not read: it is ideal_gas's of the cell's
         // density and pressure, formed here with the same operations
         // (soundspeed_of), so it has the same bits.
@@ -90,7 +112,9 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             const T p_hi = pressure[i + g.plane];
             const T rho = density0[i];
             const T c = soundspeed_of<T>(rho, p_c);
-            const T p_xm = pressure[i - 1], p_xp = pressure[i + 1], p_ym = pressure[i - g.ld], p_yp = pressure[i + g.ld];
+            const T p_xm_h = pressure[i - 1], p_xp_h = pressure[i + 1];  // loaded either way
+            const T p_xm = mxm ? p_c : p_xm_h, p_xp = mxp ? p_c : p_xp_h;
+            const T p_ym = pressure[i - g.ld], p_yp = pressure[i + g.ld];
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
@@ -604,6 +628,9 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
     // (accelerate), the node's level-0 velocity and the cell's energy0
     QuadS<S> dn{}, pn{}, qn{};
     S u0n = 0, v0n = 0, w0n = 0, e0n = 0;
+    // a mirrored x face: the node's cells j-1 (x_min face) or j (x_max
+    // face) are halo cells, read as the interior cells they mirror
+    const bool mlo = cell_xm(g, j - 1) != j - 1, mhi = cell_xm(g, j) != j;
     auto load = [&](int ii, int lp) {
         dn = quad_down_raw(density0, ii, g.ld);
         pn = quad_down_raw(pressure, ii, g.ld);
@@ -614,16 +641,16 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
         e0n = (lp <= g.z_max && jin && kin) ? energy0[ii] : (S)0;
     };
     if (node) {
-        dlo = quad_down_raw(density0, i - g.plane, g.ld);
-        plo = quad_down_raw(pressure, i - g.plane, g.ld);
-        qlo = quad_down_raw(viscosity, i - g.plane, g.ld);
+        dlo = quad_xm(quad_down_raw(density0, i - g.plane, g.ld), mlo, mhi);
+        plo = quad_xm(quad_down_raw(pressure, i - g.plane, g.ld), mlo, mhi);
+        qlo = quad_xm(quad_down_raw(viscosity, i - g.plane, g.ld), mlo, mhi);
         load(i, l0);
     }
     T e0p = 0;  // energy0 of the cell below
     const int lend = min(l1 + 1, g.z_max + 1);
     int s = 0, sp = 1;  // ring slots of the node plane in hand and the one below
     for (int lp = l0; lp <= lend; lp++, i += g.plane) {
-        const QuadS<S> dhi = dn, phi = pn, qhi = qn;
+        const QuadS<S> dhi = quad_xm(dn, mlo, mhi), phi = quad_xm(pn, mlo, mhi), qhi = quad_xm(qn, mlo, mhi);
         const S u0 = u0n, v0 = v0n, w0 = w0n;
         const T e0q = (T)e0n;
         // the next plane's loads go out before this plane's work, so they

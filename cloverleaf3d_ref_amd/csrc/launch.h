@@ -200,6 +200,28 @@ __device__ inline void ideal_gas_cell(S density, S energy, S &pressure, S &sound
     soundspeed = soundspeed_of<T>(d, p);
 }
 
+// Index of the column a fused kernel reads for column j of a cell field and
+// of a vertex field along x (Lay::xmirror): on a face whose reflective halo
+// is not exchanged, the interior column that halo would copy
+// (update_halo_kernel.f90: cells W = 1-d <- R = d and W = max+d <- max+1-d;
+// vertices W = 1-d <- 1+d and W = max+1+d <- max+1-d), else j itself.  The
+// x velocity also flips its sign there (node_xsign).
+__device__ inline int cell_xm(const Lay &g, int j)
+{
+    if ((g.xmirror & 1) && j < g.x_min) return 2 * g.x_min - 1 - j;
+    if ((g.xmirror & 2) && j > g.x_max) return 2 * g.x_max + 1 - j;
+    return j;
+}
+
+__device__ inline int node_xm(const Lay &g, int j)
+{
+    if ((g.xmirror & 1) && j < g.x_min) return 2 * g.x_min - j;
+    if ((g.xmirror & 2) && j > g.x_max + 1) return 2 * g.x_max + 2 - j;
+    return j;
+}
+
+__device__ inline bool node_xsign(const Lay &g, int j) { return node_xm(g, j) != j; }
+
 constexpr int RED_THREADS = 256;
 
 __device__ inline double wave_min(double v)

@@ -139,15 +139,16 @@ struct Raw {
     S a, rho, v0, v1, v2, v3;  // storage type: converted where used, so the loads stay in flight
 };
 
+// dxc: offset of the density read (a mirrored x face, launch.h cell_xm)
 template <typename S>
 __device__ inline Raw<S> load_raw(int mode, const Lay &g, int i, bool ok_a, bool ok_p, const S *__restrict__ mf,
                                   const S *__restrict__ vfx, const S *__restrict__ vfy, const S *__restrict__ vfz,
-                                  const S *__restrict__ density1)
+                                  const S *__restrict__ density1, int dxc)
 {
     Raw<S> r{0, 0, 0, 0, 0, 0};
     if (ok_a) r.a = mf[i];
     if (ok_p) {
-        r.rho = density1[i];
+        r.rho = density1[i + dxc];
         switch (mode) {
         case 1: r.v0 = vfy[i + g.ld]; r.v1 = vfy[i]; r.v2 = vfz[i + g.plane]; r.v3 = vfz[i]; break;
         case 2: r.v0 = vfx[i + 1]; r.v1 = vfx[i]; r.v2 = vfy[i + g.ld]; r.v3 = vfy[i]; break;
@@ -251,10 +252,12 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     const int i0 = DIR == 3 ? idx(g, j, wp, mmin) : idx(g, j, mmin, wp);  // offset at march position mmin
 #define OFF(q) (i0 + ((q) - mmin) * sM)
 
+    const int dxc = cell_xm(g, j) - j;  // a mirrored x face: the interior cell its halo copies
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
         const T a = ok_cell ? (T)mf[i] : (T)0;  // q <= mmax+3: inside the sweep's face extent
-        const T p = (ok_cell && q <= mmax + 2) ? (T)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : (T)0;
+        const T p = (ok_cell && q <= mmax + 2) ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz)
+                                               : (T)0;
         exchange<-1, -1>(xbuf[b], lane, w, a, p, F, P);
         b ^= 1;
     };
@@ -299,13 +302,13 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     Raw<S> rn{0, 0, 0, 0, 0, 0};
     S vn[3] = {0, 0, 0};
     if constexpr (ahead) {
-        rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1);
+        rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
 #pragma unroll
         for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
     }
     for (int m = m0; m <= m1; m++) {
         if constexpr (!ahead) {
-            rn = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1);
+            rn = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 2)] : (S)0;
         }
@@ -316,7 +319,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
         publish(xbuf[b], lane, w, a, p);
         if (ahead && m < m1) {
-            rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1);
+            rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 3)] : (S)0;
         }
@@ -388,11 +391,15 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     const bool store = lane >= 2 && lane <= 61 && w >= 1 && j <= g.x_max + 1 && k <= g.y_max + 1;
     const int i0 = idx(g, j, k, g.z_min);
 #define OFF(q) (i0 + ((q) - g.z_min) * g.plane)
+    // a mirrored x face: the interior cell and node its halos copy, and the
+    // sign flip of the x velocity there
+    const int dxc = cell_xm(g, j) - j, dxn = node_xm(g, j) - j;
+    const bool xneg = node_xsign(g, j);
 
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
         const T a = ok_face ? (T)mf[i] : (T)0;
-        const T p = ok_cell ? (T)density1[i] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : (T)0;
+        const T p = ok_cell ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : (T)0;
         exchange<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
         b ^= 1;
     };
@@ -402,21 +409,22 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     plane(l0 - 1, Fp, Pp, b);
     const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63, lp2 = lane < 62 ? lane + 2 : 63;
     // plane l's loads are issued during plane l-1 (after its barrier)
-    Raw<S> rn = load_raw(mode, g, OFF(l0), ok_face, ok_cell, mf, vfx, vfy, vfz, density1);
+    Raw<S> rn = load_raw(mode, g, OFF(l0), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc);
     S vn[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l0)] : (S)0;
+    for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l0) + dxn] : (S)0;
     for (int l = l0; l <= l1; l++) {
         const Raw<S> rc = rn;
         T vc[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) vc[c] = (T)vn[c];
+        if (xneg) vc[0] = -vc[0];  // the reflection of the x velocity (update_halo_kernel.f90:604-1004)
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell);
         publish(xbuf[b], lane, w, a, p);
         if (l < l1) {
-            rn = load_raw(mode, g, OFF(l + 1), ok_face, ok_cell, mf, vfx, vfy, vfz, density1);
+            rn = load_raw(mode, g, OFF(l + 1), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc);
 #pragma unroll
-            for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l + 1)] : (S)0;
+            for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l + 1) + dxn] : (S)0;
         }
         Quad<T> F, P;
         gather<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
