@@ -200,8 +200,13 @@ __device__ inline void gather(T (*buf)[W][64], int lane, int w, T a, T p, Quad<T
 }
 
 // ---------------------------------------------------------------------------
-// y and z sweeps: lanes along x (lane 0 is the j-1 halo, 63 lanes store),
-// waves along the other axis, march along the sweep axis.
+// y and z sweeps: lanes along x on 64 line-aligned columns, all of which
+// store; lane 0 also loads and publishes column j-1 (publish65), which the
+// first column's node needs.  (A 63-column layout with lane 0 as the j-1
+// halo read rows that straddle cache lines: measured at 512^3 the aligned
+// one runs FP32 mom_y 1.35 -> 1.28 ms and mom_z 1.33 -> 1.20 ms, FP64 the
+// same.)  Waves lie along the other axis; the workgroup marches along the
+// sweep axis.
 //
 // In (x, m = sweep axis, w = wave axis) offsets both sweeps have the same
 // node flux (advec_mom_kernel.f90:260-265 for y, 345-350 for z):
@@ -224,6 +229,35 @@ __device__ inline T march_node_mass(const Quad<T> &pm, const Quad<T> &p0)  // ce
     return (T)0.125 * (p0.w + p0.c + p0.xw + p0.x + pm.w + pm.c + pm.xw + pm.x);
 }
 
+// The march's x exchange with 64 owned lanes: slot s of a row holds column
+// j0-1+s, lane L publishes its own column at slot L+1 and lane 0 also the
+// column j0-1 at slot 0.
+template <typename T, int W>
+__device__ inline void publish65(T (*buf)[W][65], int lane, int w, T a, T p, T am, T pm)
+{
+    buf[0][w][lane + 1] = a;
+    buf[1][w][lane + 1] = p;
+    if (lane == 0) {
+        buf[0][w][0] = am;
+        buf[1][w][0] = pm;
+    }
+    __syncthreads();
+}
+
+template <typename T, int W>
+__device__ inline void gather65(T (*buf)[W][65], int lane, int w, T a, T p, Quad<T> &A, Quad<T> &P)
+{
+    const int wd = w > 0 ? w - 1 : 0;
+    A.c = a;
+    A.x = buf[0][w][lane];
+    A.w = buf[0][wd][lane + 1];
+    A.xw = buf[0][wd][lane];
+    P.c = p;
+    P.x = buf[1][w][lane];
+    P.w = buf[1][wd][lane + 1];
+    P.xw = buf[1][wd][lane];
+}
+
 template <int DIR, typename S>
 __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
@@ -231,7 +265,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
 {
     using T = S;  // arithmetic type of the form
     constexpr int MW = mw_of(DIR);
-    __shared__ T xbuf[2][2][MW][64];
+    __shared__ T xbuf[2][2][MW][65];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int sM = DIR == 3 ? g.plane : g.ld;
     const int wmin = DIR == 3 ? g.y_min : g.z_min, wmax = DIR == 3 ? g.y_max : g.z_max;
@@ -240,7 +274,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     const Recip rd = DIR == 3 ? geo.rz : geo.ry;
     const T V0 = (T)geo.vol;
 
-    const int j = g.x_min - 1 + bk.x * 63 + lane;
+    const int j = g.x_min + bk.x * 64 + lane;
     const int wp = wmin - 1 + bk.y * (MW - 1) + w;
     int m0, m1;
     chunk_range(bk, mmin, mmax + 1, m0, m1);
@@ -248,17 +282,23 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     // j >= x_min-1 and wp >= wmin-1 always; upper limits of the arrays read:
     // cells and the sweep's mass flux end at +2 across the sweep, vertices at +3
     const bool ok_cell = j <= g.x_max + 2 && wp <= wmax + 2;
-    const bool store = lane >= 1 && w >= 1 && j <= g.x_max + 1 && wp <= wmax + 1;
+    const bool ok_cm = lane == 0 && j - 1 <= g.x_max + 2 && wp <= wmax + 2;  // lane 0's column j-1
+    const bool store = w >= 1 && j <= g.x_max + 1 && wp <= wmax + 1;
     const int i0 = DIR == 3 ? idx(g, j, wp, mmin) : idx(g, j, mmin, wp);  // offset at march position mmin
 #define OFF(q) (i0 + ((q) - mmin) * sM)
 
     const int dxc = cell_xm(g, j) - j;  // a mirrored x face: the interior cell its halo copies
+    const int dxm = cell_xm(g, j - 1) - (j - 1);
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
         const T a = ok_cell ? (T)mf[i] : (T)0;  // q <= mmax+3: inside the sweep's face extent
         const T p = (ok_cell && q <= mmax + 2) ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz)
                                                : (T)0;
-        exchange<-1, -1>(xbuf[b], lane, w, a, p, F, P);
+        const T am = ok_cm ? (T)mf[i - 1] : (T)0;
+        const T pm = (ok_cm && q <= mmax + 2) ? (T)density1[i - 1 + dxm] * mom_post_vol(mode, g, i - 1, V0, vfx, vfy, vfz)
+                                              : (T)0;
+        publish65(xbuf[b], lane, w, a, p, am, pm);
+        gather65(xbuf[b], lane, w, a, p, F, P);
         b ^= 1;
     };
     auto vel_at = [&](int c, int q) { return store ? (T)v.in[c][OFF(q)] : (T)0; };
@@ -299,32 +339,36 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     // measured faster at 512^3 (mom_y 1.42 -> 1.35 ms, mom_z 1.36 -> 1.31 ms;
     // FP64 runs 0.01 ms slower that way).
     constexpr bool ahead = sizeof(S) == 8;
-    Raw<S> rn{0, 0, 0, 0, 0, 0};
+    Raw<S> rn{0, 0, 0, 0, 0, 0}, rnm{0, 0, 0, 0, 0, 0};
     S vn[3] = {0, 0, 0};
     if constexpr (ahead) {
         rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
+        rnm = load_raw(mode, g, OFF(m0 + 2) - 1, ok_cm, ok_cm && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxm);
 #pragma unroll
         for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
     }
     for (int m = m0; m <= m1; m++) {
         if constexpr (!ahead) {
             rn = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
+            rnm = load_raw(mode, g, OFF(m + 2) - 1, ok_cm, ok_cm && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxm);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 2)] : (S)0;
         }
-        const Raw<S> rc = rn;
+        const Raw<S> rc = rn, rcm = rnm;
         T vp2[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) vp2[c] = (T)vn[c];
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
-        publish(xbuf[b], lane, w, a, p);
+        const T am = (T)rcm.a, pm = raw_cell_mass(mode, V0, rcm, ok_cm && m + 2 <= mmax + 2);
+        publish65(xbuf[b], lane, w, a, p, am, pm);
         if (ahead && m < m1) {
             rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
+            rnm = load_raw(mode, g, OFF(m + 3) - 1, ok_cm, ok_cm && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxm);
 #pragma unroll
             for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 3)] : (S)0;
         }
         Quad<T> Fc, Pc;
-        gather<-1, -1>(xbuf[b], lane, w, a, p, Fc, Pc);
+        gather65(xbuf[b], lane, w, a, p, Fc, Pc);
         b ^= 1;
         const T nf1 = march_node_flux<DIR, T>(Fb, Fc);  // nf(m+1)
         const T mp2 = march_node_mass<DIR, T>(Pb, Pc);  // mp(m+2), garbage at mmax+3 (never used)
@@ -478,11 +522,11 @@ static hipError_t fast_advec_mom_t(const Lay &g, const Geo &m, int mode, int dir
         TIMED_LAUNCH("k_fast_mom_x", st, k_fast_mom_x<S>, grid, blk, 0, st, sh.xcd, g, m, mode, mass_flux_x, vol_flux_x,
                      vol_flux_y, vol_flux_z, density1, v);
     } else if (direction == 2) {
-        const dim3 grid((nxn + 62) / 63, (nzn + MW - 2) / (MW - 1), nchunks(nyn, sh.planes));
+        const dim3 grid((nxn + 63) / 64, (nzn + MW - 2) / (MW - 1), nchunks(nyn, sh.planes));
         TIMED_LAUNCH("k_fast_mom_y", st, (k_fast_mom_march<2, S>), grid, blk, 0, st, sh.xcd, g, m, mode, mass_flux_y,
                      vol_flux_x, vol_flux_y, vol_flux_z, density1, v);
     } else {
-        const dim3 grid((nxn + 62) / 63, (nyn + MW - 2) / (MW - 1), nchunks(nzn, sh.planes));
+        const dim3 grid((nxn + 63) / 64, (nyn + MW - 2) / (MW - 1), nchunks(nzn, sh.planes));
         TIMED_LAUNCH("k_fast_mom_z", st, (k_fast_mom_march<3, S>), grid, blk, 0, st, sh.xcd, g, m, mode, mass_flux_z,
                      vol_flux_x, vol_flux_y, vol_flux_z, density1, v);
     }
@@ -521,9 +565,9 @@ __host__ __device__ inline void dims_mom(int dir, const Lay &g, int planes, int 
     if (dir == 1) {
         gx = (nxn + 59) / 60; gy = (nyn + MW - 2) / (MW - 1); gz = nchunks(nzn, planes);
     } else if (dir == 2) {
-        gx = (nxn + 62) / 63; gy = (nzn + MW - 2) / (MW - 1); gz = nchunks(nyn, planes);
+        gx = (nxn + 63) / 64; gy = (nzn + MW - 2) / (MW - 1); gz = nchunks(nyn, planes);
     } else {
-        gx = (nxn + 62) / 63; gy = (nyn + MW - 2) / (MW - 1); gz = nchunks(nzn, planes);
+        gx = (nxn + 63) / 64; gy = (nyn + MW - 2) / (MW - 1); gz = nchunks(nzn, planes);
     }
 }
 
