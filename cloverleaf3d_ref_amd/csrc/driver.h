@@ -87,13 +87,17 @@ void host_init(const clover_deck &d, const Tile &t, HostInit &h, bool fields = t
 //                 FH_CELL1's halos, so they are left out of this exchange
 //                 altogether (neither refreshed nor, in the halo-check test
 //                 mode, overwritten).
-//   With xmirror (a tile whose two x faces are both external, Lay::xmirror
-//   3), the kernels read the interior cell or node that a reflective x halo
-//   of a cell or vertex field would copy instead of the halo (the value the
-//   reflection would write there, sign flip included), so those fields'
-//   x faces are left out of every exchange.  The face fields keep theirs:
-//   some of their x-halo corners hold reflections of values the other passes
-//   had not yet refreshed, which only the exchange itself reproduces.
+//   mirror_axes (axes along which both faces of a one-tile chunk are
+//   mirrored, Lay::mirror): there the kernels read the interior cell or node
+//   that a reflective halo of a cell or vertex field would copy instead of
+//   the halo (the value the reflection would write there, sign flip
+//   included), so those fields' faces are left out of every exchange:
+//   visc_dt and accel_pdv mirror every axis (level-0 fields, viscosity,
+//   pressure), the advection sweeps the x axis (level-1 fields; along y
+//   and z the address arithmetic cost them more than the exchanges).  The
+//   face fields keep all their exchanges: some of their corner halos hold
+//   reflections of values the other passes had not yet refreshed, which
+//   only the exchange itself reproduces.
 enum FusedHalo : int { FH_TIMESTEP, FH_VISCOSITY, FH_PRESSURE, FH_CELL1, FH_MOM1, FH_CELL2, FH_MOM2, FH_MOM3 };
 
 enum : int { AX_X = 1, AX_Y = 2, AX_Z = 4, AX_ALL = 7 };
@@ -104,7 +108,13 @@ struct HaloPlan {
     int axes[NUM_FIELDS];    // axis bits the fused kernels read
 };
 
-inline HaloPlan fused_halo_plan(int which, bool advect_x, bool xmirror = false)
+// the axes along which both faces of a tile are mirrored (Lay::mirror)
+inline int mirror_axes(const Lay &g)
+{
+    return ((g.mirror & 3) == 3 ? AX_X : 0) | ((g.mirror & 12) == 12 ? AX_Y : 0) | ((g.mirror & 48) == 48 ? AX_Z : 0);
+}
+
+inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0)
 {
     HaloPlan p{};
     auto ref = [&](std::initializer_list<int> ids) {
@@ -153,10 +163,9 @@ inline HaloPlan fused_halo_plan(int which, bool advect_x, bool xmirror = false)
         break;
     default: mom(s3); break;
     }
-    if (xmirror)
-        for (int f : {F_DENSITY0, F_DENSITY1, F_ENERGY0, F_ENERGY1, F_PRESSURE, F_VISCOSITY, F_SOUNDSPEED, F_XVEL0,
-                      F_XVEL1, F_YVEL0, F_YVEL1, F_ZVEL0, F_ZVEL1})
-            p.axes[f] &= ~AX_X;
+    for (int f : {F_DENSITY0, F_ENERGY0, F_PRESSURE, F_VISCOSITY, F_SOUNDSPEED, F_XVEL0, F_YVEL0, F_ZVEL0})
+        p.axes[f] &= ~mirror_axes;
+    for (int f : {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1}) p.axes[f] &= ~(mirror_axes & AX_X);
     return p;
 }
 

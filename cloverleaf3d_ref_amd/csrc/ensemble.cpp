@@ -136,7 +136,7 @@ void halo(clover_ensemble *e, const int *fields, int depth)
 
 void halo(clover_ensemble *e, int which)
 {
-    const HaloPlan p = fused_halo_plan(which, e->advect_x != 0, true);
+    const HaloPlan p = fused_halo_plan(which, e->advect_x != 0, AX_ALL);
     static const int axis_order[3] = {1, 0, 2};
     for (int ao = 0; ao < 3; ao++) {
         const int axis = axis_order[ao];
@@ -229,7 +229,7 @@ void prepare_grids(clover_ensemble *e)
     // the step's halo updates, for both sweep orders
     for (int ax = 0; ax < 2; ax++)
         for (int which = FH_TIMESTEP; which <= FH_MOM3; which++) {
-            const HaloPlan p = fused_halo_plan(which, ax != 0, true);
+            const HaloPlan p = fused_halo_plan(which, ax != 0, AX_ALL);
             for (int axis = 0; axis < 3; axis++) {
                 const int n = items_on_axis(p, axis);
                 if (n) grid(e, 100 + axis, p.depth, n);
@@ -434,10 +434,11 @@ int clover_ensemble_create(const clover_deck *decks, int ndecks, int device, int
             DeckDev &D = e->host[d];
             memset(&D, 0, sizeof D);
             D.g = g;
-            // every deck is one tile with external x faces: the fused kernels
-            // read the mirrored interior instead of the cell and vertex
-            // fields' reflective x halos, and the exchanges leave those out
-            D.g.xmirror = 3;
+            // every deck is one tile at least 3 cells thick with external
+            // faces: the fused kernels read the mirrored interior instead of
+            // the cell and vertex fields' reflective halos, and the
+            // exchanges leave those out
+            D.g.mirror = 63;
             // the uniform mesh as initialise_chunk.f90:37-39 computes it
             D.m = geo_of((dk.xmax - dk.xmin) / (double)(float)dk.x_cells,
                          (dk.ymax - dk.ymin) / (double)(float)dk.y_cells,

@@ -46,21 +46,30 @@ __device__ inline QuadS<S> quad_down_raw(const S *__restrict__ p, int i, int ld)
     return QuadS<S>{{p[i - 1 - ld], p[i - ld], p[i - 1], p[i]}};
 }
 
-// A quad_down_raw quad at a mirrored x face (Lay::xmirror): at the x_min
-// face (lo) the halo column j-1 takes the values of column j, which its
-// reflection copies, at the x_max face (hi) column j those of j-1.  The
-// halo values were loaded (in bounds) and are replaced where the quad is
-// used, so the loads keep their constant offsets and stay in flight.
+// A quad_down_raw quad at mirrored faces (Lay::mirror): at the x_min face
+// (xlo) the halo column j-1 takes the values of column j, which its
+// reflection copies, at the x_max face (xhi) column j those of j-1; the
+// same for the rows k-1 and k at the y faces.  The halo values were loaded
+// (in bounds) and are replaced where the quad is used, so the loads keep
+// their constant offsets and stay in flight.
 template <typename S>
-__device__ inline QuadS<S> quad_xm(QuadS<S> q, bool lo, bool hi)
+__device__ inline QuadS<S> quad_m(QuadS<S> q, bool xlo, bool xhi, bool ylo, bool yhi)
 {
-    if (lo) {
+    if (xlo) {
         q.v[0] = q.v[1];
         q.v[2] = q.v[3];
     }
-    if (hi) {
+    if (xhi) {
         q.v[1] = q.v[0];
         q.v[3] = q.v[2];
+    }
+    if (ylo) {
+        q.v[0] = q.v[2];
+        q.v[1] = q.v[3];
+    }
+    if (yhi) {
+        q.v[2] = q.v[0];
+        q.v[3] = q.v[1];
     }
     return q;
 }
@@ -94,10 +103,11 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
         T mc = 0, mu = 0, mv = 0, mw = 0, mdiv = 0;  // largest dt denominators so far (all are >= 0)
         int i = idx(g, j, k, l0);
         QuadS<S> ul = quad_up_raw(xvel0, i, g.ld), vl = quad_up_raw(yvel0, i, g.ld), wl = quad_up_raw(zvel0, i, g.ld);
-        T p_lo = pressure[i - g.plane], p_c = pressure[i];
-        // a mirrored x face: the x neighbour there is a halo cell, read as
-        // the cell itself, which its reflection copies
-        const bool mxm = cell_xm(g, j - 1) != j - 1, mxp = cell_xm(g, j + 1) != j + 1;
+        // a mirrored face (Lay::mirror): the neighbour across it is a halo
+        // cell, read as the cell itself, which its reflection copies
+        const int oxm = cell_m(g, 0, j - 1) - j, oxp = cell_m(g, 0, j + 1) - j;
+        const int oym = (cell_m(g, 1, k - 1) - k) * g.ld, oyp = (cell_m(g, 1, k + 1) - k) * g.ld;
+        T p_lo = pressure[i + (cell_m(g, 2, l0 - 1) - l0) * g.plane], p_c = pressure[i];
         // The cell's soundspeed is not read: it is ideal_gas's of the cell's
         // density and pressure, formed here with the same operations
         // (soundspeed_of), so it has the same bits.
@@ -109,12 +119,11 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
             // and measured 1.55 against 1.49 ms at 512^3
             const QuadS<S> uu = quad_up_raw(xvel0, i + g.plane, g.ld), vu = quad_up_raw(yvel0, i + g.plane, g.ld),
                            wu = quad_up_raw(zvel0, i + g.plane, g.ld);
-            const T p_hi = pressure[i + g.plane];
+            const T p_hi = pressure[i + (cell_m(g, 2, l + 1) - l) * g.plane];
             const T rho = density0[i];
             const T c = soundspeed_of<T>(rho, p_c);
-            const T p_xm_h = pressure[i - 1], p_xp_h = pressure[i + 1];  // loaded either way
-            const T p_xm = mxm ? p_c : p_xm_h, p_xp = mxp ? p_c : p_xp_h;
-            const T p_ym = pressure[i - g.ld], p_yp = pressure[i + g.ld];
+            const T p_xm = pressure[i + oxm], p_xp = pressure[i + oxp];
+            const T p_ym = pressure[i + oym], p_yp = pressure[i + oyp];
 #define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
 #define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
 #define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
@@ -628,9 +637,11 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
     // (accelerate), the node's level-0 velocity and the cell's energy0
     QuadS<S> dn{}, pn{}, qn{};
     S u0n = 0, v0n = 0, w0n = 0, e0n = 0;
-    // a mirrored x face: the node's cells j-1 (x_min face) or j (x_max
-    // face) are halo cells, read as the interior cells they mirror
-    const bool mlo = cell_xm(g, j - 1) != j - 1, mhi = cell_xm(g, j) != j;
+    // mirrored faces: the node's cells j-1 (x_min face) or j (x_max face),
+    // k-1 or k (y faces) and the planes below z_min or above z_max are halo
+    // cells, read as the interior cells they mirror
+    const bool mlo = cell_m(g, 0, j - 1) != j - 1, mhi = cell_m(g, 0, j) != j;
+    const bool ylo = cell_m(g, 1, k - 1) != k - 1, yhi = cell_m(g, 1, k) != k;
     auto load = [&](int ii, int lp) {
         dn = quad_down_raw(density0, ii, g.ld);
         pn = quad_down_raw(pressure, ii, g.ld);
@@ -641,16 +652,29 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
         e0n = (lp <= g.z_max && jin && kin) ? energy0[ii] : (S)0;
     };
     if (node) {
-        dlo = quad_xm(quad_down_raw(density0, i - g.plane, g.ld), mlo, mhi);
-        plo = quad_xm(quad_down_raw(pressure, i - g.plane, g.ld), mlo, mhi);
-        qlo = quad_xm(quad_down_raw(viscosity, i - g.plane, g.ld), mlo, mhi);
+        dlo = quad_m(quad_down_raw(density0, i - g.plane, g.ld), mlo, mhi, ylo, yhi);
+        plo = quad_m(quad_down_raw(pressure, i - g.plane, g.ld), mlo, mhi, ylo, yhi);
+        qlo = quad_m(quad_down_raw(viscosity, i - g.plane, g.ld), mlo, mhi, ylo, yhi);
         load(i, l0);
     }
     T e0p = 0;  // energy0 of the cell below
     const int lend = min(l1 + 1, g.z_max + 1);
     int s = 0, sp = 1;  // ring slots of the node plane in hand and the one below
     for (int lp = l0; lp <= lend; lp++, i += g.plane) {
-        const QuadS<S> dhi = quad_xm(dn, mlo, mhi), phi = quad_xm(pn, mlo, mhi), qhi = quad_xm(qn, mlo, mhi);
+        QuadS<S> dhi = quad_m(dn, mlo, mhi, ylo, yhi), phi = quad_m(pn, mlo, mhi, ylo, yhi),
+                 qhi = quad_m(qn, mlo, mhi, ylo, yhi);
+        // mirrored z faces: the cell plane below z_min is read as plane
+        // z_min, the one above z_max as plane z_max
+        if (cell_m(g, 2, lp - 1) != lp - 1) {
+            dlo = dhi;
+            plo = phi;
+            qlo = qhi;
+        }
+        if (cell_m(g, 2, lp) != lp) {
+            dhi = dlo;
+            phi = plo;
+            qhi = qlo;
+        }
         const S u0 = u0n, v0 = v0n, w0 = w0n;
         const T e0q = (T)e0n;
         // the next plane's loads go out before this plane's work, so they

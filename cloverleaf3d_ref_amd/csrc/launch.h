@@ -200,26 +200,34 @@ __device__ inline void ideal_gas_cell(S density, S energy, S &pressure, S &sound
     soundspeed = soundspeed_of<T>(d, p);
 }
 
-// Index of the column a fused kernel reads for column j of a cell field and
-// of a vertex field along x (Lay::xmirror): on a face whose reflective halo
-// is not exchanged, the interior column that halo would copy
+// Index a fused kernel reads for index q along axis a (0 x, 1 y, 2 z) of a
+// cell field and of a vertex field (Lay::mirror): on a face whose
+// reflective halo is not exchanged, the interior index that halo would copy
 // (update_halo_kernel.f90: cells W = 1-d <- R = d and W = max+d <- max+1-d;
-// vertices W = 1-d <- 1+d and W = max+1+d <- max+1-d), else j itself.  The
-// x velocity also flips its sign there (node_xsign).
-__device__ inline int cell_xm(const Lay &g, int j)
+// vertices W = 1-d <- 1+d and W = max+1+d <- max+1-d), else q itself.  The
+// velocity component along a also flips its sign there.
+__device__ inline int cell_m(const Lay &g, int a, int q)
 {
-    if ((g.xmirror & 1) && j < g.x_min) return 2 * g.x_min - 1 - j;
-    if ((g.xmirror & 2) && j > g.x_max) return 2 * g.x_max + 1 - j;
-    return j;
+    const int lo = a == 0 ? g.x_min : a == 1 ? g.y_min : g.z_min;
+    const int hi = a == 0 ? g.x_max : a == 1 ? g.y_max : g.z_max;
+    const int bits = g.mirror >> (2 * a);
+    if ((bits & 1) && q < lo) return 2 * lo - 1 - q;
+    if ((bits & 2) && q > hi) return 2 * hi + 1 - q;
+    return q;
 }
 
-__device__ inline int node_xm(const Lay &g, int j)
+__device__ inline int node_m(const Lay &g, int a, int q)
 {
-    if ((g.xmirror & 1) && j < g.x_min) return 2 * g.x_min - j;
-    if ((g.xmirror & 2) && j > g.x_max + 1) return 2 * g.x_max + 2 - j;
-    return j;
+    const int lo = a == 0 ? g.x_min : a == 1 ? g.y_min : g.z_min;
+    const int hi = a == 0 ? g.x_max : a == 1 ? g.y_max : g.z_max;
+    const int bits = g.mirror >> (2 * a);
+    if ((bits & 1) && q < lo) return 2 * lo - q;
+    if ((bits & 2) && q > hi + 1) return 2 * hi + 2 - q;
+    return q;
 }
 
+__device__ inline int cell_xm(const Lay &g, int j) { return cell_m(g, 0, j); }
+__device__ inline int node_xm(const Lay &g, int j) { return node_m(g, 0, j); }
 __device__ inline bool node_xsign(const Lay &g, int j) { return node_xm(g, j) != j; }
 
 constexpr int RED_THREADS = 256;

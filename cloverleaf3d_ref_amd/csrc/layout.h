@@ -24,10 +24,12 @@ struct Lay {
     int ld;       // row pitch (elements)
     int plane;    // ld * (ny + 5)
     int base;     // offset of (x_min-2, y_min-2, z_min-2)
-    // x faces whose reflective halo the fused kernels do not read: they read
-    // the interior cell or node the halo would mirror instead (bit 1 the
-    // x_min face, bit 2 the x_max face; 0 where halos are exchanged)
-    int xmirror;
+    // faces whose reflective halo the fused kernels do not read for cell and
+    // vertex fields: they read the interior cell or node the halo would
+    // mirror instead (bits 1, 2: the x_min and x_max faces; 4, 8: y; 16, 32:
+    // z; 0 where halos are exchanged).  The advection sweeps use the x bits
+    // only.
+    int mirror;
 };
 
 __host__ __device__ inline int idx(const Lay &g, int j, int k, int l)
@@ -44,7 +46,7 @@ inline Lay make_lay(int x_min, int x_max, int y_min, int y_max, int z_min, int z
     g.ld = (need + kRowAlign - 1) / kRowAlign * kRowAlign;
     g.plane = g.ld * (ny + 5);
     g.base = kFrontPad;
-    g.xmirror = 0;
+    g.mirror = 0;
     return g;
 }
 

@@ -38,7 +38,7 @@ Lay dense_lay(int x_min, int x_max, int y_min, int y_max, int z_min, int z_max, 
     g.ld = ex;
     g.plane = ex * ey;
     g.base = 0;
-    g.xmirror = 0;
+    g.mirror = 0;
     return g;
 }
 
