@@ -577,8 +577,16 @@ static void update_halo(clover_chunk *c, const int *fields, int depth, const int
 }
 
 // an exchange of the fused step, reduced to the halos its kernels read
+// (the reference's own exchange where a tile is under 3 cells thick)
 static void fused_halo(clover_chunk *c, int which)
 {
+    bool thin = false;
+    for (auto &t : c->tiles) thin = thin || t.nx < 3 || t.ny < 3 || t.nz < 3;
+    if (thin) {
+        const HaloPlan p = reference_halo_plan(which);
+        update_halo(c, p.fields, p.depth, nullptr);
+        return;
+    }
     const HaloPlan p = fused_halo_plan(which, c->advect_x != 0, c->tiles.size() == 1 ? mirror_axes(c->tiles[0].g) : 0);
     update_halo(c, p.fields, p.depth, p.axes);
 }
@@ -1265,12 +1273,12 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         hip_check(hipHostMalloc(&c->h_scalar, sizeof(double) * 8 * T, hipHostMallocDefault), "hipHostMalloc");
         for (auto &t : c->tiles) {
             t.g = make_lay(1, t.nx, 1, t.ny, 1, t.nz);
-            // one tile: every face is external.  Along each axis at least 3
-            // cells thick the fused kernels read the mirrored interior
-            // instead of the cell and vertex fields' reflective halos, and
-            // the exchanges leave those out (on a thinner tile a depth-2
-            // reflection reads a halo cell of the other face)
-            if (c->fast && T == 1) t.g.mirror = (t.nx >= 3 ? 3 : 0) | (t.ny >= 3 ? 12 : 0) | (t.nz >= 3 ? 48 : 0);
+            // one tile at least 3 cells thick along every axis: every face
+            // is external, and the fused kernels read the mirrored interior
+            // instead of the cell and vertex fields' reflective halos, which
+            // the exchanges then leave out (a thinner tile takes the
+            // reference's exchanges, fused_halo)
+            if (c->fast && T == 1 && t.nx >= 3 && t.ny >= 3 && t.nz >= 3) t.g.mirror = 63;
             const int64_t n = lay_elems(t.g);
             if (n >= (int64_t(1) << 31)) throw Error(E_ARG, "tile too large for 32-bit element offsets");
             for (int a = 0; a < NUM_ARRAYS; a++) t.f[a] = dev_alloc_bytes(c, (size_t)n * store_bytes(c->store));

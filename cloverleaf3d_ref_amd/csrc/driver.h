@@ -108,6 +108,36 @@ struct HaloPlan {
     int axes[NUM_FIELDS];    // axis bits the fused kernels read
 };
 
+// The exchange `which` as the reference makes it: every field it flags on
+// every axis (timestep.f90:68-74 and 82-84, PdV.f90:97-99, advection.f90:
+// 50-57, 71-78, 97-104, 118-125, 156-163; timestep's exchange flags xvel0
+// and yvel0 but not zvel0).  A chunk with a tile under 3 cells thick along
+// some axis takes these instead of the reduced plan below: there a
+// depth-2 reflection reads halo cells of the opposite face, whose values
+// then depend on every earlier exchange.
+inline HaloPlan reference_halo_plan(int which)
+{
+    HaloPlan p{};
+    auto ref = [&](std::initializer_list<int> ids) {
+        for (int f : ids) {
+            p.fields[f] = 1;
+            p.axes[f] = AX_ALL;
+        }
+    };
+    switch (which) {
+    case FH_TIMESTEP: ref({F_PRESSURE, F_ENERGY0, F_DENSITY0, F_XVEL0, F_YVEL0}); p.depth = 1; break;
+    case FH_VISCOSITY: ref({F_VISCOSITY}); p.depth = 1; break;
+    case FH_PRESSURE: ref({F_PRESSURE}); p.depth = 1; break;
+    case FH_CELL1:
+    case FH_CELL2: ref({F_ENERGY1, F_DENSITY1, F_VOL_FLUX_X, F_VOL_FLUX_Y, F_VOL_FLUX_Z}); p.depth = 2; break;
+    default:
+        ref({F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1, F_MASS_FLUX_X, F_MASS_FLUX_Y, F_MASS_FLUX_Z});
+        p.depth = 2;
+        break;
+    }
+    return p;
+}
+
 // the axes along which both faces of a tile are mirrored (Lay::mirror)
 inline int mirror_axes(const Lay &g)
 {

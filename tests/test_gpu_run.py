@@ -125,7 +125,7 @@ INTERIOR_FIELDS = ["density0", "energy0", "pressure", "viscosity", "xvel0", "yve
 # every axis (x tiles of 60/63 nodes, 7 rows per workgroup, 64-plane march
 # chunks)
 @pytest.mark.parametrize("nx,ny,nz,tiles", [(29, 21, 17, 1), (29, 21, 17, 4), (3, 5, 4, 1), (70, 15, 70, 1),
-                                            (130, 9, 3, 2)])
+                                            (130, 9, 3, 2), (2, 9, 6, 1), (11, 2, 5, 1), (8, 6, 2, 1)])
 def test_fused_equals_per_call(nx, ny, nz, tiles):
     """The fused kernels write exactly the per-call kernels' values: equal
     totals (same reduction tree) and bit-identical interiors after 12 steps
