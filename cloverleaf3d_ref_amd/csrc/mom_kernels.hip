@@ -333,40 +333,22 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         }
     }
 
-    // FP64: the plane m+2 the first step needs, then each step issues the
-    // next plane's loads right after its barrier.  FP32: each step loads its
-    // own plane m+2 first (no registers held across the step), which
-    // measured faster at 512^3 (mom_y 1.42 -> 1.35 ms, mom_z 1.36 -> 1.31 ms;
-    // FP64 runs 0.01 ms slower that way).
-    constexpr bool ahead = sizeof(S) == 8;
-    Raw<S> rn{0, 0, 0, 0, 0, 0}, rnm{0, 0, 0, 0, 0, 0};
-    S vn[3] = {0, 0, 0};
-    if constexpr (ahead) {
-        rn = load_raw(mode, g, OFF(m0 + 2), ok_cell, ok_cell && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
-        rnm = load_raw(mode, g, OFF(m0 + 2) - 1, ok_cm, ok_cm && m0 + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxm);
-#pragma unroll
-        for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m0 + 2)] : (S)0;
-    }
+    // each step loads its own plane m+2 first.  Loading the next plane's
+    // values one step ahead instead held them in registers across the step:
+    // at 512^3 that measured FP32 mom_y 1.42 against 1.35 ms and mom_z 1.36
+    // against 1.31 ms, and FP64 (125 against 95 VGPRs, four waves per SIMD
+    // against five) mom_y 2.55 against 2.43 ms and mom_z 2.47 against 2.33.
     for (int m = m0; m <= m1; m++) {
-        if constexpr (!ahead) {
-            rn = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
-            rnm = load_raw(mode, g, OFF(m + 2) - 1, ok_cm, ok_cm && m + 2 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxm);
-#pragma unroll
-            for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 2)] : (S)0;
-        }
-        const Raw<S> rc = rn, rcm = rnm;
+        const Raw<S> rc = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz,
+                                   density1, dxc);
+        const Raw<S> rcm = load_raw(mode, g, OFF(m + 2) - 1, ok_cm, ok_cm && m + 2 <= mmax + 2, mf, vfx, vfy, vfz,
+                                    density1, dxm);
         T vp2[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) vp2[c] = (T)vn[c];
+        for (int c = 0; c < 3; c++) vp2[c] = store ? (T)v.in[c][OFF(m + 2)] : (T)0;
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
         const T am = (T)rcm.a, pm = raw_cell_mass(mode, V0, rcm, ok_cm && m + 2 <= mmax + 2);
         publish65(xbuf[b], lane, w, a, p, am, pm);
-        if (ahead && m < m1) {
-            rn = load_raw(mode, g, OFF(m + 3), ok_cell, ok_cell && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxc);
-            rnm = load_raw(mode, g, OFF(m + 3) - 1, ok_cm, ok_cm && m + 3 <= mmax + 2, mf, vfx, vfy, vfz, density1, dxm);
-#pragma unroll
-            for (int c = 0; c < 3; c++) vn[c] = store ? v.in[c][OFF(m + 3)] : (S)0;
-        }
         Quad<T> Fc, Pc;
         gather65(xbuf[b], lane, w, a, p, Fc, Pc);
         b ^= 1;
