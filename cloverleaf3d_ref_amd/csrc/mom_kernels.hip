@@ -434,12 +434,20 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     Quad<T> Fp, Pp;  // plane l-1
     plane(l0 - 1, Fp, Pp, b);
     const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63, lp2 = lane < 62 ? lane + 2 : 63;
-    // plane l's loads are issued during plane l-1 (after its barrier)
-    Raw<S> rn = load_raw(mode, g, OFF(l0), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc);
-    S vn[3];
+    // FP64: plane l's loads are issued during plane l-1 (after its barrier);
+    // FP32: each plane's at the top of its step, which measured faster at
+    // 512^3 (1.24 -> 1.20 ms; FP64 2.19 -> 2.29 ms that way)
+    constexpr bool ahead = sizeof(S) == 8;
+    Raw<S> rn{0, 0, 0, 0, 0, 0};
+    S vn[3] = {0, 0, 0};
+    auto load_plane = [&](int q) {
+        rn = load_raw(mode, g, OFF(q), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc);
 #pragma unroll
-    for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l0) + dxn] : (S)0;
+        for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(q) + dxn] : (S)0;
+    };
+    if (ahead) load_plane(l0);
     for (int l = l0; l <= l1; l++) {
+        if (!ahead) load_plane(l);
         const Raw<S> rc = rn;
         T vc[3];
 #pragma unroll
@@ -447,11 +455,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
         if (xneg) vc[0] = -vc[0];  // the reflection of the x velocity (update_halo_kernel.f90:604-1004)
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell);
         publish(xbuf[b], lane, w, a, p);
-        if (l < l1) {
-            rn = load_raw(mode, g, OFF(l + 1), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc);
-#pragma unroll
-            for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(l + 1) + dxn] : (S)0;
-        }
+        if (ahead && l < l1) load_plane(l + 1);
         Quad<T> F, P;
         gather<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
         b ^= 1;
