@@ -595,7 +595,9 @@ static hipError_t fast_ideal_gas_t(const Lay &g, const S *density, const S *ener
 // 12 waves: in FP64 the kernel takes 144 VGPRs, three waves per SIMD, so
 // one 12-wave workgroup per CU keeps 12 waves in flight where 8-wave ones
 // kept 8 (4.14 -> 4.05 ms FP64, 2.26 -> 2.23 ms FP32 at 512^3); 10 waves
-// were slower in FP32.
+// were slower in FP32.  The FP32 kernel is held to 80 VGPRs (six waves per
+// SIMD, two workgroups per CU, one value spilled) instead of 81 (five, one
+// workgroup): 2.15 -> 2.05 ms at 512^3.
 constexpr int AW = 12;
 
 template <typename S>
@@ -720,7 +722,7 @@ __device__ __forceinline__ void accel_pdv_body(const Blk &bk, Lay g, Geo m, doub
 }
 
 template <typename S>
-__global__ void __launch_bounds__(64 * AW)
+__global__ void __launch_bounds__(64 * AW, sizeof(S) == 4 ? 6 : 1)
     k_fast_accel_pdv(int xcd, Lay g, Geo m, double dt, const S *__restrict__ density0, S *__restrict__ density1,
                      const S *__restrict__ energy0, S *__restrict__ energy1, const S *__restrict__ pressure,
                      const S *__restrict__ viscosity, const S *__restrict__ xvel0, S *__restrict__ xvel1,
@@ -910,7 +912,8 @@ kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start,
 }
 
 template <typename S>
-__global__ void __launch_bounds__(64 * AW)
+// held to six waves per SIMD in FP32 like k_fast_accel_pdv (80 VGPRs, no spill here)
+__global__ void __launch_bounds__(64 * AW, sizeof(S) == 4 ? 6 : 1)
 kb_accel_pdv(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int planes, int xcd)
 {
     FOR_TILES(xcd)
