@@ -133,6 +133,7 @@ hipError_t batch_tile_halo(Store s, bool copy, const DeckDev *tiles, const TileH
 // k-th axis in y, x, z order; nbr6: 6 neighbours per tile (from 1, -1
 // external)
 void tile_halo_items(const int *nbr6, int ntiles, const Lay *lays, const int *fields, int depth, const int *axes,
+                     const int *raxes,
                      bool poison, std::vector<TileHaloItem> (&copy)[3], std::vector<TileHaloItem> (&refl)[3][2]);
 
 // Per-tile minimum of a batched calc_dt's partials (tile t's partials at

@@ -438,12 +438,13 @@ static void tswap(clover_chunk *c, std::initializer_list<std::pair<int, int>> pa
 
 // update_halo over the tiles in one launch per pass: the three tile passes,
 // then the reflective faces axis by axis (y, x, z)
-static void batched_update_halo(clover_chunk *c, const int *fields, int depth, const int *axes)
+static void batched_update_halo(clover_chunk *c, const int *fields, int depth, const int *axes, const int *raxes)
 {
     const bool poison = axes && c->halo_check;
     std::string key(reinterpret_cast<const char *>(fields), sizeof(int) * NUM_FIELDS);
     key += std::string(1, (char)depth) + std::string(1, (char)poison);
     if (axes) key += std::string(reinterpret_cast<const char *>(axes), sizeof(int) * NUM_FIELDS);
+    if (raxes) key += std::string(reinterpret_cast<const char *>(raxes), sizeof(int) * NUM_FIELDS);
     auto it = c->halo_tabs.find(key);
     if (it == c->halo_tabs.end()) {
         const int T = (int)c->tiles.size();
@@ -454,7 +455,7 @@ static void batched_update_halo(clover_chunk *c, const int *fields, int depth, c
             lays[t] = c->tiles[t].g;
         }
         std::vector<TileHaloItem> copy[3], refl[3][2];
-        tile_halo_items(nbr.data(), T, lays.data(), fields, depth, axes, poison, copy, refl);
+        tile_halo_items(nbr.data(), T, lays.data(), fields, depth, axes, raxes ? raxes : axes, poison, copy, refl);
         clover_chunk::HaloTabs tab{};
         std::vector<TileHaloItem> all;
         static const int axis_order[3] = {1, 0, 2};
@@ -535,11 +536,14 @@ static void set_fields(int *fields, std::initializer_list<int> ids)
     for (int id : ids) fields[id] = 1;
 }
 
-// axes: null, or the per-field axis masks of a fused-step plan (driver.h)
-static void update_halo(clover_chunk *c, const int *fields, int depth, const int *axes = nullptr)
+// axes: null, or the per-field axis masks of a fused-step plan (driver.h);
+// raxes: the masks for the reflective faces (null: axes)
+static void update_halo(clover_chunk *c, const int *fields, int depth, const int *axes = nullptr,
+                        const int *raxes = nullptr)
 {
+    if (!raxes) raxes = axes;
     if (c->batched) {
-        batched_update_halo(c, fields, depth, axes);
+        batched_update_halo(c, fields, depth, axes, raxes);
         return;
     }
     {
@@ -571,7 +575,8 @@ static void update_halo(clover_chunk *c, const int *fields, int depth, const int
         for (int f = 0; f < 6; f++) ext[f] = t.nbr[f] == -1;  // one chunk: every chunk face is external
         void *fp[NUM_FIELDS];
         for (int f = 0; f < NUM_FIELDS; f++) fp[f] = t.f[f];
-        hip_check(launch_update_halo_store(c->store, t.g, fp, fields, ext, depth, c->st, axes, axes && c->halo_check),
+        hip_check(launch_update_halo_store(c->store, t.g, fp, fields, ext, depth, c->st, raxes,
+                                           axes && c->halo_check),
                   "update_halo");
     }
 }
@@ -587,8 +592,12 @@ static void fused_halo(clover_chunk *c, int which)
         update_halo(c, p.fields, p.depth, nullptr);
         return;
     }
-    const HaloPlan p = fused_halo_plan(which, c->advect_x != 0, c->tiles.size() == 1 ? mirror_axes(c->tiles[0].g) : 0);
-    update_halo(c, p.fields, p.depth, p.axes);
+    // every external face of every tile is mirrored (Lay::mirror, set
+    // where no tile is thin)
+    bool mirrored = false;
+    for (auto &t : c->tiles) mirrored = mirrored || t.g.mirror != 0;
+    const HaloPlan p = fused_halo_plan(which, c->advect_x != 0, mirrored ? AX_ALL : 0);
+    update_halo(c, p.fields, p.depth, p.axes, p.raxes);
 }
 
 // need_sound: the caller reads soundspeed too (a fetch); the fused kernels
@@ -1271,14 +1280,18 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         c->partial = dev_alloc(c, (size_t)T * kMaxPartials * 5);
         c->scalar = dev_alloc(c, (size_t)T * 8);
         hip_check(hipHostMalloc(&c->h_scalar, sizeof(double) * 8 * T, hipHostMallocDefault), "hipHostMalloc");
+        bool thin = false;
+        for (auto &t : c->tiles) thin = thin || t.nx < 3 || t.ny < 3 || t.nz < 3;
         for (auto &t : c->tiles) {
             t.g = make_lay(1, t.nx, 1, t.ny, 1, t.nz);
-            // one tile at least 3 cells thick along every axis: every face
-            // is external, and the fused kernels read the mirrored interior
-            // instead of the cell and vertex fields' reflective halos, which
-            // the exchanges then leave out (a thinner tile takes the
-            // reference's exchanges, fused_halo)
-            if (c->fast && T == 1 && t.nx >= 3 && t.ny >= 3 && t.nz >= 3) t.g.mirror = 63;
+            // tiles at least 3 cells thick: on their external faces the
+            // fused kernels read the mirrored interior instead of the cell
+            // and vertex fields' reflective halos, which the exchanges then
+            // leave out (a chunk with a thinner tile takes the reference's
+            // exchanges, fused_halo)
+            if (c->fast && !thin)
+                for (int f = 0; f < 6; f++)
+                    if (t.nbr[f] == -1) t.g.mirror |= 1 << f;
             const int64_t n = lay_elems(t.g);
             if (n >= (int64_t(1) << 31)) throw Error(E_ARG, "tile too large for 32-bit element offsets");
             for (int a = 0; a < NUM_ARRAYS; a++) t.f[a] = dev_alloc_bytes(c, (size_t)n * store_bytes(c->store));

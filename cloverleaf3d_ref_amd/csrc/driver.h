@@ -87,11 +87,12 @@ void host_init(const clover_deck &d, const Tile &t, HostInit &h, bool fields = t
 //                 FH_CELL1's halos, so they are left out of this exchange
 //                 altogether (neither refreshed nor, in the halo-check test
 //                 mode, overwritten).
-//   mirror_axes (axes along which both faces of a one-tile chunk are
-//   mirrored, Lay::mirror): there the kernels read the interior cell or node
-//   that a reflective halo of a cell or vertex field would copy instead of
-//   the halo (the value the reflection would write there, sign flip
-//   included), so those fields' faces are left out of every exchange:
+//   mirror_axes (axes whose external faces are mirrored on every tile,
+//   Lay::mirror): there the kernels read the interior cell or node that a
+//   reflective halo of a cell or vertex field would copy instead of the
+//   halo (the value the reflection would write there, sign flip included),
+//   so those fields' reflections on those axes are left out of every
+//   exchange (raxes; the tile copies of internal faces stay, axes):
 //   visc_dt and accel_pdv mirror every axis (level-0 fields, viscosity,
 //   pressure), the advection sweeps the x axis (level-1 fields; along y
 //   and z the address arithmetic cost them more than the exchanges).  The
@@ -105,7 +106,8 @@ enum : int { AX_X = 1, AX_Y = 2, AX_Z = 4, AX_ALL = 7 };
 struct HaloPlan {
     int depth;
     int fields[NUM_FIELDS];  // 1: refreshed by the reference (and not still valid)
-    int axes[NUM_FIELDS];    // axis bits the fused kernels read
+    int axes[NUM_FIELDS];    // axis bits the fused kernels read: tile copies
+    int raxes[NUM_FIELDS];   // the same for the reflective faces, less the mirrored ones
 };
 
 // The exchange `which` as the reference makes it: every field it flags on
@@ -121,7 +123,7 @@ inline HaloPlan reference_halo_plan(int which)
     auto ref = [&](std::initializer_list<int> ids) {
         for (int f : ids) {
             p.fields[f] = 1;
-            p.axes[f] = AX_ALL;
+            p.axes[f] = p.raxes[f] = AX_ALL;
         }
     };
     switch (which) {
@@ -193,9 +195,10 @@ inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0)
         break;
     default: mom(s3); break;
     }
+    for (int f = 0; f < NUM_FIELDS; f++) p.raxes[f] = p.axes[f];
     for (int f : {F_DENSITY0, F_ENERGY0, F_PRESSURE, F_VISCOSITY, F_SOUNDSPEED, F_XVEL0, F_YVEL0, F_ZVEL0})
-        p.axes[f] &= ~mirror_axes;
-    for (int f : {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1}) p.axes[f] &= ~(mirror_axes & AX_X);
+        p.raxes[f] &= ~mirror_axes;
+    for (int f : {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1}) p.raxes[f] &= ~(mirror_axes & AX_X);
     return p;
 }
 

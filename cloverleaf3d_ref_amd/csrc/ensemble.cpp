@@ -116,7 +116,7 @@ void set_fields(int *fields, std::initializer_list<int> ids)
 int items_on_axis(const HaloPlan &p, int axis)
 {
     int n = 0;
-    for (int f = 0; f < NUM_FIELDS; f++) n += p.fields[f] == 1 && ((p.axes[f] >> axis) & 1);
+    for (int f = 0; f < NUM_FIELDS; f++) n += p.fields[f] == 1 && ((p.raxes[f] >> axis) & 1);
     return 2 * n;  // both faces
 }
 
@@ -143,7 +143,7 @@ void halo(clover_ensemble *e, int which)
         const int n = items_on_axis(p, axis);
         if (n == 0) continue;
         hip_check(batch_reflect(e->store, e->dev, e->nd, grid(e, 100 + axis, p.depth, n), axis, p.depth, p.fields,
-                                e->st, p.axes),
+                                e->st, p.raxes),
                   "halo");
     }
 }

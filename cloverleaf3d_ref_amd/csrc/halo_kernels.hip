@@ -434,7 +434,8 @@ hipError_t batch_tile_halo(Store s, bool copy, const DeckDev *tiles, const TileH
 // a tiled chunk, in the order launch_tile_halo_store and
 // launch_update_halo_store would touch them tile by tile.
 void tile_halo_items(const int *nbr6, int ntiles, const Lay *lays, const int *fields, int depth, const int *axes,
-                     bool poison, std::vector<TileHaloItem> (&copy)[3], std::vector<TileHaloItem> (&refl)[3][2])
+                     const int *raxes, bool poison, std::vector<TileHaloItem> (&copy)[3],
+                     std::vector<TileHaloItem> (&refl)[3][2])
 {
     (void)depth;
     // update_tile_halo.f90:47-423: top/bottom, then left/right, then front/back
@@ -483,7 +484,7 @@ void tile_halo_items(const int *nbr6, int ntiles, const Lay *lays, const int *fi
                 (void)lays;
                 for (int f = 0; f < NUM_FIELDS; f++) {
                     if (fields[f] != 1) continue;
-                    const bool want = wanted(axes, f, axis);
+                    const bool want = wanted(raxes, f, axis);
                     if (!want && !poison) continue;
                     const HaloDesc &h = kHalo[f];
                     TileHaloItem it{};
