@@ -199,6 +199,9 @@ __device__ inline void gather(T (*buf)[W][64], int lane, int w, T a, T p, Quad<T
     P.xw = buf[1][wd][lp];
 }
 
+__device__ inline double shfl_d(double x, int src) { return __shfl(x, src, 64); }
+__device__ inline float shfl_d(float x, int src) { return __shfl(x, src, 64); }
+
 // ---------------------------------------------------------------------------
 // y and z sweeps: lanes along x on 64 line-aligned columns, all of which
 // store; lane 0 also loads and publishes column j-1 (publish65), which the
@@ -258,14 +261,63 @@ __device__ inline void gather65(T (*buf)[W][65], int lane, int w, T a, T p, Quad
     P.xw = buf[1][wd][lane];
 }
 
-template <int DIR, typename S>
-__device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
-                 const S *__restrict__ vfy, const S *__restrict__ vfz,
-                 const S *__restrict__ density1, Vel3<S> v)
+// The last x tile of a sweep holds the node columns left over by the
+// 64-column tiles: one (x_max+1) on a mesh a multiple of 64 cells wide.  In
+// the lane-along-x layout its workgroups march every plane with one live
+// lane, and at 512^3 they held 1/9 of the launch's workgroup slots (the y and
+// z sweeps ran 8 % faster on 511^3, whose nodes fill 8 tiles).  In the FP64
+// form a tail of at most kTailCols columns is marched instead with lanes
+// along the wave axis: one wave per column and 63 rows (lane 0 is the halo
+// row), each lane loading both x neighbours itself and taking the row below
+// from lane-1, in the grid's first workgroups (mom_march).  Every value is
+// formed by the same expressions from the same operands as in the 64-column
+// layout.  Measured at 512^3 FP64: mom_y 2.38 -> 2.32 ms, mom_z 2.26 -> 2.22
+// ms; 640^3 as 8 tiles (320^3 tiles, one tail column in six tiles): the
+// batched momentum launches 4.41 -> 4.16 ms.  The FP32 form keeps the
+// 64-column layout: there the tail's loads (a line per lane, 4 bytes used)
+// cost more than the slots save (mom_y 1.29 -> 1.35 ms, mom_z 1.20 -> 1.25
+// ms, batched 2.29 -> 2.64 ms).
+constexpr int kTailCols = 4;
+static_assert(kTailCols <= mw_of(2) && kTailCols <= mw_of(3), "the tail's waves fit in the tile's workgroups");
+
+__host__ __device__ inline int mom_tail_cols(const Lay &g, int gx)
+{
+    const int tw = g.x_max - g.x_min + 2 - 64 * (gx - 1);  // node columns of the last x tile
+    return tw <= kTailCols ? tw : 0;
+}
+
+// The tail's waves march at most about kTailPlanes planes of a chunk each
+// (nsub pieces of the chunk, as many as the tile's workgroups hold), so
+// that they end with the chunk's 64-column workgroups.
+constexpr int kTailPlanes = 16;
+
+struct TailSplit {
+    int tw, nrg, nsub, tb;  // columns, row groups, pieces per chunk, workgroups per chunk
+};
+
+template <int DIR>
+__device__ inline TailSplit mom_tail_split(const Lay &g, const Blk &bk, int tw)
+{
+    constexpr int MW = mw_of(DIR);
+    const int nrow = DIR == 3 ? g.y_max - g.y_min + 2 : g.z_max - g.z_min + 2;  // storing rows
+    const int npos = DIR == 3 ? g.z_max - g.z_min + 2 : g.y_max - g.y_min + 2;  // march positions
+    TailSplit t;
+    t.tw = tw;
+    t.nrg = (nrow + 62) / 63;
+    const int per = (npos + bk.gz - 1) / bk.gz;  // chunk_range's chunk length
+    t.nsub = max(1, min((per + kTailPlanes - 1) / kTailPlanes, bk.gy * MW / (tw * t.nrg)));
+    t.tb = (tw * t.nrg * t.nsub + MW - 1) / MW;  // <= bk.gy: tw * nrg <= MW * ceil(nrow / (MW-1))
+    return t;
+}
+
+template <int DIR, typename S, bool TAIL>
+__device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of(DIR)][65], TailSplit ts, Lay g,
+                                               Geo geo, int mode, const S *__restrict__ mf,
+                                               const S *__restrict__ vfx, const S *__restrict__ vfy,
+                                               const S *__restrict__ vfz, const S *__restrict__ density1, Vel3<S> v)
 {
     using T = S;  // arithmetic type of the form
     constexpr int MW = mw_of(DIR);
-    __shared__ T xbuf[2][2][MW][65];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int sM = DIR == 3 ? g.plane : g.ld;
     const int wmin = DIR == 3 ? g.y_min : g.z_min, wmax = DIR == 3 ? g.y_max : g.z_max;
@@ -274,21 +326,53 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
     const Recip rd = DIR == 3 ? geo.rz : geo.ry;
     const T V0 = (T)geo.vol;
 
-    const int j = g.x_min + bk.x * 64 + lane;
-    const int wp = wmin - 1 + bk.y * (MW - 1) + w;
-    int m0, m1;
+    int j, wp, m0, m1;
     chunk_range(bk, mmin, mmax + 1, m0, m1);
-    if (m0 > m1) return;  // the whole workgroup: before any barrier
+    if constexpr (TAIL) {
+        // wave item: piece fastest, then row group (rows wmin..wmax+1 store), then column
+        const int item = bk.y * MW + w;
+        if (item >= ts.tw * ts.nrg * ts.nsub) return;  // this wave: no barrier on this path
+        const int sub = item % ts.nsub, rest = item / ts.nsub;
+        j = g.x_min + 64 * (bk.gx - 1) + rest / ts.nrg;
+        wp = wmin - 1 + (rest % ts.nrg) * 63 + lane;
+        const int len = (m1 - m0 + 1 + ts.nsub - 1) / ts.nsub;
+        m0 += sub * len;
+        m1 = min(m0 + len - 1, m1);
+    } else {
+        j = g.x_min + bk.x * 64 + lane;
+        wp = wmin - 1 + bk.y * (MW - 1) + w;
+    }
+    if (m0 > m1) return;  // the whole workgroup (the tail form: the wave), before any barrier
     // j >= x_min-1 and wp >= wmin-1 always; upper limits of the arrays read:
     // cells and the sweep's mass flux end at +2 across the sweep, vertices at +3
     const bool ok_cell = j <= g.x_max + 2 && wp <= wmax + 2;
-    const bool ok_cm = lane == 0 && j - 1 <= g.x_max + 2 && wp <= wmax + 2;  // lane 0's column j-1
-    const bool store = w >= 1 && j <= g.x_max + 1 && wp <= wmax + 1;
+    // column j-1: loaded by lane 0 only in the 64-column layout (lane L-1 holds it for lane L)
+    const bool ok_cm = (TAIL || lane == 0) && j - 1 <= g.x_max + 2 && wp <= wmax + 2;
+    const bool store = (TAIL ? lane : w) >= 1 && j <= g.x_max + 1 && wp <= wmax + 1;
     const int i0 = DIR == 3 ? idx(g, j, wp, mmin) : idx(g, j, mmin, wp);  // offset at march position mmin
 #define OFF(q) (i0 + ((q) - mmin) * sM)
 
     const int dxc = cell_xm(g, j) - j;  // a mirrored x face: the interior cell its halo copies
     const int dxm = cell_xm(g, j - 1) - (j - 1);
+    int b = 0;
+    // the quads of a plane: columns j-1, j of rows wp-1, wp (Quad x, c, xw, w)
+    auto xchg = [&](T a, T p, T am, T pm, Quad<T> &F, Quad<T> &P) {
+        if constexpr (TAIL) {
+            const int src = lane > 0 ? lane - 1 : 0;  // lane 0 (the halo row) stores nothing
+            F.c = a;
+            F.x = am;
+            F.w = shfl_d(a, src);
+            F.xw = shfl_d(am, src);
+            P.c = p;
+            P.x = pm;
+            P.w = shfl_d(p, src);
+            P.xw = shfl_d(pm, src);
+        } else {
+            publish65(xbuf[b], lane, w, a, p, am, pm);
+            gather65(xbuf[b], lane, w, a, p, F, P);
+            b ^= 1;
+        }
+    };
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
         const T a = ok_cell ? (T)mf[i] : (T)0;  // q <= mmax+3: inside the sweep's face extent
@@ -297,13 +381,10 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         const T am = ok_cm ? (T)mf[i - 1] : (T)0;
         const T pm = (ok_cm && q <= mmax + 2) ? (T)density1[i - 1 + dxm] * mom_post_vol(mode, g, i - 1, V0, vfx, vfy, vfz)
                                               : (T)0;
-        publish65(xbuf[b], lane, w, a, p, am, pm);
-        gather65(xbuf[b], lane, w, a, p, F, P);
-        b ^= 1;
+        xchg(a, p, am, pm, F, P);
     };
     auto vel_at = [&](int c, int q) { return store ? (T)v.in[c][OFF(q)] : (T)0; };
 
-    int b = 0;
     Quad<T> Fa, Pa, Fb, Pb;
     // prologue: nf(m0-2..m0), node masses at m0-1, m0, m0+1, velocities m0-2..m0+1
     plane(m0 - 2, Fa, Pa, b);
@@ -348,10 +429,8 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
         for (int c = 0; c < 3; c++) vp2[c] = store ? (T)v.in[c][OFF(m + 2)] : (T)0;
         const T a = (T)rc.a, p = raw_cell_mass(mode, V0, rc, ok_cell && m + 2 <= mmax + 2);
         const T am = (T)rcm.a, pm = raw_cell_mass(mode, V0, rcm, ok_cm && m + 2 <= mmax + 2);
-        publish65(xbuf[b], lane, w, a, p, am, pm);
         Quad<T> Fc, Pc;
-        gather65(xbuf[b], lane, w, a, p, Fc, Pc);
-        b ^= 1;
+        xchg(a, p, am, pm, Fc, Pc);
         const T nf1 = march_node_flux<DIR, T>(Fb, Fc);  // nf(m+1)
         const T mp2 = march_node_mass<DIR, T>(Pb, Pc);  // mp(m+2), garbage at mmax+3 (never used)
         const T mq1 = mp1 - nf0 + nf1;              // mq(m+1)
@@ -376,12 +455,43 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, Lay g, Geo geo, in
 #undef OFF
 }
 
+// One workgroup of a y or z sweep.  With a narrow tail the grid's first
+// workgroups (in launch order) march the tail, TailSplit::tb per chunk: its
+// marches take longest per plane (each load touches a line per lane), and
+// dispatched last they ran past the rest of the launch.  The 64-column
+// tiles follow in the grid's order with gx-1 tiles to a row, and the
+// workgroups left over exit at once.
+template <int DIR, typename S>
+__device__ __forceinline__ void mom_march(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf,
+                                          const S *__restrict__ vfx, const S *__restrict__ vfy,
+                                          const S *__restrict__ vfz, const S *__restrict__ density1, Vel3<S> v)
+{
+    __shared__ S xbuf[2][2][mw_of(DIR)][65];
+    const int tw = sizeof(S) == 8 ? mom_tail_cols(g, bk.gx) : 0;  // the FP64 form only (mom_tail_cols)
+    if (!tw) {
+        mom_march_body<DIR, S, false>(bk, xbuf, TailSplit{}, g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+        return;
+    }
+    const TailSplit ts = mom_tail_split<DIR>(g, bk, tw);
+    int lin = bk.x + bk.gx * (bk.y + bk.gy * bk.z);
+    if (lin < ts.tb * bk.gz) {
+        mom_march_body<DIR, S, true>(Blk{bk.gx - 1, lin % ts.tb, lin / ts.tb, bk.gx, bk.gy, bk.gz}, xbuf, ts, g,
+                                     geo, mode, mf, vfx, vfy, vfz, density1, v);
+        return;
+    }
+    lin -= ts.tb * bk.gz;
+    const int mx = bk.gx - 1;  // 64-column tiles
+    if (lin >= mx * bk.gy * bk.gz) return;
+    mom_march_body<DIR, S, false>(Blk{lin % mx, (lin / mx) % bk.gy, lin / (mx * bk.gy), bk.gx, bk.gy, bk.gz}, xbuf,
+                                  TailSplit{}, g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+}
+
 template <int DIR, typename S>
 __global__ void __launch_bounds__(64 * mw_of(DIR)) k_fast_mom_march(int xcd, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
-    mom_march_body<DIR, S>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+    mom_march<DIR, S>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -389,8 +499,6 @@ __global__ void __launch_bounds__(64 * mw_of(DIR)) k_fast_mom_march(int xcd, Lay
 // along y, march along z.  advec_mom_kernel.f90:172-206:
 //   nf(j)  = 0.125*(f(0,-1,0)+f(0,0,0)+f(1,-1,0)+f(1,0,0)+f(0,-1,-1)+f(0,0,-1)+f(1,-1,-1)+f(1,0,-1))
 //   mp(j)  = cells (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1)
-__device__ inline double shfl_d(double x, int src) { return __shfl(x, src, 64); }
-__device__ inline float shfl_d(float x, int src) { return __shfl(x, src, 64); }
 
 template <typename S>
 __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
@@ -588,15 +696,16 @@ __device__ __forceinline__ void mom_tile(const DeckDev *__restrict__ decks, cons
     if constexpr (DIR == 1)
         mom_x_body<S>(bk, D.g, D.m, mode, static_cast<const S *>(D.f[F_MASS_FLUX_X]), vfx, vfy, vfz, rho, v);
     else
-        mom_march_body<DIR, S>(bk, D.g, D.m, mode,
+        mom_march<DIR, S>(bk, D.g, D.m, mode,
                                static_cast<const S *>(D.f[DIR == 2 ? F_MASS_FLUX_Y : F_MASS_FLUX_Z]), vfx, vfy,
                                vfz, rho, v);
 }
 
 template <int DIR, typename S>
-// The FP64 y/z sweeps held to five waves per SIMD (96 VGPRs, one value
-// spilled; the tile walk takes them to 101, four waves): the batched
-// momentum launches of 640^3 as 8 tiles 4.81 -> 4.35 ms.
+// The FP64 y/z sweeps held to five waves per SIMD (96 VGPRs; the tile walk
+// takes them to 101, four waves): the batched momentum launches of 640^3 as
+// 8 tiles 4.81 -> 4.35 ms with one value spilled.  With the tail form five
+// values spill (20 bytes per lane) and the launches still run 4.41 -> 4.16 ms.
 __global__ void __launch_bounds__(64 * mw_of(DIR), DIR != 1 && sizeof(S) == 8 ? 5 : 1)
 kb_mom(const DeckDev *__restrict__ decks, const int *__restrict__ start, int nd, int mode, int planes, int xcd)
 {

@@ -123,9 +123,11 @@ INTERIOR_FIELDS = ["density0", "energy0", "pressure", "viscosity", "xvel0", "yve
 
 # 70x15x70 and 130x9x3 span several workgroups of the fused kernels along
 # every axis (x tiles of 60/63 nodes, 7 rows per workgroup, 64-plane march
-# chunks)
+# chunks); 64x70x9, 65x9x70 and 128x9x5 as two tiles leave one or two node
+# columns in the y/z momentum sweeps' last 64-column x tile
 @pytest.mark.parametrize("nx,ny,nz,tiles", [(29, 21, 17, 1), (29, 21, 17, 4), (3, 5, 4, 1), (70, 15, 70, 1),
-                                            (130, 9, 3, 2), (2, 9, 6, 1), (11, 2, 5, 1), (8, 6, 2, 1)])
+                                            (130, 9, 3, 2), (2, 9, 6, 1), (11, 2, 5, 1), (8, 6, 2, 1),
+                                            (64, 70, 9, 1), (65, 9, 70, 1), (128, 9, 5, 2)])
 def test_fused_equals_per_call(nx, ny, nz, tiles):
     """The fused kernels write exactly the per-call kernels' values: equal
     totals (same reduction tree) and bit-identical interiors after 12 steps
