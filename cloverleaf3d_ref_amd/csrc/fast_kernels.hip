@@ -74,7 +74,6 @@ __device__ inline QuadS<S> quad_m(QuadS<S> q, bool xlo, bool xhi, bool ylo, bool
     return q;
 }
 
-
 // ---------------------------------------------------------------------------
 // viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
 // viscosity only at the cell itself, so it runs on the value just computed
@@ -277,6 +276,43 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
 // (PdV.f90:103-107) and the predicted soundspeed is recomputed by the next
 // ideal_gas before calc_dt reads it.  Fortran sums a+b+c+d+a+b+c+d as a left
 // fold, so the 8-term face sums continue from the 4-term partial.
+//
+// PdV(predict), revert and ideal_gas at one cell: the node velocities of
+// its planes l (ul, vl, wl) and l+1 (uu, vu, wu) as quad_up_raw quads, its
+// level-0 density, energy and viscosity -> the predicted pressure
+template <typename T>
+__device__ inline T predict_pressure(const QuadS<T> &ul, const QuadS<T> &vl, const QuadS<T> &wl, const QuadS<T> &uu,
+                                     const QuadS<T> &vu, const QuadS<T> &wu, T d0, T e0v, T q0, T xa, T ya, T za,
+                                     T vol, T tdt)
+{
+    const T p0 = pressure_of<T>(d0, e0v);
+#define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
+#define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
+#define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
+    const T sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+    const T sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+    const T sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+    const T stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+    const T sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+    const T sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+#undef U
+#undef V
+#undef W
+    const T left_flux = (xa * sl) * (T)0.125 * tdt * (T)0.5;
+    const T right_flux = (xa * sr) * (T)0.125 * tdt * (T)0.5;
+    const T bottom_flux = (ya * sb) * (T)0.125 * tdt * (T)0.5;
+    const T top_flux = (ya * stp) * (T)0.125 * tdt * (T)0.5;
+    const T back_flux = (za * sk) * (T)0.125 * tdt * (T)0.5;
+    const T front_flux = (za * sf) * (T)0.125 * tdt * (T)0.5;
+    const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
+    const T vol_change_s = quo(vol, vol + total_flux);
+    const T recip_volume = (T)1 / vol;
+    const T energy_change = (quo(p0, d0) + quo_sparse(q0, d0)) * total_flux * recip_volume;
+    const T e1 = e0v - energy_change;
+    const T d1 = d0 * vol_change_s;
+    return (T)(1.4 - 1.0) * d1 * e1;
+}
+
 template <typename S>
 __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, double dt, const S *__restrict__ density0, const S *__restrict__ energy0,
                    S *__restrict__ pressure, const S *__restrict__ viscosity,
@@ -303,7 +339,6 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
     for (int l = l0; l <= l1; l++, i += g.plane) {
         const QuadS<S> uu = un, vu = vn, wu = wn;
         const T d0 = dn, e0v = en, q0 = qn;
-        const T p0 = pressure_of<T>(d0, e0v);
         if (l < l1) {
             const int i1 = i + g.plane, i2 = i + 2 * g.plane;
             un = quad_up_raw(xvel0, i2, g.ld);
@@ -313,31 +348,77 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
             en = energy0[i1];
             qn = viscosity[i1];
         }
-#define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
-#define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
-#define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
-        const T sl = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1) + U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
-        const T sr = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1) + U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
-        const T sb = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1) + V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
-        const T stp = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1) + V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
-        const T sk = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0) + W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
-        const T sf = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1) + W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
-#undef U
-#undef V
-#undef W
-        const T left_flux = (xa * sl) * (T)0.125 * tdt * (T)0.5;
-        const T right_flux = (xa * sr) * (T)0.125 * tdt * (T)0.5;
-        const T bottom_flux = (ya * sb) * (T)0.125 * tdt * (T)0.5;
-        const T top_flux = (ya * stp) * (T)0.125 * tdt * (T)0.5;
-        const T back_flux = (za * sk) * (T)0.125 * tdt * (T)0.5;
-        const T front_flux = (za * sf) * (T)0.125 * tdt * (T)0.5;
-        const T total_flux = right_flux - left_flux + top_flux - bottom_flux + front_flux - back_flux;
-        const T vol_change_s = quo(vol, vol + total_flux);
-        const T recip_volume = (T)1 / vol;
-        const T energy_change = (quo(p0, d0) + quo_sparse(q0, d0)) * total_flux * recip_volume;
-        const T e1 = e0v - energy_change;
-        const T d1 = d0 * vol_change_s;
-        pressure[i] = (T)(1.4 - 1.0) * d1 * e1;
+        pressure[i] = predict_pressure<T>(ul, vl, wl, uu, vu, wu, d0, e0v, q0, xa, ya, za, vol, tdt);
+        ul = uu;
+        vl = vu;
+        wl = wu;
+    }
+}
+
+// The FP32 form with two cells (j, j+1, j even from x_min) per thread: its
+// cells and nodes j, j+1 come in 8-byte loads and node j+2 in a 4-byte one,
+// and the pressures go out in one 8-byte store, so each load carries twice
+// the bytes of the one-cell form's (76 VGPRs against 54).  Each cell's
+// pressure comes from predict_pressure, as in pdv_predict_body.  At 512^3:
+// 834 -> 750 us, three rounds on one machine, same bits.  The batched
+// kernel (kb_pdv_predict) keeps the one-cell form.
+struct NodePair {
+    float2 r0, r1;  // nodes j, j+1 of rows k and k+1
+    float n0, n1;   // node j+2 of rows k and k+1
+};
+
+__device__ inline NodePair load_node_pair(const float *__restrict__ p, int i, int ld)
+{
+    return NodePair{*reinterpret_cast<const float2 *>(p + i), *reinterpret_cast<const float2 *>(p + i + ld), p[i + 2],
+                    p[i + ld + 2]};
+}
+
+__device__ inline QuadS<float> quad_a(const NodePair &n) { return QuadS<float>{{n.r0.x, n.r0.y, n.r1.x, n.r1.y}}; }
+__device__ inline QuadS<float> quad_b(const NodePair &n) { return QuadS<float>{{n.r0.y, n.n0, n.r1.y, n.n1}}; }
+
+__device__ __forceinline__ void pdv_predict_pair_body(const Blk &bk, Lay g, Geo m, double dt,
+                                                      const float *__restrict__ density0,
+                                                      const float *__restrict__ energy0, float *__restrict__ pressure,
+                                                      const float *__restrict__ viscosity,
+                                                      const float *__restrict__ xvel0, const float *__restrict__ yvel0,
+                                                      const float *__restrict__ zvel0)
+{
+    const int j = g.x_min + 2 * (bk.x * (int)blockDim.x + (int)threadIdx.x);
+    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
+    if (j > g.x_max || k > g.y_max) return;
+    int l0, l1;
+    chunk_range(bk, g.z_min, g.z_max, l0, l1);
+    if (l0 > l1) return;
+    const bool two = j < g.x_max;  // cell j+1 is interior (else it is the halo cell x_max+1, loaded, not stored)
+    const float xa = (float)m.xa, ya = (float)m.ya, za = (float)m.za, vol = (float)m.vol, tdt = (float)dt;
+    int i = idx(g, j, k, l0);  // even: x_min sits on a 32-element boundary
+    NodePair ul = load_node_pair(xvel0, i, g.ld), vl = load_node_pair(yvel0, i, g.ld),
+             wl = load_node_pair(zvel0, i, g.ld);
+    NodePair un = load_node_pair(xvel0, i + g.plane, g.ld), vn = load_node_pair(yvel0, i + g.plane, g.ld),
+             wn = load_node_pair(zvel0, i + g.plane, g.ld);
+    auto cells = [&](const float *__restrict__ p, int ii) { return *reinterpret_cast<const float2 *>(p + ii); };
+    float2 dn = cells(density0, i), en = cells(energy0, i), qn = cells(viscosity, i);
+    for (int l = l0; l <= l1; l++, i += g.plane) {
+        const NodePair uu = un, vu = vn, wu = wn;
+        const float2 d0 = dn, e0 = en, q0 = qn;
+        if (l < l1) {
+            const int i1 = i + g.plane, i2 = i + 2 * g.plane;
+            un = load_node_pair(xvel0, i2, g.ld);
+            vn = load_node_pair(yvel0, i2, g.ld);
+            wn = load_node_pair(zvel0, i2, g.ld);
+            dn = cells(density0, i1);
+            en = cells(energy0, i1);
+            qn = cells(viscosity, i1);
+        }
+        const float pa = predict_pressure<float>(quad_a(ul), quad_a(vl), quad_a(wl), quad_a(uu), quad_a(vu),
+                                                 quad_a(wu), d0.x, e0.x, q0.x, xa, ya, za, vol, tdt);
+        if (two) {
+            const float pb = predict_pressure<float>(quad_b(ul), quad_b(vl), quad_b(wl), quad_b(uu), quad_b(vu),
+                                                     quad_b(wu), d0.y, e0.y, q0.y, xa, ya, za, vol, tdt);
+            *reinterpret_cast<float2 *>(pressure + i) = make_float2(pa, pb);
+        } else {
+            pressure[i] = pa;
+        }
         ul = uu;
         vl = vu;
         wl = wu;
@@ -350,7 +431,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_fast_pdv_predict(int xcd, Lay g, 
                    const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                    const S *__restrict__ zvel0)
 {
-    pdv_predict_body<S>(hw_blk(xcd), g, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
+    if constexpr (sizeof(S) == 4)
+        pdv_predict_pair_body(hw_blk(xcd), g, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
+    else
+        pdv_predict_body<S>(hw_blk(xcd), g, m, dt, density0, energy0, pressure, viscosity, xvel0, yvel0, zvel0);
 }
 
 template <typename S>
@@ -360,7 +444,8 @@ static hipError_t fast_pdv_predict_t(const Lay &g, const Geo &m, double dt, cons
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const dim3 blk = block_of(sh);
-    const dim3 grid((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
+    const int cpt = sizeof(S) == 4 ? 2 : 1;  // cells per thread along x
+    const dim3 grid((nx + cpt * blk.x - 1) / (cpt * blk.x), (ny + blk.y - 1) / blk.y, nchunks(nz, sh.planes));
     TIMED_LAUNCH("k_fast_pdv_predict", st, k_fast_pdv_predict<S>, grid, blk, 0, st, sh.xcd, g, m, dt, density0,
                  energy0, pressure, viscosity, xvel0, yvel0, zvel0);
     return hipGetLastError();
