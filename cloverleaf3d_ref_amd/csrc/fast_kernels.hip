@@ -74,6 +74,122 @@ __device__ inline QuadS<S> quad_m(QuadS<S> q, bool xlo, bool xhi, bool ylo, bool
     return q;
 }
 
+// The FP32 forms of visc_dt and the PdV predictor take two cells (j, j+1,
+// j even from x_min) per thread: the node velocities of columns j, j+1 come
+// in one 8-byte load per row and column j+2 in a 4-byte one, and the cell
+// pairs in 8-byte loads and stores.  quad_a and quad_b are the two cells'
+// quad_up_raw quads.
+struct NodePair {
+    float2 r0, r1;  // nodes j, j+1 of rows k and k+1
+    float n0, n1;   // node j+2 of rows k and k+1
+};
+
+__device__ inline NodePair load_node_pair(const float *__restrict__ p, int i, int ld)
+{
+    return NodePair{*reinterpret_cast<const float2 *>(p + i), *reinterpret_cast<const float2 *>(p + i + ld), p[i + 2],
+                    p[i + ld + 2]};
+}
+
+__device__ inline QuadS<float> quad_a(const NodePair &n) { return QuadS<float>{{n.r0.x, n.r0.y, n.r1.x, n.r1.y}}; }
+__device__ inline QuadS<float> quad_b(const NodePair &n) { return QuadS<float>{{n.r0.y, n.n0, n.r1.y, n.n1}}; }
+
+// viscosity_kernel.f90:71-134 and calc_dt_kernel.f90:102-126 at one cell:
+// the node velocities of its planes l (ul, vl, wl) and l+1 (uu, vu, wu) as
+// quad_up_raw quads, its density and pressure, the pressures of its six
+// neighbours -> its viscosity q; raises the running maxima of calc_dt's five
+// denominators (mc, mu, mv, mw, mdiv) with the cell's
+template <typename T>
+__device__ inline T visc_cell(const QuadS<T> &ul, const QuadS<T> &vl, const QuadS<T> &wl, const QuadS<T> &uu,
+                              const QuadS<T> &vu, const QuadS<T> &wu, T rho, T p_c, T p_lo, T p_hi, T p_xm, T p_xp,
+                              T p_ym, T p_yp, const Geo &m, T ds, T dx, T dy, T dz, T xa, T ya, T za, T &mc, T &mu,
+                              T &mv, T &mw, T &mdiv)
+{
+    const T c = soundspeed_of<T>(rho, p_c);
+#define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
+#define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
+#define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
+    const T ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
+    const T ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
+    const T ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
+    const T ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
+    const T ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
+    const T ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
+    const T vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
+    const T vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
+    const T vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
+    const T vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
+    const T vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
+    const T vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
+    const T wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
+    const T wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
+    const T wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
+    const T wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
+    const T wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
+    const T wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
+    // calc_dt_kernel.f90:124-125 sums the z-face nodes in its own order
+    const T dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
+    const T dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
+#undef U
+#undef V
+#undef W
+    const T div = (xa * (ugradx2 - ugradx1) + ya * (vgrady2 - vgrady1)) + za * (wgradz2 - wgradz1);
+    const T xx = by_const_scaled<T>(ugradx2 - ugradx1, 0.25, m.rx);
+    const T yy = by_const_scaled<T>(vgrady2 - vgrady1, 0.25, m.ry);
+    const T zz = by_const_scaled<T>(wgradz2 - wgradz1, 0.25, m.rz);
+    const T xy = by_const_scaled<T>(ugrady2 - ugrady1, 0.25, m.ry) + by_const_scaled<T>(vgradx2 - vgradx1, 0.25, m.rx);
+    const T xz = by_const_scaled<T>(ugradz2 - ugradz1, 0.25, m.rz) + by_const_scaled<T>(wgradx2 - wgradx1, 0.25, m.rx);
+    const T yz = by_const_scaled<T>(vgradz2 - vgradz1, 0.25, m.rz) + by_const_scaled<T>(wgrady2 - wgrady1, 0.25, m.ry);
+    // p difference / (celld + celld): halving the difference instead
+    // of doubling the divisor gives the same quotient (both exact)
+    T pgradx = by_const_scaled<T>(p_xp - p_xm, 0.5, m.rx);
+    T pgrady = by_const_scaled<T>(p_yp - p_ym, 0.5, m.ry);
+    T pgradz = by_const_scaled<T>(p_hi - p_lo, 0.5, m.rz);
+    const T pgradx2 = pgradx * pgradx;
+    const T pgrady2 = pgrady * pgrady;
+    const T pgradz2 = pgradz * pgradz;
+    // the limiter's denominator is positive, so its sign is the
+    // numerator's, and its value is needed only where q is formed
+    // (a positive quotient that underflows to zero gives q = 0 there)
+    const T lim_num = xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
+                      xz * pgradx * pgradz + yz * pgrady * pgradz;
+    const T lim_den = tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16);
+    T limiter = 0;
+    if constexpr (sizeof(T) == 4) limiter = quo(lim_num, lim_den);  // cheaper than a branch
+    T q;
+    if ((lim_num > (T)0) || (div >= (T)0)) {
+        q = (T)0;
+    } else {
+        if constexpr (sizeof(T) == 8) limiter = quo(lim_num, lim_den);
+        pgradx = copysign(tmax((T)1.0e-16, fabs(pgradx)), pgradx);
+        pgrady = copysign(tmax((T)1.0e-16, fabs(pgrady)), pgrady);
+        pgradz = copysign(tmax((T)1.0e-16, fabs(pgradz)), pgradz);
+        const T pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
+        const T xgrad = fabs(quo(dx * pgrad, pgradx));
+        const T ygrad = fabs(quo(dy * pgrad, pgrady));
+        const T zgrad = fabs(quo(dz * pgrad, pgradz));
+        const T grad = tmin(tmin(xgrad, ygrad), zgrad);
+        const T grad2 = grad * grad;
+        q = (T)2 * rho * grad2 * limiter * limiter;
+    }
+
+    // calc_dt: the denominators of the five candidates (visc_dt_body
+    // forms the candidates from their maxima over a thread's cells)
+    T cc = c * c;
+    // in FP64 the divide is skipped where q is zero (adding 2*0/rho
+    // leaves cc as it is); the FP32 reciprocal is cheaper than a branch
+    if (sizeof(T) == 4 || q != (T)0) cc = cc + quo((T)2 * q, rho);
+    mc = tmax(mc, ds * cc);
+    const T du1 = ugradx1 * xa, du2 = ugradx2 * xa;
+    mu = tmax(mu, tmax(fabs(du1), fabs(du2)));
+    const T dv1 = vgrady1 * ya, dv2 = vgrady2 * ya;
+    mv = tmax(mv, tmax(fabs(dv1), fabs(dv2)));
+    const T dw1 = dw1s * za, dw2 = dw2s * za;
+    mw = tmax(mw, tmax(fabs(dw1), fabs(dw2)));
+    const T dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
+    mdiv = tmax(mdiv, fabs(dv));
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // viscosity_kernel.f90:71-134 + calc_dt_kernel.f90:102-149.  calc_dt reads
 // viscosity only at the cell itself, so it runs on the value just computed
@@ -120,92 +236,10 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
                            wu = quad_up_raw(zvel0, i + g.plane, g.ld);
             const T p_hi = pressure[i + (cell_m(g, 2, l + 1) - l) * g.plane];
             const T rho = density0[i];
-            const T c = soundspeed_of<T>(rho, p_c);
             const T p_xm = pressure[i + oxm], p_xp = pressure[i + oxp];
             const T p_ym = pressure[i + oym], p_yp = pressure[i + oyp];
-#define U(a, b, c) ((c) ? uu : ul).v[(a) + 2 * (b)]
-#define V(a, b, c) ((c) ? vu : vl).v[(a) + 2 * (b)]
-#define W(a, b, c) ((c) ? wu : wl).v[(a) + 2 * (b)]
-            const T ugradx1 = U(0, 0, 0) + U(0, 1, 0) + U(0, 0, 1) + U(0, 1, 1);
-            const T ugradx2 = U(1, 0, 0) + U(1, 1, 0) + U(1, 0, 1) + U(1, 1, 1);
-            const T ugrady1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 0, 1) + U(1, 0, 1);
-            const T ugrady2 = U(0, 1, 0) + U(1, 1, 0) + U(0, 1, 1) + U(1, 1, 1);
-            const T ugradz1 = U(0, 0, 0) + U(1, 0, 0) + U(0, 1, 0) + U(1, 1, 0);
-            const T ugradz2 = U(0, 0, 1) + U(1, 0, 1) + U(0, 1, 1) + U(1, 1, 1);
-            const T vgradx1 = V(0, 0, 0) + V(0, 1, 0) + V(0, 0, 1) + V(0, 1, 1);
-            const T vgradx2 = V(1, 0, 0) + V(1, 1, 0) + V(1, 0, 1) + V(1, 1, 1);
-            const T vgrady1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 0, 1) + V(1, 0, 1);
-            const T vgrady2 = V(0, 1, 0) + V(1, 1, 0) + V(0, 1, 1) + V(1, 1, 1);
-            const T vgradz1 = V(0, 0, 0) + V(1, 0, 0) + V(0, 1, 0) + V(1, 1, 0);
-            const T vgradz2 = V(0, 0, 1) + V(1, 0, 1) + V(0, 1, 1) + V(1, 1, 1);
-            const T wgradx1 = W(0, 0, 0) + W(0, 1, 0) + W(0, 0, 1) + W(0, 1, 1);
-            const T wgradx2 = W(1, 0, 0) + W(1, 1, 0) + W(1, 0, 1) + W(1, 1, 1);
-            const T wgrady1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 0, 1) + W(1, 0, 1);
-            const T wgrady2 = W(0, 1, 0) + W(1, 1, 0) + W(0, 1, 1) + W(1, 1, 1);
-            const T wgradz1 = W(0, 0, 0) + W(1, 0, 0) + W(0, 1, 0) + W(1, 1, 0);
-            const T wgradz2 = W(0, 0, 1) + W(1, 0, 1) + W(0, 1, 1) + W(1, 1, 1);
-            // calc_dt_kernel.f90:124-125 sums the z-face nodes in its own order
-            const T dw1s = W(0, 0, 0) + W(0, 1, 0) + W(1, 0, 0) + W(1, 1, 0);
-            const T dw2s = W(0, 0, 1) + W(0, 1, 1) + W(1, 0, 1) + W(1, 1, 1);
-#undef U
-#undef V
-#undef W
-            const T div = (xa * (ugradx2 - ugradx1) + ya * (vgrady2 - vgrady1)) + za * (wgradz2 - wgradz1);
-            const T xx = by_const_scaled<T>(ugradx2 - ugradx1, 0.25, m.rx);
-            const T yy = by_const_scaled<T>(vgrady2 - vgrady1, 0.25, m.ry);
-            const T zz = by_const_scaled<T>(wgradz2 - wgradz1, 0.25, m.rz);
-            const T xy = by_const_scaled<T>(ugrady2 - ugrady1, 0.25, m.ry) + by_const_scaled<T>(vgradx2 - vgradx1, 0.25, m.rx);
-            const T xz = by_const_scaled<T>(ugradz2 - ugradz1, 0.25, m.rz) + by_const_scaled<T>(wgradx2 - wgradx1, 0.25, m.rx);
-            const T yz = by_const_scaled<T>(vgradz2 - vgradz1, 0.25, m.rz) + by_const_scaled<T>(wgrady2 - wgrady1, 0.25, m.ry);
-            // p difference / (celld + celld): halving the difference instead
-            // of doubling the divisor gives the same quotient (both exact)
-            T pgradx = by_const_scaled<T>(p_xp - p_xm, 0.5, m.rx);
-            T pgrady = by_const_scaled<T>(p_yp - p_ym, 0.5, m.ry);
-            T pgradz = by_const_scaled<T>(p_hi - p_lo, 0.5, m.rz);
-            const T pgradx2 = pgradx * pgradx;
-            const T pgrady2 = pgrady * pgrady;
-            const T pgradz2 = pgradz * pgradz;
-            // the limiter's denominator is positive, so its sign is the
-            // numerator's, and its value is needed only where q is formed
-            // (a positive quotient that underflows to zero gives q = 0 there)
-            const T lim_num = xx * pgradx2 + yy * pgrady2 + zz * pgradz2 + xy * pgradx * pgrady +
-                              xz * pgradx * pgradz + yz * pgrady * pgradz;
-            const T lim_den = tmax(pgradx2 + pgrady2 + pgradz2, (T)1.0e-16);
-            T limiter = 0;
-            if constexpr (sizeof(T) == 4) limiter = quo(lim_num, lim_den);  // cheaper than a branch
-            T q;
-            if ((lim_num > (T)0) || (div >= (T)0)) {
-                q = (T)0;
-            } else {
-                if constexpr (sizeof(T) == 8) limiter = quo(lim_num, lim_den);
-                pgradx = copysign(tmax((T)1.0e-16, fabs(pgradx)), pgradx);
-                pgrady = copysign(tmax((T)1.0e-16, fabs(pgrady)), pgrady);
-                pgradz = copysign(tmax((T)1.0e-16, fabs(pgradz)), pgradz);
-                const T pgrad = sqrt(pgradx * pgradx + pgrady * pgrady + pgradz * pgradz);
-                const T xgrad = fabs(quo(dx * pgrad, pgradx));
-                const T ygrad = fabs(quo(dy * pgrad, pgrady));
-                const T zgrad = fabs(quo(dz * pgrad, pgradz));
-                const T grad = tmin(tmin(xgrad, ygrad), zgrad);
-                const T grad2 = grad * grad;
-                q = (T)2 * rho * grad2 * limiter * limiter;
-            }
-            viscosity[i] = q;
-
-            // calc_dt: the denominators of the five candidates (their
-            // maxima over the thread's cells, see below)
-            T cc = c * c;
-            // in FP64 the divide is skipped where q is zero (adding 2*0/rho
-            // leaves cc as it is); the FP32 reciprocal is cheaper than a branch
-            if (sizeof(T) == 4 || q != (T)0) cc = cc + quo((T)2 * q, rho);
-            mc = tmax(mc, ds * cc);
-            const T du1 = ugradx1 * xa, du2 = ugradx2 * xa;
-            mu = tmax(mu, tmax(fabs(du1), fabs(du2)));
-            const T dv1 = vgrady1 * ya, dv2 = vgrady2 * ya;
-            mv = tmax(mv, tmax(fabs(dv1), fabs(dv2)));
-            const T dw1 = dw1s * za, dw2 = dw2s * za;
-            mw = tmax(mw, tmax(fabs(dw1), fabs(dw2)));
-            const T dv = du2 - du1 + dv2 - dv1 + dw2 - dw1;
-            mdiv = tmax(mdiv, fabs(dv));
+            viscosity[i] = visc_cell<T>(ul, vl, wl, uu, vu, wu, rho, p_c, p_lo, p_hi, p_xm, p_xp, p_ym, p_yp, m, ds,
+                                        dx, dy, dz, xa, ya, za, mc, mu, mv, mw, mdiv);
             ul = uu;
             vl = vu;
             wl = wu;
@@ -239,14 +273,99 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
     }
 }
 
+// The FP32 form with two cells per thread (NodePair).  Cell j's x
+// neighbours are j-1 (a 4-byte load) and j+1 (the pair's second pressure,
+// or cell j itself at a mirrored x_max face), cell j+1's are j (the pair's
+// first) and j+2 (a 4-byte load); each cell goes through visc_cell as in
+// visc_dt_body, and the dt candidates come from the thread's maxima.  110
+// VGPRs against 88; at 512^3: 827 -> 743 us, three rounds on one machine,
+// same bits.  The batched kernel (kb_visc_dt) keeps the one-cell form.
+__device__ __forceinline__ void visc_dt_pair_body(const Blk &bk, Lay g, Geo m, double dtc_safe, double dtu_safe,
+                                                  double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
+                                                  const float *__restrict__ density0,
+                                                  const float *__restrict__ pressure, float *__restrict__ viscosity,
+                                                  const float *__restrict__ xvel0, const float *__restrict__ yvel0,
+                                                  const float *__restrict__ zvel0, double *__restrict__ partial)
+{
+    using T = float;
+    const int j = g.x_min + 2 * (bk.x * (int)blockDim.x + (int)threadIdx.x);
+    const int k = g.y_min + bk.y * (int)blockDim.y + (int)threadIdx.y;
+    int l0, l1;
+    chunk_range(bk, g.z_min, g.z_max, l0, l1);
+    const bool in = j <= g.x_max && k <= g.y_max && l0 <= l1;  // every thread reaches the reduction
+    const double mm = dmin(dmin(m.dx, m.dy), m.dz);
+    const T ds = (T)(1.0 / (mm * mm));
+    const T dx = (T)m.dx, dy = (T)m.dy, dz = (T)m.dz;
+    const T xa = (T)m.xa, ya = (T)m.ya, za = (T)m.za, vol = (T)m.vol;
+    const T small = (T)g_small;
+    double dtmin_thread = 1.0e300;
+    if (in) {
+        const bool two = j < g.x_max;  // cell j+1 is interior (else the halo cell x_max+1: loaded, not stored)
+        T mc = 0, mu = 0, mv = 0, mw = 0, mdiv = 0;
+        int i = idx(g, j, k, l0);  // even: x_min sits on a 32-element boundary
+        auto pair = [&](const float *__restrict__ p, int ii) { return *reinterpret_cast<const float2 *>(p + ii); };
+        NodePair ul = load_node_pair(xvel0, i, g.ld), vl = load_node_pair(yvel0, i, g.ld),
+                 wl = load_node_pair(zvel0, i, g.ld);
+        const int oxm = cell_m(g, 0, j - 1) - j, oxp2 = cell_m(g, 0, j + 2) - j;
+        const bool xp_self = cell_m(g, 0, j + 1) != j + 1;  // j = x_max on a mirrored face
+        const int oym = (cell_m(g, 1, k - 1) - k) * g.ld, oyp = (cell_m(g, 1, k + 1) - k) * g.ld;
+        float2 p_lo = pair(pressure, i + (cell_m(g, 2, l0 - 1) - l0) * g.plane), p_c = pair(pressure, i);
+        for (int l = l0; l <= l1; l++, i += g.plane) {
+            const NodePair uu = load_node_pair(xvel0, i + g.plane, g.ld), vu = load_node_pair(yvel0, i + g.plane, g.ld),
+                           wu = load_node_pair(zvel0, i + g.plane, g.ld);
+            const float2 p_hi = pair(pressure, i + (cell_m(g, 2, l + 1) - l) * g.plane);
+            const float2 rho = pair(density0, i);
+            const T p_xm = pressure[i + oxm], p_xp2 = pressure[i + oxp2];
+            const float2 p_ym = pair(pressure, i + oym), p_yp = pair(pressure, i + oyp);
+            const T qa = visc_cell<T>(quad_a(ul), quad_a(vl), quad_a(wl), quad_a(uu), quad_a(vu), quad_a(wu), rho.x,
+                                      p_c.x, p_lo.x, p_hi.x, p_xm, xp_self ? p_c.x : p_c.y, p_ym.x, p_yp.x, m, ds, dx,
+                                      dy, dz, xa, ya, za, mc, mu, mv, mw, mdiv);
+            if (two) {
+                const T qb = visc_cell<T>(quad_b(ul), quad_b(vl), quad_b(wl), quad_b(uu), quad_b(vu), quad_b(wu),
+                                          rho.y, p_c.y, p_lo.y, p_hi.y, p_c.x, p_xp2, p_ym.y, p_yp.y, m, ds, dx, dy, dz,
+                                          xa, ya, za, mc, mu, mv, mw, mdiv);
+                *reinterpret_cast<float2 *>(viscosity + i) = make_float2(qa, qb);
+            } else {
+                viscosity[i] = qa;
+            }
+            ul = uu;
+            vl = vu;
+            wl = wu;
+            p_lo = p_c;
+            p_c = p_hi;
+        }
+        const T dtct = quo((T)dtc_safe * (T)1, tmax(sqrt(mc), small));
+        const T dtut = quo((T)dtu_safe * (T)4 * vol, tmax(mu, (T)1.0e-5 * vol));
+        const T dtvt = quo((T)dtv_safe * (T)4 * vol, tmax(mv, (T)1.0e-5 * vol));
+        const T dtwt = quo((T)dtw_safe * (T)4 * vol, tmax(mw, (T)1.0e-5 * vol));
+        const T dtdivt = quo((T)dtdiv_safe * (T)4 * (vol), tmax(vol * (T)1.0e-05, mdiv));
+        dtmin_thread = (double)tmin(tmin(tmin(tmin(dtct, dtut), dtvt), dtwt), dtdivt);
+    }
+    __shared__ double red[kMaxBlock / 64];
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const int nwaves = (int)(blockDim.x * blockDim.y) / 64;
+    const double w = wave_min(dtmin_thread);
+    if ((tid & 63) == 0) red[tid >> 6] = w;
+    __syncthreads();
+    if (tid == 0) {
+        double b = red[0];
+        for (int q = 1; q < nwaves; q++) b = dmin(b, red[q]);
+        partial[(bk.z * bk.gy + bk.y) * bk.gx + bk.x] = b;
+    }
+}
+
 template <typename S>
 __global__ void __launch_bounds__(kMaxBlock) k_fast_visc_dt(int xcd, Lay g, Geo m, double dtc_safe, double dtu_safe, double dtv_safe, double dtw_safe, double dtdiv_safe,
                double g_small, const S *__restrict__ density0, const S *__restrict__ pressure,
                S *__restrict__ viscosity, const S *__restrict__ xvel0, const S *__restrict__ yvel0,
                const S *__restrict__ zvel0, double *__restrict__ partial)
 {
-    visc_dt_body<S>(hw_blk(xcd), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0, pressure,
-                    viscosity, xvel0, yvel0, zvel0, partial);
+    if constexpr (sizeof(S) == 4)
+        visc_dt_pair_body(hw_blk(xcd), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0,
+                          pressure, viscosity, xvel0, yvel0, zvel0, partial);
+    else
+        visc_dt_body<S>(hw_blk(xcd), g, m, dtc_safe, dtu_safe, dtv_safe, dtw_safe, dtdiv_safe, g_small, density0,
+                        pressure, viscosity, xvel0, yvel0, zvel0, partial);
 }
 
 template <typename S>
@@ -258,7 +377,8 @@ static hipError_t fast_visc_dt_t(const Lay &g, const Geo &m, double dtc_safe, do
 {
     const int nx = g.x_max - g.x_min + 1, ny = g.y_max - g.y_min + 1, nz = g.z_max - g.z_min + 1;
     const dim3 blk = block_of(sh);
-    const int bx = (nx + (int)blk.x - 1) / (int)blk.x, by = (ny + (int)blk.y - 1) / (int)blk.y;
+    const int cpt = sizeof(S) == 4 ? 2 : 1;  // cells per thread along x
+    const int bx = (nx + cpt * (int)blk.x - 1) / (cpt * (int)blk.x), by = (ny + (int)blk.y - 1) / (int)blk.y;
     if (bx * by > kMaxDtPartials) return hipErrorInvalidValue;  // caller keeps tiles within the partial buffer
     int gz = nchunks(nz, sh.planes);
     if (bx * by * gz > kMaxDtPartials) gz = kMaxDtPartials / (bx * by);
@@ -355,26 +475,11 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
     }
 }
 
-// The FP32 form with two cells (j, j+1, j even from x_min) per thread: its
-// cells and nodes j, j+1 come in 8-byte loads and node j+2 in a 4-byte one,
-// and the pressures go out in one 8-byte store, so each load carries twice
-// the bytes of the one-cell form's (76 VGPRs against 54).  Each cell's
-// pressure comes from predict_pressure, as in pdv_predict_body.  At 512^3:
-// 834 -> 750 us, three rounds on one machine, same bits.  The batched
+// The FP32 form with two cells per thread (NodePair): each load carries
+// twice the bytes of the one-cell form's (76 VGPRs against 54).  Each
+// cell's pressure comes from predict_pressure, as in pdv_predict_body.  At
+// 512^3: 834 -> 750 us, three rounds on one machine, same bits.  The batched
 // kernel (kb_pdv_predict) keeps the one-cell form.
-struct NodePair {
-    float2 r0, r1;  // nodes j, j+1 of rows k and k+1
-    float n0, n1;   // node j+2 of rows k and k+1
-};
-
-__device__ inline NodePair load_node_pair(const float *__restrict__ p, int i, int ld)
-{
-    return NodePair{*reinterpret_cast<const float2 *>(p + i), *reinterpret_cast<const float2 *>(p + i + ld), p[i + 2],
-                    p[i + ld + 2]};
-}
-
-__device__ inline QuadS<float> quad_a(const NodePair &n) { return QuadS<float>{{n.r0.x, n.r0.y, n.r1.x, n.r1.y}}; }
-__device__ inline QuadS<float> quad_b(const NodePair &n) { return QuadS<float>{{n.r0.y, n.n0, n.r1.y, n.n1}}; }
 
 __device__ __forceinline__ void pdv_predict_pair_body(const Blk &bk, Lay g, Geo m, double dt,
                                                       const float *__restrict__ density0,
