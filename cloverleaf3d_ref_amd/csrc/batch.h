@@ -44,7 +44,7 @@ enum BatchKernel : int {
     BK_IDEAL_GAS, BK_VISC_DT, BK_PDV_PREDICT, BK_ACCEL_PDV, BK_SUMMARY,
     BK_CELL_X, BK_CELL_Y, BK_CELL_Z, BK_MOM_X, BK_MOM_Y, BK_MOM_Z, BK_COUNT
 };
-int batch_blocks(BatchKernel k, const Lay &g, int planes);
+int batch_blocks(BatchKernel k, const Lay &g, int planes, Store s);
 // reflective halo along `axis` for `nitems` (field, face) items
 int batch_blocks_reflect(const Lay &g, int axis, int depth, int nitems);
 

@@ -390,7 +390,7 @@ static const BatchGrid &tgrid(clover_chunk *c, int k, int planes)
     if (it != c->grids.end()) return it->second;
     const int T = (int)c->tiles.size();
     std::vector<int> start(T + 1, 0);
-    for (int t = 0; t < T; t++) start[t + 1] = start[t] + batch_blocks((BatchKernel)k, c->tiles[t].g, planes);
+    for (int t = 0; t < T; t++) start[t + 1] = start[t] + batch_blocks((BatchKernel)k, c->tiles[t].g, planes, c->store);
     int *p = nullptr;
     hip_check(hipMalloc(&p, sizeof(int) * (T + 1)), "hipMalloc");
     c->tables.push_back(p);
@@ -1321,7 +1321,7 @@ int clover_chunk_create_ex(const clover_deck *deck, int device, int flags, clove
         // march chunk the tuner may pick
         if (c->fast && T > 1) {
             long long need = 0;
-            for (auto &t : c->tiles) need += batch_blocks(BK_VISC_DT, t.g, kTunePlanes[0]);
+            for (auto &t : c->tiles) need += batch_blocks(BK_VISC_DT, t.g, kTunePlanes[0], c->store);
             if (need <= (long long)T * kMaxPartials * 5) {
                 c->batched = true;
                 tiles_table_init(c);

@@ -92,7 +92,7 @@ const BatchGrid &grid(clover_ensemble *e, int kind, int a, int b = 0)
     if (cs != hipStreamCaptureStatusNone) throw Error(E_STATE, "ensemble: launch table missing during capture");
     std::vector<int> counts(e->nd);
     for (int d = 0; d < e->nd; d++)
-        counts[d] = kind < 100 ? batch_blocks((BatchKernel)kind, e->host[d].g, a)
+        counts[d] = kind < 100 ? batch_blocks((BatchKernel)kind, e->host[d].g, a, e->store)
                                : batch_blocks_reflect(e->host[d].g, kind - 100, a, b);
     return e->grids[key] = make_grid(e, counts);
 }

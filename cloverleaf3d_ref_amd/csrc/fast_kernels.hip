@@ -279,7 +279,8 @@ __device__ __forceinline__ void visc_dt_body(const Blk &bk, Lay g, Geo m, double
 // first) and j+2 (a 4-byte load); each cell goes through visc_cell as in
 // visc_dt_body, and the dt candidates come from the thread's maxima.  110
 // VGPRs against 88; at 512^3: 827 -> 743 us, three rounds on one machine,
-// same bits.  The batched kernel (kb_visc_dt) keeps the one-cell form.
+// same bits.  The batched kernel (kb_visc_dt) uses it too: 640^3 as 8
+// tiles 1.68 -> 1.59 ms (the 512-deck ensemble: unchanged).
 __device__ __forceinline__ void visc_dt_pair_body(const Blk &bk, Lay g, Geo m, double dtc_safe, double dtu_safe,
                                                   double dtv_safe, double dtw_safe, double dtdiv_safe, double g_small,
                                                   const float *__restrict__ density0,
@@ -479,7 +480,7 @@ __device__ __forceinline__ void pdv_predict_body(const Blk &bk, Lay g, Geo m, do
 // twice the bytes of the one-cell form's (76 VGPRs against 54).  Each
 // cell's pressure comes from predict_pressure, as in pdv_predict_body.  At
 // 512^3: 834 -> 750 us, three rounds on one machine, same bits.  The batched
-// kernel (kb_pdv_predict) keeps the one-cell form.
+// kernel (kb_pdv_predict) uses it too: 640^3 as 8 tiles 1.66 -> 1.53 ms.
 
 __device__ __forceinline__ void pdv_predict_pair_body(const Blk &bk, Lay g, Geo m, double dt,
                                                       const float *__restrict__ density0,
@@ -1014,9 +1015,10 @@ __host__ __device__ inline void dims_ideal(const Lay &g, int &gx, int &gy, int &
     gz = g.z_max - g.z_min + 1;
 }
 
-__host__ __device__ inline void dims_cells(const Lay &g, int planes, int &gx, int &gy, int &gz)
+// cpt: cells per thread along x (2 in the FP32 visc_dt and predictor)
+__host__ __device__ inline void dims_cells(const Lay &g, int planes, int cpt, int &gx, int &gy, int &gz)
 {
-    gx = (g.x_max - g.x_min + BX) / BX;
+    gx = (g.x_max - g.x_min + cpt * BX) / (cpt * BX);
     gy = (g.y_max - g.y_min + BY) / BY;
     gz = nchunks(g.z_max - g.z_min + 1, planes);
 }
@@ -1080,10 +1082,15 @@ kb_visc_dt(const DeckDev *__restrict__ decks, const int *__restrict__ start, int
 {
     FOR_TILES(xcd)
     {
-        DECK_TILE(t_, dims_cells(D.g, planes, gx, gy, gz))
-        visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
-                        CF(F_DENSITY0), CF(F_PRESSURE), MF(F_VISCOSITY), CF(F_XVEL0),
-                        CF(F_YVEL0), CF(F_ZVEL0), partial + start[d]);
+        DECK_TILE(t_, dims_cells(D.g, planes, sizeof(S) == 4 ? 2 : 1, gx, gy, gz))
+        if constexpr (sizeof(S) == 4)
+            visc_dt_pair_body(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
+                              CF(F_DENSITY0), CF(F_PRESSURE), MF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0),
+                              CF(F_ZVEL0), partial + start[d]);
+        else
+            visc_dt_body<S>(bk, D.g, D.m, D.dtc_safe, D.dtu_safe, D.dtv_safe, D.dtw_safe, D.dtdiv_safe, g_small,
+                            CF(F_DENSITY0), CF(F_PRESSURE), MF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0),
+                            partial + start[d]);
         __syncthreads();
     }
 }
@@ -1094,9 +1101,13 @@ kb_pdv_predict(const DeckDev *__restrict__ decks, const int *__restrict__ start,
 {
     FOR_TILES(xcd)
     {
-        DECK_TILE(t_, dims_cells(D.g, planes, gx, gy, gz))
-        pdv_predict_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE), CF(F_VISCOSITY),
-                            CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
+        DECK_TILE(t_, dims_cells(D.g, planes, sizeof(S) == 4 ? 2 : 1, gx, gy, gz))
+        if constexpr (sizeof(S) == 4)
+            pdv_predict_pair_body(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE),
+                                  CF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
+        else
+            pdv_predict_body<S>(bk, D.g, D.m, D.dt, CF(F_DENSITY0), CF(F_ENERGY0), MF(F_PRESSURE),
+                                CF(F_VISCOSITY), CF(F_XVEL0), CF(F_YVEL0), CF(F_ZVEL0));
         __syncthreads();
     }
 }
@@ -1142,13 +1153,13 @@ kb_field_summary(const DeckDev *__restrict__ decks, const int *__restrict__ star
 int batch_blocks_mom(int dir, const Lay &g, int planes);
 int batch_blocks_cell(int dir, const Lay &g, int planes);
 
-int batch_blocks(BatchKernel k, const Lay &g, int planes)
+int batch_blocks(BatchKernel k, const Lay &g, int planes, Store s)
 {
     int gx = 1, gy = 1, gz = 1;
     switch (k) {
     case BK_IDEAL_GAS: dims_ideal(g, gx, gy, gz); break;
     case BK_VISC_DT:
-    case BK_PDV_PREDICT: dims_cells(g, planes, gx, gy, gz); break;
+    case BK_PDV_PREDICT: dims_cells(g, planes, s == ST_F32 ? 2 : 1, gx, gy, gz); break;
     case BK_ACCEL_PDV: dims_accel_pdv(g, planes, gx, gy, gz); break;
     case BK_SUMMARY: dims_summary(g, gx, gy, gz); break;
     case BK_CELL_X: return batch_blocks_cell(1, g, planes);
