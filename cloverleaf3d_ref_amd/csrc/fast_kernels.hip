@@ -78,7 +78,10 @@ __device__ inline QuadS<S> quad_m(QuadS<S> q, bool xlo, bool xhi, bool ylo, bool
 // j even from x_min) per thread: the node velocities of columns j, j+1 come
 // in one 8-byte load per row and column j+2 in a 4-byte one, and the cell
 // pairs in 8-byte loads and stores.  quad_a and quad_b are the two cells'
-// quad_up_raw quads.
+// quad_up_raw quads.  The pairs are 8-byte aligned: x_min sits on a
+// 32-element boundary of every row (layout.h) and every field array starts
+// on a 256-byte boundary (its own allocation in a chunk, a 256-byte rounded
+// slot of the ensemble's arena).
 struct NodePair {
     float2 r0, r1;  // nodes j, j+1 of rows k and k+1
     float n0, n1;   // node j+2 of rows k and k+1
