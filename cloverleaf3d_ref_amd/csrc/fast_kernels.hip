@@ -791,7 +791,9 @@ static hipError_t fast_ideal_gas_t(const Lay &g, const S *density, const S *ener
 // kept 8 (4.14 -> 4.05 ms FP64, 2.26 -> 2.23 ms FP32 at 512^3); 10 waves
 // were slower in FP32.  The FP32 kernel is held to 80 VGPRs (six waves per
 // SIMD, two workgroups per CU, one value spilled) instead of 81 (five, one
-// workgroup): 2.15 -> 2.05 ms at 512^3.
+// workgroup): 2.15 -> 2.05 ms at 512^3.  16 waves (FP64 held to 128 VGPRs,
+// four waves per SIMD, seven values spilled) ran slower: 4.04 -> 4.55 ms
+// FP64, 2.39 -> 2.55 ms FP32.
 constexpr int AW = 12;
 
 template <typename S>
