@@ -97,7 +97,7 @@ __device__ inline float shfl_c(float x, int src) { return __shfl(x, src, 64); }
 // cell/face j = jt - 2 + i.  A wave walks `rows` consecutive rows (k
 // fastest) and issues the next row's loads before the current row's work;
 // one row per wave leaves too few bytes in flight in the FP32 form.
-template <typename S>
+template <typename S, bool TAN = false>  // TAN: vol_flux_y/z at tan_xm (mom_kernels.hip mom_march_body)
 __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mode, int rows, const S *__restrict__ vfx,
               const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
@@ -122,15 +122,16 @@ __device__ __forceinline__ void cell_x_body(const Blk &bk, Lay g, Geo m, int mod
     S rn = 0, en = 0, fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0, fz0 = 0, fz1 = 0;
     auto row_at = [&](int r) { return idx(g, j, g.y_min + r % ny, g.z_min + r / ny); };
     const int dxc = cell_xm(g, j) - j;  // a mirrored x face: the interior cell its halo copies
+    const int dxt = TAN ? tan_xm(g, j) - j : 0;  // the same for vol_flux_y and vol_flux_z
     auto load = [&](int ii) {
         rn = ok_cell ? density1[ii + dxc] : (S)0;
         en = ok_cell ? energy1[ii + dxc] : (S)0;
         fx0 = ok_face ? vfx[ii] : (S)0;
         fx1 = ok_cell ? vfx[ii + 1] : (S)0;
-        fy0 = (ok_cell && all3) ? vfy[ii] : (S)0;
-        fy1 = (ok_cell && all3) ? vfy[ii + g.ld] : (S)0;
-        fz0 = (ok_cell && all3) ? vfz[ii] : (S)0;
-        fz1 = (ok_cell && all3) ? vfz[ii + g.plane] : (S)0;
+        fy0 = (ok_cell && all3) ? vfy[ii + dxt] : (S)0;
+        fy1 = (ok_cell && all3) ? vfy[ii + dxt + g.ld] : (S)0;
+        fz0 = (ok_cell && all3) ? vfz[ii + dxt] : (S)0;
+        fz1 = (ok_cell && all3) ? vfz[ii + dxt + g.plane] : (S)0;
     };
     int in = row_at(r0);
     load(in);
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(64 * CROWS) k_fast_cell_x(int xcd, Lay g, Geo 
               const S *__restrict__ energy1, S *__restrict__ mass_flux_x, S *__restrict__ rho_out,
               S *__restrict__ e_out, S *__restrict__ p_out, int rows)
 {
-    cell_x_body<S>(hw_blk(xcd), g, m, mode, rows, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out,
+    cell_x_body<S, sizeof(S) == 4>(hw_blk(xcd), g, m, mode, rows, vfx, vfy, vfz, density1, energy1, mass_flux_x, rho_out, e_out,
                    p_out);
 }
 

@@ -596,7 +596,9 @@ static void fused_halo(clover_chunk *c, int which)
     // where no tile is thin)
     bool mirrored = false;
     for (auto &t : c->tiles) mirrored = mirrored || t.g.mirror != 0;
-    const HaloPlan p = fused_halo_plan(which, c->advect_x != 0, mirrored ? AX_ALL : 0);
+    // the single-tile FP32 kernels read vol_flux_y/z's x halos as the mirrored columns (launch.h tan_xm)
+    const HaloPlan p =
+        fused_halo_plan(which, c->advect_x != 0, mirrored ? AX_ALL : 0, c->store == ST_F32 && !c->batched);
     update_halo(c, p.fields, p.depth, p.axes, p.raxes);
 }
 

@@ -146,7 +146,7 @@ inline int mirror_axes(const Lay &g)
     return ((g.mirror & 3) == 3 ? AX_X : 0) | ((g.mirror & 12) == 12 ? AX_Y : 0) | ((g.mirror & 48) == 48 ? AX_Z : 0);
 }
 
-inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0)
+inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0, bool tan_x = false)
 {
     HaloPlan p{};
     auto ref = [&](std::initializer_list<int> ids) {
@@ -199,6 +199,8 @@ inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0)
     for (int f : {F_DENSITY0, F_ENERGY0, F_PRESSURE, F_VISCOSITY, F_SOUNDSPEED, F_XVEL0, F_YVEL0, F_ZVEL0})
         p.raxes[f] &= ~mirror_axes;
     for (int f : {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1}) p.raxes[f] &= ~(mirror_axes & AX_X);
+    if (tan_x)
+        for (int f : {F_VOL_FLUX_Y, F_VOL_FLUX_Z}) p.raxes[f] &= ~(mirror_axes & AX_X);
     return p;
 }
 

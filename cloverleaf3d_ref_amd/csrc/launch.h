@@ -226,6 +226,19 @@ __device__ inline int node_m(const Lay &g, int a, int q)
     return q;
 }
 
+// The same for vol_flux_y and vol_flux_z across the x faces: update_halo
+// reflects them about the edge cell's centre (update_halo_kernel.f90:
+// 1173-1193 and 1306-1324: W = 1-d <- 1+d, W = max+d <- max-d), with no
+// sign change.  Its later y and z passes reflect the x halos' own values, so
+// every corner equals the reflection of the mirrored interior column's
+// value.
+__device__ inline int tan_xm(const Lay &g, int j)
+{
+    if ((g.mirror & 1) && j < g.x_min) return 2 * g.x_min - j;
+    if ((g.mirror & 2) && j > g.x_max) return 2 * g.x_max - j;
+    return j;
+}
+
 __device__ inline int cell_xm(const Lay &g, int j) { return cell_m(g, 0, j); }
 __device__ inline int node_xm(const Lay &g, int j) { return node_m(g, 0, j); }
 __device__ inline bool node_xsign(const Lay &g, int j) { return node_xm(g, j) != j; }

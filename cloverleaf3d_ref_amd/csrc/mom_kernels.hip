@@ -51,14 +51,17 @@ struct Vel3 {
 // 1 sweep 1 along x, 2 sweep 1 along z, 3 sweep 2 with advect_x,
 // 4 sweep 2 without, 5 and 6 sweep 3
 template <typename S>
+// (dxt: the offset of the column vol_flux_y and vol_flux_z are read at, on a
+// mirrored x face, launch.h tan_xm)
 __device__ inline S mom_post_vol(int mode, const Lay &g, int i, S V0, const S *__restrict__ vfx,
-                                 const S *__restrict__ vfy, const S *__restrict__ vfz)
+                                 const S *__restrict__ vfy, const S *__restrict__ vfz, int dxt)
 {
 #define D(x) (x)
+    const int t = i + dxt;
     switch (mode) {
-    case 1: return V0 + D(vfy[i + g.ld]) - D(vfy[i]) + D(vfz[i + g.plane]) - D(vfz[i]);
-    case 2: return V0 + D(vfx[i + 1]) - D(vfx[i]) + D(vfy[i + g.ld]) - D(vfy[i]);
-    case 3: return V0 + D(vfz[i + g.plane]) - D(vfz[i]);
+    case 1: return V0 + D(vfy[t + g.ld]) - D(vfy[t]) + D(vfz[t + g.plane]) - D(vfz[t]);
+    case 2: return V0 + D(vfx[i + 1]) - D(vfx[i]) + D(vfy[t + g.ld]) - D(vfy[t]);
+    case 3: return V0 + D(vfz[t + g.plane]) - D(vfz[t]);
     case 4: return V0 + D(vfx[i + 1]) - D(vfx[i]);
     default: return V0;
     }
@@ -139,20 +142,22 @@ struct Raw {
     S a, rho, v0, v1, v2, v3;  // storage type: converted where used, so the loads stay in flight
 };
 
-// dxc: offset of the density read (a mirrored x face, launch.h cell_xm)
+// dxc: offset of the density read (a mirrored x face, launch.h cell_xm);
+// dxt: of the vol_flux_y and vol_flux_z reads (tan_xm)
 template <typename S>
 __device__ inline Raw<S> load_raw(int mode, const Lay &g, int i, bool ok_a, bool ok_p, const S *__restrict__ mf,
                                   const S *__restrict__ vfx, const S *__restrict__ vfy, const S *__restrict__ vfz,
-                                  const S *__restrict__ density1, int dxc)
+                                  const S *__restrict__ density1, int dxc, int dxt)
 {
     Raw<S> r{0, 0, 0, 0, 0, 0};
     if (ok_a) r.a = mf[i];
     if (ok_p) {
         r.rho = density1[i + dxc];
+        const int t = i + dxt;
         switch (mode) {
-        case 1: r.v0 = vfy[i + g.ld]; r.v1 = vfy[i]; r.v2 = vfz[i + g.plane]; r.v3 = vfz[i]; break;
-        case 2: r.v0 = vfx[i + 1]; r.v1 = vfx[i]; r.v2 = vfy[i + g.ld]; r.v3 = vfy[i]; break;
-        case 3: r.v0 = vfz[i + g.plane]; r.v1 = vfz[i]; break;
+        case 1: r.v0 = vfy[t + g.ld]; r.v1 = vfy[t]; r.v2 = vfz[t + g.plane]; r.v3 = vfz[t]; break;
+        case 2: r.v0 = vfx[i + 1]; r.v1 = vfx[i]; r.v2 = vfy[t + g.ld]; r.v3 = vfy[t]; break;
+        case 3: r.v0 = vfz[t + g.plane]; r.v1 = vfz[t]; break;
         case 4: r.v0 = vfx[i + 1]; r.v1 = vfx[i]; break;
         default: break;
         }
@@ -310,7 +315,10 @@ __device__ inline TailSplit mom_tail_split(const Lay &g, const Blk &bk, int tw)
     return t;
 }
 
-template <int DIR, typename S, bool TAIL>
+// TAN: vol_flux_y/z read at the mirrored column on mirrored x faces
+// (tan_xm); the single-tile FP32 chunk only, whose exchanges leave those x
+// halos out (fused_halo_plan)
+template <int DIR, typename S, bool TAIL, bool TAN = false>
 __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of(DIR)][65], TailSplit ts, Lay g,
                                                Geo geo, int mode, const S *__restrict__ mf,
                                                const S *__restrict__ vfx, const S *__restrict__ vfy,
@@ -354,6 +362,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of
 
     const int dxc = cell_xm(g, j) - j;  // a mirrored x face: the interior cell its halo copies
     const int dxm = cell_xm(g, j - 1) - (j - 1);
+    const int dxt = TAN ? tan_xm(g, j) - j : 0, dxtm = TAN ? tan_xm(g, j - 1) - (j - 1) : 0;  // vol_flux_y/z
     int b = 0;
     // the quads of a plane: columns j-1, j of rows wp-1, wp (Quad x, c, xw, w)
     auto xchg = [&](T a, T p, T am, T pm, Quad<T> &F, Quad<T> &P) {
@@ -376,11 +385,13 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
         const T a = ok_cell ? (T)mf[i] : (T)0;  // q <= mmax+3: inside the sweep's face extent
-        const T p = (ok_cell && q <= mmax + 2) ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz)
-                                               : (T)0;
+        const T p = (ok_cell && q <= mmax + 2)
+                        ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz, dxt)
+                        : (T)0;
         const T am = ok_cm ? (T)mf[i - 1] : (T)0;
-        const T pm = (ok_cm && q <= mmax + 2) ? (T)density1[i - 1 + dxm] * mom_post_vol(mode, g, i - 1, V0, vfx, vfy, vfz)
-                                              : (T)0;
+        const T pm = (ok_cm && q <= mmax + 2)
+                         ? (T)density1[i - 1 + dxm] * mom_post_vol(mode, g, i - 1, V0, vfx, vfy, vfz, dxtm)
+                         : (T)0;
         xchg(a, p, am, pm, F, P);
     };
     auto vel_at = [&](int c, int q) { return store ? (T)v.in[c][OFF(q)] : (T)0; };
@@ -421,9 +432,9 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of
     // against five) mom_y 2.55 against 2.43 ms and mom_z 2.47 against 2.33.
     for (int m = m0; m <= m1; m++) {
         const Raw<S> rc = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz,
-                                   density1, dxc);
+                                   density1, dxc, dxt);
         const Raw<S> rcm = load_raw(mode, g, OFF(m + 2) - 1, ok_cm, ok_cm && m + 2 <= mmax + 2, mf, vfx, vfy, vfz,
-                                    density1, dxm);
+                                    density1, dxm, dxtm);
         T vp2[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) vp2[c] = store ? (T)v.in[c][OFF(m + 2)] : (T)0;
@@ -461,7 +472,7 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of
 // dispatched last they ran past the rest of the launch.  The 64-column
 // tiles follow in the grid's order with gx-1 tiles to a row, and the
 // workgroups left over exit at once.
-template <int DIR, typename S>
+template <int DIR, typename S, bool TAN = false>
 __device__ __forceinline__ void mom_march(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf,
                                           const S *__restrict__ vfx, const S *__restrict__ vfy,
                                           const S *__restrict__ vfz, const S *__restrict__ density1, Vel3<S> v)
@@ -469,20 +480,20 @@ __device__ __forceinline__ void mom_march(const Blk &bk, Lay g, Geo geo, int mod
     __shared__ S xbuf[2][2][mw_of(DIR)][65];
     const int tw = sizeof(S) == 8 ? mom_tail_cols(g, bk.gx) : 0;  // the FP64 form only (mom_tail_cols)
     if (!tw) {
-        mom_march_body<DIR, S, false>(bk, xbuf, TailSplit{}, g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+        mom_march_body<DIR, S, false, TAN>(bk, xbuf, TailSplit{}, g, geo, mode, mf, vfx, vfy, vfz, density1, v);
         return;
     }
     const TailSplit ts = mom_tail_split<DIR>(g, bk, tw);
     int lin = bk.x + bk.gx * (bk.y + bk.gy * bk.z);
     if (lin < ts.tb * bk.gz) {
-        mom_march_body<DIR, S, true>(Blk{bk.gx - 1, lin % ts.tb, lin / ts.tb, bk.gx, bk.gy, bk.gz}, xbuf, ts, g,
+        mom_march_body<DIR, S, true, TAN>(Blk{bk.gx - 1, lin % ts.tb, lin / ts.tb, bk.gx, bk.gy, bk.gz}, xbuf, ts, g,
                                      geo, mode, mf, vfx, vfy, vfz, density1, v);
         return;
     }
     lin -= ts.tb * bk.gz;
     const int mx = bk.gx - 1;  // 64-column tiles
     if (lin >= mx * bk.gy * bk.gz) return;
-    mom_march_body<DIR, S, false>(Blk{lin % mx, (lin / mx) % bk.gy, lin / (mx * bk.gy), bk.gx, bk.gy, bk.gz}, xbuf,
+    mom_march_body<DIR, S, false, TAN>(Blk{lin % mx, (lin / mx) % bk.gy, lin / (mx * bk.gy), bk.gx, bk.gy, bk.gz}, xbuf,
                                   TailSplit{}, g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
@@ -491,7 +502,7 @@ __global__ void __launch_bounds__(64 * mw_of(DIR)) k_fast_mom_march(int xcd, Lay
                  const S *__restrict__ vfy, const S *__restrict__ vfz,
                  const S *__restrict__ density1, Vel3<S> v)
 {
-    mom_march<DIR, S>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+    mom_march<DIR, S, sizeof(S) == 4>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -500,7 +511,7 @@ __global__ void __launch_bounds__(64 * mw_of(DIR)) k_fast_mom_march(int xcd, Lay
 //   nf(j)  = 0.125*(f(0,-1,0)+f(0,0,0)+f(1,-1,0)+f(1,0,0)+f(0,-1,-1)+f(0,0,-1)+f(1,-1,-1)+f(1,0,-1))
 //   mp(j)  = cells (0,-1,0),(0,0,0),(-1,-1,0),(-1,0,0),(0,-1,-1),(0,0,-1),(-1,-1,-1),(-1,0,-1)
 
-template <typename S>
+template <typename S, bool TAN = false>  // TAN: as in mom_march_body
 __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mode, const S *__restrict__ mf, const S *__restrict__ vfx,
              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
              Vel3<S> v)
@@ -527,13 +538,13 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
 #define OFF(q) (i0 + ((q) - g.z_min) * g.plane)
     // a mirrored x face: the interior cell and node its halos copy, and the
     // sign flip of the x velocity there
-    const int dxc = cell_xm(g, j) - j, dxn = node_xm(g, j) - j;
+    const int dxc = cell_xm(g, j) - j, dxn = node_xm(g, j) - j, dxt = TAN ? tan_xm(g, j) - j : 0;
     const bool xneg = node_xsign(g, j);
 
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
         const T a = ok_face ? (T)mf[i] : (T)0;
-        const T p = ok_cell ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz) : (T)0;
+        const T p = ok_cell ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz, dxt) : (T)0;
         exchange<+1, -1>(xbuf[b], lane, w, a, p, F, P);  // mass flux at j+1, cell mass at j-1
         b ^= 1;
     };
@@ -549,7 +560,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     Raw<S> rn{0, 0, 0, 0, 0, 0};
     S vn[3] = {0, 0, 0};
     auto load_plane = [&](int q) {
-        rn = load_raw(mode, g, OFF(q), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc);
+        rn = load_raw(mode, g, OFF(q), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc, dxt);
 #pragma unroll
         for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(q) + dxn] : (S)0;
     };
@@ -594,7 +605,7 @@ __global__ void __launch_bounds__(64 * mw_of(1)) k_fast_mom_x(int xcd, Lay g, Ge
              const S *__restrict__ vfy, const S *__restrict__ vfz, const S *__restrict__ density1,
              Vel3<S> v)
 {
-    mom_x_body<S>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
+    mom_x_body<S, sizeof(S) == 4>(hw_blk(xcd), g, geo, mode, mf, vfx, vfy, vfz, density1, v);
 }
 
 template <typename S>
