@@ -146,6 +146,10 @@ inline int mirror_axes(const Lay &g)
     return ((g.mirror & 3) == 3 ? AX_X : 0) | ((g.mirror & 12) == 12 ? AX_Y : 0) | ((g.mirror & 48) == 48 ? AX_Z : 0);
 }
 
+// mirror_axes: the axes whose faces the fused kernels mirror (their level-0
+// fields everywhere, level-1 ones across x); tan_x: the single-tile FP32
+// kernels also mirror vol_flux_y/z and mass_flux_y/z across x (launch.h
+// tan_xm), so their x reflections are left out too
 inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0, bool tan_x = false)
 {
     HaloPlan p{};
@@ -200,7 +204,7 @@ inline HaloPlan fused_halo_plan(int which, bool advect_x, int mirror_axes = 0, b
         p.raxes[f] &= ~mirror_axes;
     for (int f : {F_DENSITY1, F_ENERGY1, F_XVEL1, F_YVEL1, F_ZVEL1}) p.raxes[f] &= ~(mirror_axes & AX_X);
     if (tan_x)
-        for (int f : {F_VOL_FLUX_Y, F_VOL_FLUX_Z}) p.raxes[f] &= ~(mirror_axes & AX_X);
+        for (int f : {F_VOL_FLUX_Y, F_VOL_FLUX_Z, F_MASS_FLUX_Y, F_MASS_FLUX_Z}) p.raxes[f] &= ~(mirror_axes & AX_X);
     return p;
 }
 

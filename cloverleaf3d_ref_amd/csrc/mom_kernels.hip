@@ -143,14 +143,16 @@ struct Raw {
 };
 
 // dxc: offset of the density read (a mirrored x face, launch.h cell_xm);
-// dxt: of the vol_flux_y and vol_flux_z reads (tan_xm)
+// dxt: of the vol_flux_y and vol_flux_z reads (tan_xm); dma: of the mass
+// flux read (the same in the y and z sweeps, whose mass fluxes follow
+// tan_xm too)
 template <typename S>
 __device__ inline Raw<S> load_raw(int mode, const Lay &g, int i, bool ok_a, bool ok_p, const S *__restrict__ mf,
                                   const S *__restrict__ vfx, const S *__restrict__ vfy, const S *__restrict__ vfz,
-                                  const S *__restrict__ density1, int dxc, int dxt)
+                                  const S *__restrict__ density1, int dxc, int dxt, int dma)
 {
     Raw<S> r{0, 0, 0, 0, 0, 0};
-    if (ok_a) r.a = mf[i];
+    if (ok_a) r.a = mf[i + dma];
     if (ok_p) {
         r.rho = density1[i + dxc];
         const int t = i + dxt;
@@ -315,9 +317,9 @@ __device__ inline TailSplit mom_tail_split(const Lay &g, const Blk &bk, int tw)
     return t;
 }
 
-// TAN: vol_flux_y/z read at the mirrored column on mirrored x faces
-// (tan_xm); the single-tile FP32 chunk only, whose exchanges leave those x
-// halos out (fused_halo_plan)
+// TAN: vol_flux_y/z and the sweep's mass flux (y or z) read at the
+// mirrored column on mirrored x faces (tan_xm); the single-tile FP32 chunk
+// only, whose exchanges leave those x halos out (fused_halo_plan)
 template <int DIR, typename S, bool TAIL, bool TAN = false>
 __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of(DIR)][65], TailSplit ts, Lay g,
                                                Geo geo, int mode, const S *__restrict__ mf,
@@ -384,11 +386,11 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of
     };
     auto plane = [&](int q, Quad<T> &F, Quad<T> &P, int &b) {
         const int i = OFF(q);
-        const T a = ok_cell ? (T)mf[i] : (T)0;  // q <= mmax+3: inside the sweep's face extent
+        const T a = ok_cell ? (T)mf[i + dxt] : (T)0;  // q <= mmax+3: inside the sweep's face extent
         const T p = (ok_cell && q <= mmax + 2)
                         ? (T)density1[i + dxc] * mom_post_vol(mode, g, i, V0, vfx, vfy, vfz, dxt)
                         : (T)0;
-        const T am = ok_cm ? (T)mf[i - 1] : (T)0;
+        const T am = ok_cm ? (T)mf[i - 1 + dxtm] : (T)0;
         const T pm = (ok_cm && q <= mmax + 2)
                          ? (T)density1[i - 1 + dxm] * mom_post_vol(mode, g, i - 1, V0, vfx, vfy, vfz, dxtm)
                          : (T)0;
@@ -432,9 +434,9 @@ __device__ __forceinline__ void mom_march_body(const Blk &bk, S (*xbuf)[2][mw_of
     // against five) mom_y 2.55 against 2.43 ms and mom_z 2.47 against 2.33.
     for (int m = m0; m <= m1; m++) {
         const Raw<S> rc = load_raw(mode, g, OFF(m + 2), ok_cell, ok_cell && m + 2 <= mmax + 2, mf, vfx, vfy, vfz,
-                                   density1, dxc, dxt);
+                                   density1, dxc, dxt, dxt);
         const Raw<S> rcm = load_raw(mode, g, OFF(m + 2) - 1, ok_cm, ok_cm && m + 2 <= mmax + 2, mf, vfx, vfy, vfz,
-                                    density1, dxm, dxtm);
+                                    density1, dxm, dxtm, dxtm);
         T vp2[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) vp2[c] = store ? (T)v.in[c][OFF(m + 2)] : (T)0;
@@ -560,7 +562,7 @@ __device__ __forceinline__ void mom_x_body(const Blk &bk, Lay g, Geo geo, int mo
     Raw<S> rn{0, 0, 0, 0, 0, 0};
     S vn[3] = {0, 0, 0};
     auto load_plane = [&](int q) {
-        rn = load_raw(mode, g, OFF(q), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc, dxt);
+        rn = load_raw(mode, g, OFF(q), ok_face, ok_cell, mf, vfx, vfy, vfz, density1, dxc, dxt, 0);
 #pragma unroll
         for (int c = 0; c < 3; c++) vn[c] = ok_vert ? v.in[c][OFF(q) + dxn] : (S)0;
     };
