@@ -88,3 +88,29 @@ def test_fp32_start_state_is_rounded_fp64():
             assert np.array_equal(a, b.astype(np.float32).astype(np.float64)), name
         s32, s64 = c32.field_summary(), c64.field_summary()
         assert abs(s32["mass"] / s64["mass"] - 1) < 1e-7
+
+
+# single-tile decks (the FP32 kernels mirror vol_flux_y/z and mass_flux_y/z
+# across x there) whose x extent leaves 1, 2, 6 or 7 node columns in the
+# last 64-column tile, over several workgroups along every axis
+@pytest.mark.parametrize("nx,ny,nz", [(29, 21, 17), (70, 15, 70), (64, 70, 9), (3, 5, 4)])
+def test_fp32_reads_no_skipped_halo(nx, ny, nz):
+    """In the poisoned-halo mode every halo the fused FP32 step leaves out
+    of its exchanges holds a wrong value: a kernel that read one would
+    change the interiors.  The FP32 run in that mode equals the plain FP32
+    run bit for bit: totals, dt and every interior field after 12 steps."""
+    from test_gpu_run import INTERIOR_FIELDS, ODD_DECK
+    deck = parse_deck(ODD_DECK.format(nx=nx, ny=ny, nz=nz, steps=12, tiles=1, td="3d"))
+    out = []
+    for check in (False, True):
+        print(f"[gpu] fp32 chunk {nx}x{ny}x{nz} halo_check={check}", flush=True)
+        with Chunk(deck, fp32=True, halo_check=check) as c:
+            dts = [c.step() for _ in range(12)]
+            rows = c.field_summary()
+            fields = {n: c.fetch(n) for n in INTERIOR_FIELDS}
+        out.append((dts, rows, fields))
+    (d0, r0, f0), (d1, r1, f1) = out
+    assert d0 == d1 and r0 == r1
+    for name, a in f0.items():
+        inner = tuple(slice(2, s - 2) for s in a.shape)
+        assert np.array_equal(a[inner].view(np.uint64), f1[name][inner].view(np.uint64)), name
